@@ -1,0 +1,369 @@
+"""RADLite input/output files ("L4" formats): writers and readers.
+
+These keep RADLite's run-directory contract, so the same directory can
+be rendered by the reference ``RADlite`` executable or by this package.
+Every reader states the reference routine whose semantics it follows;
+the writers emit full-precision numbers wherever the reference reads
+list-directed, and the reference's fixed formats where it reads
+formatted (moldata.dat), so both sides see identical doubles.
+
+Array convention used throughout the package: cell fields are numpy
+arrays of shape ``[nr, nth_half]`` (theta index fastest), the order in
+which the reference reads them (``setup.F:778-785``, ``line.F:915-919``).
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+F17 = "{:.17e}"
+
+
+def _fmt(x: float) -> str:
+    return F17.format(float(x))
+
+
+# ----------------------------------------------------------------------
+# writers
+# ----------------------------------------------------------------------
+def write_radlite_inp(path, nphi, dbdr, b_extra, outer_bc=0, incl_deg=45.0):
+    """radlite.inp, format 104 (main.F:535-597; setup.F:1117-1137).
+
+    ``strip_comments`` (tools.F:6-19) keeps only columns 1-16, so every
+    value is left-justified in a 16-column field.
+    """
+    rows = [
+        "104", "=" * 40, "-1", "2", "0", "1.000e-06", "0", "1", "=" * 40,
+        "41", "16", "2", "0.1", "1.0", "3", "=" * 40,
+        str(outer_bc), "2", "0", "=" * 40,
+        repr(float(incl_deg))[:15], str(nphi), str(dbdr), str(b_extra), "=" * 40,
+        "1", "0", "-1", "0", "-1", "0", "0", "0", "=" * 40,
+        "-551", "0", "1", "1", "1", "=" * 40,
+    ]
+    with open(os.path.join(path, "radlite.inp"), "w") as f:
+        for r in rows:
+            if r.startswith("="):
+                f.write(r + "\n")
+            else:
+                assert len(r) <= 15, r
+                f.write(f"{r:<16s}\n")
+
+
+def write_linespectrum_inp(path, vmax, dv, incl_deg, nlines, ilinestart=1,
+                           molfile="moldata.dat", vrad=0.0):
+    """linespectrum.inp, object format 2 (telescope.F:104-238)."""
+    with open(os.path.join(path, "linespectrum.inp"), "w") as f:
+        f.write("1\n1\n" + "-" * 63 + "\n2\n" + "-" * 63 + "\n")
+        f.write(_fmt(vmax) + "\n" + _fmt(dv) + "\n" + "-" * 65 + "\n")
+        f.write("2\n" + molfile + "\n0\n1.0\n")
+        f.write(_fmt(incl_deg) + "\n" + _fmt(vrad) + "\n")
+        f.write(f"{nlines}\n{ilinestart}\n")
+
+
+def write_line_inp(path, nlines, umass_av=2.3, nfreq=40, width=80.0, fix_lw=0.0):
+    """line.inp, format 2 (line.F:168-279). The telescope passband
+    overrides nfreq/width; ``fix_lw`` is the fallback width that
+    line.F:3020-3033 uses for aksmax where the local width is <= 0."""
+    with open(os.path.join(path, "line.inp"), "w") as f:
+        f.write(f"2\n1\n{umass_av!r}\n" + "=" * 60 + "\n")
+        f.write("-" * 60 + "\n1\n0\n" + f"{nfreq}\n{_fmt(width)}\n1\n")
+        if fix_lw > 0:
+            f.write("1\n" + _fmt(fix_lw) + "\n")
+        f.write("0\n")
+        for _ in range(1, nlines):
+            f.write("-" * 60 + "\n0\n")
+
+
+def quantize_moldata(E_cm, g, A):
+    """Values as the reference sees them after its formatted reads:
+    energies F12.4, weights F7.1, Einstein A E12.3 (line.F:1902,1934)."""
+    Eq = np.array([float(f"{e:12.4f}") for e in E_cm])
+    gq = np.array([float(f"{x:7.1f}") for x in g])
+    Aq = np.array([float(f"{a:12.3e}") for a in A])
+    return Eq, gq, Aq
+
+
+def write_moldata(path, name, weight, E_cm, g, lev_up, lev_low, A,
+                  fname="moldata.dat"):
+    """LAMDA-like molecule file read by read_species_lambda
+    (line.F:1865-1935). Levels 1-based; lev_up > lev_low required
+    (line.F:1756-1762)."""
+    E_cm = np.asarray(E_cm, float)
+    with open(os.path.join(path, fname), "w") as f:
+        f.write(f"!MOLECULE\n{name}\n!MOLECULAR WEIGHT\n{weight:4.1f}\n")
+        f.write(f"!NUMBER OF ENERGY LEVELS\n{len(E_cm):6d}\n")
+        f.write("!LEVEL + ENERGIES(cm^-1) + WEIGHT\n")
+        for i, (e, gg) in enumerate(zip(E_cm, g)):
+            f.write(f"{i + 1:5d}{e:12.4f}{gg:7.1f}\n")
+        f.write(f"!NUMBER OF RADIATIVE TRANSITIONS\n{len(A):6d}\n")
+        f.write("!TRANS + UP + LOW + EINSTEINA(s^-1)\n")
+        for i, (u, l, a) in enumerate(zip(lev_up, lev_low, A)):
+            f.write(f"{i + 1:5d}{int(u):5d}{int(l):5d}{a:12.3e}\n")
+
+
+def write_levelpop(path, pops, E_erg=None, g=None, molname="moldata"):
+    """levelpop.info + levelpop_<mol>.dat (line.F:842-919).
+    ``pops`` has shape [nr, nth_half, nlev]."""
+    nr, nth, nlev = pops.shape
+    with open(os.path.join(path, "levelpop.info"), "w") as f:
+        f.write(f"-3\nlevelpop_{molname}.dat\n0\n")
+    E_erg = np.zeros(nlev) if E_erg is None else E_erg
+    g = np.ones(nlev) if g is None else g
+    with open(os.path.join(path, f"levelpop_{molname}.dat"), "w") as f:
+        f.write(f"{nr} {nth} {nlev} 1\n")
+        f.write(" ".join(_fmt(e) for e in E_erg) + "\n")
+        f.write(" ".join(_fmt(x) for x in g) + "\n")
+        flat = pops.reshape(nr * nth, nlev)
+        f.write("\n".join(" ".join(_fmt(x) for x in row) for row in flat))
+        f.write("\n")
+
+
+def _write_scalar(fname, header, arr):
+    with open(fname, "w") as f:
+        f.write(header + "\n")
+        f.write("\n".join(_fmt(x) for x in np.asarray(arr).ravel()))
+        f.write("\n")
+
+
+def write_fields(path, r, theta_half, rho_gas, abund, vel, lw_kms,
+                 rho_dust, t_dust):
+    """radius/theta/density/abundance/velocity/turbulence/dustdens/
+    dusttemp files (grid.F:716-727,1103-1165; setup.F:775-786,824-844,
+    902-911,1484-1485,701-725,159-250)."""
+    nr, nth = rho_gas.shape
+    _write_scalar(os.path.join(path, "radius.inp"), f"{nr}", r)
+    _write_scalar(os.path.join(path, "theta.inp"), f"{nth} 1", theta_half)
+    _write_scalar(os.path.join(path, "density.inp"), f"{nr} {nth} 1", rho_gas)
+    with open(os.path.join(path, "abundance.inp"), "w") as f:
+        f.write(f"{nr} {nth}\n")
+        f.write("\n".join(_fmt(a) + " 0.0" for a in abund.ravel()) + "\n")
+    with open(os.path.join(path, "velocity.inp"), "w") as f:
+        f.write(f"{nr} {nth}\n")
+        f.write("\n".join(" ".join(_fmt(v) for v in row)
+                          for row in vel.reshape(nr * nth, 3)) + "\n")
+    with open(os.path.join(path, "turbulence.inp"), "w") as f:
+        f.write(f"1\n{nr} {nth}\n")
+        f.write("\n".join(_fmt(x) for x in lw_kms.ravel()) + "\n")
+    _write_scalar(os.path.join(path, "dustdens.inp"), f"1 {nr} {nth} 1",
+                  rho_dust)
+    with open(os.path.join(path, "dusttemp.info"), "w") as f:
+        f.write("-2\n1\n1\n1\n1\n")
+    _write_scalar(os.path.join(path, "dusttemp_final.dat"),
+                  f"1 {nr} {nth} 1\n1", t_dust)
+
+
+def write_dust_and_star(path, nu, kappa_abs, kappa_scat, rstar, fnu_1pc):
+    """frequency.inp, dustopac.inp + dustopac_1.inp, starinfo.inp,
+    starspectrum.inp (grid.F:1451-1469; dust.F:92-245,317-520;
+    star.F:449-620)."""
+    _write_scalar(os.path.join(path, "frequency.inp"), f"{len(nu)}", nu)
+    with open(os.path.join(path, "dustopac.inp"), "w") as f:
+        f.write("2\n1\n" + "=" * 40 + "\n-1\n0\n1\n" + "-" * 40 + "\n")
+    with open(os.path.join(path, "dustopac_1.inp"), "w") as f:
+        f.write(f"{len(nu)} 1\n\n")
+        f.write("\n".join(_fmt(k) for k in kappa_abs) + "\n")
+        f.write("\n".join(_fmt(k) for k in kappa_scat) + "\n")
+    with open(os.path.join(path, "starinfo.inp"), "w") as f:
+        f.write("1\n" + _fmt(rstar) + "\n")
+    with open(os.path.join(path, "starspectrum.inp"), "w") as f:
+        f.write(f"{len(nu)}\n")
+        f.write("\n".join(_fmt(a) + " " + _fmt(b) for a, b in zip(nu, fnu_1pc)))
+        f.write("\n")
+
+
+# ----------------------------------------------------------------------
+# readers (semantics of the reference's own readers)
+# ----------------------------------------------------------------------
+def _tokens(fname):
+    with open(fname) as f:
+        return f.read().split()
+
+
+@dataclass
+class RunInputs:
+    """Everything the reference reads for a line spectrum, as doubles."""
+    r: np.ndarray
+    theta_half: np.ndarray
+    rho_gas: np.ndarray            # [nr, nth]
+    abund: np.ndarray              # [nr, nth]
+    vel: np.ndarray                # [nr, nth, 3] cm/s
+    lw_kms: np.ndarray             # [nr, nth] km/s
+    pops: np.ndarray               # [nr, nth, nlev]
+    rho_dust: np.ndarray           # [nspec, nr, nth]
+    t_dust: np.ndarray             # [nspec, nr, nth]
+    cont_nu: np.ndarray            # frequency.inp
+    kappa_abs: np.ndarray          # [nspec, nfreq]
+    kappa_scat: np.ndarray         # [nspec, nfreq]
+    rstar: float
+    star_fnu: np.ndarray           # F_nu at 1 pc on cont_nu
+    umass_av: float
+    fix_lw: float
+    E_cm: np.ndarray
+    g: np.ndarray
+    lev_up: np.ndarray
+    lev_low: np.ndarray
+    A: np.ndarray
+    molname: str
+    nphi: int
+    dbdr: int
+    b_extra: int
+    outer_bc: int
+    vmax: float
+    dv: float
+    incl_deg: float
+    vrad: float
+    nlines_run: int
+    iline_start: int
+    extra: dict = field(default_factory=dict)
+
+
+def read_run_dir(path) -> RunInputs:
+    """Read a RADLite run directory the way main.F:79-238 does."""
+    P = lambda n: os.path.join(path, n)  # noqa: E731
+    # radlite.inp: 16-column fields (tools.F:6-19), order main.F:535-597
+    with open(P("radlite.inp")) as f:
+        rows = [ln[:16].split()[0] for ln in f
+                if ln.strip() and ln[0] not in "#;%="]
+    outer_bc, nphi, dbdr, b_extra = int(rows[13]), int(rows[17]), int(rows[18]), int(rows[19])
+    t = _tokens(P("radius.inp"))
+    nr = int(t[0]); r = np.array(t[1:1 + nr], float)
+    t = _tokens(P("theta.inp"))
+    nth = int(t[0]); theta = np.array(t[2:2 + nth], float)
+    t = _tokens(P("density.inp"))
+    rho = np.array(t[3:3 + nr * nth], float).reshape(nr, nth)
+    t = _tokens(P("abundance.inp"))
+    ab = np.array(t[2:2 + 2 * nr * nth], float).reshape(nr * nth, 2)[:, 0].reshape(nr, nth)
+    t = _tokens(P("velocity.inp"))
+    vel = np.array(t[2:2 + 3 * nr * nth], float).reshape(nr, nth, 3)
+    t = _tokens(P("turbulence.inp"))
+    lw = np.array(t[3:3 + nr * nth], float).reshape(nr, nth)
+    # dust density: nspec nr nt imirt, then per species (setup.F:558-630)
+    t = _tokens(P("dustdens.inp"))
+    nsp = int(t[0])
+    rhod = np.array(t[4:4 + nsp * nr * nth], float).reshape(nsp, nr, nth)
+    t = _tokens(P("dusttemp_final.dat"))
+    nsp_t = int(t[0]); k = 4; td = []
+    for _ in range(nsp_t):
+        nsize = int(t[k]); k += 1
+        assert nsize == 1
+        td.append(np.array(t[k:k + nr * nth], float).reshape(nr, nth)); k += nr * nth
+    td = np.array(td)
+    t = _tokens(P("frequency.inp"))
+    nf = int(t[0]); nu = np.array(t[1:1 + nf], float)
+    with open(P("dustopac.inp")) as f:
+        dl = [ln.split()[0] for ln in f if ln.strip()]
+    nspec = int(dl[1]); ka, ks = [], []
+    for isp in range(nspec):
+        idx = int(dl[5 + 4 * isp])
+        t = _tokens(P(f"dustopac_{idx}.inp"))
+        assert int(t[0]) == nf and int(t[1]) == 1
+        ka.append(np.array(t[2:2 + nf], float))
+        ks.append(np.array(t[2 + nf:2 + 2 * nf], float))
+    t = _tokens(P("starinfo.inp"))
+    rstar = float(t[1])
+    t = _tokens(P("starspectrum.inp"))
+    sp = np.array(t[1:1 + 2 * nf], float).reshape(nf, 2)
+    # line.inp (line.F:168-279)
+    with open(P("line.inp")) as f:
+        ll = [ln.split()[0] for ln in f if ln.strip()]
+    umass = float(ll[2])
+    fix_lw = 0.0
+    # first line block: comment, iinfo=1, symm, nfreq, width, proftype, imore...
+    j = 5
+    j += 5
+    while True:
+        imore = int(ll[j]); j += 1
+        if imore == 0:
+            break
+        fix_lw = float(ll[j]); j += 1
+    # linespectrum.inp (telescope.F:104-238)
+    with open(P("linespectrum.inp")) as f:
+        ls = [ln.split()[0] for ln in f if ln.strip()]
+    vmax, dv = float(ls[5]), float(ls[6])
+    molfile = ls[9]
+    incl_deg, vrad = float(ls[12]), float(ls[13])
+    nlines_run, istart = int(ls[14]), int(ls[15])
+    molname = molfile[:molfile.rfind(".dat")]
+    # moldata (line.F:1865-1935), fixed columns
+    with open(P(molfile)) as f:
+        ml = f.read().splitlines()
+    nlev = int(ml[5][:6])
+    E_cm, g = [], []
+    for i in range(nlev):
+        s = ml[7 + i]
+        E_cm.append(float(s[5:17])); g.append(float(s[17:24]))
+    nlin = int(ml[8 + nlev][:6])
+    up, lo, A = [], [], []
+    for i in range(nlin):
+        s = ml[10 + nlev + i]
+        up.append(int(s[5:10])); lo.append(int(s[10:15])); A.append(float(s[15:27]))
+    # level populations (line.F:842-919)
+    t = _tokens(P(f"levelpop_{molname}.dat"))
+    nlp = int(t[2])
+    pops = np.array(t[4 + 2 * nlp:4 + 2 * nlp + nr * nth * nlp], float).reshape(nr, nth, nlp)
+    return RunInputs(
+        r=r, theta_half=theta, rho_gas=rho, abund=ab, vel=vel, lw_kms=lw,
+        pops=pops, rho_dust=rhod, t_dust=td, cont_nu=nu,
+        kappa_abs=np.array(ka), kappa_scat=np.array(ks), rstar=rstar,
+        star_fnu=sp[:, 1], umass_av=umass, fix_lw=fix_lw,
+        E_cm=np.array(E_cm), g=np.array(g), lev_up=np.array(up),
+        lev_low=np.array(lo), A=np.array(A), molname=molname, nphi=nphi,
+        dbdr=dbdr, b_extra=b_extra, outer_bc=outer_bc, vmax=vmax, dv=dv,
+        incl_deg=incl_deg, vrad=vrad, nlines_run=nlines_run,
+        iline_start=istart)
+
+
+def write_line_spectrum_file(fname, molname, nfr_max, incl_rad, blocks,
+                             distance_pc=1.0, vrad=0.0, style=1):
+    """Spectrum output with the reference's header and per-line blocks
+    (telescope.F:1662-1804, formats at :1675-1682, :1773-1802):
+    rows ``v  F_nu`` in E13.6 for inu = nfr..1.
+
+    ``blocks`` is a list of dicts with keys lev_up, lev_low, nu0, nu
+    (channel frequencies, ascending), flux (same length)."""
+    def e(x, w, d):
+        return _fortran_e(x, w, d)
+    with open(fname, "w") as f:
+        f.write(f"{1:2d}\n{style:2d}\n{molname:<80s}\n{molname + '.dat':<80s}\n")
+        f.write(f"{len(blocks):10d}{nfr_max:10d}\n")
+        f.write(e(distance_pc, 12, 4) + e(vrad, 12, 4) + f"{incl_rad * 57.2957795129:7.3f}\n")
+        for b in blocks:
+            nu0 = b["nu0"]
+            f.write("\n" + f"{b['lev_up']:5d}{b['lev_low']:5d}\n")
+            f.write(e(nu0, 14, 9) + "\n" + e(-1.0, 10, 5) + "\n")
+            f.write(f"{len(b['flux']):5d}\n\n")
+            for k in range(len(b["flux"]) - 1, -1, -1):
+                v = -2.99792458e5 * (b["nu"][k] - nu0) / nu0 + vrad
+                f.write(e(v, 13, 6) + " " + e(b["flux"][k], 13, 6) + "\n")
+
+
+def _fortran_e(x: float, w: int, d: int) -> str:
+    """Fortran Ew.d edit descriptor: 0.ddddE+xx, right-justified."""
+    x = float(x)
+    if x == 0.0:
+        s = "0." + "0" * d + "E+00"
+    else:
+        m, ex = f"{abs(x):.{d - 1}e}".split("e")
+        digits = m.replace(".", "")
+        ex = int(ex) + 1
+        s = ("-" if x < 0 else "") + "0." + digits + ("E%+03d" % ex)
+    return s.rjust(w)
+
+
+def read_line_spectrum_file(fname):
+    """Parse a spectrum file the way pyradlite does (radlite.py:2825-2850)."""
+    with open(fname) as f:
+        lines = f.read().splitlines()
+    nl = int(lines[4].split()[0])
+    out, k = [], 6
+    for _ in range(nl):
+        k += 1
+        up, lo = (int(x) for x in lines[k].split()); k += 1
+        nu0 = float(lines[k]); k += 2
+        nfr = int(lines[k]); k += 2
+        rows = np.array([[float(x) for x in lines[k + i].split()] for i in range(nfr)])
+        k += nfr
+        out.append(dict(lev_up=up, lev_low=lo, nu0=nu0, v=rows[:, 0], flux=rows[:, 1]))
+    return out

@@ -1,0 +1,76 @@
+"""Golden parity cases: seeded models rendered by the compiled reference.
+
+Each case is a run directory produced by radlite_amd.synth from a seed;
+make_golden.py runs the reference build (oracle/_ref/RADlite_probe) on
+it and stores the reference's full-precision results in <name>.npz.
+The tests regenerate the identical inputs (checked by a hash of the
+files) and compare the oracle and the HIP engine against them.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+
+import numpy as np
+
+from radlite_amd import synth
+
+# name -> parameters.  Small grids exercise every branch the reference
+# takes (refinement rings, dbdr insertions, the incl < 0.1 rad clamp,
+# near edge-on views, CMB outer boundary, several lines in one run).
+CASES = {
+    "t1": dict(nr=8, nth=6, nphi=8, b_extra=-2, dbdr=1, nfr=12, vmax=30.0, incl=45.0, seed=1),
+    "t2": dict(nr=10, nth=8, nphi=6, b_extra=-1, dbdr=2, nfr=15, vmax=25.0, incl=20.0, seed=2),
+    "t3": dict(nr=12, nth=10, nphi=12, b_extra=-3, dbdr=1, nfr=21, vmax=40.0, incl=80.0, seed=3),
+    "t4": dict(nr=8, nth=6, nphi=8, b_extra=-1, dbdr=1, nfr=9, vmax=20.0, incl=3.0, seed=4),
+    "t5": dict(nr=9, nth=7, nphi=10, b_extra=-2, dbdr=3, nfr=16, vmax=35.0, incl=60.0, seed=5,
+               outer_bc=2),
+    "t6": dict(nr=10, nth=8, nphi=8, b_extra=-2, dbdr=1, nfr=24, vmax=30.0, incl=35.0, seed=6,
+               nlines=3),
+    "t7": dict(nr=16, nth=12, nphi=16, b_extra=-4, dbdr=2, nfr=30, vmax=30.0, incl=45.0, seed=7,
+               t_scale=0.2),
+    # the CPU-runnable workload: nr=100 x nth=60, 110 rings x 64 phi, 50 ch, 45 deg
+    "c1": dict(nr=100, nth=60, nphi=64, b_extra=-1, dbdr=1, nfr=50, vmax=30.0, incl=45.0, seed=0,
+               sigma_t=0.01, image_stride=7),
+}
+
+
+def build_case(name, path):
+    c = dict(CASES[name])
+    disk = synth.make_disk(c["nr"], c["nth"], seed=c["seed"], sigma_t=c.get("sigma_t", 0.05),
+                           t_scale=c.get("t_scale", 1.0))
+    if c.get("nlines", 1) == 1:
+        lines = synth.single_line()
+    else:
+        lines = synth.random_lines(c["nlines"], seed=c["seed"] + 100, Jmax=10)
+    pops = synth.lte_pops(disk, lines,
+                          q_mode="t_over_2.77" if c.get("nlines", 1) == 1 else "norm")
+    vmax, dv = synth.passband(c["vmax"], c["nfr"])
+    synth.write_run_dir(path, disk, lines, pops, nphi=c["nphi"], dbdr=c["dbdr"],
+                        b_extra=c["b_extra"], vmax=vmax, dv=dv, incl_deg=c["incl"],
+                        outer_bc=c.get("outer_bc", 0))
+    return c
+
+
+INPUT_FILES = ("radlite.inp", "linespectrum.inp", "line.inp", "moldata.dat",
+               "levelpop_moldata.dat", "radius.inp", "theta.inp", "density.inp",
+               "abundance.inp", "velocity.inp", "turbulence.inp", "dustdens.inp",
+               "dusttemp_final.dat", "frequency.inp", "dustopac_1.inp",
+               "starinfo.inp", "starspectrum.inp")
+
+
+def input_hash(path):
+    h = hashlib.sha256()
+    for f in INPUT_FILES:
+        with open(os.path.join(path, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def golden_path(name):
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), f"{name}.npz")
+
+
+def load_golden(name):
+    z = np.load(golden_path(name), allow_pickle=False)
+    return {k: z[k] for k in z.files}

@@ -1,0 +1,100 @@
+"""Write the golden fixtures from the compiled reference.
+
+Runs in this (build) container only: it needs oracle/_ref/RADlite_probe,
+built by ``make -C oracle/refbuild`` from the reference sources.  For
+each case in cases.py it generates the run directory, runs the
+reference, and stores its full-precision results (probe_dump.F) in
+tests/golden/<case>.npz:
+
+  flux[nline, nfr]      spec_flux_observer           telescope.F:1435
+  image[nline, nrows, nfr], image_rows   per-ray intensities (imcir_int)
+  cmask[nline, nrays, nfr]   integrated channels (NONREDUNDANT)
+  minvel/maxvel[nline, nrays]  REAL*4 velocity window
+  overshoot[nline], maser[nline], aksmax, nu0[nline], dnu[nline, nfr]
+  traj_*                trajectories of selected rays (make_trajectory_c)
+  input_sha256          hash of the generated input files
+
+Usage: python tests/golden/make_golden.py [case ...]
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, HERE)
+
+import refprobe  # noqa: E402
+from cases import CASES, build_case, golden_path, input_hash  # noqa: E402
+
+PROBE = os.path.join(ROOT, "oracle", "_ref", "RADlite_probe")
+
+
+def ray_rows(rec):
+    """imcir_int [ring][phi][ch] -> [ray][ch] in ray order (centre first)."""
+    nphi, nrr = rec["nphi"], rec["nrr"]
+    return np.concatenate([rec["image"][0, :1, :], rec["image"][1:].reshape(nrr * nphi, -1)])
+
+
+def main(names):
+    for name in names:
+        c = CASES[name]
+        with tempfile.TemporaryDirectory() as d:
+            build_case(name, d)
+            nrays = 1 + (abs(c["b_extra"]) + 10 + (c["nr"] - 1) * c["dbdr"]) * c["nphi"]
+            stride = c.get("image_stride", 1)
+            tr_stride = 1 if nrays < 2000 else 41
+            sel = list(range(1, nrays + 1, tr_stride))
+            # always include the whole outermost ring
+            sel = sorted(set(sel) | set(range(nrays - c["nphi"] + 1, nrays + 1)))
+            with open(os.path.join(d, "probe_rays.inp"), "w") as f:
+                f.write(f"{len(sel)}\n" + "\n".join(map(str, sel)) + "\n")
+            r = subprocess.run([PROBE], cwd=d, capture_output=True, text=True, timeout=3600)
+            if r.returncode != 0 or not os.path.exists(os.path.join(d, "radlite.success")):
+                print(r.stdout[-3000:], r.stderr[-3000:])
+                raise SystemExit(f"reference failed on {name}")
+            recs = refprobe.read_probe_out(os.path.join(d, "probe_out.bin"))
+            trajs = refprobe.read_probe_traj(os.path.join(d, "probe_traj.bin"))
+            ntr = len(sel)
+            trajs = trajs[:ntr]          # trajectories are line-independent
+            rows = np.arange(0, nrays, stride)
+            out = dict(
+                input_sha256=np.array(input_hash(d)),
+                flux=np.array([x["flux"] for x in recs]),
+                dnu=np.array([x["dnu"] for x in recs]),
+                nu0=np.array([x["nu0"] for x in recs]),
+                image_rows=rows,
+                image=np.array([ray_rows(x)[rows] for x in recs]),
+                cmask=np.array([np.concatenate([np.ones((1, x["nfr"]), np.int32),
+                                                x["cmask"].reshape(-1, x["nfr"])]) for x in recs]
+                               ).astype(np.int8),
+                minvel=np.array([x["minvel"] for x in recs]),
+                maxvel=np.array([x["maxvel"] for x in recs]),
+                overshoot=np.array([x["overshoot"] for x in recs]),
+                maser=np.array([x["maser"] for x in recs]),
+                aksmax=np.array(recs[0]["aksmax"]),
+                ring_ri=recs[0]["ring_ri"],
+                x0=recs[0]["x0"], z0=recs[0]["z0"], theta0=np.array(recs[0]["theta0"]),
+                traj_rays=np.array([t["iray"] for t in trajs]) - 1,
+                traj_n=np.array([len(t["s"]) for t in trajs]),
+                traj_s=np.concatenate([t["s"] for t in trajs]),
+                traj_r=np.concatenate([t["r"] for t in trajs]),
+                traj_theta=np.concatenate([t["theta"] for t in trajs]),
+                traj_mu=np.concatenate([t["mu"] for t in trajs]),
+                traj_phi=np.concatenate([t["phi"] for t in trajs]),
+                traj_icross=np.concatenate([t["icross"] for t in trajs]).astype(np.int8),
+                traj_ir=np.concatenate([t["ir"] for t in trajs]).astype(np.int16),
+                traj_it=np.concatenate([t["it"] for t in trajs]).astype(np.int16),
+            )
+            np.savez_compressed(golden_path(name), **out)
+            print(name, "nrays", nrays, "lines", len(recs), "->",
+                  os.path.getsize(golden_path(name)), "bytes")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or list(CASES))
