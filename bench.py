@@ -1,0 +1,145 @@
+"""Benchmark: ray*channel integrations per second of the circular-image
+line raytrace on one MI355X.
+
+Workload (default, one GPU): the nr=400 x nth=200 disk with 200
+channels over +-40 km/s, 113+10+399 rings x 256 phi = 133,633 rays per
+inclination, inclinations 30 and 60 deg, T with lognormal(0.1) noise, one
+CO-like line at 4.7 um.  One step = the whole hot path for both
+inclinations: ray set, trajectories (FP64), node records, formal
+solution, flux sums (telescope.F:1465 calc_line_spectrum per item).
+
+Prints one JSON line.  value = nominal ray*channels (rays x channels x
+inclinations, the reference's own yardstick) / seconds per step.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--skip", default="none", choices=["none", "reference"])
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--cpu-stride", type=int, default=8,
+                    help="CPU baseline traces every n-th ray of the workload")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding of (line, inclination) items is described in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from radlite_amd import engine, synth
+    ri, w = synth.workload_inputs(args.workload, seed=0)
+    nfr, vmax = engine.passband(ri.vmax, ri.dv)
+    incs = [x * 0.0174532925199 for x in w["incl"]]
+    skip = engine.SKIP_NONE if args.skip == "none" else engine.SKIP_REFERENCE
+    rt = engine.Raytracer(0, args.precision)
+    rt.load_run(ri)
+    items = [(0, i, nfr, vmax, skip) for i in range(len(incs))]
+
+    def step():
+        rt.setup_rays(incs, ri.nphi, ri.b_extra, ri.dbdr)   # rays + trajectories redone
+        fl, _, _ = rt.trace(items)
+        return fl, rt.stats()
+
+    for _ in range(args.warmup):
+        step()
+    rt.L.rl_sync(rt.ctx)
+    t_formal, t_total, stats = [], [], None
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fl, stats = step()
+        t_formal.append(stats["t_formal_ms"])
+        t_total.append(stats["t_total_ms"])
+    rt.L.rl_sync(rt.ctx)
+    dt = (time.perf_counter() - t0) / args.steps
+    nominal = sum(rt.nrays[i] * nfr for i in range(len(incs)))
+    value = nominal / dt
+    nodes = stats["nodes"]
+    rays = stats["rays"]
+    nseg = (nodes - rays) / rays                      # mean segments per ray
+    bytes_per_rc = (28.0 if args.precision == "fp32" else 56.0) * nseg
+    formal_s = float(np.mean(t_formal)) / 1e3
+    achieved_gbs = bytes_per_rc * nominal / formal_s / 1e9
+    flops_per_rc = 40.0 * nseg
+    achieved_tf = flops_per_rc * nominal / formal_s / 1e12
+
+    out = {
+        "metric": "ray*channel integrations/s (forward)",
+        "value": value,
+        "unit": "ray*channel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: nr={w['nr']} x nth={w['nth']}, {rays // len(incs)} rays"
+                        f" x {nfr} ch x {len(incs)} incl, skip={args.skip}",
+            "nr": w["nr"], "nth_half": w["nth"], "rays_per_incl": rays // len(incs),
+            "channels": nfr, "inclinations": list(w["incl"]), "lines": 1,
+            "precision": args.precision, "skip_mode": args.skip,
+            "mean_segments_per_ray": nseg,
+        },
+        "stage_ms": {k: stats[k] for k in ("t_traj_ms", "t_nodes_ms", "t_formal_ms",
+                                           "t_flux_ms", "t_total_ms")},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved_gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_formal",
+            "bytes_per_ray_channel": bytes_per_rc,
+            "kernel_ms": formal_s * 1e3,
+            "valu_tflops_algorithmic": achieved_tf,
+            "valu_frac_of_fp32_peak": achieved_tf / FP32_PEAK_TFLOPS,
+        },
+    }
+    rt.close()
+
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyorc
+        orc = pyorc.Oracle(pyorc.Prepared(ri))
+        cores = min(os.cpu_count() or 1, 16)
+        t0 = time.perf_counter()
+        nrc = 0
+        for i in range(len(incs)):
+            o = orc.line_spectrum(0, incl_rad=incs[i], skip_mode=1 if skip else 0,
+                                  nthreads=cores, ray_stride=args.cpu_stride,
+                                  want_image=False)
+            nrc += ((o["nrays"] + args.cpu_stride - 1) // args.cpu_stride) * nfr
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": nrc / tc, "unit": "ray*channel/s", "cores": cores, "kind": "port",
+            "sample": f"every {args.cpu_stride}th ray of both inclinations "
+                      f"({nrc} ray*channels, {tc:.1f} s), FP64 C restatement, "
+                      f"skip={args.skip}",
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

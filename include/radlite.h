@@ -1,0 +1,150 @@
+/*
+ * radlite.h -- C ABI of the MI355X line raytracer.
+ *
+ * Drop-in for RADLite's image/spectrum raytrace: the per-line driver
+ * calc_line_spectrum (RADLITE/telescope.F:1465) and everything below it
+ * (make_image_circular :339, make_trajectory_c :2787, charintline
+ * :3889, get_line_dust_values line.F:3965, clever_integrate_element_
+ * linedust line.F:4636, integrate_element_linedust line.F:4515,
+ * voigt_profile line.F:2280, qdr_src_2 transfer.F:1498, and the flux
+ * sum calc_freq_flux_observer telescope.F:1320).
+ *
+ * The reference passes everything through COMMON blocks; here the same
+ * quantities are passed explicitly, as plain pointers and sizes.  All
+ * host arrays are caller-owned and only read during the call (they are
+ * copied to device memory).  Cell fields are [nr][nth_half] with theta
+ * fastest -- the order RADLite's readers fill them (setup.F:778-785).
+ *
+ * Return values: 0 on success, otherwise the reference's stop code for
+ * the same condition (13 input/capacity, 124 star resolved by the
+ * central beam, 749 ds<0, 7454/137 hole logic, 6023/6024 interpolation
+ * fraction out of [0,1], 192/987/988 trajectory extra points), or
+ * RL_E_HIP for a device-runtime failure.  rl_last_error() describes the
+ * last failure.  One host thread per context; contexts are independent.
+ */
+#ifndef RADLITE_H
+#define RADLITE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rl_ctx rl_ctx;
+
+enum { RL_FP64 = 0, RL_FP32 = 1 };
+/* RL_SKIP_REFERENCE reproduces the reference's NONREDUNDANT channel
+ * skipping (telescope.F:550-613); RL_SKIP_NONE integrates every channel. */
+enum { RL_SKIP_REFERENCE = 0, RL_SKIP_NONE = 1 };
+enum { RL_WARN_MASER = 1u, RL_WARN_OVERSHOOT = 2u };
+enum { RL_E_HIP = 900, RL_E_ARG = 13 };
+
+/* One (line, inclination) work item.  line: 0-based line index;
+ * incl: 0-based index into the inclinations given to
+ * rl_setup_rays_circular; nfr: channels (>=2); vmax_kms: half width of
+ * the passband (telescope.F:1465 argument "passband"). */
+typedef struct {
+    int line;
+    int incl;
+    int nfr;
+    double vmax_kms;
+    int skip_mode;
+} rl_item;
+
+/* Context: owns device buffers and a stream on `device`.  precision is
+ * RL_FP64 (parity mode) or RL_FP32 (performance mode; FP64 is kept for
+ * the trajectory roots and path lengths). */
+int  rl_create(rl_ctx **ctx, int device, int precision);
+void rl_destroy(rl_ctx *ctx);
+const char *rl_last_error(const rl_ctx *ctx);
+
+/* Spatial grid: radius.inp radii [nr] and theta.inp values [nth_half];
+ * mirrored and given ghost cells as grid.F:711-819,1098-1176 do. */
+int rl_set_grid(rl_ctx *ctx, int nr, const double *r_cm, int nth_half,
+                const double *theta_half);
+
+/* Medium (setup.F:754-911,1453): gas density [g/cm^3], molecular
+ * abundance, velocity [nr][nth][3] (v_r, v_theta, v_phi in cm/s), local
+ * line width lw [km/s] (turbulence.inp), mean molecular weight, and the
+ * maximum line width aksmax [km/s] (line.F:3020-3033). */
+int rl_set_medium(rl_ctx *ctx, const double *rho, const double *abund,
+                  const double *vel_cms, const double *lw_kms,
+                  double umass_av, double aksmax_kms);
+
+/* Level populations [nr][nth][nlev] (line.F:915-919). */
+int rl_set_levels(rl_ctx *ctx, int nlev, const double *pops);
+
+/* Lines: rest frequency nu0 [Hz] (line.F:1980-1983), Einstein A, upper
+ * and lower statistical weights, 0-based level indices. */
+int rl_set_lines(rl_ctx *ctx, int nlines, const double *nu0,
+                 const double *A, const double *gu, const double *gl,
+                 const int *lev_up, const int *lev_low);
+
+/* Grey dust emissivity/extinction at each line centre,
+ * [nr][nth][nlines] (line.F:3687-3743). */
+int rl_set_dust_line(rl_ctx *ctx, const double *jd, const double *ad);
+
+/* Star: radius and specific intensity at the stellar surface on the
+ * continuum grid cont_nu (star.F:508-525); ncont may be 0 (dark star). */
+int rl_set_star(rl_ctx *ctx, double rstar_cm, int ncont,
+                const double *cont_nu, const double *istar);
+
+/* Outer boundary: 0 = black, 2 = CMB (telescope.F:3984-4028). */
+int rl_set_outer_bc(rl_ctx *ctx, int type);
+
+/* Circular-image ray sets, one per inclination (setup_rays_circular,
+ * telescope.F:715-1191 with imethod=1, nrref=10).  Outputs (optional,
+ * may be NULL): per inclination the ray and ring counts, and the ring
+ * radii [nrings+1] and ring edges [nrings+2] of the first inclination. */
+int rl_setup_rays_circular(rl_ctx *ctx, int nincl, const double *incl_rad,
+                           int nphi, int nrext, int dbdr, int *n_rays,
+                           int *n_rings, double *ring_r, double *ring_edge);
+
+/* Trace a batch of (line, inclination) items.  flux: concatenation of
+ * every item's nfr values in erg/s/cm^2/Hz at 1 pc (telescope.F:1435).
+ * image (optional): concatenation per item of [nrays][nfr] intensities
+ * (ray 0 is the central beam, then ring-major, phi fastest).
+ * warn_bits (optional): per item RL_WARN_* flags. */
+int rl_trace_lines_forward(rl_ctx *ctx, int nitems, const rl_item *items,
+                           double *flux, double *image, unsigned *warn_bits);
+
+/* Adjoint of rl_trace_lines_forward for the same items: given
+ * dL/dflux (same layout as flux), accumulate dL/d(cell table) for the
+ * quantities the trace reads: g_nup / g_nlow [nlines][nr][nth] (upper
+ * and lower level number densities), g_jd / g_ad [nlines][nr][nth],
+ * g_lw [nr][nth].  Any output may be NULL.  Outputs are overwritten. */
+int rl_trace_lines_adjoint(rl_ctx *ctx, int nitems, const rl_item *items,
+                           const double *dL_dflux, double *g_nup,
+                           double *g_nlow, double *g_jd, double *g_ad,
+                           double *g_lw);
+
+/* Wait for all work of the context. */
+int rl_sync(rl_ctx *ctx);
+
+/* Milliseconds of device time of the last trace call, by stage:
+ * t[0] trajectories, t[1] node records, t[2] formal solution,
+ * t[3] flux sum, t[4] total; and counts c[0] rays, c[1] nodes,
+ * c[2] integrated ray*channels, c[3] nominal ray*channels. */
+int rl_last_stats(const rl_ctx *ctx, double *t_ms, long long *counts);
+
+/* ---- host pre-pass with the reference's arithmetic (R13) ---------- */
+/* nu0 = 1.509160e26 * 1.986468498e-16 * (E_up - E_low)[cm^-1]. */
+int rl_prepare_line_freqs(int nlines, const double *E_cm, const int *lev_up,
+                          const int *lev_low, double *nu0);
+/* grey dust j, alpha at each line centre from dust density/temperature
+ * [nspec][nr][nth] and kappa_abs/kappa_scat [nspec][nfreq]. */
+int rl_prepare_line_dust(int nr, int nth_half, int nspec, const double *rho_d,
+                         const double *t_d, int nfreq, const double *cont_nu,
+                         const double *kabs, const double *kscat, int nlines,
+                         const double *nu0, double *jd, double *ad);
+/* starspectrum.inp F_nu at 1 pc -> I_nu at the stellar surface. */
+int rl_prepare_star(int nfreq, const double *fnu_1pc, double rstar,
+                    double *istar);
+/* aksmax over cells (lw <= 0 falls back to fix_lw). */
+double rl_prepare_aksmax(int ncell, const double *lw_kms, double fix_lw);
+/* nfr = int(2 vmax/dv + 1), vmax := (nfr-1) dv / 2 (main.F:209-210). */
+int rl_passband(double vmax, double dv, int *nfr, double *vmax_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -173,7 +173,8 @@ class Oracle:
         return out
 
     def line_spectrum(self, iline, incl_rad=None, nphi=None, nrext=None, dbdr=None,
-                      nfr=None, vmax=None, skip_mode=0, nthreads=0, want_image=True):
+                      nfr=None, vmax=None, skip_mode=0, nthreads=0, want_image=True,
+                      ray_stride=1):
         p = self.p
         incl_rad = p.incl_rad if incl_rad is None else incl_rad
         nphi = p.ri.nphi if nphi is None else nphi
@@ -193,6 +194,7 @@ class Oracle:
         rc = lib().orc_line_spectrum(
             C.byref(self.m), C.byref(self.g), C.byref(rays), C.c_int(iline),
             C.c_double(vmax), C.c_int(nfr), C.c_int(skip_mode), C.c_int(nthreads),
+            C.c_int(ray_stride),
             _p(flux), _p(image) if want_image else None,
             _p(cmask, ip) if want_image else None, _p(mn, fp), _p(mx, fp), _p(dnu),
             C.byref(maser), C.byref(ovs), C.byref(nint))

@@ -907,7 +907,7 @@ static double star_at(const orc_model *m, double freq)
 
 int orc_line_spectrum(const orc_model *m, const orc_grid *g,
                       const orc_rays *rays, int il, double width,
-                      int nfr, int skip_mode, int nthreads,
+                      int nfr, int skip_mode, int nthreads, int ray_stride,
                       double *flux, double *image, int *cmask,
                       float *minvel, float *maxvel, double *dnu_out,
                       int *maser_out, long *overshoot_out, long *nint_out)
@@ -949,6 +949,7 @@ int orc_line_spectrum(const orc_model *m, const orc_grid *g,
 #endif
     for (int iray = 0; iray < nrays; iray++) {
         orc_traj t;
+        if (ray_stride > 1 && iray % ray_stride != 0) continue;
         raycache_t cache;
         line_ctx L;
         int e, ch, mz = 0;
@@ -1009,7 +1010,7 @@ int orc_line_spectrum(const orc_model *m, const orc_grid *g,
         orc_traj_free(&t);
         free_cache(&cache);
     }
-    if (!err) {
+    if (!err && ray_stride <= 1) {
         /* calc_freq_flux_observer, telescope.F:1386-1435 */
         const double *ri = rays->ring_ri;
         int nphi = rays->nphi, nrings = rays->nrings;

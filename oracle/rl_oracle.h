@@ -75,10 +75,12 @@ void orc_traj_free(orc_traj *t);
 /* One line at one inclination (calc_line_spectrum, telescope.F:1465).
  * skip_mode 0 = the reference's NONREDUNDANT channel skipping,
  * 1 = integrate every channel.  image: [nrays][nfr] or NULL.
+ * ray_stride > 1 traces only every ray_stride-th ray (image rows of the
+ * others are left untouched and the flux is not computed). 
  * Returns 0 or a reference stop code. */
 int orc_line_spectrum(const orc_model *m, const orc_grid *g,
                       const orc_rays *rays, int iline, double vmax_kms,
-                      int nfr, int skip_mode, int nthreads,
+                      int nfr, int skip_mode, int nthreads, int ray_stride,
                       double *flux, double *image, int *cmask,
                       float *minvel, float *maxvel, double *dnu_out,
                       int *maser, long *overshoot, long *n_integrated);

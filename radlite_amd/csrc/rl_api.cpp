@@ -1,0 +1,678 @@
+// C ABI of the raytracer (include/radlite.h): context, device tables,
+// ray sets, and the per-item launch sequence.  Host arithmetic that
+// feeds the kernels (ring radii, passband, line constants) follows the
+// reference's evaluation order so that FP64 mode reproduces it.
+#include "radlite.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rl_engine.h"
+
+using rl::CellRec;
+using rl::GeoNode;
+using rl::RayInfo;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            if (p) (void)hipFree(p);
+            p = o.p; n = o.n; o.p = nullptr; o.n = 0;
+        }
+        return *this;
+    }
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= n && p) return hipSuccess;
+        if (p) { (void)hipFree(p); p = nullptr; n = 0; }
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 8);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct RaySet {
+    double theta0 = 0.0;
+    int nrays = 0, nrings = 0, nphi = 0;
+    std::vector<double> x0, z0, ring_r, ring_ri;
+    DevBuf d_x0, d_z0;
+    // trajectory CSR (built lazily, shared by every line of this inclination)
+    bool traj_ready = false;
+    long long nnodes = 0;
+    std::vector<long long> off;
+    DevBuf d_off, d_nodes;
+    DevBuf d_ringw;
+    double center_w = 0.0;
+};
+
+}  // namespace
+
+struct rl_ctx {
+    int device = 0, precision = RL_FP64;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // grid
+    int nr = 0, nh = 0;
+    std::vector<double> rc, tc;
+    DevBuf d_rc, d_tc;
+    // medium
+    bool have_medium = false;
+    double umass_av = 2.3, aksmax = 0.0;
+    DevBuf d_rho, d_abund, d_vel, d_lw;
+    int nlev = 0;
+    DevBuf d_pops;
+    // lines
+    int nlines = 0;
+    std::vector<double> nu0, A, gu, gl;
+    std::vector<int> lev_up, lev_low;
+    DevBuf d_jd, d_ad;
+    bool have_dust = false;
+    // star, boundary
+    double rstar = 0.0;
+    std::vector<double> cont_nu, istar;
+    int outer_bc = 0;
+    // rays
+    std::vector<RaySet> incl;
+    // per-item work buffers
+    DevBuf d_cells, d_rec, d_rinfo, d_image, d_chan, d_partial, d_flux, d_err, d_ovs,
+        d_maser, d_scr;
+    int cells_line = -1;
+    // stats
+    double t_ms[5] = {0, 0, 0, 0, 0};
+    long long counts[4] = {0, 0, 0, 0};
+    hipEvent_t ev[6] = {};
+};
+
+static int fail(rl_ctx *c, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+static int fail(rl_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, x)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess)                                                       \
+            return fail(ctx, RL_E_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                        \
+    } while (0)
+
+static int upload(rl_ctx *c, DevBuf &b, const void *src, size_t bytes) {
+    HIPCHK(c, b.ensure(bytes));
+    HIPCHK(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------
+extern "C" int rl_create(rl_ctx **out, int device, int precision) {
+    if (!out) return RL_E_ARG;
+    *out = nullptr;
+    if (precision != RL_FP64 && precision != RL_FP32) return RL_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+        return RL_E_HIP;
+    if (hipSetDevice(device) != hipSuccess) return RL_E_HIP;
+    rl_ctx *c = new rl_ctx();
+    c->device = device;
+    c->precision = precision;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return RL_E_HIP;
+    }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) { delete c; return RL_E_HIP; }
+    *out = c;
+    return 0;
+}
+
+extern "C" void rl_destroy(rl_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char *rl_last_error(const rl_ctx *c) {
+    return c ? c->err.c_str() : "no context";
+}
+
+extern "C" int rl_sync(rl_ctx *c) {
+    if (!c) return RL_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int rl_last_stats(const rl_ctx *c, double *t, long long *n) {
+    if (!c) return RL_E_ARG;
+    if (t) for (int i = 0; i < 5; i++) t[i] = c->t_ms[i];
+    if (n) for (int i = 0; i < 4; i++) n[i] = c->counts[i];
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// inputs
+// ---------------------------------------------------------------------
+extern "C" int rl_set_grid(rl_ctx *c, int nr, const double *r, int nh, const double *th) {
+    if (!c || !r || !th || nr < 3 || nh < 2) return fail(c, RL_E_ARG, "rl_set_grid: bad args");
+    HIPCHK(c, hipSetDevice(c->device));
+    for (int i = 1; i < nr; i++)
+        if (!(r[i] > r[i - 1])) return fail(c, RL_E_ARG, "radius grid not increasing");
+    for (int i = 0; i < nh; i++)
+        if (th[i] < 0.0 || th[i] > 1.5707963268)          // grid.F:1150-1157
+            return fail(c, RL_E_ARG, "theta.inp value out of [0, pi/2]");
+    const int ny = 2 * nh;
+    c->nr = nr;
+    c->nh = nh;
+    c->rc.assign(nr + 4, 0.0);
+    c->tc.assign(ny + 4, 0.0);
+    auto R = [&](int i) -> double & { return c->rc[i + 1]; };
+    auto T = [&](int i) -> double & { return c->tc[i + 1]; };
+    for (int i = 1; i <= nr; i++) R(i) = r[i - 1];
+    R(0) = R(1) * R(1) / R(2);                              // grid.F:753-760
+    R(-1) = R(0) * R(0) / R(1);
+    R(nr + 1) = R(nr) * R(nr) / R(nr - 1);
+    R(nr + 2) = R(nr + 1) * R(nr + 1) / R(nr);
+    for (int i = 1; i <= nh; i++) T(i) = th[i - 1];
+    for (int i = 1; i <= nh; i++) T(2 * nh + 1 - i) = 3.14159265359 - T(i);  // :1160-1163
+    const float twopi_sp = 2 * 3.1415926f;                 // grid.F:1173-1176
+    T(0) = -T(1);
+    T(-1) = -T(2);
+    T(ny + 1) = (double)twopi_sp - T(ny);
+    T(ny + 2) = (double)twopi_sp - T(ny - 1);
+    if (int e = upload(c, c->d_rc, c->rc.data(), c->rc.size() * 8)) return e;
+    if (int e = upload(c, c->d_tc, c->tc.data(), c->tc.size() * 8)) return e;
+    c->incl.clear();
+    c->cells_line = -1;
+    return 0;
+}
+
+extern "C" int rl_set_medium(rl_ctx *c, const double *rho, const double *abund,
+                             const double *vel, const double *lw, double umass_av,
+                             double aksmax) {
+    if (!c || !c->nr) return fail(c, RL_E_ARG, "rl_set_medium: set the grid first");
+    if (!rho || !abund || !vel || !lw || !(umass_av > 0.0))
+        return fail(c, RL_E_ARG, "rl_set_medium: bad args");
+    HIPCHK(c, hipSetDevice(c->device));
+    size_t nc = (size_t)c->nr * c->nh;
+    if (int e = upload(c, c->d_rho, rho, nc * 8)) return e;
+    if (int e = upload(c, c->d_abund, abund, nc * 8)) return e;
+    if (int e = upload(c, c->d_vel, vel, nc * 24)) return e;
+    if (int e = upload(c, c->d_lw, lw, nc * 8)) return e;
+    c->umass_av = umass_av;
+    c->aksmax = aksmax;
+    c->have_medium = true;
+    for (auto &s : c->incl) s.traj_ready = false;    // u depends on the velocity
+    c->cells_line = -1;
+    return 0;
+}
+
+extern "C" int rl_set_levels(rl_ctx *c, int nlev, const double *pops) {
+    if (!c || !c->nr || nlev < 2 || !pops) return fail(c, RL_E_ARG, "rl_set_levels: bad args");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->nlev = nlev;
+    c->cells_line = -1;
+    return upload(c, c->d_pops, pops, (size_t)c->nr * c->nh * nlev * 8);
+}
+
+extern "C" int rl_set_lines(rl_ctx *c, int nl, const double *nu0, const double *A,
+                            const double *gu, const double *gl, const int *lu, const int *ll) {
+    if (!c || nl < 1 || !nu0 || !A || !gu || !gl || !lu || !ll)
+        return fail(c, RL_E_ARG, "rl_set_lines: bad args");
+    for (int i = 0; i < nl; i++) {                           // line.F:1739-1770
+        if (lu[i] < 0 || ll[i] < 0 || (c->nlev && (lu[i] >= c->nlev || ll[i] >= c->nlev)))
+            return fail(c, 13, "line %d: level index out of range", i);
+        if (lu[i] <= ll[i]) return fail(c, 13, "line %d: upper level must follow lower", i);
+        if (nu0[i] == 0.0) return fail(c, 13, "line %d: nu0 = 0", i);
+    }
+    c->nlines = nl;
+    c->nu0.assign(nu0, nu0 + nl);
+    c->A.assign(A, A + nl);
+    c->gu.assign(gu, gu + nl);
+    c->gl.assign(gl, gl + nl);
+    c->lev_up.assign(lu, lu + nl);
+    c->lev_low.assign(ll, ll + nl);
+    c->have_dust = false;
+    c->cells_line = -1;
+    return 0;
+}
+
+extern "C" int rl_set_dust_line(rl_ctx *c, const double *jd, const double *ad) {
+    if (!c || !c->nr || !c->nlines || !jd || !ad)
+        return fail(c, RL_E_ARG, "rl_set_dust_line: set grid and lines first");
+    HIPCHK(c, hipSetDevice(c->device));
+    size_t n = (size_t)c->nr * c->nh * c->nlines * 8;
+    if (int e = upload(c, c->d_jd, jd, n)) return e;
+    if (int e = upload(c, c->d_ad, ad, n)) return e;
+    c->have_dust = true;
+    c->cells_line = -1;
+    return 0;
+}
+
+extern "C" int rl_set_star(rl_ctx *c, double rstar, int ncont, const double *nu,
+                           const double *istar) {
+    if (!c || rstar < 0.0 || ncont < 0 || (ncont > 0 && (!nu || !istar)))
+        return fail(c, RL_E_ARG, "rl_set_star: bad args");
+    if (c->nr && rstar >= c->rc[2]) return fail(c, 13, "star is bigger than inner grid radius");
+    c->rstar = rstar;
+    c->cont_nu.assign(nu, nu + ncont);
+    c->istar.assign(istar, istar + ncont);
+    for (auto &s : c->incl) s.traj_ready = false;
+    return 0;
+}
+
+extern "C" int rl_set_outer_bc(rl_ctx *c, int type) {
+    if (!c) return RL_E_ARG;
+    if (type != 0 && type != 2)
+        return fail(c, 13, "outer boundary type %d is not supported (0 or 2)", type);
+    c->outer_bc = type;
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// rays: setup_rays_circular (telescope.F:715-1191) + ring edges
+// (telescope.F:434-488) + flux weights (telescope.F:1386-1418)
+// ---------------------------------------------------------------------
+static int build_rayset(rl_ctx *c, RaySet &rs, double anginf, int nphi, int nrext, int dbdr) {
+    const int nrref = 10;
+    const double epsxyz = 1.0e2 * 1.0e-10, epsrrr = 1.0e3 * 1.0e-10;
+    const int nr = c->nr;
+    int nrrextra = nrext < 0 ? -nrext : nrext;
+    auto R = [&](int i) { return c->rc[i + 1]; };
+    if (nrrextra == 0) return fail(c, 13, "b_extra must be non-zero (main.F:628-635)");
+    if (dbdr < 1 || nphi < 1) return fail(c, 13, "nphi and dbdr must be >= 1");
+    if (std::fabs(anginf) < 1.0e-1) anginf = (double)0.1f * std::fabs(anginf) / anginf;
+    if (c->rstar > R(1)) return fail(c, 91991, "star larger than the inner radius");
+    const double theta0 = anginf + 1.0e-4;
+    const double sinth0 = std::sin(theta0);
+    const double dphi = 6.28318530718 / (1.0 * nphi);
+    double phi = 0.5 * dphi;
+    std::vector<double> xh(nphi), zh(nphi);
+    for (int k = 0; k < nphi; k++) {
+        zh[k] = -std::sin(phi) / sinth0;
+        double zh02 = zh[k] * zh[k];
+        double dum = 1.0 - zh02 * sinth0 * sinth0;
+        dum = dum + epsxyz;
+        if (dum < 0.0) return fail(c, 13, "ray direction setup failed");
+        xh[k] = std::cos(phi) > 0.0 ? std::sqrt(dum) : -std::sqrt(dum);
+        phi = phi + dphi;
+    }
+    const int nrings = nrrextra + nrref + (nr - 1) * dbdr;
+    rs.theta0 = theta0;
+    rs.nrings = nrings;
+    rs.nphi = nphi;
+    rs.nrays = 1 + nrings * nphi;
+    rs.x0.assign(rs.nrays, 0.0);
+    rs.z0.assign(rs.nrays, 0.0);
+    rs.ring_r.assign(nrings + 1, 0.0);
+    rs.ring_ri.assign(nrings + 2, 0.0);
+    int ir = 1, irad = 1;
+    auto ring = [&](double r_c) {
+        rs.ring_r[irad++] = r_c;
+        for (int k = 0; k < nphi; k++, ir++) {
+            rs.x0[ir] = r_c * xh[k];
+            rs.z0[ir] = r_c * zh[k];
+        }
+    };
+    double ref1 = 0.5, ref2 = 0.0, rstar = c->rstar;
+    for (int ix = 1; ix <= nrrextra + nrref; ix++) {
+        if (ix <= nrrextra) {
+            ring(((double)(ix - 1) * (R(1) - rstar) / nrrextra) + rstar);
+        } else {
+            ref2 = ref2 + ref1;
+            ref1 = ref1 / 2;
+            ring((((double)nrrextra + ref2) * (R(1) - rstar) / ((double)nrrextra + 1.0)) +
+                 rstar);
+        }
+    }
+    for (int ix = 1; ix <= nr - 1; ix++) {
+        ring(R(ix) * (1.0 + epsrrr));
+        if (dbdr > 1) {
+            double dr = (R(ix + 1) - R(ix)) / (1.0 * dbdr);
+            for (int iins = 1; iins <= dbdr - 1; iins++) ring(R(ix) + iins * dr);
+        }
+    }
+    auto &rr = rs.ring_r;
+    auto &ri = rs.ring_ri;
+    ri[0] = 0.0;
+    for (int k = 1; k <= nrings; k++) ri[k] = 0.5 * (rr[k] + rr[k - 1]);
+    ri[nrings + 1] = R(nr);
+    for (int k = 1; k <= nrings - 1; k++)
+        if (rr[k] - ri[k] > 2 * (ri[k + 1] - rr[k])) ri[k] = rr[k] - 2 * (ri[k + 1] - rr[k]);
+    if (ri[1] < rstar) {
+        if (rr[1] < rstar) return fail(c, 1, "internal error in ray setup");
+        ri[1] = rstar;
+    }
+    std::vector<double> w(nrings);
+    for (int k = 1; k <= nrings; k++)
+        w[k - 1] = 3.14159265359 * (ri[k + 1] * ri[k + 1] - ri[k] * ri[k]);
+    rs.center_w = 3.14159265359 * (ri[1] * ri[1]);
+    if (int e = upload(c, rs.d_x0, rs.x0.data(), rs.x0.size() * 8)) return e;
+    if (int e = upload(c, rs.d_z0, rs.z0.data(), rs.z0.size() * 8)) return e;
+    if (int e = upload(c, rs.d_ringw, w.data(), w.size() * 8)) return e;
+    rs.traj_ready = false;
+    return 0;
+}
+
+extern "C" int rl_setup_rays_circular(rl_ctx *c, int nincl, const double *incl, int nphi,
+                                      int nrext, int dbdr, int *n_rays, int *n_rings,
+                                      double *ring_r, double *ring_edge) {
+    if (!c || !c->nr || nincl < 1 || !incl)
+        return fail(c, RL_E_ARG, "rl_setup_rays_circular: set the grid first");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->incl.clear();
+    c->incl.resize(nincl);
+    for (int i = 0; i < nincl; i++) {
+        if (int e = build_rayset(c, c->incl[i], incl[i], nphi, nrext, dbdr)) return e;
+        if (n_rays) n_rays[i] = c->incl[i].nrays;
+        if (n_rings) n_rings[i] = c->incl[i].nrings;
+    }
+    if (ring_r) std::memcpy(ring_r, c->incl[0].ring_r.data(), c->incl[0].ring_r.size() * 8);
+    if (ring_edge)
+        std::memcpy(ring_edge, c->incl[0].ring_ri.data(), c->incl[0].ring_ri.size() * 8);
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// trace
+// ---------------------------------------------------------------------
+static int check_err(rl_ctx *c) {
+    int e = 0;
+    HIPCHK(c, hipMemcpyAsync(&e, c->d_err.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (e) return fail(c, e, "device stop code %d", e);
+    return 0;
+}
+
+static int build_traj(rl_ctx *c, RaySet &rs) {
+    if (rs.traj_ready) return 0;
+    rl::GridDev g{c->nr, 2 * c->nh, c->d_rc.as<double>(), c->d_tc.as<double>(), c->rstar};
+    const int n = rs.nrays;
+    const int iyeq = c->nh;
+    HIPCHK(c, c->d_scr.ensure((size_t)(iyeq + 1) * n * 20));
+    double *s1 = c->d_scr.as<double>();
+    double *s2 = s1 + (size_t)(iyeq + 1) * n;
+    int *iy = reinterpret_cast<int *>(s2 + (size_t)(iyeq + 1) * n);
+    DevBuf cnt;
+    HIPCHK(c, cnt.ensure((size_t)n * 4));
+    HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));
+    double rbeam0 = rs.ring_ri[1];
+    rl::launch_traj(c->stream, g, c->d_vel.as<double>(), c->nh, n, rs.d_x0.as<double>(),
+                    rs.d_z0.as<double>(), rs.theta0, rbeam0, s1, s2, iy, cnt.as<int>(),
+                    nullptr, nullptr, c->d_err.as<int>());
+    HIPCHK(c, hipGetLastError());
+    std::vector<int> h(n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), cnt.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (int e = check_err(c)) return e;
+    rs.off.assign(n + 1, 0);
+    for (int i = 0; i < n; i++) rs.off[i + 1] = rs.off[i] + h[i];
+    rs.nnodes = rs.off[n];
+    if (int e = upload(c, rs.d_off, rs.off.data(), (size_t)(n + 1) * 8)) return e;
+    HIPCHK(c, rs.d_nodes.ensure((size_t)rs.nnodes * sizeof(GeoNode)));
+    rl::launch_traj(c->stream, g, c->d_vel.as<double>(), c->nh, n, rs.d_x0.as<double>(),
+                    rs.d_z0.as<double>(), rs.theta0, rbeam0, s1, s2, iy, nullptr,
+                    rs.d_off.as<long long>(), rs.d_nodes.as<GeoNode>(), c->d_err.as<int>());
+    HIPCHK(c, hipGetLastError());
+    if (int e = check_err(c)) return e;
+    rs.traj_ready = true;
+    return 0;
+}
+
+static double star_at(const rl_ctx *c, double freq) {      // line.F:3821-3843
+    const int n = (int)c->cont_nu.size();
+    if (n < 2) return 0.0;
+    const double *x = c->cont_nu.data() - 1, *s = c->istar.data() - 1;
+    bool asc = x[n] > x[1];
+    int lo = 0, hi = n + 1;
+    while (hi - lo != 1) {
+        int m = (hi + lo) >> 1;
+        if ((freq > x[m]) == asc) lo = m; else hi = m;
+    }
+    if (lo == 0 || lo == n) return 0.0;
+    double wgt = (freq - x[lo]) / (x[lo + 1] - x[lo]);
+    return (1.0 - wgt) * s[lo] + wgt * s[lo + 1];
+}
+
+extern "C" int rl_trace_lines_forward(rl_ctx *c, int nitems, const rl_item *items,
+                                      double *flux, double *image, unsigned *warn) {
+    if (!c || nitems < 0 || (nitems && (!items || !flux)))
+        return fail(c, RL_E_ARG, "rl_trace_lines_forward: bad args");
+    if (!c->have_medium || !c->nlev || !c->nlines || !c->have_dust || c->incl.empty())
+        return fail(c, 13, "inputs incomplete: grid, medium, levels, lines, dust, rays");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->d_err.ensure(4));
+    HIPCHK(c, c->d_ovs.ensure(8));
+    HIPCHK(c, c->d_maser.ensure(4));
+    for (int i = 0; i < 5; i++) c->t_ms[i] = 0.0;
+    for (int i = 0; i < 4; i++) c->counts[i] = 0;
+    const size_t rsz = c->precision == RL_FP64 ? 64 : 32;
+    size_t fo = 0, io = 0;
+    for (int ii = 0; ii < nitems; ii++) {
+        const rl_item &it = items[ii];
+        if (it.line < 0 || it.line >= c->nlines || it.incl < 0 ||
+            it.incl >= (int)c->incl.size() || it.nfr < 2 || !(it.vmax_kms > 0.0) ||
+            (it.skip_mode != RL_SKIP_REFERENCE && it.skip_mode != RL_SKIP_NONE))
+            return fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii);
+        RaySet &rs = c->incl[it.incl];
+        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        if (int e = build_traj(c, rs)) return e;
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        // cell table for this line
+        const double molpg = 1.0 / (c->umass_av * 1.6726e-24);
+        const int ncell = c->nr * c->nh;
+        HIPCHK(c, c->d_cells.ensure((size_t)ncell * sizeof(CellRec)));
+        if (c->cells_line != it.line) {
+            rl::launch_cells(c->stream, ncell, c->nlev, c->nlines, it.line,
+                             c->lev_up[it.line], c->lev_low[it.line], molpg,
+                             c->d_rho.as<double>(), c->d_abund.as<double>(),
+                             c->d_lw.as<double>(), c->d_pops.as<double>(),
+                             c->d_jd.as<double>(), c->d_ad.as<double>(),
+                             c->d_cells.as<CellRec>());
+            c->cells_line = it.line;
+        }
+        // channel tables: line_setup_passband (line.F:462-469)
+        const int nfr = it.nfr;
+        const double nu0s = c->nu0[it.line], nu0 = std::fabs(nu0s);
+        const double passb = 3.33567e-6 * nu0 * it.vmax_kms;
+        const double nu1 = 0.0 - passb, dnu = 2.0 * passb / (nfr - 1.0);
+        std::vector<double> chan(3 * (size_t)nfr);
+        for (int k = 0; k < nfr; k++) {
+            double dn = nu1 + k * dnu;
+            chan[k] = dn;
+            chan[nfr + k] = dn / nu0s;
+            chan[2 * nfr + k] = star_at(c, nu0 + dn);
+        }
+        if (int e = upload(c, c->d_chan, chan.data(), chan.size() * 8)) return e;
+        rl::LineConst lc;
+        lc.nu0 = nu0s;
+        lc.c_aa = 3.33567e-6 * nu0;
+        lc.hknu = 5.27296241956e-28 * nu0s;
+        lc.A = c->A[it.line];
+        lc.Bud = 6.78171833781e46 * lc.A / (nu0s * nu0s * nu0s);      // line.F:1778
+        lc.Bdu = lc.Bud * (c->gu[it.line] / c->gl[it.line]);
+        {
+            double q = c->rstar / rs.ring_ri[1];
+            lc.starfract = q * q;
+            if (rs.ring_ri[1] < c->rstar) return fail(c, 124, "central beam resolves the star");
+        }
+        lc.I0 = 0.0;
+        if (c->outer_bc == 2)
+            lc.I0 = 1.47455253991e-47 * (nu0s * nu0s * nu0s) /
+                    (std::exp(4.7991598e-11 * nu0s / 2.728) - 1.0);
+        const int nrays = rs.nrays;
+        const int rpc = 16, nchunks = (rs.nrings + rpc - 1) / rpc;
+        HIPCHK(c, c->d_rec.ensure((size_t)rs.nnodes * rsz));
+        HIPCHK(c, c->d_rinfo.ensure((size_t)nrays * sizeof(RayInfo)));
+        HIPCHK(c, c->d_image.ensure((size_t)nrays * nfr * 8));
+        HIPCHK(c, c->d_partial.ensure((size_t)nchunks * nfr * 8));
+        HIPCHK(c, c->d_flux.ensure((size_t)nfr * 8));
+        HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_ovs.p, 0, 8, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_maser.p, 0, 4, c->stream));
+        rl::ItemLaunch L;
+        L.nrays = nrays; L.nfr = nfr; L.nrings = rs.nrings; L.nphi = rs.nphi;
+        L.skip_mode = it.skip_mode;
+        L.off = rs.d_off.as<long long>();
+        L.geo = rs.d_nodes.as<GeoNode>();
+        L.cells = c->d_cells.as<CellRec>();
+        L.rec = c->d_rec.p;
+        L.rinfo = c->d_rinfo.as<RayInfo>();
+        L.dnu = c->d_chan.as<double>();
+        L.velo = L.dnu + nfr;
+        L.istar = L.dnu + 2 * nfr;
+        L.lc = lc;
+        L.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
+        L.image = c->d_image.as<double>();
+        L.ovs = c->d_ovs.as<unsigned long long>();
+        L.maser = c->d_maser.as<int>();
+        L.ring_w = rs.d_ringw.as<double>();
+        L.center_w = rs.center_w;
+        L.nchunks = nchunks;
+        L.rings_per_chunk = rpc;
+        L.partial = c->d_partial.as<double>();
+        L.d2 = (1.0 * 3.08572e18) * (1.0 * 3.08572e18);
+        L.flux = c->d_flux.as<double>();
+        L.ev_nodes = c->ev[2];
+        L.ev_formal = c->ev[3];
+        if (c->precision == RL_FP64) rl::launch_item<double>(c->stream, L);
+        else rl::launch_item<float>(c->stream, L);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+        HIPCHK(c, hipMemcpyAsync(flux + fo, c->d_flux.p, (size_t)nfr * 8,
+                                 hipMemcpyDeviceToHost, c->stream));
+        if (image)
+            HIPCHK(c, hipMemcpyAsync(image + io, c->d_image.p, (size_t)nrays * nfr * 8,
+                                     hipMemcpyDeviceToHost, c->stream));
+        unsigned long long ovs = 0;
+        int maser = 0;
+        HIPCHK(c, hipMemcpyAsync(&ovs, c->d_ovs.p, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&maser, c->d_maser.p, 4, hipMemcpyDeviceToHost, c->stream));
+        if (int e = check_err(c)) return e;
+        if (warn) warn[ii] = (maser ? RL_WARN_MASER : 0u) | (ovs ? RL_WARN_OVERSHOOT : 0u);
+        float t01 = 0, t12 = 0, t23 = 0, t34 = 0;
+        (void)hipEventElapsedTime(&t01, c->ev[0], c->ev[1]);
+        (void)hipEventElapsedTime(&t12, c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&t23, c->ev[2], c->ev[3]);
+        (void)hipEventElapsedTime(&t34, c->ev[3], c->ev[4]);
+        c->t_ms[0] += t01; c->t_ms[1] += t12; c->t_ms[2] += t23; c->t_ms[3] += t34;
+        c->t_ms[4] += t01 + t12 + t23 + t34;
+        c->counts[0] += nrays;
+        c->counts[1] += rs.nnodes;
+        c->counts[3] += (long long)nrays * nfr;
+        c->counts[2] = (long long)ovs;   // overshoot count of the last item (diagnostic)
+        fo += nfr;
+        io += (size_t)nrays * nfr;
+    }
+    return 0;
+}
+
+extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int, const rl_item *, const double *,
+                                      double *, double *, double *, double *, double *) {
+    return fail(c, 13, "adjoint not available in this build");
+}
+
+// ---------------------------------------------------------------------
+// host pre-pass (R13) with the reference's arithmetic
+// ---------------------------------------------------------------------
+extern "C" int rl_prepare_line_freqs(int nl, const double *E, const int *lu, const int *ll,
+                                     double *nu0) {
+    if (nl < 1 || !E || !lu || !ll || !nu0) return RL_E_ARG;
+    for (int i = 0; i < nl; i++) {                      // line.F:1903, 1980-1983
+        double eu = 1.986468498e-16 * E[lu[i]], el = 1.986468498e-16 * E[ll[i]];
+        nu0[i] = 1.509160e26 * (eu - el);
+    }
+    return 0;
+}
+
+static double bplanck(double temp, double nu) {          // setup.F:937-952
+    if (temp == 0.0) return 0.0;
+    return 1.47455e-47 * nu * nu * nu / (std::exp(4.7989e-11 * nu / temp) - 1.0) + 1.0e-290;
+}
+
+extern "C" int rl_prepare_line_dust(int nr, int nh, int nspec, const double *rho_d,
+                                    const double *t_d, int nf, const double *cn,
+                                    const double *kabs, const double *kscat, int nl,
+                                    const double *nu0, double *jd, double *ad) {
+    if (nr < 1 || nh < 1 || nspec < 0 || nf < 1 || nl < 1 || !cn || !nu0 || !jd || !ad)
+        return RL_E_ARG;
+    const double *x = cn - 1;
+    const bool asc = x[nf] > x[1];
+    const size_t ncell = (size_t)nr * nh;
+    for (int il = 0; il < nl; il++) {                   // line.F:3560-3607
+        const double freq = nu0[il];
+        int lo = 0, hi = nf + 1;
+        while (hi - lo != 1) {
+            int m = (hi + lo) >> 1;
+            if ((freq > x[m]) == asc) lo = m; else hi = m;
+        }
+        for (size_t cell = 0; cell < ncell; cell++) {
+            double src = 0.0, alp = 0.0;
+            if (lo != 0 && lo != nf) {
+                double wgt = (freq - x[lo]) / (x[lo + 1] - x[lo]);
+                for (int s = 0; s < nspec; s++) {
+                    const double *ka = kabs + (size_t)s * nf - 1;
+                    double kw = wgt * ka[lo + 1] + (1.0 - wgt) * ka[lo];
+                    double rd = rho_d[s * ncell + cell];
+                    src = src + rd * kw * bplanck(t_d[s * ncell + cell], freq);
+                    alp = alp + rd * kw;
+                }
+                src = src + wgt * 0.0 + (1.0 - wgt) * 0.0;
+                for (int s = 0; s < nspec; s++) {
+                    const double *ks = kscat + (size_t)s * nf - 1;
+                    double kw = wgt * ks[lo + 1] + (1.0 - wgt) * ks[lo];
+                    alp = alp + rho_d[s * ncell + cell] * kw;
+                }
+            }
+            jd[cell * nl + il] = src;
+            ad[cell * nl + il] = alp;
+        }
+    }
+    return 0;
+}
+
+extern "C" int rl_prepare_star(int nf, const double *fnu, double rstar, double *is) {
+    if (nf < 0 || (nf && (!fnu || !is)) || !(rstar > 0.0)) return RL_E_ARG;
+    for (int i = 0; i < nf; i++) is[i] = 3.0308410e36 * fnu[i] / (rstar * rstar);
+    return 0;
+}
+
+extern "C" double rl_prepare_aksmax(int n, const double *lw, double fix_lw) {
+    double mx = 0.0;                                     // line.F:3020-3033
+    for (int i = 0; i < n; i++) {
+        double a = lw[i];
+        if (a <= 0.0) a = fix_lw;
+        if (a > mx) mx = a;
+    }
+    return mx;
+}
+
+extern "C" int rl_passband(double vmax, double dv, int *nfr, double *vmax_out) {
+    if (!(dv > 0.0) || !(vmax > 0.0) || !nfr || !vmax_out) return RL_E_ARG;
+    int n = (int)(2.0 * vmax / dv + 1);                  // main.F:209-210
+    *nfr = n;
+    *vmax_out = 0.5 * (n - 1) * dv;
+    return n >= 2 ? 0 : 13;
+}

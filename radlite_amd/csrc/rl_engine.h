@@ -1,0 +1,70 @@
+// Host/device interface between the C ABI (rl_api.cpp) and the kernels
+// (rl_engine.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace rl {
+
+struct GridDev {
+    int nr, ny;
+    const double *rc, *tc;    // ghost-extended, index i+1
+    double rstar;
+};
+
+// Channel-independent geometry of one trajectory node (FP64).
+struct GeoNode {
+    double ds;                // s_k - s_{k-1} (0 for the first node)
+    double u;                 // Omega.v/c at the node (line.F:2650-2662)
+    double dr, dt;            // interpolation fractions (telescope.F:4088-4091)
+    int c00, c01, c10, c11;   // stencil cells (r0,t0) (r0,t1) (r1,t0) (r1,t1)
+    int flag;                 // icross | node flags << 2
+};
+
+enum { CR_JD = 0, CR_AD = 1, CR_LW = 2, CR_NU = 3, CR_NL = 4 };
+struct CellRec { double v[5]; };
+
+struct RayInfo {
+    float mn, mx;             // NONREDUNDANT window (REAL*4)
+    int cont;                 // first skipped channel, -1 if none
+    int n;                    // nodes on the ray
+};
+
+struct LineConst {
+    double nu0, c_aa, hknu, A, Bud, Bdu;
+    double starfract;         // (R*/rbeam0)^2 for the central beam
+    double I0;                // outer boundary intensity
+};
+
+struct ItemLaunch {
+    int nrays, nfr, nrings, nphi, skip_mode;
+    const long long *off;
+    const GeoNode *geo;
+    const CellRec *cells;
+    void *rec;                // NodeRec<T>[nnodes]
+    RayInfo *rinfo;
+    const double *dnu, *velo, *istar;   // per channel
+    LineConst lc;
+    double aks2;              // 2*aksmax/c
+    double *image;            // [nrays][nfr]
+    unsigned long long *ovs;
+    int *maser;
+    const double *ring_w;     // per ring pi*(ri_{k+1}^2 - ri_k^2)
+    double center_w;
+    int nchunks, rings_per_chunk;
+    double *partial;
+    double d2;                // distance^2
+    double *flux;             // device [nfr]
+    hipEvent_t ev_nodes, ev_formal;
+};
+
+void launch_traj(hipStream_t st, const GridDev &g, const double *vel, int nh, int nrays,
+                 const double *x0, const double *z0, double theta0, double rbeam0,
+                 double *s1, double *s2, int *iy, int *count, const long long *off,
+                 GeoNode *nodes, int *err);
+void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int lev_up,
+                  int lev_low, double molpg, const double *rho, const double *abund,
+                  const double *lw, const double *pops, const double *jd, const double *ad,
+                  CellRec *out);
+template <typename T> void launch_item(hipStream_t st, const ItemLaunch &it);
+
+}  // namespace rl

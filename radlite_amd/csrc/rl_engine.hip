@@ -1,0 +1,383 @@
+// Kernels of the raytracer and their launchers.
+//
+//   k_traj        one thread per ray: trajectory (FP64), node geometry
+//                 (segment length, projected velocity, interpolation
+//                 stencil), CSR by ray.  Run twice: count, then write.
+//   k_cells       per line: cell table {jd, ad, lw, N_up, N_low}
+//   k_nodes<T>    one wave per ray: interpolated node records for one
+//                 line + the NONREDUNDANT window and continuum channel
+//   k_formal<T>   one lane per (ray, channel): formal solution along the
+//                 ray (charintline, telescope.F:3889-4312)
+//   k_fixup<T>    one thread per ray: the reference's lazy continuum
+//                 copies for skipped channels (telescope.F:583-611)
+//   k_flux        ring-weighted sums of the image into the spectrum
+//                 (calc_freq_flux_observer, telescope.F:1386-1435)
+#include "rl_engine.h"
+
+#include <hip/hip_runtime.h>
+
+#include "rl_traj.h"
+
+namespace rl {
+
+// ---------------------------------------------------------------------
+// trajectories
+// ---------------------------------------------------------------------
+struct CellView {
+    int nh;                   // nth_half
+    const double *vel;        // [cell][3]
+    __device__ int cell(int ir, int it) const { return (ir - 1) * nh + (it - 1); }
+};
+
+__global__ void __launch_bounds__(128)
+k_traj(GridView g, CellView cv, int nrays, const double *__restrict__ x0,
+       const double *__restrict__ z0, double theta0, double rbeam0,
+       double *scr_s1, double *scr_s2, int *scr_iy, int *__restrict__ count,
+       const long long *__restrict__ off, GeoNode *__restrict__ nodes, int *err) {
+    int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= nrays) return;
+    Traj t(g, x0[ray], z0[ray], theta0, scr_s1 + ray, scr_s2 + ray, scr_iy + ray, nrays);
+    if (off == nullptr) {
+        auto sink = [&](int, const TrajNode &) {};
+        int n = trajectory(t, sink, false);
+        if (n < 0) { set_err(err, -n); n = 0; }
+        count[ray] = n;
+        return;
+    }
+    const long long base = off[ray];
+    const int cap = (int)(off[ray + 1] - base);
+    const int nr = g.nr, ny = g.ny;
+    double s_prev = 0.0;
+    int ir_prev = -1, icr_prev = -1, star_done = 0;
+    const bool central = ray == 0 && rbeam0 > 0.0;
+    auto sink = [&](int k, const TrajNode &nd) {
+        if (k >= cap) { set_err(err, E_CAP); return; }
+        GeoNode o;
+        int ir = nd.ir, it = nd.it;
+        if (ir == 0) { set_err(err, E_HOLE); ir = 1; }
+        double dr = (nd.r - g.R(ir)) / (g.R(ir + 1) - g.R(ir));
+        double dt = (nd.theta - g.TH(it)) / (g.TH(it + 1) - g.TH(it));
+        if (dr < 0.0 || dr > 1.0) set_err(err, E_DR);
+        if (dt < 0.0 || dt > 1.0) set_err(err, E_DT);
+        int r0 = ir, r1, t0 = it, t1;
+        if (dr > 0.0) { r1 = ir + 1; if (r1 > nr) r1 = nr; }
+        else { r1 = ir - 1; if (r1 < 1) r1 = 1; }
+        t1 = dt > 0.0 ? it + 1 : it - 1;
+        t0 = ridx_it(t0, ny);
+        t1 = ridx_it(t1, ny);
+        o.c00 = cv.cell(r0, t0);
+        o.c01 = cv.cell(r0, t1);
+        o.c10 = cv.cell(r1, t0);
+        o.c11 = cv.cell(r1, t1);
+        o.dr = dr;
+        o.dt = dt;
+        // projected velocity: get_line_dust_values interpolation of the
+        // velocity vector + omega_dot_v (line.F:4039-4208, 2650-2662)
+        double v[3];
+        const double *V = cv.vel;
+        for (int q = 0; q < 3; q++) {
+            double a00 = V[o.c00 * 3 + q], a01 = V[o.c01 * 3 + q];
+            double a10 = V[o.c10 * 3 + q], a11 = V[o.c11 * 3 + q];
+            if (nd.icross == 1) v[q] = (1.0 - dt) * a00 + dt * a01;
+            else if (nd.icross == 2) v[q] = (1.0 - dr) * a00 + dr * a10;
+            else v[q] = (1.0 - dr) * ((1.0 - dt) * a00 + dt * a01) +
+                        dr * ((1.0 - dt) * a10 + dt * a11);
+        }
+        if (nd.mu > 1.0) set_err(err, 393);
+        o.u = 3.335668e-11 * (nd.mu * v[0] + sqrt(1.0 - nd.mu * nd.mu) *
+                                              (v[1] * sin(nd.phi) + v[2] * cos(nd.phi)));
+        int flag = nd.icross;
+        if (k == 0) {
+            o.ds = 0.0;
+        } else {
+            o.ds = nd.s - s_prev;
+            if (o.ds < 0.0) set_err(err, E_DSNEG);
+            // hole chord, telescope.F:4095-4104
+            if (ir == 1 && ir_prev == 1 && nd.icross == 1 && icr_prev == 1)
+                flag |= NF_RESET << 2;
+            // central-beam star blend, telescope.F:4144-4193
+            if (central && ir == 1 && star_done == 0) {
+                flag |= NF_BLEND << 2;
+                star_done = 1;
+            }
+        }
+        o.flag = flag;
+        nodes[base + k] = o;
+        s_prev = nd.s;
+        ir_prev = ir;
+        icr_prev = nd.icross;
+    };
+    int n = trajectory(t, sink, true);
+    if (n < 0) set_err(err, -n);
+    else if (n != cap) set_err(err, E_CAP);
+}
+
+// ---------------------------------------------------------------------
+// per-line cell tables: N = pop*abund*rho*molpg (line.F:4039-4048)
+// ---------------------------------------------------------------------
+__global__ void k_cells(int ncell, int nlev, int nlines, int line, int lev_up,
+                        int lev_low, double molpg, const double *__restrict__ rho,
+                        const double *__restrict__ abund, const double *__restrict__ lw,
+                        const double *__restrict__ pops, const double *__restrict__ jd,
+                        const double *__restrict__ ad, CellRec *__restrict__ out) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    CellRec r;
+    r.v[CR_JD] = jd[(size_t)c * nlines + line];
+    r.v[CR_AD] = ad[(size_t)c * nlines + line];
+    r.v[CR_LW] = lw[c];
+    r.v[CR_NU] = pops[(size_t)c * nlev + lev_up] * abund[c] * rho[c] * molpg;
+    r.v[CR_NL] = pops[(size_t)c * nlev + lev_low] * abund[c] * rho[c] * molpg;
+    out[c] = r;
+}
+
+// ---------------------------------------------------------------------
+// node records (one wave per ray) + window + continuum channel
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_nodes(int nrays, const long long *__restrict__ off, const GeoNode *__restrict__ geo,
+        const CellRec *__restrict__ cells, NodeRec<T> *__restrict__ rec,
+        RayInfo *__restrict__ rinfo, int nfr, const double *__restrict__ velo,
+        double aks2, int skip_mode) {
+    const int ray = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (ray >= nrays) return;
+    const long long b = off[ray];
+    const int n = (int)(off[ray + 1] - b);
+    double umin = 1.0e300, umax = -1.0e300;
+    for (int k = lane; k < n; k += 64) {
+        GeoNode gn = geo[b + k];
+        const CellRec c00 = cells[gn.c00], c01 = cells[gn.c01];
+        const CellRec c10 = cells[gn.c10], c11 = cells[gn.c11];
+        const double dr = gn.dr, dt = gn.dt;
+        const int icr = gn.flag & 3;
+        double v[5];
+        const double *a = c00.v, *p = c01.v, *q = c10.v, *w = c11.v;
+        for (int f = 0; f < 5; f++) {
+            if (icr == 1) v[f] = (1.0 - dt) * a[f] + dt * p[f];
+            else if (icr == 2) v[f] = (1.0 - dr) * a[f] + dr * q[f];
+            else v[f] = (1.0 - dr) * ((1.0 - dt) * a[f] + dt * p[f]) +
+                        dr * ((1.0 - dt) * q[f] + dt * w[f]);
+        }
+        NodeRec<T> r;
+        r.ds = (T)gn.ds; r.u = (T)gn.u;
+        r.jd = (T)v[0]; r.ad = (T)v[1]; r.lw = (T)v[2]; r.nu = (T)v[3]; r.nl = (T)v[4];
+        r.flag = (T)(gn.flag >> 2);
+        rec[b + k] = r;
+        // window from the node preceding each segment (telescope.F:4265-4270)
+        if (k + 1 < n && v[3] + v[4] > LEVTHRES) {
+            umin = fmin(umin, gn.u);
+            umax = fmax(umax, gn.u);
+        }
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        umin = fmin(umin, __shfl_xor(umin, m));
+        umax = fmax(umax, __shfl_xor(umax, m));
+    }
+    float mn = umin < 1.0 ? (float)umin : 1.0f;
+    float mx = umax > -1.0 ? (float)umax : -1.0f;
+    int cont = nfr;
+    if (skip_mode == 0 && ray > 0) {
+        for (int ch = lane; ch < nfr; ch += 64) {
+            bool inwin = velo[ch] <= (double)mx + aks2 && velo[ch] >= (double)mn - aks2;
+            if (!inwin) { cont = ch; break; }
+        }
+        for (int m = 32; m >= 1; m >>= 1) cont = min(cont, __shfl_xor(cont, m));
+    }
+    if (lane == 0) {
+        RayInfo ri;
+        ri.mn = mn; ri.mx = mx; ri.cont = cont < nfr ? cont : -1; ri.n = n;
+        rinfo[ray] = ri;
+    }
+}
+
+// ---------------------------------------------------------------------
+// formal solution
+// ---------------------------------------------------------------------
+template <typename T>
+__device__ inline T trace_one(const NodeRec<T> *__restrict__ rec, int n, const LineK<T> &L,
+                              T I0, T istar, T starfract, int &ovs, int &maser) {
+    T I = I0;
+    Carry<T> c;
+    c.init = 1; c.phi0 = T(0); c.srcl0 = T(0); c.alpl0 = T(0);
+    NodeRec<T> a = rec[0];
+    Pt<T> pa{a.jd, a.ad, a.u, a.nu, a.nl};
+    T lwa = a.lw;
+    for (int k = 1; k < n; k++) {
+        NodeRec<T> b = rec[k];
+        T ds = b.ds;
+        int fl = (int)b.flag;
+        if (fl & NF_RESET) { ds = T(0); c.init = 1; }
+        if (fl & NF_BLEND) {
+            I = (T(1) - starfract) * I + starfract * istar;
+            c.init = 1;
+        }
+        Pt<T> pb{b.jd, b.ad, b.u, b.nu, b.nl};
+        clever_elem<T>(L, I, ds, pa, pb, lwa, b.lw, c, ovs);
+        if (c.alpl0 * ds < T(-0.01)) maser = 1;
+        pa = pb;
+        lwa = b.lw;
+    }
+    return I;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_formal(int nrays, int nfr, const long long *__restrict__ off,
+         const NodeRec<T> *__restrict__ rec, const RayInfo *__restrict__ rinfo,
+         const double *__restrict__ dnu, const double *__restrict__ velo,
+         const double *__restrict__ istar, LineConst lc, double aks2, int skip_mode,
+         double *__restrict__ image, unsigned long long *ovs_out, int *maser_out) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int ovs = 0, maser = 0;
+    if (g < (long long)nrays * nfr) {
+        const int ray = (int)(g / nfr), ch = (int)(g - (long long)ray * nfr);
+        const RayInfo ri = rinfo[ray];
+        bool doit = true;
+        if (skip_mode == 0 && ray > 0 && ch > 0 && ch != ri.cont) {
+            doit = velo[ch] <= (double)ri.mx + aks2 && velo[ch] >= (double)ri.mn - aks2;
+        }
+        double out;
+        if (doit) {
+            LineK<T> L;
+            L.nu0 = (T)lc.nu0; L.c_aa = (T)lc.c_aa; L.hknu = (T)lc.hknu;
+            L.A = (T)lc.A; L.Bud = (T)lc.Bud; L.Bdu = (T)lc.Bdu;
+            L.dnu = (T)dnu[ch]; L.uch = (T)velo[ch];
+            T sf = ray == 0 ? (T)lc.starfract : T(0);
+            out = (double)trace_one<T>(rec + off[ray], ri.n, L, (T)lc.I0, (T)istar[ch], sf,
+                                       ovs, maser);
+        } else {
+            out = __longlong_as_double(0x7ff8dead00000000ll);   // filled by k_fixup
+        }
+        image[g] = out;
+    }
+    // wave-level pre-reduction of the warning counters
+    for (int m = 32; m >= 1; m >>= 1) {
+        ovs += __shfl_xor(ovs, m);
+        maser |= __shfl_xor(maser, m);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (ovs) atomicAdd(ovs_out, (unsigned long long)ovs);
+        if (maser) atomicOr(maser_out, 1);
+    }
+}
+
+// lazy continuum (telescope.F:583-611): the first skipped channel of a
+// ray was integrated by k_formal; later skipped channels copy it, unless
+// it is exactly 0, in which case they are integrated in order.
+template <typename T>
+__global__ void k_fixup(int nrays, int nfr, const long long *__restrict__ off,
+                        const NodeRec<T> *__restrict__ rec, const RayInfo *__restrict__ rinfo,
+                        const double *__restrict__ dnu, const double *__restrict__ velo,
+                        const double *__restrict__ istar, LineConst lc, double aks2,
+                        double *__restrict__ image, unsigned long long *ovs_out,
+                        int *maser_out) {
+    int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray == 0 || ray >= nrays) return;
+    const RayInfo ri = rinfo[ray];
+    if (ri.cont < 0) return;
+    double *row = image + (size_t)ray * nfr;
+    double cont = row[ri.cont];
+    int ovs = 0, maser = 0;
+    for (int ch = ri.cont + 1; ch < nfr; ch++) {
+        bool inwin = velo[ch] <= (double)ri.mx + aks2 && velo[ch] >= (double)ri.mn - aks2;
+        if (inwin) continue;
+        if (cont != 0.0) {
+            row[ch] = cont;
+        } else {
+            LineK<T> L;
+            L.nu0 = (T)lc.nu0; L.c_aa = (T)lc.c_aa; L.hknu = (T)lc.hknu;
+            L.A = (T)lc.A; L.Bud = (T)lc.Bud; L.Bdu = (T)lc.Bdu;
+            L.dnu = (T)dnu[ch]; L.uch = (T)velo[ch];
+            row[ch] = (double)trace_one<T>(rec + off[ray], ri.n, L, (T)lc.I0, (T)istar[ch],
+                                           T(0), ovs, maser);
+            cont = row[ch];
+        }
+    }
+    if (ovs) atomicAdd(ovs_out, (unsigned long long)ovs);
+    if (maser) atomicOr(maser_out, 1);
+}
+
+// ---------------------------------------------------------------------
+// flux: partial ring sums, then a fixed-order sum of the partials
+// ---------------------------------------------------------------------
+__global__ void k_flux_partial(int nfr, int nrings, int nphi, int rings_per_chunk,
+                               const double *__restrict__ image,
+                               const double *__restrict__ ring_w, double center_w,
+                               double *__restrict__ partial) {
+    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = blockIdx.y;
+    if (ch >= nfr) return;
+    double acc = 0.0;
+    if (chunk == 0) acc = center_w * image[ch];
+    int r0 = chunk * rings_per_chunk, r1 = min(nrings, r0 + rings_per_chunk);
+    for (int ir = r0; ir < r1; ir++) {
+        const double *row = image + (size_t)(1 + ir * nphi) * nfr + ch;
+        double s = 0.0;
+        for (int ip = 0; ip < nphi; ip++) s = s + row[(size_t)ip * nfr];
+        s = s / (1.0 * nphi);
+        acc = acc + s * ring_w[ir];
+    }
+    partial[(size_t)chunk * nfr + ch] = acc;
+}
+
+__global__ void k_flux_final(int nfr, int nchunks, const double *__restrict__ partial,
+                             double d2, double *__restrict__ flux) {
+    int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ch >= nfr) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunks; c++) s = s + partial[(size_t)c * nfr + ch];
+    flux[ch] = s / d2;
+}
+
+// ---------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------
+void launch_traj(hipStream_t st, const GridDev &gd, const double *vel, int nh, int nrays,
+                 const double *x0, const double *z0, double theta0, double rbeam0,
+                 double *s1, double *s2, int *iy, int *count, const long long *off,
+                 GeoNode *nodes, int *err) {
+    GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar};
+    CellView cv{nh, vel};
+    dim3 blk(128), grd((nrays + 127) / 128);
+    hipLaunchKernelGGL(k_traj, grd, blk, 0, st, g, cv, nrays, x0, z0, theta0, rbeam0, s1,
+                       s2, iy, count, off, nodes, err);
+}
+
+void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int lev_up,
+                  int lev_low, double molpg, const double *rho, const double *abund,
+                  const double *lw, const double *pops, const double *jd, const double *ad,
+                  CellRec *out) {
+    hipLaunchKernelGGL(k_cells, dim3((ncell + 255) / 256), dim3(256), 0, st, ncell, nlev,
+                       nlines, line, lev_up, lev_low, molpg, rho, abund, lw, pops, jd, ad,
+                       out);
+}
+
+template <typename T>
+void launch_item(hipStream_t st, const ItemLaunch &it) {
+    auto *rec = reinterpret_cast<NodeRec<T> *>(it.rec);
+    hipLaunchKernelGGL(k_nodes<T>, dim3(it.nrays), dim3(64), 0, st, it.nrays, it.off,
+                       it.geo, it.cells, rec, it.rinfo, it.nfr, it.velo, it.aks2,
+                       it.skip_mode);
+    if (it.ev_nodes) hipEventRecord(it.ev_nodes, st);
+    long long total = (long long)it.nrays * it.nfr;
+    hipLaunchKernelGGL(k_formal<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                       it.nrays, it.nfr, it.off, rec, it.rinfo, it.dnu, it.velo, it.istar,
+                       it.lc, it.aks2, it.skip_mode, it.image, it.ovs, it.maser);
+    if (it.skip_mode == 0)
+        hipLaunchKernelGGL(k_fixup<T>, dim3((it.nrays + 127) / 128), dim3(128), 0, st,
+                           it.nrays, it.nfr, it.off, rec, it.rinfo, it.dnu, it.velo,
+                           it.istar, it.lc, it.aks2, it.image, it.ovs, it.maser);
+    if (it.ev_formal) hipEventRecord(it.ev_formal, st);
+    dim3 gp((it.nfr + 63) / 64, it.nchunks);
+    hipLaunchKernelGGL(k_flux_partial, gp, dim3(64), 0, st, it.nfr, it.nrings, it.nphi,
+                       it.rings_per_chunk, it.image, it.ring_w, it.center_w, it.partial);
+    hipLaunchKernelGGL(k_flux_final, dim3((it.nfr + 63) / 64), dim3(64), 0, st, it.nfr,
+                       it.nchunks, it.partial, it.d2, it.flux);
+}
+
+template void launch_item<double>(hipStream_t, const ItemLaunch &);
+template void launch_item<float>(hipStream_t, const ItemLaunch &);
+
+}  // namespace rl

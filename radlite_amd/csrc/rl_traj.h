@@ -1,0 +1,310 @@
+// Ray/cell crossing geometry on the device, FP64, one thread per ray.
+//
+// Restates make_trajectory_c (RADLITE/telescope.F:2787-3720) with the
+// reference's root formulas, eps filters and extra-point rule, but
+// without its per-ray arrays: the theta-cone and R-sphere crossing
+// lists are generated on demand from their index (only the cone roots
+// sar1/sar2 and their cone indices are kept, in a strided scratch
+// buffer), the extra points live in a small private list, and the
+// three-way merge streams nodes straight to the caller's sink.
+#pragma once
+#include "rl_device.h"
+
+namespace rl {
+
+struct GridView {
+    int nr, ny;               // ny = mirrored theta count
+    const double *rc;         // rc[i+1] = R_i, i=-1..nr+2
+    const double *tc;         // tc[i+1] = Theta_i, i=-1..ny+2
+    double rstar;
+    __device__ double R(int i) const { return rc[i + 1]; }
+    __device__ double TH(int i) const { return tc[i + 1]; }
+};
+
+struct TrajNode {
+    double s, r, theta, mu, phi;
+    int icross, ir, it;
+};
+
+struct Traj {
+    const GridView &g;
+    double x0, z0, theta0;
+    double costh0, sinth0, costh02, sinth02, pitheta0;
+    int iyeq, iyend;
+    // theta cones
+    double *sar1, *sar2;
+    int *iyar;
+    int stride;
+    int isnr, isdblnr, iup, branchA, ith_amount;
+    // R spheres
+    int ir_min, ir_amount, half;
+    double bimpact;
+
+    __device__ Traj(const GridView &g_, double x0_, double z0_, double th0,
+                    double *s1, double *s2, int *iy, int strd)
+        : g(g_), x0(x0_), z0(z0_), theta0(th0), sar1(s1), sar2(s2), iyar(iy),
+          stride(strd) {}
+
+    __device__ double thr(double s) const {
+        return sqrt(x0 * x0 + z0 * z0 + s * s + 2.0 * z0 * costh0 * s);
+    }
+    __device__ double theta_of_s(double s) const {
+        double th = atan(sqrt(x0 * x0 + sinth02 * s * s) / (z0 + costh0 * s));
+        if (th < 0.0) th = th + PICONST;
+        return th;
+    }
+    __device__ int hunt_r(double x) const {
+        return hunt_bisect([&](int i) { return g.R(i); }, g.nr, x);
+    }
+    __device__ int hunt_t(double x) const {
+        return hunt_bisect([&](int i) { return g.TH(i); }, g.ny, x);
+    }
+    __device__ double S1(int is) const { return sar1[(size_t)is * stride]; }
+    __device__ double S2(int is) const { return sar2[(size_t)is * stride]; }
+    __device__ int IY(int is) const { return iyar[(size_t)is * stride]; }
+
+    // th list entry iss (1-based): ordering of telescope.F:3024-3194
+    __device__ void th_entry(int iss, double &s, int &itheta) const {
+        if (iss > ith_amount) { s = 1.0e30; itheta = 0; return; }
+        int is, root, upq;
+        const int n = isnr, d = isdblnr;
+        if (branchA) {
+            if (iss <= n - d) { is = d + iss; root = 1; upq = 0; }
+            else if (iss <= 2 * n - 2 * d) { is = n - (iss - (n - d + 1)); root = 2; upq = 1; }
+            else if (iss <= 2 * n - d) { is = d - (iss - (2 * n + 1 - 2 * d)); root = 1; upq = 1; }
+            else { is = 1 + (iss - (2 * n + 1 - d)); root = 2; upq = 1; }
+        } else {
+            if (iss <= d) { is = d - (iss - 1); root = 1; upq = 0; }
+            else if (iss <= 2 * d) { is = 1 + (iss - d - 1); root = 2; upq = 0; }
+            else if (iss <= n + d) { is = d + 1 + (iss - 2 * d - 1); root = 1; upq = 0; }
+            else { is = n - (iss - n - d - 1); root = 2; upq = 1; }
+        }
+        s = root == 1 ? S1(is) : S2(is);
+        int iy = IY(is);
+        if (upq) itheta = iup == 1 ? iy : iyend + 1 - iy;
+        else itheta = iup == 1 ? iyend + 1 - iy : iy;
+    }
+    __device__ double th_s(int iss) const {
+        double s; int it;
+        th_entry(iss, s, it);
+        return s;
+    }
+    // r list entry k (1-based), telescope.F:3284-3335
+    __device__ void r_entry(int k, double &s, double &r, int &ix) const {
+        if (k > ir_amount) { s = 1.0e30; r = 0.0; ix = 0; return; }
+        if (k < 1) { s = 0.0; r = 0.0; ix = 0; return; }   // r_s(0): see header
+        int is = k <= half ? k : ir_amount + 1 - k;
+        ix = g.nr - is + 1;
+        r = ix == 0 ? g.rstar : g.R(ix);
+        double a = 1.0, b = 2.0 * costh0 * z0, c = x0 * x0 + z0 * z0 - r * r;
+        double sd = sqrt(b * b - 4.0 * a * c + TELESC_EPS * b * b);
+        s = k <= half ? (-b - sd) / (2.0 * a) : (-b + sd) / (2.0 * a);
+    }
+    __device__ double r_s(int k) const {
+        double s, r; int ix;
+        r_entry(k, s, r, ix);
+        return s;
+    }
+};
+
+// Builds the trajectory and streams its nodes to sink(k, node).
+// Returns the node count (>= 0) or minus a stop code.
+template <typename Sink>
+__device__ int trajectory(Traj &t, Sink &&sink, bool want_dir) {
+    const GridView &g = t.g;
+    const double eps = TELESC_EPS, epsplus = 1.0e1 * TELESC_EPS;
+    const double pi = PICONST, send = 1.0e30;
+    const int nr = g.nr, ny = g.ny;
+    const double x0 = t.x0, z0 = t.z0;
+
+    t.pitheta0 = 0.5 * pi - t.theta0;
+    t.costh0 = cos(t.theta0);
+    t.sinth0 = sin(t.theta0);
+    t.costh02 = t.costh0 * t.costh0;
+    t.sinth02 = t.sinth0 * t.sinth0;
+    const double costh0 = t.costh0, sinth02 = t.sinth02, costh02 = t.costh02;
+    t.iyeq = ny / 2;
+    t.iyend = ny;
+
+    // theta = const cones (telescope.F:2968-3000)
+    int is = 1;
+    for (int iy = 1; iy <= t.iyeq; iy++) {
+        double theta = g.TH(iy), tn = tan(theta), tanth2 = tn * tn;
+        double a = tanth2 * costh02 - sinth02;
+        double b = 2.0 * tanth2 * costh0 * z0;
+        double c = tanth2 * z0 * z0 - x0 * x0;
+        double sd = b * b - 4.0 * a * c;
+        if (sd > 0.0) {
+            sd = sqrt(sd);
+            double s1 = (-b - sd) / (2.0 * a), s2 = (-b + sd) / (2.0 * a);
+            if (s1 > s2) { double d = s1; s1 = s2; s2 = d; }
+            t.iyar[(size_t)is * t.stride] = iy;
+            t.sar1[(size_t)is * t.stride] = s1;
+            t.sar2[(size_t)is * t.stride] = s2;
+            is++;
+        }
+    }
+    t.isnr = is - 1;
+    t.isdblnr = 0;
+    for (is = 1; is <= t.isnr; is++) {
+        double pit = 0.5 * pi - g.TH(t.IY(is));
+        if (fabs(pit) > fabs(t.pitheta0)) t.isdblnr++;
+    }
+    t.iup = t.pitheta0 > 0.0 ? 1 : 0;
+    t.branchA = z0 * t.pitheta0 > 0.0 ? 1 : 0;
+    t.ith_amount = 2 * t.isnr;
+
+    // R = const spheres (telescope.F:3241-3290)
+    double bimpact = sqrt(x0 * x0 + z0 * z0 * (1.0 - costh02));
+    int ix;
+    for (ix = 1; ix <= nr; ix++)
+        if (g.R(ix) > bimpact) break;
+    if (ix > nr) return -E_INPUT;
+    t.ir_min = bimpact <= g.rstar ? 0 : ix;
+    t.bimpact = bimpact;
+    t.ir_amount = 2 * (nr + 1 - t.ir_min);
+    t.half = nr + 1 - t.ir_min;
+    {   // discriminant sanity (telescope.F:3293-3300)
+        double r = g.R(nr), b = 2.0 * costh0 * z0, c = x0 * x0 + z0 * z0 - r * r;
+        if (b * b - 4.0 * c + eps * b * b < 0.0) return -E_INPUT;
+    }
+
+    // extra points (telescope.F:3417-3606)
+    double ex_s[EXMAX + 1], ex_r[EXMAX + 1], ex_t[EXMAX + 1];
+    int ex_ir[EXMAX + 1], ex_it[EXMAX + 1];
+    int nex = 0;
+    if (bimpact > g.R(1)) {
+        double s2 = 0.0 - z0 * costh0;
+        if (nex + 1 + 4 * RAYADPT > EXMAX) return -E_EXOVF;
+        nex++;
+        ex_s[nex] = s2; ex_r[nex] = bimpact; ex_t[nex] = t.theta_of_s(s2);
+        ex_ir[nex] = t.ir_min - 1; ex_it[nex] = t.hunt_t(ex_t[nex]);
+        double s3 = s2;
+        int isrt = hunt_bisect([&](int i) { return t.r_s(i); }, t.ir_amount, s2);
+        for (int irng = 1; irng <= 4; irng++) {
+            double s1;
+            if (irng == 1) { s1 = s3; s2 = t.r_s(isrt + 1); }
+            else if (irng == 2) { s1 = t.r_s(isrt); s2 = s3; }
+            else if (irng == 3) { s1 = t.r_s(isrt + 1); s2 = t.r_s(isrt + 2); }
+            else { s1 = t.r_s(isrt - 1); s2 = t.r_s(isrt); }
+            double ds = (s2 - s1) / (1.0 + 1.0 * RAYADPT);
+            for (int iad = 1; iad <= RAYADPT; iad++) {
+                double s0 = iad * ds + s1;
+                nex++;
+                ex_s[nex] = s0; ex_r[nex] = t.thr(s0); ex_t[nex] = t.theta_of_s(s0);
+                ex_ir[nex] = t.hunt_r(ex_r[nex]); ex_it[nex] = t.hunt_t(ex_t[nex]);
+            }
+        }
+    }
+    {
+        double s2 = x0 * x0 * costh0 / (z0 * sinth02);
+        double rr = t.thr(s2);
+        if (rr > g.R(1) && rr < g.R(nr)) {
+            int ixh = t.hunt_r(rr);
+            if (ixh == 0 || ixh == nr) return -E_HUNT;
+            if (nex + 1 + 4 * RAYADPT > EXMAX) return -E_EXOVF;
+            nex++;
+            ex_s[nex] = s2; ex_r[nex] = rr; ex_t[nex] = t.theta_of_s(s2);
+            ex_ir[nex] = ixh; ex_it[nex] = t.hunt_t(ex_t[nex]);
+            double s3 = s2;
+            int isrt = hunt_bisect([&](int i) { return t.th_s(i); }, t.ith_amount, s3);
+            if (!(isrt - RAYRNPT + 1 < 1 || isrt + RAYRNPT > t.ith_amount)) {
+                for (int irng = 1; irng <= 4; irng++) {
+                    double s1;
+                    if (irng == 1) { s1 = s3; s2 = t.th_s(isrt + 1); }
+                    else if (irng == 2) { s1 = t.th_s(isrt); s2 = s3; }
+                    else if (irng == 3) { s1 = t.th_s(isrt + 1); s2 = t.th_s(isrt + 2); }
+                    else { s1 = t.th_s(isrt - 1); s2 = t.th_s(isrt); }
+                    if (s2 == s1) return -E_EXEQ;
+                    if (s2 < s1) return -E_EXORD;
+                    double ds = (s2 - s1) / (1.0 + 1.0 * RAYADPT);
+                    for (int iad = 1; iad <= RAYADPT; iad++) {
+                        double s0 = iad * ds + s1, r2 = t.thr(s0);
+                        int ixx = t.hunt_r(r2);
+                        if (ixx == 0 || ixx == nr) continue;        // :3580
+                        nex++;
+                        ex_s[nex] = s0; ex_r[nex] = r2; ex_t[nex] = t.theta_of_s(s0);
+                        ex_ir[nex] = ixx; ex_it[nex] = t.hunt_t(ex_t[nex]);
+                    }
+                }
+            }
+        }
+    }
+    // ray_sort (nrecip.F:833): ascending in s, insertion sort (<= 34)
+    for (int i = 2; i <= nex; i++) {
+        double ks = ex_s[i], kr = ex_r[i], kt = ex_t[i];
+        int ki = ex_ir[i], kj = ex_it[i], j;
+        for (j = i - 1; j >= 1 && ex_s[j] > ks; j--) {
+            ex_s[j + 1] = ex_s[j]; ex_r[j + 1] = ex_r[j]; ex_t[j + 1] = ex_t[j];
+            ex_ir[j + 1] = ex_ir[j]; ex_it[j + 1] = ex_it[j];
+        }
+        ex_s[j + 1] = ks; ex_r[j + 1] = kr; ex_t[j + 1] = kt;
+        ex_ir[j + 1] = ki; ex_it[j + 1] = kj;
+    }
+
+    // merge (telescope.F:3610-3677) and local direction (:3682-3718)
+    const double rmaxt = g.R(nr) * (1.0 - epsplus), rmaxr = g.R(nr) * (1.0 + epsplus);
+    const double rmint = g.R(1) * (1.0 + epsplus), rminr = g.R(1) * (1.0 - epsplus);
+    const double sbeg = -1.0e30;
+    const double znew = z0 * sinth02, bnew = sqrt(x0 * x0 + z0 * z0 * sinth02);
+    double sprev = -1.0e30;
+    int nout = 0, isex = 1, ist = 1, isr = 1;
+    double ths, rs, rr_;
+    int thit, rix;
+    t.th_entry(ist, ths, thit);
+    t.r_entry(isr, rs, rr_, rix);
+    auto emit = [&](int icr, double r, double th, int ir, int it, double s) {
+        TrajNode nd;
+        nd.s = s; nd.r = r; nd.theta = th; nd.icross = icr; nd.ir = ir; nd.it = it;
+        if (want_dir) {
+            double snew = s + z0 * costh0;
+            nd.mu = snew / sqrt(bnew * bnew + snew * snew);
+            double q = znew + snew * costh0;
+            double dummy = bnew * sqrt(bnew * bnew + snew * snew - q * q);
+            double sinphi = dummy > 0.0 ? (bnew * bnew * costh0 - znew * snew) / dummy
+                                        : epsplus;
+            sinphi = x0 < 0.0 ? asin(sinphi) : pi - asin(sinphi);
+            while (sinphi < 0.0) sinphi = sinphi + 2.0 * pi;
+            while (sinphi >= 2.0 * pi) sinphi = sinphi - 2.0 * pi;
+            nd.phi = sinphi;
+        } else {
+            nd.mu = 0.0; nd.phi = 0.0;
+        }
+        sink(nout, nd);
+        nout++;
+        sprev = s;
+    };
+    const int tr_total = t.ir_amount + t.ith_amount;
+    for (int iss = 1; iss <= tr_total; iss++) {
+        for (;;) {
+            double m = ths < rs ? ths : rs;
+            double es = isex <= nex ? ex_s[isex] : 1.0e30;
+            if (!(es < m)) break;
+            if (ex_r[isex] <= rmaxt && ex_r[isex] >= rmint &&
+                (es - sprev) > eps * ex_r[isex] && es >= sbeg && es <= send)
+                emit(3, ex_r[isex], ex_t[isex], ex_ir[isex], ex_it[isex], es);
+            isex++;
+        }
+        if (ths < rs) {
+            double r = t.thr(ths);
+            if (r <= rmaxt && r >= rmint && (ths - sprev) > eps * r &&
+                ths >= sbeg && ths <= send) {
+                int ir = r < g.R(1) ? 0 : (r > g.R(nr) ? nr : t.hunt_r(r));
+                emit(2, r, g.TH(thit), ir, thit, ths);
+            }
+            ist++;
+            t.th_entry(ist, ths, thit);
+        } else {
+            if (rr_ <= rmaxr && rr_ >= rminr && (rs - sprev) > eps * rr_ &&
+                rs >= sbeg && rs <= send) {
+                double th = t.theta_of_s(rs);
+                emit(1, rr_, th, rix, t.hunt_t(th), rs);
+            }
+            isr++;
+            t.r_entry(isr, rs, rr_, rix);
+        }
+    }
+    return nout;
+}
+
+}  // namespace rl

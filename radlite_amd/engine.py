@@ -1,0 +1,325 @@
+"""Python host of the MI355X raytracer: ctypes binding of librl.so.
+
+This mirrors the reference's operator interface for the image/spectrum
+raytrace: a model (grid, medium, levels, lines, dust, star) is loaded
+once, a circular-image ray set is built per inclination
+(setup_rays_circular, telescope.F:715), and calc_line_spectrum
+(telescope.F:1465) is run for a batch of (line, inclination) items.
+Errors carry the reference's stop codes.
+
+librl.so must be built (``python -m radlite_amd.build``); there is no
+CPU fallback -- importing the engine without the library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import l4io
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librl.so")
+_LIB = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+RL_FP64, RL_FP32 = 0, 1
+SKIP_REFERENCE, SKIP_NONE = 0, 1
+WARN_MASER, WARN_OVERSHOOT = 1, 2
+
+
+class RLItem(C.Structure):
+    _fields_ = [("line", C.c_int), ("incl", C.c_int), ("nfr", C.c_int),
+                ("vmax_kms", C.c_double), ("skip_mode", C.c_int)]
+
+
+class RadliteError(RuntimeError):
+    """A reference stop code (or RL_E_HIP=900) raised by the engine."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"stop {code}: {msg}")
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with python -m radlite_amd.build")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    sig = {
+        "rl_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int]),
+        "rl_destroy": (None, [vp]),
+        "rl_last_error": (C.c_char_p, [vp]),
+        "rl_set_grid": (C.c_int, [vp, C.c_int, _dp, C.c_int, _dp]),
+        "rl_set_medium": (C.c_int, [vp, _dp, _dp, _dp, _dp, C.c_double, C.c_double]),
+        "rl_set_levels": (C.c_int, [vp, C.c_int, _dp]),
+        "rl_set_lines": (C.c_int, [vp, C.c_int, _dp, _dp, _dp, _dp, _ip, _ip]),
+        "rl_set_dust_line": (C.c_int, [vp, _dp, _dp]),
+        "rl_set_star": (C.c_int, [vp, C.c_double, C.c_int, _dp, _dp]),
+        "rl_set_outer_bc": (C.c_int, [vp, C.c_int]),
+        "rl_setup_rays_circular": (C.c_int, [vp, C.c_int, _dp, C.c_int, C.c_int, C.c_int,
+                                             _ip, _ip, _dp, _dp]),
+        "rl_trace_lines_forward": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
+                                             C.POINTER(C.c_uint)]),
+        "rl_trace_lines_adjoint": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
+                                             _dp, _dp, _dp, _dp]),
+        "rl_sync": (C.c_int, [vp]),
+        "rl_last_stats": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong)]),
+        "rl_prepare_line_freqs": (C.c_int, [C.c_int, _dp, _ip, _ip, _dp]),
+        "rl_prepare_line_dust": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int,
+                                           _dp, _dp, _dp, C.c_int, _dp, _dp, _dp]),
+        "rl_prepare_star": (C.c_int, [C.c_int, _dp, C.c_double, _dp]),
+        "rl_prepare_aksmax": (C.c_double, [C.c_int, _dp, C.c_double]),
+        "rl_passband": (C.c_int, [C.c_double, C.c_double, _ip, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+@dataclass
+class LineTables:
+    """Host pre-pass results (the engine's inputs besides the fields)."""
+    nu0: np.ndarray
+    A: np.ndarray
+    gu: np.ndarray
+    gl: np.ndarray
+    lev_up: np.ndarray     # 0-based
+    lev_low: np.ndarray
+    jd: np.ndarray         # [nr, nth, nlines]
+    ad: np.ndarray
+    istar: np.ndarray
+    cont_nu: np.ndarray
+    aksmax: float
+
+
+def prepare_tables(ri: l4io.RunInputs) -> LineTables:
+    """R13 pre-pass (line.F:1980-1983, 3560-3607; star.F:508-525;
+    line.F:3020-3033) with the reference's arithmetic, in librl."""
+    L = load_library()
+    nl = len(ri.A)
+    lu, lup = _i(ri.lev_up - 1)
+    ll, llp = _i(ri.lev_low - 1)
+    E, Ep = _d(ri.E_cm)
+    nu0 = np.zeros(nl)
+    rc = L.rl_prepare_line_freqs(nl, Ep, lup, llp, nu0.ctypes.data_as(_dp))
+    if rc:
+        raise RadliteError(rc, "rl_prepare_line_freqs")
+    nr, nth = ri.rho_gas.shape
+    jd = np.zeros((nr, nth, nl))
+    ad = np.zeros((nr, nth, nl))
+    rd, rdp = _d(ri.rho_dust)
+    td, tdp = _d(ri.t_dust)
+    nu, nup = _d(ri.cont_nu)
+    ka, kap = _d(ri.kappa_abs)
+    ks, ksp = _d(ri.kappa_scat)
+    rc = L.rl_prepare_line_dust(nr, nth, ri.rho_dust.shape[0], rdp, tdp, len(nu), nup, kap,
+                                ksp, nl, nu0.ctypes.data_as(_dp), jd.ctypes.data_as(_dp),
+                                ad.ctypes.data_as(_dp))
+    if rc:
+        raise RadliteError(rc, "rl_prepare_line_dust")
+    istar = np.zeros(len(nu))
+    fn, fnp = _d(ri.star_fnu)
+    rc = L.rl_prepare_star(len(nu), fnp, ri.rstar, istar.ctypes.data_as(_dp))
+    if rc:
+        raise RadliteError(rc, "rl_prepare_star")
+    lw, lwp = _d(ri.lw_kms)
+    aks = L.rl_prepare_aksmax(lw.size, lwp, ri.fix_lw)
+    return LineTables(nu0=nu0, A=np.asarray(ri.A, float), gu=ri.g[ri.lev_up - 1].astype(float),
+                      gl=ri.g[ri.lev_low - 1].astype(float), lev_up=lu, lev_low=ll, jd=jd, ad=ad,
+                      istar=istar, cont_nu=nu, aksmax=aks)
+
+
+def passband(vmax, dv):
+    """(nfr, vmax) as main.F:209-210 derive them."""
+    L = load_library()
+    n = C.c_int(0)
+    v = C.c_double(0.0)
+    rc = L.rl_passband(vmax, dv, C.byref(n), C.byref(v))
+    if rc:
+        raise RadliteError(rc, "rl_passband")
+    return n.value, v.value
+
+
+class Raytracer:
+    """One engine context on one GPU."""
+
+    def __init__(self, device=0, precision="fp64"):
+        self.L = load_library()
+        self.ctx = C.c_void_p()
+        prec = {"fp64": RL_FP64, "fp32": RL_FP32}[precision]
+        self.precision = precision
+        rc = self.L.rl_create(C.byref(self.ctx), device, prec)
+        if rc:
+            raise RadliteError(rc, "rl_create (no HIP device?)")
+        self.nrays = []
+        self.nrings = []
+        self.ring_r = None
+        self.ring_edge = None
+        self.nr = self.nth = 0
+
+    def close(self):
+        if self.ctx:
+            self.L.rl_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = self.L.rl_last_error(self.ctx)
+            raise RadliteError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    # ---- model -----------------------------------------------------
+    def set_grid(self, r, theta_half):
+        r, rp = _d(r)
+        t, tp = _d(theta_half)
+        self.nr, self.nth = len(r), len(t)
+        self._chk(self.L.rl_set_grid(self.ctx, len(r), rp, len(t), tp), "rl_set_grid")
+
+    def set_medium(self, rho, abund, vel, lw_kms, umass_av, aksmax):
+        a = [_d(x) for x in (rho, abund, vel, lw_kms)]
+        self._chk(self.L.rl_set_medium(self.ctx, a[0][1], a[1][1], a[2][1], a[3][1],
+                                       umass_av, aksmax), "rl_set_medium")
+
+    def set_levels(self, pops):
+        p, pp = _d(pops)
+        self._chk(self.L.rl_set_levels(self.ctx, p.shape[-1], pp), "rl_set_levels")
+
+    def set_lines(self, nu0, A, gu, gl, lev_up, lev_low):
+        a = [_d(x) for x in (nu0, A, gu, gl)]
+        u, up = _i(lev_up)
+        l, lp = _i(lev_low)
+        self._chk(self.L.rl_set_lines(self.ctx, len(a[0][0]), a[0][1], a[1][1], a[2][1],
+                                      a[3][1], up, lp), "rl_set_lines")
+
+    def set_dust_line(self, jd, ad):
+        j, jp = _d(jd)
+        a, ap = _d(ad)
+        self._chk(self.L.rl_set_dust_line(self.ctx, jp, ap), "rl_set_dust_line")
+
+    def set_star(self, rstar, cont_nu, istar):
+        n, npt = _d(cont_nu)
+        s, sp = _d(istar)
+        self._chk(self.L.rl_set_star(self.ctx, rstar, len(n), npt, sp), "rl_set_star")
+
+    def set_outer_bc(self, t):
+        self._chk(self.L.rl_set_outer_bc(self.ctx, int(t)), "rl_set_outer_bc")
+
+    def load_run(self, ri: l4io.RunInputs, tables: LineTables | None = None):
+        """Load a RADLite run directory's contents (see l4io.read_run_dir)."""
+        tb = prepare_tables(ri) if tables is None else tables
+        self.tables = tb
+        self.set_grid(ri.r, ri.theta_half)
+        self.set_star(ri.rstar, tb.cont_nu, tb.istar)
+        self.set_medium(ri.rho_gas, ri.abund, ri.vel, ri.lw_kms, ri.umass_av, tb.aksmax)
+        self.set_levels(ri.pops)
+        self.set_lines(tb.nu0, tb.A, tb.gu, tb.gl, tb.lev_up, tb.lev_low)
+        self.set_dust_line(tb.jd, tb.ad)
+        self.set_outer_bc(ri.outer_bc)
+        return tb
+
+    # ---- rays and trace --------------------------------------------
+    def setup_rays(self, incl_rad, nphi, nrext, dbdr):
+        inc, ip_ = _d(np.atleast_1d(incl_rad))
+        n = len(inc)
+        nrays = np.zeros(n, np.int32)
+        nrings = np.zeros(n, np.int32)
+        rc = self.L.rl_setup_rays_circular(self.ctx, n, ip_, nphi, nrext, dbdr,
+                                           nrays.ctypes.data_as(_ip),
+                                           nrings.ctypes.data_as(_ip), None, None)
+        self._chk(rc, "rl_setup_rays_circular")
+        rr = np.zeros(nrings[0] + 1)
+        re = np.zeros(nrings[0] + 2)
+        rc = self.L.rl_setup_rays_circular(self.ctx, n, ip_, nphi, nrext, dbdr, None, None,
+                                           rr.ctypes.data_as(_dp), re.ctypes.data_as(_dp))
+        self._chk(rc, "rl_setup_rays_circular")
+        self.nrays, self.nrings = list(nrays), list(nrings)
+        self.ring_r, self.ring_edge = rr, re
+        return self.nrays
+
+    def trace(self, items, want_image=False):
+        """items: list of (line, incl, nfr, vmax_kms, skip_mode).
+        Returns (list of flux arrays, list of images or None, warn bits)."""
+        arr = (RLItem * len(items))(*[RLItem(*it) for it in items])
+        nf = [it[2] for it in items]
+        flux = np.zeros(sum(nf))
+        img = None
+        if want_image:
+            img = np.zeros(sum(self.nrays[it[1]] * it[2] for it in items))
+        warn = np.zeros(len(items), np.uint32)
+        rc = self.L.rl_trace_lines_forward(
+            self.ctx, len(items), arr, flux.ctypes.data_as(_dp),
+            img.ctypes.data_as(_dp) if img is not None else None,
+            warn.ctypes.data_as(C.POINTER(C.c_uint)))
+        self._chk(rc, "rl_trace_lines_forward")
+        fl, im, o, io = [], [], 0, 0
+        for it in items:
+            fl.append(flux[o:o + it[2]])
+            o += it[2]
+            if img is not None:
+                n = self.nrays[it[1]] * it[2]
+                im.append(img[io:io + n].reshape(self.nrays[it[1]], it[2]))
+                io += n
+        return fl, (im if img is not None else None), warn
+
+    def stats(self):
+        t = np.zeros(5)
+        n = np.zeros(4, np.int64)
+        self.L.rl_last_stats(self.ctx, t.ctypes.data_as(_dp),
+                             n.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return dict(t_traj_ms=t[0], t_nodes_ms=t[1], t_formal_ms=t[2], t_flux_ms=t[3],
+                    t_total_ms=t[4], rays=int(n[0]), nodes=int(n[1]),
+                    overshoot=int(n[2]), nominal=int(n[3]))
+
+
+def run_directory(path, precision="fp64", skip_mode=SKIP_REFERENCE, device=0,
+                  write_output=True):
+    """The RADlite executable's job for a run directory: every requested
+    line at the linespectrum.inp inclination, written as
+    linespectrum_<mol>.dat (main.F:1037-1049, telescope.F:1662-1804)."""
+    ri = l4io.read_run_dir(path)
+    rt = Raytracer(device=device, precision=precision)
+    tb = rt.load_run(ri)
+    nfr, vmax = passband(ri.vmax, ri.dv)
+    incl = ri.incl_deg * 0.0174532925199                      # telescope.F:188
+    rt.setup_rays([incl], ri.nphi, ri.b_extra, ri.dbdr)
+    lines = list(range(ri.iline_start - 1, ri.iline_start - 1 + ri.nlines_run))
+    fl, _, warn = rt.trace([(il, 0, nfr, vmax, skip_mode) for il in lines])
+    blocks = []
+    for il, f in zip(lines, fl):
+        nu0 = abs(tb.nu0[il])
+        passb = 3.33567e-6 * nu0 * vmax
+        dnu = 2.0 * passb / (nfr - 1.0)
+        nu = nu0 + (-passb + np.arange(nfr) * dnu)
+        blocks.append(dict(lev_up=int(ri.lev_up[il]), lev_low=int(ri.lev_low[il]), nu0=nu0,
+                           nu=nu, flux=f))
+    if write_output:
+        l4io.write_line_spectrum_file(os.path.join(path, f"linespectrum_{ri.molname}.dat"),
+                                      ri.molname, nfr, incl, blocks, vrad=ri.vrad)
+    rt.close()
+    return blocks, warn

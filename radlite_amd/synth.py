@@ -188,3 +188,31 @@ def write_workload(path, name, seed=0, **over):
     write_run_dir(path, disk, lines, pops, nphi=w["nphi"], dbdr=w["dbdr"],
                   b_extra=w["b_extra"], vmax=vmax, dv=dv, incl_deg=w["incl"][0])
     return w
+
+
+def run_inputs(disk: Disk, lines: LineSet, pops, *, nphi, dbdr, b_extra, vmax, dv,
+               incl_deg, outer_bc=0, fix_lw=0.0, umass_av=2.3, nfreq=40):
+    """The same RunInputs that l4io.read_run_dir would return for the
+    directory write_run_dir writes (every value round-trips through the
+    %.17e text), built in memory."""
+    nu, ka, ks, rstar, fnu = star_and_dust(nfreq=nfreq)
+    return l4io.RunInputs(
+        r=disk.r, theta_half=disk.theta_half, rho_gas=disk.rho_gas, abund=disk.abund,
+        vel=disk.vel, lw_kms=disk.lw_kms, pops=pops, rho_dust=disk.rho_dust[None],
+        t_dust=disk.t_dust[None], cont_nu=nu, kappa_abs=ka[None], kappa_scat=ks[None],
+        rstar=rstar, star_fnu=fnu, umass_av=umass_av, fix_lw=fix_lw, E_cm=lines.E_cm,
+        g=lines.g, lev_up=lines.lev_up, lev_low=lines.lev_low, A=lines.A, molname="moldata",
+        nphi=nphi, dbdr=dbdr, b_extra=b_extra, outer_bc=outer_bc, vmax=vmax, dv=dv,
+        incl_deg=incl_deg, vrad=0.0, nlines_run=len(lines.A), iline_start=1)
+
+
+def workload_inputs(name, seed=0, **over):
+    """RunInputs of a named workload (first inclination in incl_deg)."""
+    w = dict(WORKLOADS[name]); w.update(over)
+    disk = make_disk(w["nr"], w["nth"], seed=seed, sigma_t=w["sigma_t"])
+    lines = single_line()
+    pops = lte_pops(disk, lines)
+    vmax, dv = passband(w["vmax"], w["nfr"])
+    ri = run_inputs(disk, lines, pops, nphi=w["nphi"], dbdr=w["dbdr"], b_extra=w["b_extra"],
+                    vmax=vmax, dv=dv, incl_deg=w["incl"][0])
+    return ri, w

@@ -1,0 +1,184 @@
+"""HIP engine parity: FP64 mode against the reference's golden vectors and
+the CPU oracle, FP32 mode against the FP64 oracle, batching, errors.
+
+Tolerances (stated per test): FP64 <= 1e-10 relative per channel (and
+per ray for intensities); FP32 <= 1e-4 relative on the line-integrated
+flux (sum over channels of F - F_continuum, the continuum taken at the
+passband edges).  All GPU calls go through librl.so's C ABI.
+"""
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import pyorc
+from cases import CASES, build_case, input_hash, load_golden
+from radlite_amd import engine, l4io, synth
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_LINE_TOL = 1e-4
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _rel(a, b, floor=0.0):
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+def _line_integral(f):
+    cont = 0.5 * (f[0] + f[-1])
+    return float(np.sum(f - cont))
+
+
+_CACHE = {}
+
+
+def _case(name):
+    if name not in _CACHE:
+        g = load_golden(name)
+        with tempfile.TemporaryDirectory() as d:
+            build_case(name, d)
+            assert str(g["input_sha256"]) == input_hash(d)
+            ri = l4io.read_run_dir(d)
+        _CACHE[name] = (g, ri)
+    return _CACHE[name]
+
+
+def _engine(ri, precision):
+    rt = engine.Raytracer(0, precision)
+    rt.load_run(ri)
+    nfr, vmax = engine.passband(ri.vmax, ri.dv)
+    rt.setup_rays([ri.incl_deg * 0.0174532925199], ri.nphi, ri.b_extra, ri.dbdr)
+    return rt, nfr, vmax
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_matches_reference_golden(name):
+    g, ri = _case(name)
+    say(f"[gpu] fp64 golden {name}")
+    rt, nfr, vmax = _engine(ri, "fp64")
+    nl = len(g["flux"])
+    fl, im, warn = rt.trace([(il, 0, nfr, vmax, engine.SKIP_REFERENCE) for il in range(nl)],
+                            want_image=True)
+    for il in range(nl):
+        rel = _rel(fl[il], g["flux"][il])
+        assert rel.max() <= FP64_TOL, (name, il, rel.max())
+        img = im[il][g["image_rows"]]
+        scale = np.abs(g["image"][il]).max()
+        rel = _rel(img, g["image"][il], floor=1e-12 * scale)
+        assert rel.max() <= FP64_TOL, (name, il, rel.max(), np.unravel_index(rel.argmax(), rel.shape))
+        assert bool(warn[il] & engine.WARN_MASER) == bool(g["maser"][il])
+    rt.close()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_all_channels_matches_oracle(name):
+    g, ri = _case(name)
+    say(f"[gpu] fp64 all-channels {name}")
+    orc = pyorc.Oracle(pyorc.Prepared(ri))
+    rt, nfr, vmax = _engine(ri, "fp64")
+    fl, im, _ = rt.trace([(0, 0, nfr, vmax, engine.SKIP_NONE)], want_image=True)
+    o = orc.line_spectrum(0, skip_mode=1)
+    assert _rel(fl[0], o["flux"]).max() <= FP64_TOL
+    scale = np.abs(o["image"]).max()
+    assert _rel(im[0], o["image"], floor=1e-12 * scale).max() <= FP64_TOL
+    rt.close()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp32_line_flux_within_1e4(name):
+    g, ri = _case(name)
+    say(f"[gpu] fp32 {name}")
+    rt, nfr, vmax = _engine(ri, "fp32")
+    nl = len(g["flux"])
+    fl, _, _ = rt.trace([(il, 0, nfr, vmax, engine.SKIP_REFERENCE) for il in range(nl)])
+    for il in range(nl):
+        ref = g["flux"][il]
+        li_ref = _line_integral(ref)
+        li = _line_integral(fl[il])
+        err_line = abs(li - li_ref) / max(abs(li_ref), 1e-300)
+        err_ch = _rel(fl[il], ref).max()
+        say(f"    {name} line {il}: fp32 line-integral rel err {err_line:.2e}, "
+            f"max per-channel rel err {err_ch:.2e}")
+        # a line weaker than 1e-6 of the continuum carries no line flux to compare
+        if abs(li_ref) > 1e-6 * np.abs(ref).max() * len(ref):
+            assert err_line <= FP32_LINE_TOL, (name, il, err_line)
+        assert err_ch <= 1e-4, (name, il, err_ch)
+    rt.close()
+
+
+def test_batch_lines_and_inclinations():
+    g, ri = _case("t6")
+    say("[gpu] batch t6 x 2 inclinations")
+    orc = pyorc.Oracle(pyorc.Prepared(ri))
+    rt = engine.Raytracer(0, "fp64")
+    rt.load_run(ri)
+    nfr, vmax = engine.passband(ri.vmax, ri.dv)
+    incs = [35.0 * 0.0174532925199, 62.0 * 0.0174532925199]
+    rt.setup_rays(incs, ri.nphi, ri.b_extra, ri.dbdr)
+    items = [(il, ii, nfr - 3 * il, vmax * (1 + 0.1 * il), engine.SKIP_REFERENCE)
+             for ii in range(2) for il in range(3)]
+    fl, _, _ = rt.trace(items)
+    for k, (il, ii, n, vm, sk) in enumerate(items):
+        o = orc.line_spectrum(il, incl_rad=incs[ii], nfr=n, vmax=vm, skip_mode=sk,
+                              want_image=False)
+        assert _rel(fl[k], o["flux"]).max() <= FP64_TOL, (k, il, ii)
+    rt.close()
+
+
+def test_stop_codes():
+    g, ri = _case("t1")
+    say("[gpu] stop codes")
+    rt, nfr, vmax = _engine(ri, "fp64")
+    with pytest.raises(engine.RadliteError) as e:
+        rt.trace([(5, 0, nfr, vmax, 0)])            # no such line
+    assert e.value.code == 13
+    with pytest.raises(engine.RadliteError) as e:
+        rt.trace([(0, 0, 1, vmax, 0)])              # nfr = 1 (line.F:456-459)
+    assert e.value.code == 13
+    # a valid call still works afterwards
+    fl, _, _ = rt.trace([(0, 0, nfr, vmax, 0)])
+    assert _rel(fl[0], g["flux"][0]).max() <= FP64_TOL
+    rt.close()
+
+
+def test_workload_c2_sampled_rays_and_symmetry():
+    """nr=400 x nth=200, 133,633 rays, 200 channels, 30 and 60 deg: the
+    FP64 engine's per-ray intensities against the oracle on a sample of
+    rays (every 97th), the spectrum's v -> -v symmetry (axisymmetric
+    Keplerian disk), and FP32 against FP64 on the line-integrated flux."""
+    ri, w = synth.workload_inputs("c2", seed=0)
+    say("[gpu] c2 full grid")
+    nfr, vmax = engine.passband(ri.vmax, ri.dv)
+    incs = [x * 0.0174532925199 for x in w["incl"]]
+    rt = engine.Raytracer(0, "fp64")
+    rt.load_run(ri)
+    rt.setup_rays(incs, ri.nphi, ri.b_extra, ri.dbdr)
+    assert rt.nrays == [133633, 133633]
+    fl, im, _ = rt.trace([(0, i, nfr, vmax, engine.SKIP_NONE) for i in range(2)], want_image=True)
+    rt.close()
+    orc = pyorc.Oracle(pyorc.Prepared(ri))
+    for i in range(2):
+        o = orc.line_spectrum(0, incl_rad=incs[i], skip_mode=1, ray_stride=97)
+        rows = np.arange(0, o["nrays"], 97)
+        scale = np.abs(o["image"][rows]).max()
+        rel = _rel(im[i][rows], o["image"][rows], floor=1e-12 * scale)
+        say(f"    c2 incl {w['incl'][i]}: sampled-ray max rel {rel.max():.2e}")
+        assert rel.max() <= FP64_TOL
+        sym = _rel(fl[i], fl[i][::-1]).max()
+        say(f"    c2 incl {w['incl'][i]}: |F(v)-F(-v)|/F max {sym:.2e}")
+        assert sym <= 1e-6
+    rt32 = engine.Raytracer(0, "fp32")
+    rt32.load_run(ri)
+    rt32.setup_rays(incs, ri.nphi, ri.b_extra, ri.dbdr)
+    f32, _, _ = rt32.trace([(0, i, nfr, vmax, engine.SKIP_NONE) for i in range(2)])
+    rt32.close()
+    for i in range(2):
+        e = abs(_line_integral(f32[i]) - _line_integral(fl[i])) / abs(_line_integral(fl[i]))
+        say(f"    c2 incl {w['incl'][i]}: fp32 line-integral rel err {e:.2e}")
+        assert e <= FP32_LINE_TOL
