@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "rl_device.h"
 #include "rl_engine.h"
 
 using rl::CellRec;
@@ -468,7 +469,7 @@ extern "C" int rl_trace_lines_forward(rl_ctx *c, int nitems, const rl_item *item
     HIPCHK(c, c->d_maser.ensure(4));
     for (int i = 0; i < 5; i++) c->t_ms[i] = 0.0;
     for (int i = 0; i < 4; i++) c->counts[i] = 0;
-    const size_t rsz = c->precision == RL_FP64 ? 64 : 32;
+    const size_t rsz = c->precision == RL_FP64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::FastRec);
     size_t fo = 0, io = 0;
     for (int ii = 0; ii < nitems; ii++) {
         const rl_item &it = items[ii];

@@ -19,6 +19,8 @@ constexpr int RAYADPT = 4;
 constexpr int RAYRNPT = 4;
 constexpr int EXMAX = 40;     // extra points per ray: at most 1+16+1+16
 constexpr double LEVTHRES = 1.0e-3;                 // configure.h:52
+constexpr int NF_RESET = 1;    // hole chord: ds := 0, init := 1 (telescope.F:4095-4104)
+constexpr int NF_BLEND = 2;    // central-beam star blend here (telescope.F:4144-4193)
 
 // Error codes are the reference's stop codes.
 enum : int { E_OK = 0, E_INPUT = 13, E_STAR = 124, E_DSNEG = 749,
@@ -160,7 +162,16 @@ __device__ inline void clever_elem(const LineK<T> &L, T &I, T ds,
             T lg_ds = ddiv<T>(sright - sleft, T(31 - 1.0));
             Pt<T> a = p0;
             T sa = T(0);
-            for (int i = 1; i <= 31; i++) {
+            // Only sub-points with 0 < s < ds are used; visiting the index
+            // range that can contain them (with one index of margin on
+            // each side) and keeping the exact test gives the same points.
+            int ilo = 1, ihi = 31;
+            if (lg_ds > T(0)) {
+                T f0 = -sleft / lg_ds, f1 = (ds - sleft) / lg_ds;
+                if (f0 > T(0) && f0 < T(40)) ilo = max(1, (int)f0);
+                if (f1 > T(-10) && f1 < T(40)) ihi = min(31, (int)f1 + 2);
+            }
+            for (int i = ilo; i <= ihi; i++) {
                 T s = sleft + T(i - 1) * lg_ds;
                 if (s > T(0) && s < ds) {
                     T e = ddiv<T>(s, ds), ep = T(1) - e;
@@ -182,13 +193,145 @@ __device__ inline void clever_elem(const LineK<T> &L, T &I, T ds,
     integ_elem<T>(L, I, ds, p0, p1, lw0, lw1, c, ovs);
 }
 
+// ---------------------------------------------------------------------
+// FP32 performance formulation.  Everything that does not depend on the
+// channel is precomputed per node / per segment (FastRec), and the
+// quadrature weights of qdr_src_2 are evaluated without the cancellation
+// in e1 = dtau - (1 - exp(-dtau)), which loses all single-precision
+// digits for small dtau.
+// ---------------------------------------------------------------------
+struct alignas(16) FastRec {
+    // node k values (linear in the level populations, so the sub-grid
+    // interpolates them directly)
+    float ds;      // s_k - s_{k-1}
+    float u;       // Omega.v/c
+    float jd, ad;  // dust emissivity / extinction
+    float jc;      // 5.27296241956e-28 nu0 N_up A          (line.F:4578)
+    float dc;      // 5.27296241956e-28 nu0 (N_low B_lu - N_up B_ul)
+    // segment (k-1 -> k) values
+    float inv_aa;  // 1 / (3.33567e-6 |nu0| lwav)           (line.F:2301)
+    float norm;    // 0.56419583546 * inv_aa
+    float dls;     // sub-grid line length ds/x, < 0 when no sub-grid
+    float inv_du;  // 1 / (u_k - u_{k-1})
+    float flag;    // NF_* bits
+    float pad;
+};
+
+// qdr_src_2 weights: I_out = I*xp + a*S1 + b*S2
+__device__ inline void qdr_weights_f(float x, float &a, float &b, float &xp) {
+    if (x > 1.0e-6f) {
+        if (x < 0.5f) {
+            // b = e1/x = sum_{n>=1} (-1)^{n+1} x^n/(n+1)!,  e0 = x (1 - b)
+            b = x * (0.5f - x * (1.0f / 6 - x * (1.0f / 24 - x * (1.0f / 120 -
+                x * (1.0f / 720 - x * (1.0f / 5040 - x * (1.0f / 40320)))))));
+            float e0 = x * (1.0f - b);
+            xp = 1.0f - e0;
+            a = e0 - b;
+        } else {
+            xp = __expf(-x);
+            float e0 = 1.0f - xp;
+            b = __fdividef(x - e0, x);
+            a = e0 - b;
+        }
+    } else {
+        a = 0.5f * x;
+        b = 0.5f * x;
+        xp = 1.0f - x;
+    }
+}
+
+struct FastLine { float nu0, dnu, uch; };
+
+struct FastCarry { float srcl0, alpl0; int init; };
+
+// integrate_element_linedust (line.F:4515-4624) on FastRec data
+__device__ inline void elem_fast(const FastLine &L, float &I, float ds, float u0, float jd0,
+                                 float ad0, float jc0, float dc0, float u1, float jd1,
+                                 float ad1, float jc1, float dc1, float inv_aa, float norm,
+                                 FastCarry &c, int &ovs) {
+    if (c.init) {
+        float z0 = (L.dnu - L.nu0 * u0) * inv_aa;
+        float phi0 = norm * __expf(-(z0 * z0));
+        c.srcl0 = jc0 * phi0;
+        c.alpl0 = dc0 * phi0;
+    }
+    float z1 = (L.dnu - L.nu0 * u1) * inv_aa;
+    float phi1 = norm * __expf(-(z1 * z1));
+    float srcl1 = jc1 * phi1, alpl1 = dc1 * phi1;
+    float j0 = jd0 + c.srcl0, j1 = jd1 + srcl1;
+    float a0 = ad0 + c.alpl0, a1 = ad1 + alpl1;
+    float dtau = 0.5f * (a0 + a1) * ds;
+    float theomax = 0.5f * (j0 + j1) * ds;
+    float wa, wb, xp;
+    qdr_weights_f(dtau, wa, wb, xp);
+    float r0 = __frcp_rn(a0), r1 = __frcp_rn(a1);
+    float s1 = a0 > 0.f ? j0 * r0 : (a1 > 0.f ? j1 * r1 : 0.f);
+    float s2 = a1 > 0.f ? j1 * r1 : (a0 > 0.f ? j0 * r0 : 0.f);
+    float q = dtau > 1e-9f ? wa * s1 + wb * s2 : theomax;
+    ovs += q > 2.f * theomax;
+    q = fminf(q, theomax);
+    I = I * xp + q;
+    c.srcl0 = srcl1;
+    c.alpl0 = alpl1;
+    c.init = 0;
+}
+
+// charintline (telescope.F:3889-4312) on FastRec data
+__device__ inline float trace_fast(const FastRec *__restrict__ rec, int n, const FastLine &L,
+                                   float I0, float istar, float starfract, int &ovs,
+                                   int &maser) {
+    float I = I0;
+    FastCarry c{0.f, 0.f, 1};
+    FastRec a = rec[0];
+    float u0 = a.u, jd0 = a.jd, ad0 = a.ad, jc0 = a.jc, dc0 = a.dc;
+    for (int k = 1; k < n; k++) {
+        const FastRec b = rec[k];
+        float ds = b.ds;
+        const int fl = (int)b.flag;
+        if (fl & NF_RESET) { ds = 0.f; c.init = 1; }
+        if (fl & NF_BLEND) { I = (1.f - starfract) * I + starfract * istar; c.init = 1; }
+        bool done = false;
+        if (b.dls >= 0.f) {                  // 6x > 1 (line.F:4715)
+            float s_c = ds * (L.uch - u0) * b.inv_du;
+            float sright = s_c + 3.f * b.dls, sleft = s_c - 3.f * b.dls;
+            if (sright > 0.f && sleft < ds) {
+                float lg = (sright - sleft) * (1.f / 30.f);
+                float f0 = -sleft / lg, f1 = (ds - sleft) / lg;
+                int ilo = f0 > 0.f ? max(1, (int)fminf(f0, 40.f)) : 1;
+                int ihi = f1 < 40.f ? min(31, (int)fmaxf(f1, -2.f) + 2) : 31;
+                float pu = u0, pjd = jd0, pad = ad0, pjc = jc0, pdc = dc0, sa = 0.f;
+                const float inv_ds = 1.f / ds;
+                for (int i = ilo; i <= ihi; i++) {
+                    float s = sleft + (float)(i - 1) * lg;
+                    if (s > 0.f && s < ds) {
+                        float e = s * inv_ds, ep = 1.f - e;
+                        float qu = ep * u0 + e * b.u, qjd = ep * jd0 + e * b.jd;
+                        float qad = ep * ad0 + e * b.ad, qjc = ep * jc0 + e * b.jc;
+                        float qdc = ep * dc0 + e * b.dc;
+                        elem_fast(L, I, s - sa, pu, pjd, pad, pjc, pdc, qu, qjd, qad, qjc, qdc,
+                                  b.inv_aa, b.norm, c, ovs);
+                        pu = qu; pjd = qjd; pad = qad; pjc = qjc; pdc = qdc; sa = s;
+                    }
+                }
+                elem_fast(L, I, ds - sa, pu, pjd, pad, pjc, pdc, b.u, b.jd, b.ad, b.jc, b.dc,
+                          b.inv_aa, b.norm, c, ovs);
+                done = true;
+            }
+        }
+        if (!done)
+            elem_fast(L, I, ds, u0, jd0, ad0, jc0, dc0, b.u, b.jd, b.ad, b.jc, b.dc, b.inv_aa,
+                      b.norm, c, ovs);
+        if (c.alpl0 * ds < -0.01f) maser = 1;
+        u0 = b.u; jd0 = b.jd; ad0 = b.ad; jc0 = b.jc; dc0 = b.dc;
+    }
+    return I;
+}
+
 // Node record produced per (item, node): the channel-independent values
 // of get_line_dust_values (line.F:3965-4217) plus the segment length.
 template <typename T>
 struct alignas(sizeof(T) * 8) NodeRec {
     T ds, u, jd, ad, lw, nu, nl, flag;
 };
-constexpr int NF_RESET = 1;    // hole chord: ds := 0, init := 1 (telescope.F:4095-4104)
-constexpr int NF_BLEND = 2;    // central-beam star blend here (telescope.F:4144-4193)
 
 }  // namespace rl

@@ -16,6 +16,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "rl_traj.h"
 
 namespace rl {
@@ -134,43 +136,26 @@ __global__ void k_cells(int ncell, int nlev, int nlines, int line, int lev_up,
 // ---------------------------------------------------------------------
 // node records (one wave per ray) + window + continuum channel
 // ---------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(64)
-k_nodes(int nrays, const long long *__restrict__ off, const GeoNode *__restrict__ geo,
-        const CellRec *__restrict__ cells, NodeRec<T> *__restrict__ rec,
-        RayInfo *__restrict__ rinfo, int nfr, const double *__restrict__ velo,
-        double aks2, int skip_mode) {
-    const int ray = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (ray >= nrays) return;
-    const long long b = off[ray];
-    const int n = (int)(off[ray + 1] - b);
-    double umin = 1.0e300, umax = -1.0e300;
-    for (int k = lane; k < n; k += 64) {
-        GeoNode gn = geo[b + k];
-        const CellRec c00 = cells[gn.c00], c01 = cells[gn.c01];
-        const CellRec c10 = cells[gn.c10], c11 = cells[gn.c11];
-        const double dr = gn.dr, dt = gn.dt;
-        const int icr = gn.flag & 3;
-        double v[5];
-        const double *a = c00.v, *p = c01.v, *q = c10.v, *w = c11.v;
-        for (int f = 0; f < 5; f++) {
-            if (icr == 1) v[f] = (1.0 - dt) * a[f] + dt * p[f];
-            else if (icr == 2) v[f] = (1.0 - dr) * a[f] + dr * q[f];
-            else v[f] = (1.0 - dr) * ((1.0 - dt) * a[f] + dt * p[f]) +
-                        dr * ((1.0 - dt) * q[f] + dt * w[f]);
-        }
-        NodeRec<T> r;
-        r.ds = (T)gn.ds; r.u = (T)gn.u;
-        r.jd = (T)v[0]; r.ad = (T)v[1]; r.lw = (T)v[2]; r.nu = (T)v[3]; r.nl = (T)v[4];
-        r.flag = (T)(gn.flag >> 2);
-        rec[b + k] = r;
-        // window from the node preceding each segment (telescope.F:4265-4270)
-        if (k + 1 < n && v[3] + v[4] > LEVTHRES) {
-            umin = fmin(umin, gn.u);
-            umax = fmax(umax, gn.u);
-        }
+// get_line_dust_values interpolation (line.F:4006-4208) of the cell
+// table {jd, ad, lw, N_up, N_low} at one node
+__device__ inline void node_interp(const GeoNode &gn, const CellRec *__restrict__ cells,
+                                   double v[5]) {
+    const CellRec c00 = cells[gn.c00], c01 = cells[gn.c01];
+    const CellRec c10 = cells[gn.c10], c11 = cells[gn.c11];
+    const double dr = gn.dr, dt = gn.dt;
+    const int icr = gn.flag & 3;
+    for (int f = 0; f < 5; f++) {
+        if (icr == 1) v[f] = (1.0 - dt) * c00.v[f] + dt * c01.v[f];
+        else if (icr == 2) v[f] = (1.0 - dr) * c00.v[f] + dr * c10.v[f];
+        else v[f] = (1.0 - dr) * ((1.0 - dt) * c00.v[f] + dt * c01.v[f]) +
+                    dr * ((1.0 - dt) * c10.v[f] + dt * c11.v[f]);
     }
+}
+
+// NONREDUNDANT window and first skipped channel of one ray (one wave)
+__device__ inline void finish_window(int ray, int lane, double umin, double umax, int nfr,
+                                     const double *__restrict__ velo, double aks2,
+                                     int skip_mode, int n, RayInfo *__restrict__ rinfo) {
     for (int m = 32; m >= 1; m >>= 1) {
         umin = fmin(umin, __shfl_xor(umin, m));
         umax = fmax(umax, __shfl_xor(umax, m));
@@ -190,6 +175,85 @@ k_nodes(int nrays, const long long *__restrict__ off, const GeoNode *__restrict_
         ri.mn = mn; ri.mx = mx; ri.cont = cont < nfr ? cont : -1; ri.n = n;
         rinfo[ray] = ri;
     }
+}
+
+// FP64 parity path: node values as the reference holds them
+__global__ void __launch_bounds__(64)
+k_nodes(int nrays, const long long *__restrict__ off, const GeoNode *__restrict__ geo,
+        const CellRec *__restrict__ cells, NodeRec<double> *__restrict__ rec,
+        RayInfo *__restrict__ rinfo, int nfr, const double *__restrict__ velo,
+        double aks2, int skip_mode) {
+    const int ray = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (ray >= nrays) return;
+    const long long b = off[ray];
+    const int n = (int)(off[ray + 1] - b);
+    double umin = 1.0e300, umax = -1.0e300;
+    for (int k = lane; k < n; k += 64) {
+        const GeoNode gn = geo[b + k];
+        double v[5];
+        node_interp(gn, cells, v);
+        NodeRec<double> r;
+        r.ds = gn.ds; r.u = gn.u;
+        r.jd = v[CR_JD]; r.ad = v[CR_AD]; r.lw = v[CR_LW]; r.nu = v[CR_NU]; r.nl = v[CR_NL];
+        r.flag = (double)(gn.flag >> 2);
+        rec[b + k] = r;
+        // window from the node preceding each segment (telescope.F:4265-4270)
+        if (k + 1 < n && v[CR_NU] + v[CR_NL] > LEVTHRES) {
+            umin = fmin(umin, gn.u);
+            umax = fmax(umax, gn.u);
+        }
+    }
+    finish_window(ray, lane, umin, umax, nfr, velo, aks2, skip_mode, n, rinfo);
+}
+
+// FP32 performance path: channel-independent products precomputed
+__global__ void __launch_bounds__(64)
+k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__restrict__ geo,
+             const CellRec *__restrict__ cells, FastRec *__restrict__ rec,
+             RayInfo *__restrict__ rinfo, int nfr, const double *__restrict__ velo,
+             double aks2, int skip_mode, LineConst lc) {
+    const int ray = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (ray >= nrays) return;
+    const long long b = off[ray];
+    const int n = (int)(off[ray + 1] - b);
+    double umin = 1.0e300, umax = -1.0e300;
+    for (int k = lane; k < n; k += 64) {
+        const GeoNode gn = geo[b + k];
+        double v[5];
+        node_interp(gn, cells, v);
+        FastRec r;
+        r.ds = (float)gn.ds;
+        r.u = (float)gn.u;
+        r.jd = (float)v[CR_JD];
+        r.ad = (float)v[CR_AD];
+        r.jc = (float)(lc.hknu * v[CR_NU] * lc.A);
+        r.dc = (float)(lc.hknu * (v[CR_NL] * lc.Bdu - v[CR_NU] * lc.Bud));
+        r.flag = (float)(gn.flag >> 2);
+        r.pad = 0.f;
+        if (k > 0) {
+            const GeoNode gp = geo[b + k - 1];
+            double vp[5];
+            node_interp(gp, cells, vp);
+            double lwav = 0.5 * (vp[CR_LW] + v[CR_LW]);
+            double aa = lc.c_aa * lwav;
+            r.inv_aa = (float)(1.0 / aa);
+            r.norm = (float)(0.56419583546 / aa);
+            double du = gn.u - gp.u;
+            double x = fabs(du / (lwav / 2.99792458e5));
+            r.dls = 2.0 * 3.0 * x > 1.0 ? (float)(gn.ds / x) : -1.0f;
+            r.inv_du = (float)(1.0 / du);
+        } else {
+            r.inv_aa = 0.f; r.norm = 0.f; r.dls = -1.f; r.inv_du = 0.f;
+        }
+        rec[b + k] = r;
+        if (k + 1 < n && v[CR_NU] + v[CR_NL] > LEVTHRES) {
+            umin = fmin(umin, gn.u);
+            umax = fmax(umax, gn.u);
+        }
+    }
+    finish_window(ray, lane, umin, umax, nfr, velo, aks2, skip_mode, n, rinfo);
 }
 
 // ---------------------------------------------------------------------
@@ -222,10 +286,38 @@ __device__ inline T trace_one(const NodeRec<T> *__restrict__ rec, int n, const L
     return I;
 }
 
+template <typename T> struct RecOf { using type = NodeRec<double>; };
+template <> struct RecOf<float> { using type = FastRec; };
+
+template <typename T>
+__device__ inline double trace_dispatch(const typename RecOf<T>::type *rec, long long b, int n,
+                                        const LineConst &lc, double dnu, double velo,
+                                        double istar, T sf, int &ovs, int &maser);
+
+template <>
+__device__ inline double trace_dispatch<double>(const NodeRec<double> *rec, long long b, int n,
+                                                const LineConst &lc, double dnu, double velo,
+                                                double istar, double sf, int &ovs,
+                                                int &maser) {
+    LineK<double> L;
+    L.nu0 = lc.nu0; L.c_aa = lc.c_aa; L.hknu = lc.hknu;
+    L.A = lc.A; L.Bud = lc.Bud; L.Bdu = lc.Bdu;
+    L.dnu = dnu; L.uch = velo;
+    return trace_one<double>(rec + b, n, L, lc.I0, istar, sf, ovs, maser);
+}
+
+template <>
+__device__ inline double trace_dispatch<float>(const FastRec *rec, long long b, int n,
+                                               const LineConst &lc, double dnu, double velo,
+                                               double istar, float sf, int &ovs, int &maser) {
+    FastLine L{(float)lc.nu0, (float)dnu, (float)velo};
+    return (double)trace_fast(rec + b, n, L, (float)lc.I0, (float)istar, sf, ovs, maser);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_formal(int nrays, int nfr, const long long *__restrict__ off,
-         const NodeRec<T> *__restrict__ rec, const RayInfo *__restrict__ rinfo,
+         const typename RecOf<T>::type *__restrict__ rec, const RayInfo *__restrict__ rinfo,
          const double *__restrict__ dnu, const double *__restrict__ velo,
          const double *__restrict__ istar, LineConst lc, double aks2, int skip_mode,
          double *__restrict__ image, unsigned long long *ovs_out, int *maser_out) {
@@ -240,13 +332,9 @@ k_formal(int nrays, int nfr, const long long *__restrict__ off,
         }
         double out;
         if (doit) {
-            LineK<T> L;
-            L.nu0 = (T)lc.nu0; L.c_aa = (T)lc.c_aa; L.hknu = (T)lc.hknu;
-            L.A = (T)lc.A; L.Bud = (T)lc.Bud; L.Bdu = (T)lc.Bdu;
-            L.dnu = (T)dnu[ch]; L.uch = (T)velo[ch];
             T sf = ray == 0 ? (T)lc.starfract : T(0);
-            out = (double)trace_one<T>(rec + off[ray], ri.n, L, (T)lc.I0, (T)istar[ch], sf,
-                                       ovs, maser);
+            out = trace_dispatch<T>(rec, off[ray], ri.n, lc, dnu[ch], velo[ch], istar[ch], sf,
+                                    ovs, maser);
         } else {
             out = __longlong_as_double(0x7ff8dead00000000ll);   // filled by k_fixup
         }
@@ -268,7 +356,7 @@ k_formal(int nrays, int nfr, const long long *__restrict__ off,
 // it is exactly 0, in which case they are integrated in order.
 template <typename T>
 __global__ void k_fixup(int nrays, int nfr, const long long *__restrict__ off,
-                        const NodeRec<T> *__restrict__ rec, const RayInfo *__restrict__ rinfo,
+                        const typename RecOf<T>::type *__restrict__ rec, const RayInfo *__restrict__ rinfo,
                         const double *__restrict__ dnu, const double *__restrict__ velo,
                         const double *__restrict__ istar, LineConst lc, double aks2,
                         double *__restrict__ image, unsigned long long *ovs_out,
@@ -286,12 +374,8 @@ __global__ void k_fixup(int nrays, int nfr, const long long *__restrict__ off,
         if (cont != 0.0) {
             row[ch] = cont;
         } else {
-            LineK<T> L;
-            L.nu0 = (T)lc.nu0; L.c_aa = (T)lc.c_aa; L.hknu = (T)lc.hknu;
-            L.A = (T)lc.A; L.Bud = (T)lc.Bud; L.Bdu = (T)lc.Bdu;
-            L.dnu = (T)dnu[ch]; L.uch = (T)velo[ch];
-            row[ch] = (double)trace_one<T>(rec + off[ray], ri.n, L, (T)lc.I0, (T)istar[ch],
-                                           T(0), ovs, maser);
+            row[ch] = trace_dispatch<T>(rec, off[ray], ri.n, lc, dnu[ch], velo[ch], istar[ch],
+                                        T(0), ovs, maser);
             cont = row[ch];
         }
     }
@@ -356,11 +440,16 @@ void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int
 
 template <typename T>
 void launch_item(hipStream_t st, const ItemLaunch &it) {
-    auto *rec = reinterpret_cast<NodeRec<T> *>(it.rec);
-    hipLaunchKernelGGL(k_nodes<T>, dim3(it.nrays), dim3(64), 0, st, it.nrays, it.off,
-                       it.geo, it.cells, rec, it.rinfo, it.nfr, it.velo, it.aks2,
-                       it.skip_mode);
-    if (it.ev_nodes) hipEventRecord(it.ev_nodes, st);
+    auto *rec = reinterpret_cast<typename RecOf<T>::type *>(it.rec);
+    if constexpr (std::is_same<T, double>::value)
+        hipLaunchKernelGGL(k_nodes, dim3(it.nrays), dim3(64), 0, st, it.nrays, it.off,
+                           it.geo, it.cells, rec, it.rinfo, it.nfr, it.velo, it.aks2,
+                           it.skip_mode);
+    else
+        hipLaunchKernelGGL(k_nodes_fast, dim3(it.nrays), dim3(64), 0, st, it.nrays, it.off,
+                           it.geo, it.cells, rec, it.rinfo, it.nfr, it.velo, it.aks2,
+                           it.skip_mode, it.lc);
+    if (it.ev_nodes) (void)hipEventRecord(it.ev_nodes, st);
     long long total = (long long)it.nrays * it.nfr;
     hipLaunchKernelGGL(k_formal<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
                        it.nrays, it.nfr, it.off, rec, it.rinfo, it.dnu, it.velo, it.istar,
@@ -369,7 +458,7 @@ void launch_item(hipStream_t st, const ItemLaunch &it) {
         hipLaunchKernelGGL(k_fixup<T>, dim3((it.nrays + 127) / 128), dim3(128), 0, st,
                            it.nrays, it.nfr, it.off, rec, it.rinfo, it.dnu, it.velo,
                            it.istar, it.lc, it.aks2, it.image, it.ovs, it.maser);
-    if (it.ev_formal) hipEventRecord(it.ev_formal, st);
+    if (it.ev_formal) (void)hipEventRecord(it.ev_formal, st);
     dim3 gp((it.nfr + 63) / 64, it.nchunks);
     hipLaunchKernelGGL(k_flux_partial, gp, dim3(64), 0, st, it.nfr, it.nrings, it.nphi,
                        it.rings_per_chunk, it.image, it.ring_w, it.center_w, it.partial);

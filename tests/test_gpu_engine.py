@@ -1,10 +1,15 @@
 """HIP engine parity: FP64 mode against the reference's golden vectors and
 the CPU oracle, FP32 mode against the FP64 oracle, batching, errors.
 
-Tolerances (stated per test): FP64 <= 1e-10 relative per channel (and
-per ray for intensities); FP32 <= 1e-4 relative on the line-integrated
-flux (sum over channels of F - F_continuum, the continuum taken at the
-passband edges).  All GPU calls go through librl.so's C ABI.
+Tolerances (stated per test): FP64 <= 1e-10 relative per channel of the
+spectrum.  Per-ray intensities are held to 1e-8: qdr_src_2 forms
+e1 = dtau - (1 - exp(-dtau)) (transfer.F:1519-1521), which turns a
+one-ulp difference between the device exp and the host libm exp into up
+to ~6e-9 relative on single rays near dtau ~ 1e-6 (measured on the CPU
+by perturbing the oracle's exp by one ulp: 6.1e-9 per ray, 1.4e-12 per
+spectrum channel).  FP32 <= 1e-4 relative on the line-integrated flux
+(sum over channels of F - F_continuum, the continuum taken as the mean
+of the two passband edges).  All GPU calls go through librl.so's C ABI.
 """
 import sys
 import tempfile
@@ -19,6 +24,7 @@ from radlite_amd import engine, l4io, synth
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-10
+RAY_TOL = 1e-8
 FP32_LINE_TOL = 1e-4
 
 
@@ -71,7 +77,7 @@ def test_fp64_matches_reference_golden(name):
         img = im[il][g["image_rows"]]
         scale = np.abs(g["image"][il]).max()
         rel = _rel(img, g["image"][il], floor=1e-12 * scale)
-        assert rel.max() <= FP64_TOL, (name, il, rel.max(), np.unravel_index(rel.argmax(), rel.shape))
+        assert rel.max() <= RAY_TOL, (name, il, rel.max(), np.unravel_index(rel.argmax(), rel.shape))
         assert bool(warn[il] & engine.WARN_MASER) == bool(g["maser"][il])
     rt.close()
 
@@ -86,7 +92,7 @@ def test_fp64_all_channels_matches_oracle(name):
     o = orc.line_spectrum(0, skip_mode=1)
     assert _rel(fl[0], o["flux"]).max() <= FP64_TOL
     scale = np.abs(o["image"]).max()
-    assert _rel(im[0], o["image"], floor=1e-12 * scale).max() <= FP64_TOL
+    assert _rel(im[0], o["image"], floor=1e-12 * scale).max() <= RAY_TOL
     rt.close()
 
 
@@ -169,7 +175,7 @@ def test_workload_c2_sampled_rays_and_symmetry():
         scale = np.abs(o["image"][rows]).max()
         rel = _rel(im[i][rows], o["image"][rows], floor=1e-12 * scale)
         say(f"    c2 incl {w['incl'][i]}: sampled-ray max rel {rel.max():.2e}")
-        assert rel.max() <= FP64_TOL
+        assert rel.max() <= RAY_TOL
         sym = _rel(fl[i], fl[i][::-1]).max()
         say(f"    c2 incl {w['incl'][i]}: |F(v)-F(-v)|/F max {sym:.2e}")
         assert sym <= 1e-6
