@@ -48,7 +48,7 @@ struct DevBuf {
 };
 
 struct RaySet {
-    double theta0 = 0.0;
+    double theta0 = 0.0, costh0 = 1.0, sinth0 = 0.0;
     int nrays = 0, nrings = 0, nphi = 0;
     std::vector<double> x0, z0, ring_r, ring_ri;
     DevBuf d_x0, d_z0;
@@ -70,7 +70,7 @@ struct rl_ctx {
     // grid
     int nr = 0, nh = 0;
     std::vector<double> rc, tc;
-    DevBuf d_rc, d_tc;
+    DevBuf d_rc, d_tc, d_tan2;
     // medium
     bool have_medium = false;
     double umass_av = 2.3, aksmax = 0.0;
@@ -91,7 +91,7 @@ struct rl_ctx {
     std::vector<RaySet> incl;
     // per-item work buffers
     DevBuf d_cells, d_rec, d_rinfo, d_image, d_chan, d_partial, d_flux, d_err, d_ovs,
-        d_maser, d_scr;
+        d_maser, d_scr, d_cross;
     int cells_line = -1;
     // stats
     double t_ms[5] = {0, 0, 0, 0, 0};
@@ -206,6 +206,14 @@ extern "C" int rl_set_grid(rl_ctx *c, int nr, const double *r, int nh, const dou
     T(ny + 2) = (double)twopi_sp - T(ny - 1);
     if (int e = upload(c, c->d_rc, c->rc.data(), c->rc.size() * 8)) return e;
     if (int e = upload(c, c->d_tc, c->tc.data(), c->tc.size() * 8)) return e;
+    // tan^2 of the cone angles (telescope.F:2973), ray independent
+    std::vector<double> t2(nh + 1, 0.0);
+    for (int iy = 1; iy <= nh; iy++) {
+        double tn = std::tan(T(iy));
+        t2[iy] = tn * tn;
+    }
+    if (int e = upload(c, c->d_tan2, t2.data(), t2.size() * 8)) return e;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     c->incl.clear();
     c->cells_line = -1;
     return 0;
@@ -323,6 +331,8 @@ static int build_rayset(rl_ctx *c, RaySet &rs, double anginf, int nphi, int nrex
     }
     const int nrings = nrrextra + nrref + (nr - 1) * dbdr;
     rs.theta0 = theta0;
+    rs.costh0 = std::cos(theta0);                   // telescope.F:2884-2885
+    rs.sinth0 = std::sin(theta0);
     rs.nrings = nrings;
     rs.nphi = nphi;
     rs.nrays = 1 + nrings * nphi;
@@ -420,10 +430,10 @@ static int build_traj(rl_ctx *c, RaySet &rs) {
     DevBuf cnt;
     HIPCHK(c, cnt.ensure((size_t)n * 4));
     HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));
-    double rbeam0 = rs.ring_ri[1];
-    rl::launch_traj(c->stream, g, c->d_vel.as<double>(), c->nh, n, rs.d_x0.as<double>(),
-                    rs.d_z0.as<double>(), rs.theta0, rbeam0, s1, s2, iy, cnt.as<int>(),
-                    nullptr, nullptr, c->d_err.as<int>());
+    const int blend = rs.ring_ri[1] > 0.0 ? 1 : 0;
+    rl::launch_cross(c->stream, g, c->d_tan2.as<double>(), n, rs.d_x0.as<double>(),
+                     rs.d_z0.as<double>(), rs.theta0, rs.costh0, rs.sinth0, blend, s1, s2, iy,
+                     cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, c->d_err.as<int>());
     HIPCHK(c, hipGetLastError());
     std::vector<int> h(n);
     HIPCHK(c, hipMemcpyAsync(h.data(), cnt.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -433,9 +443,18 @@ static int build_traj(rl_ctx *c, RaySet &rs) {
     rs.nnodes = rs.off[n];
     if (int e = upload(c, rs.d_off, rs.off.data(), (size_t)(n + 1) * 8)) return e;
     HIPCHK(c, rs.d_nodes.ensure((size_t)rs.nnodes * sizeof(GeoNode)));
-    rl::launch_traj(c->stream, g, c->d_vel.as<double>(), c->nh, n, rs.d_x0.as<double>(),
-                    rs.d_z0.as<double>(), rs.theta0, rbeam0, s1, s2, iy, nullptr,
-                    rs.d_off.as<long long>(), rs.d_nodes.as<GeoNode>(), c->d_err.as<int>());
+    HIPCHK(c, c->d_cross.ensure((size_t)rs.nnodes * 16));
+    double *cs = c->d_cross.as<double>();
+    int *ccode = reinterpret_cast<int *>(cs + rs.nnodes);
+    int *cray = ccode + rs.nnodes;
+    rl::launch_cross(c->stream, g, c->d_tan2.as<double>(), n, rs.d_x0.as<double>(),
+                     rs.d_z0.as<double>(), rs.theta0, rs.costh0, rs.sinth0, blend, s1, s2, iy,
+                     nullptr, rs.d_off.as<long long>(), cs, ccode, cray, c->d_err.as<int>());
+    HIPCHK(c, hipGetLastError());
+    rl::launch_geo(c->stream, g, c->d_vel.as<double>(), c->nh, rs.nnodes,
+                   rs.d_off.as<long long>(), rs.d_x0.as<double>(), rs.d_z0.as<double>(),
+                   rs.costh0, rs.sinth0, cs, ccode, cray, rs.d_nodes.as<GeoNode>(),
+                   c->d_err.as<int>());
     HIPCHK(c, hipGetLastError());
     if (int e = check_err(c)) return e;
     rs.traj_ready = true;

@@ -74,7 +74,7 @@ struct LineK {            // per (line, channel) constants
 };
 
 template <typename T>
-struct Carry { T phi0, srcl0, alpl0; int init; };
+struct Carry { T phi0, srcl0, alpl0, S0; int init; };
 
 template <typename T>
 struct Pt { T srcd, alpd, u, nu, nl; };     // one integration point
@@ -90,7 +90,8 @@ __device__ inline T voigt(const LineK<T> &L, T aks, T dnu) {
 
 // qdr_src_2, transfer.F:1498-1571 -- I <- I*xp + min(Q, theomax)
 template <typename T>
-__device__ inline T qdr(T I, T js1, T alp1, T js2, T alp2, T ds, int &ovs) {
+__device__ inline T qdr(T I, T js1, T alp1, T js2, T alp2, T ds, int &ovs, T &S0,
+                        bool have_s0) {
     T dtau1 = T(0.5) * (alp1 + alp2) * ds;
     T theomax = T(0.5) * (js1 + js2) * ds;
     T a, b, xp;
@@ -105,13 +106,13 @@ __device__ inline T qdr(T I, T js1, T alp1, T js2, T alp2, T ds, int &ovs) {
         b = T(0.5) * dtau1;
         xp = T(1) - dtau1;
     }
-    T src1, src2;
-    if (alp1 > T(0)) src1 = ddiv<T>(js1, alp1);
-    else if (alp2 > T(0)) src1 = ddiv<T>(js2, alp2);
-    else src1 = T(0);
-    if (alp2 > T(0)) src2 = ddiv<T>(js2, alp2);
-    else if (alp1 > T(0)) src2 = ddiv<T>(js1, alp1);
-    else src2 = T(0);
+    // js1/alp1 equals the previous element's js2/alp2 bit for bit when the
+    // carry is valid, so it is reused rather than divided again
+    T q2 = alp2 > T(0) ? ddiv<T>(js2, alp2) : T(0);
+    T q1 = have_s0 ? S0 : (alp1 > T(0) ? ddiv<T>(js1, alp1) : T(0));
+    T src1 = alp1 > T(0) ? q1 : (alp2 > T(0) ? q2 : T(0));
+    T src2 = alp2 > T(0) ? q2 : (alp1 > T(0) ? q1 : T(0));
+    S0 = q2;
     T q;
     if (dtau1 > T((double)1e-9f)) q = a * src1 + b * src2;   // 1e-9 is REAL*4
     else q = T(0.5) * (js1 + js2) * ds;
@@ -138,7 +139,8 @@ __device__ inline void integ_elem(const LineK<T> &L, T &I, T ds,
     T alpl1 = L.hknu * phi1 * (p1.nl * L.Bdu - p1.nu * L.Bud);
     T src0 = p0.srcd + c.srcl0, src1 = p1.srcd + srcl1;
     T alp0 = p0.alpd + c.alpl0, alp1 = p1.alpd + alpl1;
-    I = qdr<T>(I, src0, alp0, src1, alp1, ds, ovs);
+    const bool have = c.init == 0;
+    I = qdr<T>(I, src0, alp0, src1, alp1, ds, ovs, c.S0, have);
     c.phi0 = phi1;
     c.srcl0 = srcl1;
     c.alpl0 = alpl1;
@@ -242,7 +244,7 @@ __device__ inline void qdr_weights_f(float x, float &a, float &b, float &xp) {
 
 struct FastLine { float nu0, dnu, uch; };
 
-struct FastCarry { float srcl0, alpl0; int init; };
+struct FastCarry { float srcl0, alpl0, S0; int init; };
 
 // integrate_element_linedust (line.F:4515-4624) on FastRec data
 __device__ inline void elem_fast(const FastLine &L, float &I, float ds, float u0, float jd0,
@@ -264,9 +266,15 @@ __device__ inline void elem_fast(const FastLine &L, float &I, float ds, float u0
     float theomax = 0.5f * (j0 + j1) * ds;
     float wa, wb, xp;
     qdr_weights_f(dtau, wa, wb, xp);
-    float r0 = __frcp_rn(a0), r1 = __frcp_rn(a1);
-    float s1 = a0 > 0.f ? j0 * r0 : (a1 > 0.f ? j1 * r1 : 0.f);
-    float s2 = a1 > 0.f ? j1 * r1 : (a0 > 0.f ? j0 * r0 : 0.f);
+    // source functions (transfer.F:1528-1541) with IEEE division: near an
+    // empty region alpha is a float denormal and j*rcp(alpha) overflows.
+    // The start point equals the previous element's end point bit for bit
+    // unless init is set, so its j/alpha is carried.
+    float q1 = a1 > 0.f ? j1 / a1 : 0.f;
+    float q0 = c.init ? (a0 > 0.f ? j0 / a0 : 0.f) : c.S0;
+    float s1 = a0 > 0.f ? q0 : (a1 > 0.f ? q1 : 0.f);
+    float s2 = a1 > 0.f ? q1 : (a0 > 0.f ? q0 : 0.f);
+    c.S0 = q1;
     float q = dtau > 1e-9f ? wa * s1 + wb * s2 : theomax;
     ovs += q > 2.f * theomax;
     q = fminf(q, theomax);
@@ -281,7 +289,7 @@ __device__ inline float trace_fast(const FastRec *__restrict__ rec, int n, const
                                    float I0, float istar, float starfract, int &ovs,
                                    int &maser) {
     float I = I0;
-    FastCarry c{0.f, 0.f, 1};
+    FastCarry c{0.f, 0.f, 0.f, 1};
     FastRec a = rec[0];
     float u0 = a.u, jd0 = a.jd, ad0 = a.ad, jc0 = a.jc, dc0 = a.dc;
     for (int k = 1; k < n; k++) {

@@ -31,87 +31,151 @@ struct CellView {
     __device__ int cell(int ir, int it) const { return (ir - 1) * nh + (it - 1); }
 };
 
+// Stage 1, one thread per ray: the crossing list (s, surface, index) in
+// the reference's order, with the hole-chord and central-beam flags.
+// code = icross | rext << 2 | NF_* << 3 | idx << 5
 __global__ void __launch_bounds__(128)
-k_traj(GridView g, CellView cv, int nrays, const double *__restrict__ x0,
-       const double *__restrict__ z0, double theta0, double rbeam0,
-       double *scr_s1, double *scr_s2, int *scr_iy, int *__restrict__ count,
-       const long long *__restrict__ off, GeoNode *__restrict__ nodes, int *err) {
+k_cross(GridView g, const double *__restrict__ tan2, int nrays,
+        const double *__restrict__ x0, const double *__restrict__ z0, double theta0,
+        double costh0, double sinth0, int central_blend, double *scr_s1, double *scr_s2,
+        int *scr_iy, int *__restrict__ count, const long long *__restrict__ off,
+        double *__restrict__ cs, int *__restrict__ ccode, int *__restrict__ cray, int *err) {
     int ray = blockIdx.x * blockDim.x + threadIdx.x;
     if (ray >= nrays) return;
-    Traj t(g, x0[ray], z0[ray], theta0, scr_s1 + ray, scr_s2 + ray, scr_iy + ray, nrays);
+    Traj t(g, tan2, x0[ray], z0[ray], theta0, costh0, sinth0, scr_s1 + ray, scr_s2 + ray,
+           scr_iy + ray, nrays);
     if (off == nullptr) {
-        auto sink = [&](int, const TrajNode &) {};
-        int n = trajectory(t, sink, false);
+        auto sink = [&](int, const Crossing &) {};
+        int n = trajectory(t, sink);
         if (n < 0) { set_err(err, -n); n = 0; }
         count[ray] = n;
         return;
     }
     const long long base = off[ray];
     const int cap = (int)(off[ray + 1] - base);
-    const int nr = g.nr, ny = g.ny;
-    double s_prev = 0.0;
-    int ir_prev = -1, icr_prev = -1, star_done = 0;
-    const bool central = ray == 0 && rbeam0 > 0.0;
-    auto sink = [&](int k, const TrajNode &nd) {
+    int icr_prev = -1, idx_prev = -1, star_done = 0;
+    const bool central = ray == 0 && central_blend;
+    auto sink = [&](int k, const Crossing &c) {
         if (k >= cap) { set_err(err, E_CAP); return; }
-        GeoNode o;
-        int ir = nd.ir, it = nd.it;
-        if (ir == 0) { set_err(err, E_HOLE); ir = 1; }
-        double dr = (nd.r - g.R(ir)) / (g.R(ir + 1) - g.R(ir));
-        double dt = (nd.theta - g.TH(it)) / (g.TH(it + 1) - g.TH(it));
-        if (dr < 0.0 || dr > 1.0) set_err(err, E_DR);
-        if (dt < 0.0 || dt > 1.0) set_err(err, E_DT);
-        int r0 = ir, r1, t0 = it, t1;
-        if (dr > 0.0) { r1 = ir + 1; if (r1 > nr) r1 = nr; }
-        else { r1 = ir - 1; if (r1 < 1) r1 = 1; }
-        t1 = dt > 0.0 ? it + 1 : it - 1;
-        t0 = ridx_it(t0, ny);
-        t1 = ridx_it(t1, ny);
-        o.c00 = cv.cell(r0, t0);
-        o.c01 = cv.cell(r0, t1);
-        o.c10 = cv.cell(r1, t0);
-        o.c11 = cv.cell(r1, t1);
-        o.dr = dr;
-        o.dt = dt;
-        // projected velocity: get_line_dust_values interpolation of the
-        // velocity vector + omega_dot_v (line.F:4039-4208, 2650-2662)
-        double v[3];
-        const double *V = cv.vel;
-        for (int q = 0; q < 3; q++) {
-            double a00 = V[o.c00 * 3 + q], a01 = V[o.c01 * 3 + q];
-            double a10 = V[o.c10 * 3 + q], a11 = V[o.c11 * 3 + q];
-            if (nd.icross == 1) v[q] = (1.0 - dt) * a00 + dt * a01;
-            else if (nd.icross == 2) v[q] = (1.0 - dr) * a00 + dr * a10;
-            else v[q] = (1.0 - dr) * ((1.0 - dt) * a00 + dt * a01) +
-                        dr * ((1.0 - dt) * a10 + dt * a11);
-        }
-        if (nd.mu > 1.0) set_err(err, 393);
-        o.u = 3.335668e-11 * (nd.mu * v[0] + sqrt(1.0 - nd.mu * nd.mu) *
-                                              (v[1] * sin(nd.phi) + v[2] * cos(nd.phi)));
-        int flag = nd.icross;
-        if (k == 0) {
-            o.ds = 0.0;
-        } else {
-            o.ds = nd.s - s_prev;
-            if (o.ds < 0.0) set_err(err, E_DSNEG);
-            // hole chord, telescope.F:4095-4104
-            if (ir == 1 && ir_prev == 1 && nd.icross == 1 && icr_prev == 1)
-                flag |= NF_RESET << 2;
-            // central-beam star blend, telescope.F:4144-4193
-            if (central && ir == 1 && star_done == 0) {
-                flag |= NF_BLEND << 2;
+        int flag = 0;
+        if (k > 0) {
+            // hole chord (telescope.F:4095-4104): two consecutive crossings of R_1
+            if (c.icross == 1 && c.idx == 1 && icr_prev == 1 && idx_prev == 1) flag |= NF_RESET;
+            // central-beam star blend at the first node on R_1 (telescope.F:4144)
+            if (central && star_done == 0 && c.icross == 1 && c.idx == 1) {
+                flag |= NF_BLEND;
                 star_done = 1;
             }
         }
-        o.flag = flag;
-        nodes[base + k] = o;
-        s_prev = nd.s;
-        ir_prev = ir;
-        icr_prev = nd.icross;
+        cs[base + k] = c.s;
+        ccode[base + k] = c.icross | (c.rext << 2) | (flag << 3) | (c.idx << 5);
+        cray[base + k] = ray;
+        icr_prev = c.icross;
+        idx_prev = c.idx;
     };
-    int n = trajectory(t, sink, true);
+    int n = trajectory(t, sink);
     if (n < 0) set_err(err, -n);
     else if (n != cap) set_err(err, E_CAP);
+}
+
+// Stage 2, one thread per node: position, cell, interpolation stencil,
+// local direction (telescope.F:3682-3718) and projected velocity
+// (get_line_dust_values + omega_dot_v, line.F:4039-4208, 2650-2662).
+__global__ void __launch_bounds__(256)
+k_geo(GridView g, CellView cv, long long nnodes, const long long *__restrict__ off,
+      const double *__restrict__ x0a, const double *__restrict__ z0a, double costh0,
+      double sinth0, const double *__restrict__ cs, const int *__restrict__ ccode,
+      const int *__restrict__ cray, GeoNode *__restrict__ nodes, int *err) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnodes) return;
+    const double pi = PICONST, epsplus = 1.0e1 * TELESC_EPS;
+    const int nr = g.nr, ny = g.ny;
+    const int ray = cray[k];
+    const double x0 = x0a[ray], z0 = z0a[ray];
+    const double sinth02 = sinth0 * sinth0, costh02 = costh0 * costh0;
+    const double s = cs[k];
+    const int code = ccode[k];
+    const int icross = code & 3, rext = (code >> 2) & 1, flag = (code >> 3) & 3, idx = code >> 5;
+    auto thr = [&](double q) { return sqrt(x0 * x0 + z0 * z0 + q * q + 2.0 * z0 * costh0 * q); };
+    auto theta_of_s = [&](double q) {
+        double th = atan(sqrt(x0 * x0 + sinth02 * q * q) / (z0 + costh0 * q));
+        if (th < 0.0) th = th + pi;
+        return th;
+    };
+    auto hunt_r = [&](double x) { return hunt_bisect([&](int i) { return g.R(i); }, nr, x); };
+    auto hunt_t = [&](double x) { return hunt_bisect([&](int i) { return g.TH(i); }, ny, x); };
+    double r, theta;
+    int ir, it;
+    if (icross == 1) {
+        ir = idx;
+        r = idx == 0 ? g.rstar : g.R(idx);
+        theta = theta_of_s(s);
+        it = hunt_t(theta);
+    } else if (icross == 2) {
+        r = thr(s);
+        ir = r < g.R(1) ? 0 : (r > g.R(nr) ? nr : hunt_r(r));
+        it = idx;
+        theta = g.TH(idx);
+    } else {
+        if (rext) {
+            r = sqrt(x0 * x0 + z0 * z0 * (1.0 - costh02));
+            ir = idx;
+        } else {
+            r = thr(s);
+            ir = hunt_r(r);
+        }
+        theta = theta_of_s(s);
+        it = hunt_t(theta);
+    }
+    GeoNode o;
+    if (ir == 0) { set_err(err, E_HOLE); ir = 1; }
+    double dr = (r - g.R(ir)) / (g.R(ir + 1) - g.R(ir));
+    double dt = (theta - g.TH(it)) / (g.TH(it + 1) - g.TH(it));
+    if (dr < 0.0 || dr > 1.0) set_err(err, E_DR);
+    if (dt < 0.0 || dt > 1.0) set_err(err, E_DT);
+    int r1, t1;
+    if (dr > 0.0) { r1 = ir + 1; if (r1 > nr) r1 = nr; }
+    else { r1 = ir - 1; if (r1 < 1) r1 = 1; }
+    t1 = dt > 0.0 ? it + 1 : it - 1;
+    const int t0 = ridx_it(it, ny);
+    t1 = ridx_it(t1, ny);
+    o.c00 = cv.cell(ir, t0);
+    o.c01 = cv.cell(ir, t1);
+    o.c10 = cv.cell(r1, t0);
+    o.c11 = cv.cell(r1, t1);
+    o.dr = dr;
+    o.dt = dt;
+    // local direction, telescope.F:3682-3718
+    const double znew = z0 * sinth02, bnew = sqrt(x0 * x0 + z0 * z0 * sinth02);
+    const double snew = s + z0 * costh0;
+    const double mu = snew / sqrt(bnew * bnew + snew * snew);
+    const double q = znew + snew * costh0;
+    const double dummy = bnew * sqrt(bnew * bnew + snew * snew - q * q);
+    double sinphi = dummy > 0.0 ? (bnew * bnew * costh0 - znew * snew) / dummy : epsplus;
+    sinphi = x0 < 0.0 ? asin(sinphi) : pi - asin(sinphi);
+    while (sinphi < 0.0) sinphi = sinphi + 2.0 * pi;
+    while (sinphi >= 2.0 * pi) sinphi = sinphi - 2.0 * pi;
+    double v[3];
+    const double *V = cv.vel;
+    for (int c = 0; c < 3; c++) {
+        double a00 = V[o.c00 * 3 + c], a01 = V[o.c01 * 3 + c];
+        double a10 = V[o.c10 * 3 + c], a11 = V[o.c11 * 3 + c];
+        if (icross == 1) v[c] = (1.0 - dt) * a00 + dt * a01;
+        else if (icross == 2) v[c] = (1.0 - dr) * a00 + dr * a10;
+        else v[c] = (1.0 - dr) * ((1.0 - dt) * a00 + dt * a01) +
+                    dr * ((1.0 - dt) * a10 + dt * a11);
+    }
+    if (mu > 1.0) set_err(err, 393);
+    o.u = 3.335668e-11 * (mu * v[0] + sqrt(1.0 - mu * mu) *
+                                         (v[1] * sin(sinphi) + v[2] * cos(sinphi)));
+    if (k == off[ray]) {
+        o.ds = 0.0;
+    } else {
+        o.ds = s - cs[k - 1];
+        if (o.ds < 0.0) set_err(err, E_DSNEG);
+    }
+    o.flag = icross | (flag << 2);
+    nodes[k] = o;
 }
 
 // ---------------------------------------------------------------------
@@ -264,7 +328,7 @@ __device__ inline T trace_one(const NodeRec<T> *__restrict__ rec, int n, const L
                               T I0, T istar, T starfract, int &ovs, int &maser) {
     T I = I0;
     Carry<T> c;
-    c.init = 1; c.phi0 = T(0); c.srcl0 = T(0); c.alpl0 = T(0);
+    c.init = 1; c.phi0 = T(0); c.srcl0 = T(0); c.alpl0 = T(0); c.S0 = T(0);
     NodeRec<T> a = rec[0];
     Pt<T> pa{a.jd, a.ad, a.u, a.nu, a.nl};
     T lwa = a.lw;
@@ -418,15 +482,25 @@ __global__ void k_flux_final(int nfr, int nchunks, const double *__restrict__ pa
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
-void launch_traj(hipStream_t st, const GridDev &gd, const double *vel, int nh, int nrays,
-                 const double *x0, const double *z0, double theta0, double rbeam0,
-                 double *s1, double *s2, int *iy, int *count, const long long *off,
-                 GeoNode *nodes, int *err) {
+void launch_cross(hipStream_t st, const GridDev &gd, const double *tan2, int nrays,
+                  const double *x0, const double *z0, double theta0, double costh0,
+                  double sinth0, int central_blend, double *s1, double *s2, int *iy, int *count,
+                  const long long *off, double *cs, int *ccode, int *cray, int *err) {
+    GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar};
+    dim3 blk(128), grd((nrays + 127) / 128);
+    hipLaunchKernelGGL(k_cross, grd, blk, 0, st, g, tan2, nrays, x0, z0, theta0, costh0,
+                       sinth0, central_blend, s1, s2, iy, count, off, cs, ccode, cray, err);
+}
+
+void launch_geo(hipStream_t st, const GridDev &gd, const double *vel, int nh, long long nnodes,
+                const long long *off, const double *x0, const double *z0, double costh0,
+                double sinth0, const double *cs, const int *ccode, const int *cray,
+                GeoNode *nodes, int *err) {
     GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar};
     CellView cv{nh, vel};
-    dim3 blk(128), grd((nrays + 127) / 128);
-    hipLaunchKernelGGL(k_traj, grd, blk, 0, st, g, cv, nrays, x0, z0, theta0, rbeam0, s1,
-                       s2, iy, count, off, nodes, err);
+    dim3 blk(256), grd((unsigned)((nnodes + 255) / 256));
+    hipLaunchKernelGGL(k_geo, grd, blk, 0, st, g, cv, nnodes, off, x0, z0, costh0, sinth0, cs,
+                       ccode, cray, nodes, err);
 }
 
 void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int lev_up,

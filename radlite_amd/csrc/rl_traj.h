@@ -21,13 +21,17 @@ struct GridView {
     __device__ double TH(int i) const { return tc[i + 1]; }
 };
 
-struct TrajNode {
-    double s, r, theta, mu, phi;
-    int icross, ir, it;
+// A crossing as the merge emits it: which surface was crossed and where.
+// icross 1: R = R_idx sphere; 2: Theta = Theta_idx cone; 3: extra point
+// (idx = its radial cell when it is the R-extremum, rext = 1).
+struct Crossing {
+    double s;
+    int icross, idx, rext;
 };
 
 struct Traj {
     const GridView &g;
+    const double *tan2;       // tan^2(Theta_iy), iy = 1..ny/2, from the host libm
     double x0, z0, theta0;
     double costh0, sinth0, costh02, sinth02, pitheta0;
     int iyeq, iyend;
@@ -40,10 +44,11 @@ struct Traj {
     int ir_min, ir_amount, half;
     double bimpact;
 
-    __device__ Traj(const GridView &g_, double x0_, double z0_, double th0,
-                    double *s1, double *s2, int *iy, int strd)
-        : g(g_), x0(x0_), z0(z0_), theta0(th0), sar1(s1), sar2(s2), iyar(iy),
-          stride(strd) {}
+    __device__ Traj(const GridView &g_, const double *tan2_, double x0_, double z0_,
+                    double th0, double cth0, double sth0, double *s1, double *s2, int *iy,
+                    int strd)
+        : g(g_), tan2(tan2_), x0(x0_), z0(z0_), theta0(th0), costh0(cth0), sinth0(sth0),
+          sar1(s1), sar2(s2), iyar(iy), stride(strd) {}
 
     __device__ double thr(double s) const {
         return sqrt(x0 * x0 + z0 * z0 + s * s + 2.0 * z0 * costh0 * s);
@@ -107,19 +112,18 @@ struct Traj {
     }
 };
 
-// Builds the trajectory and streams its nodes to sink(k, node).
+// Builds the crossing list and streams it to sink(k, crossing).
 // Returns the node count (>= 0) or minus a stop code.
 template <typename Sink>
-__device__ int trajectory(Traj &t, Sink &&sink, bool want_dir) {
+__device__ int trajectory(Traj &t, Sink &&sink) {
     const GridView &g = t.g;
     const double eps = TELESC_EPS, epsplus = 1.0e1 * TELESC_EPS;
     const double pi = PICONST, send = 1.0e30;
     const int nr = g.nr, ny = g.ny;
+    (void)ny;
     const double x0 = t.x0, z0 = t.z0;
 
     t.pitheta0 = 0.5 * pi - t.theta0;
-    t.costh0 = cos(t.theta0);
-    t.sinth0 = sin(t.theta0);
     t.costh02 = t.costh0 * t.costh0;
     t.sinth02 = t.sinth0 * t.sinth0;
     const double costh0 = t.costh0, sinth02 = t.sinth02, costh02 = t.costh02;
@@ -129,7 +133,7 @@ __device__ int trajectory(Traj &t, Sink &&sink, bool want_dir) {
     // theta = const cones (telescope.F:2968-3000)
     int is = 1;
     for (int iy = 1; iy <= t.iyeq; iy++) {
-        double theta = g.TH(iy), tn = tan(theta), tanth2 = tn * tn;
+        double tanth2 = t.tan2[iy];
         double a = tanth2 * costh02 - sinth02;
         double b = 2.0 * tanth2 * costh0 * z0;
         double c = tanth2 * z0 * z0 - x0 * x0;
@@ -170,15 +174,15 @@ __device__ int trajectory(Traj &t, Sink &&sink, bool want_dir) {
     }
 
     // extra points (telescope.F:3417-3606)
-    double ex_s[EXMAX + 1], ex_r[EXMAX + 1], ex_t[EXMAX + 1];
+    double ex_s[EXMAX + 1], ex_r[EXMAX + 1];
     int ex_ir[EXMAX + 1], ex_it[EXMAX + 1];
     int nex = 0;
     if (bimpact > g.R(1)) {
         double s2 = 0.0 - z0 * costh0;
         if (nex + 1 + 4 * RAYADPT > EXMAX) return -E_EXOVF;
         nex++;
-        ex_s[nex] = s2; ex_r[nex] = bimpact; ex_t[nex] = t.theta_of_s(s2);
-        ex_ir[nex] = t.ir_min - 1; ex_it[nex] = t.hunt_t(ex_t[nex]);
+        ex_s[nex] = s2; ex_r[nex] = bimpact;
+        ex_ir[nex] = t.ir_min - 1; ex_it[nex] = 1;
         double s3 = s2;
         int isrt = hunt_bisect([&](int i) { return t.r_s(i); }, t.ir_amount, s2);
         for (int irng = 1; irng <= 4; irng++) {
@@ -191,8 +195,8 @@ __device__ int trajectory(Traj &t, Sink &&sink, bool want_dir) {
             for (int iad = 1; iad <= RAYADPT; iad++) {
                 double s0 = iad * ds + s1;
                 nex++;
-                ex_s[nex] = s0; ex_r[nex] = t.thr(s0); ex_t[nex] = t.theta_of_s(s0);
-                ex_ir[nex] = t.hunt_r(ex_r[nex]); ex_it[nex] = t.hunt_t(ex_t[nex]);
+                ex_s[nex] = s0; ex_r[nex] = t.thr(s0);
+                ex_ir[nex] = 0; ex_it[nex] = 0;
             }
         }
     }
@@ -204,8 +208,8 @@ __device__ int trajectory(Traj &t, Sink &&sink, bool want_dir) {
             if (ixh == 0 || ixh == nr) return -E_HUNT;
             if (nex + 1 + 4 * RAYADPT > EXMAX) return -E_EXOVF;
             nex++;
-            ex_s[nex] = s2; ex_r[nex] = rr; ex_t[nex] = t.theta_of_s(s2);
-            ex_ir[nex] = ixh; ex_it[nex] = t.hunt_t(ex_t[nex]);
+            ex_s[nex] = s2; ex_r[nex] = rr;
+            ex_ir[nex] = 0; ex_it[nex] = 0;
             double s3 = s2;
             int isrt = hunt_bisect([&](int i) { return t.th_s(i); }, t.ith_amount, s3);
             if (!(isrt - RAYRNPT + 1 < 1 || isrt + RAYRNPT > t.ith_amount)) {
@@ -223,8 +227,8 @@ __device__ int trajectory(Traj &t, Sink &&sink, bool want_dir) {
                         int ixx = t.hunt_r(r2);
                         if (ixx == 0 || ixx == nr) continue;        // :3580
                         nex++;
-                        ex_s[nex] = s0; ex_r[nex] = r2; ex_t[nex] = t.theta_of_s(s0);
-                        ex_ir[nex] = ixx; ex_it[nex] = t.hunt_t(ex_t[nex]);
+                        ex_s[nex] = s0; ex_r[nex] = r2;
+                        ex_ir[nex] = 0; ex_it[nex] = 0;
                     }
                 }
             }
@@ -232,45 +236,30 @@ __device__ int trajectory(Traj &t, Sink &&sink, bool want_dir) {
     }
     // ray_sort (nrecip.F:833): ascending in s, insertion sort (<= 34)
     for (int i = 2; i <= nex; i++) {
-        double ks = ex_s[i], kr = ex_r[i], kt = ex_t[i];
+        double ks = ex_s[i], kr = ex_r[i];
         int ki = ex_ir[i], kj = ex_it[i], j;
         for (j = i - 1; j >= 1 && ex_s[j] > ks; j--) {
-            ex_s[j + 1] = ex_s[j]; ex_r[j + 1] = ex_r[j]; ex_t[j + 1] = ex_t[j];
+            ex_s[j + 1] = ex_s[j]; ex_r[j + 1] = ex_r[j];
             ex_ir[j + 1] = ex_ir[j]; ex_it[j + 1] = ex_it[j];
         }
-        ex_s[j + 1] = ks; ex_r[j + 1] = kr; ex_t[j + 1] = kt;
+        ex_s[j + 1] = ks; ex_r[j + 1] = kr;
         ex_ir[j + 1] = ki; ex_it[j + 1] = kj;
     }
 
-    // merge (telescope.F:3610-3677) and local direction (:3682-3718)
+    // merge (telescope.F:3610-3677)
     const double rmaxt = g.R(nr) * (1.0 - epsplus), rmaxr = g.R(nr) * (1.0 + epsplus);
     const double rmint = g.R(1) * (1.0 + epsplus), rminr = g.R(1) * (1.0 - epsplus);
     const double sbeg = -1.0e30;
-    const double znew = z0 * sinth02, bnew = sqrt(x0 * x0 + z0 * z0 * sinth02);
     double sprev = -1.0e30;
     int nout = 0, isex = 1, ist = 1, isr = 1;
     double ths, rs, rr_;
     int thit, rix;
     t.th_entry(ist, ths, thit);
     t.r_entry(isr, rs, rr_, rix);
-    auto emit = [&](int icr, double r, double th, int ir, int it, double s) {
-        TrajNode nd;
-        nd.s = s; nd.r = r; nd.theta = th; nd.icross = icr; nd.ir = ir; nd.it = it;
-        if (want_dir) {
-            double snew = s + z0 * costh0;
-            nd.mu = snew / sqrt(bnew * bnew + snew * snew);
-            double q = znew + snew * costh0;
-            double dummy = bnew * sqrt(bnew * bnew + snew * snew - q * q);
-            double sinphi = dummy > 0.0 ? (bnew * bnew * costh0 - znew * snew) / dummy
-                                        : epsplus;
-            sinphi = x0 < 0.0 ? asin(sinphi) : pi - asin(sinphi);
-            while (sinphi < 0.0) sinphi = sinphi + 2.0 * pi;
-            while (sinphi >= 2.0 * pi) sinphi = sinphi - 2.0 * pi;
-            nd.phi = sinphi;
-        } else {
-            nd.mu = 0.0; nd.phi = 0.0;
-        }
-        sink(nout, nd);
+    auto emit = [&](int icr, double s, int idx, int rext) {
+        Crossing c;
+        c.s = s; c.icross = icr; c.idx = idx; c.rext = rext;
+        sink(nout, c);
         nout++;
         sprev = s;
     };
@@ -282,24 +271,20 @@ __device__ int trajectory(Traj &t, Sink &&sink, bool want_dir) {
             if (!(es < m)) break;
             if (ex_r[isex] <= rmaxt && ex_r[isex] >= rmint &&
                 (es - sprev) > eps * ex_r[isex] && es >= sbeg && es <= send)
-                emit(3, ex_r[isex], ex_t[isex], ex_ir[isex], ex_it[isex], es);
+                emit(3, es, ex_ir[isex], ex_it[isex]);
             isex++;
         }
         if (ths < rs) {
             double r = t.thr(ths);
             if (r <= rmaxt && r >= rmint && (ths - sprev) > eps * r &&
-                ths >= sbeg && ths <= send) {
-                int ir = r < g.R(1) ? 0 : (r > g.R(nr) ? nr : t.hunt_r(r));
-                emit(2, r, g.TH(thit), ir, thit, ths);
-            }
+                ths >= sbeg && ths <= send)
+                emit(2, ths, thit, 0);
             ist++;
             t.th_entry(ist, ths, thit);
         } else {
             if (rr_ <= rmaxr && rr_ >= rminr && (rs - sprev) > eps * rr_ &&
-                rs >= sbeg && rs <= send) {
-                double th = t.theta_of_s(rs);
-                emit(1, rr_, th, rix, t.hunt_t(th), rs);
-            }
+                rs >= sbeg && rs <= send)
+                emit(1, rs, rix, 0);
             isr++;
             t.r_entry(isr, rs, rr_, rix);
         }
