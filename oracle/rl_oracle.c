@@ -1116,3 +1116,34 @@ double orc_aksmax(int ncell, const double *lw, double fix_lw)
     }
     return mx;
 }
+
+/* Test hook: the channel-independent node values of one ray for one line
+ * (get_line_dust_values at every trajectory node) with the hole-reset and
+ * star-blend flags, as charintline consumes them.  Arrays of size >= the
+ * node count; returns the count, or minus a stop code. */
+int orc_ray_nodes(const orc_model *m, const orc_grid *g, const orc_rays *rays, int il,
+                  int iray, int cap, double *ds, double *u, double *jd, double *ad,
+                  double *lw, double *nu, double *nl, int *flags)
+{
+    orc_traj t;
+    raycache_t c;
+    int e, k, n;
+    double rbeam0 = iray == 0 ? rays->ring_ri[1] : -1.0;
+    memset(&t, 0, sizeof t);
+    memset(&c, 0, sizeof c);
+    e = orc_trajectory(&t, g, m->rstar, rays->x0[iray], rays->z0[iray], rays->theta0, 0.0);
+    if (!e) e = build_cache(m, g, &t, il, rbeam0, &c);
+    n = t.n;
+    if (!e && n > cap) e = 13;
+    if (!e) {
+        for (k = 0; k < n; k++) {
+            ds[k] = c.ds[k]; u[k] = c.nv[k].dvmu; jd[k] = c.nv[k].src; ad[k] = c.nv[k].alp;
+            lw[k] = c.nv[k].lw; nu[k] = c.nv[k].nup; nl[k] = c.nv[k].ndown;
+            flags[k] = (c.reset[k] ? 1 : 0) | (c.blend[k] ? 2 : 0);
+        }
+        if (iray == 0) ds[0] = c.starfract;
+    }
+    orc_traj_free(&t);
+    free_cache(&c);
+    return e ? -e : n;
+}

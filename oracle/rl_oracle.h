@@ -99,6 +99,11 @@ void orc_star_intensity(int nfreq, const double *fnu_1pc, double rstar,
 double orc_aksmax(int ncell, const double *lw, double fix_lw);
 int  orc_passband_nfr(double vmax, double dv, double *vmax_out);
 
+/* Test hook: node values of one ray (ds[0] carries starfract for ray 0). */
+int orc_ray_nodes(const orc_model *m, const orc_grid *g, const orc_rays *rays, int il,
+                  int iray, int cap, double *ds, double *u, double *jd, double *ad,
+                  double *lw, double *nu, double *nl, int *flags);
+
 /* Exposed pieces for unit tests. */
 double orc_qdr_src_2(double I, double js1, double alp1, double js2,
                      double alp2, double ds, long *overshoot);

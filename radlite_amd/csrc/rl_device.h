@@ -203,19 +203,25 @@ __device__ inline void clever_elem(const LineK<T> &L, T &I, T ds,
 // digits for small dtau.
 // ---------------------------------------------------------------------
 struct alignas(16) FastRec {
-    // node k values (linear in the level populations, so the sub-grid
-    // interpolates them directly)
+    // node k: emissivities and opacities are stored divided by a per-node
+    // scale sig (the dust opacity where there is dust), so that j/alpha is
+    // formed from O(1) numbers even where j and alpha themselves are below
+    // the single-precision range.  j = sig*(jdh + jch*phi), alpha =
+    // sig*(adh + dch*phi) with phi the line profile.
     float ds;      // s_k - s_{k-1}
     float u;       // Omega.v/c
-    float jd, ad;  // dust emissivity / extinction
-    float jc;      // 5.27296241956e-28 nu0 N_up A          (line.F:4578)
-    float dc;      // 5.27296241956e-28 nu0 (N_low B_lu - N_up B_ul)
-    // segment (k-1 -> k) values
-    float inv_aa;  // 1 / (3.33567e-6 |nu0| lwav)           (line.F:2301)
+    float sig;     // scale (may underflow to 0 in float; never used as divisor)
+    float jdh, adh;// dust j, alpha / sig
+    float jch;     // 5.27296241956e-28 nu0 N_up A / sig         (line.F:4578)
+    float dch;     // 5.27296241956e-28 nu0 (N_low B_lu - N_up B_ul) / sig
+    float flag;    // NF_* bits
+    // segment (k-1 -> k)
+    float inv_aa;  // 1 / (3.33567e-6 |nu0| lwav)                 (line.F:2301)
     float norm;    // 0.56419583546 * inv_aa
     float dls;     // sub-grid line length ds/x, < 0 when no sub-grid
     float inv_du;  // 1 / (u_k - u_{k-1})
-    float flag;    // NF_* bits
+    float rel0, rel1;  // sig_{k-1}/sig_max, sig_k/sig_max (sub-grid weights)
+    float sigm;    // sig_max = max(sig_{k-1}, sig_k)
     float pad;
 };
 
@@ -244,43 +250,43 @@ __device__ inline void qdr_weights_f(float x, float &a, float &b, float &xp) {
 
 struct FastLine { float nu0, dnu, uch; };
 
-struct FastCarry { float srcl0, alpl0, S0; int init; };
+struct FPt { float u, sig, jd, ad, jc, dc; };     // one point, scaled values
 
-// integrate_element_linedust (line.F:4515-4624) on FastRec data
-__device__ inline void elem_fast(const FastLine &L, float &I, float ds, float u0, float jd0,
-                                 float ad0, float jc0, float dc0, float u1, float jd1,
-                                 float ad1, float jc1, float dc1, float inv_aa, float norm,
-                                 FastCarry &c, int &ovs) {
+struct FastCarry { float jl0, al0, S0; int init; };  // line j, alpha / sig at the start
+
+// integrate_element_linedust (line.F:4515-4624) on scaled values
+__device__ inline void elem_fast(const FastLine &L, float &I, float ds, const FPt &p0,
+                                 const FPt &p1, float inv_aa, float norm, FastCarry &c,
+                                 int &ovs) {
     if (c.init) {
-        float z0 = (L.dnu - L.nu0 * u0) * inv_aa;
+        float z0 = (L.dnu - L.nu0 * p0.u) * inv_aa;
         float phi0 = norm * __expf(-(z0 * z0));
-        c.srcl0 = jc0 * phi0;
-        c.alpl0 = dc0 * phi0;
+        c.jl0 = p0.jc * phi0;
+        c.al0 = p0.dc * phi0;
     }
-    float z1 = (L.dnu - L.nu0 * u1) * inv_aa;
+    float z1 = (L.dnu - L.nu0 * p1.u) * inv_aa;
     float phi1 = norm * __expf(-(z1 * z1));
-    float srcl1 = jc1 * phi1, alpl1 = dc1 * phi1;
-    float j0 = jd0 + c.srcl0, j1 = jd1 + srcl1;
-    float a0 = ad0 + c.alpl0, a1 = ad1 + alpl1;
+    float jl1 = p1.jc * phi1, al1 = p1.dc * phi1;
+    float jh0 = p0.jd + c.jl0, jh1 = p1.jd + jl1;      // j / sig
+    float ah0 = p0.ad + c.al0, ah1 = p1.ad + al1;      // alpha / sig
+    float a0 = p0.sig * ah0, a1 = p1.sig * ah1;
     float dtau = 0.5f * (a0 + a1) * ds;
-    float theomax = 0.5f * (j0 + j1) * ds;
+    float theomax = 0.5f * (p0.sig * jh0 + p1.sig * jh1) * ds;
     float wa, wb, xp;
     qdr_weights_f(dtau, wa, wb, xp);
-    // source functions (transfer.F:1528-1541) with IEEE division: near an
-    // empty region alpha is a float denormal and j*rcp(alpha) overflows.
-    // The start point equals the previous element's end point bit for bit
-    // unless init is set, so its j/alpha is carried.
-    float q1 = a1 > 0.f ? j1 / a1 : 0.f;
-    float q0 = c.init ? (a0 > 0.f ? j0 / a0 : 0.f) : c.S0;
-    float s1 = a0 > 0.f ? q0 : (a1 > 0.f ? q1 : 0.f);
-    float s2 = a1 > 0.f ? q1 : (a0 > 0.f ? q0 : 0.f);
-    c.S0 = q1;
+    // source functions (transfer.F:1528-1541) from the scaled values; the
+    // start point equals the previous end point unless init is set
+    float q1 = ah1 > 0.f ? jh1 / ah1 : 0.f;
+    float q0 = c.init ? (ah0 > 0.f ? jh0 / ah0 : 0.f) : c.S0;
+    float s1 = ah0 > 0.f ? q0 : (ah1 > 0.f ? q1 : 0.f);
+    float s2 = ah1 > 0.f ? q1 : (ah0 > 0.f ? q0 : 0.f);
     float q = dtau > 1e-9f ? wa * s1 + wb * s2 : theomax;
     ovs += q > 2.f * theomax;
     q = fminf(q, theomax);
     I = I * xp + q;
-    c.srcl0 = srcl1;
-    c.alpl0 = alpl1;
+    c.jl0 = jl1;
+    c.al0 = al1;
+    c.S0 = q1;
     c.init = 0;
 }
 
@@ -290,47 +296,51 @@ __device__ inline float trace_fast(const FastRec *__restrict__ rec, int n, const
                                    int &maser) {
     float I = I0;
     FastCarry c{0.f, 0.f, 0.f, 1};
-    FastRec a = rec[0];
-    float u0 = a.u, jd0 = a.jd, ad0 = a.ad, jc0 = a.jc, dc0 = a.dc;
+    const FastRec a = rec[0];
+    FPt p0{a.u, a.sig, a.jdh, a.adh, a.jch, a.dch};
     for (int k = 1; k < n; k++) {
         const FastRec b = rec[k];
+        const FPt p1{b.u, b.sig, b.jdh, b.adh, b.jch, b.dch};
         float ds = b.ds;
         const int fl = (int)b.flag;
         if (fl & NF_RESET) { ds = 0.f; c.init = 1; }
         if (fl & NF_BLEND) { I = (1.f - starfract) * I + starfract * istar; c.init = 1; }
         bool done = false;
         if (b.dls >= 0.f) {                  // 6x > 1 (line.F:4715)
-            float s_c = ds * (L.uch - u0) * b.inv_du;
+            float s_c = ds * (L.uch - p0.u) * b.inv_du;
             float sright = s_c + 3.f * b.dls, sleft = s_c - 3.f * b.dls;
             if (sright > 0.f && sleft < ds) {
                 float lg = (sright - sleft) * (1.f / 30.f);
                 float f0 = -sleft / lg, f1 = (ds - sleft) / lg;
                 int ilo = f0 > 0.f ? max(1, (int)fminf(f0, 40.f)) : 1;
                 int ihi = f1 < 40.f ? min(31, (int)fmaxf(f1, -2.f) + 2) : 31;
-                float pu = u0, pjd = jd0, pad = ad0, pjc = jc0, pdc = dc0, sa = 0.f;
+                FPt pp = p0;
+                float sa = 0.f;
                 const float inv_ds = 1.f / ds;
                 for (int i = ilo; i <= ihi; i++) {
                     float s = sleft + (float)(i - 1) * lg;
                     if (s > 0.f && s < ds) {
+                        // linear interpolation of the unscaled quantities
+                        // (line.F:4775-4790), done with relative weights
                         float e = s * inv_ds, ep = 1.f - e;
-                        float qu = ep * u0 + e * b.u, qjd = ep * jd0 + e * b.jd;
-                        float qad = ep * ad0 + e * b.ad, qjc = ep * jc0 + e * b.jc;
-                        float qdc = ep * dc0 + e * b.dc;
-                        elem_fast(L, I, s - sa, pu, pjd, pad, pjc, pdc, qu, qjd, qad, qjc, qdc,
-                                  b.inv_aa, b.norm, c, ovs);
-                        pu = qu; pjd = qjd; pad = qad; pjc = qjc; pdc = qdc; sa = s;
+                        float w0 = ep * b.rel0, w1 = e * b.rel1, W = w0 + w1;
+                        float iw = 1.f / W;
+                        w0 *= iw; w1 *= iw;
+                        FPt q{ep * p0.u + e * p1.u, b.sigm * W, w0 * p0.jd + w1 * p1.jd,
+                              w0 * p0.ad + w1 * p1.ad, w0 * p0.jc + w1 * p1.jc,
+                              w0 * p0.dc + w1 * p1.dc};
+                        elem_fast(L, I, s - sa, pp, q, b.inv_aa, b.norm, c, ovs);
+                        pp = q;
+                        sa = s;
                     }
                 }
-                elem_fast(L, I, ds - sa, pu, pjd, pad, pjc, pdc, b.u, b.jd, b.ad, b.jc, b.dc,
-                          b.inv_aa, b.norm, c, ovs);
+                elem_fast(L, I, ds - sa, pp, p1, b.inv_aa, b.norm, c, ovs);
                 done = true;
             }
         }
-        if (!done)
-            elem_fast(L, I, ds, u0, jd0, ad0, jc0, dc0, b.u, b.jd, b.ad, b.jc, b.dc, b.inv_aa,
-                      b.norm, c, ovs);
-        if (c.alpl0 * ds < -0.01f) maser = 1;
-        u0 = b.u; jd0 = b.jd; ad0 = b.ad; jc0 = b.jc; dc0 = b.dc;
+        if (!done) elem_fast(L, I, ds, p0, p1, b.inv_aa, b.norm, c, ovs);
+        if (b.sig * c.al0 * ds < -0.01f) maser = 1;
+        p0 = p1;
     }
     return I;
 }

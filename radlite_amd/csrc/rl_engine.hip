@@ -271,6 +271,17 @@ k_nodes(int nrays, const long long *__restrict__ off, const GeoNode *__restrict_
     finish_window(ray, lane, umin, umax, nfr, velo, aks2, skip_mode, n, rinfo);
 }
 
+// per-node scale for the FP32 records: the dust opacity where there is
+// dust, else the largest line coefficient, else 1
+__device__ inline void scaled_node(const double v[5], const LineConst &lc, double &sig,
+                                   double &jc, double &dc) {
+    jc = lc.hknu * v[CR_NU] * lc.A;
+    dc = lc.hknu * (v[CR_NL] * lc.Bdu - v[CR_NU] * lc.Bud);
+    sig = v[CR_AD];
+    if (!(sig > 0.0)) sig = fmax(fmax(fabs(dc), fabs(jc)), fabs(v[CR_JD]));
+    if (!(sig > 0.0)) sig = 1.0;
+}
+
 // FP32 performance path: channel-independent products precomputed
 __global__ void __launch_bounds__(64)
 k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__restrict__ geo,
@@ -288,18 +299,22 @@ k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__rest
         double v[5];
         node_interp(gn, cells, v);
         FastRec r;
+        double sig, jc, dc;
+        scaled_node(v, lc, sig, jc, dc);
         r.ds = (float)gn.ds;
         r.u = (float)gn.u;
-        r.jd = (float)v[CR_JD];
-        r.ad = (float)v[CR_AD];
-        r.jc = (float)(lc.hknu * v[CR_NU] * lc.A);
-        r.dc = (float)(lc.hknu * (v[CR_NL] * lc.Bdu - v[CR_NU] * lc.Bud));
+        r.sig = (float)sig;
+        r.jdh = (float)(v[CR_JD] / sig);
+        r.adh = (float)(v[CR_AD] / sig);
+        r.jch = (float)(jc / sig);
+        r.dch = (float)(dc / sig);
         r.flag = (float)(gn.flag >> 2);
         r.pad = 0.f;
         if (k > 0) {
             const GeoNode gp = geo[b + k - 1];
-            double vp[5];
+            double vp[5], sp, jcp, dcp;
             node_interp(gp, cells, vp);
+            scaled_node(vp, lc, sp, jcp, dcp);
             double lwav = 0.5 * (vp[CR_LW] + v[CR_LW]);
             double aa = lc.c_aa * lwav;
             r.inv_aa = (float)(1.0 / aa);
@@ -308,8 +323,13 @@ k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__rest
             double x = fabs(du / (lwav / 2.99792458e5));
             r.dls = 2.0 * 3.0 * x > 1.0 ? (float)(gn.ds / x) : -1.0f;
             r.inv_du = (float)(1.0 / du);
+            double sm = fmax(sp, sig);
+            r.rel0 = (float)(sp / sm);
+            r.rel1 = (float)(sig / sm);
+            r.sigm = (float)sm;
         } else {
             r.inv_aa = 0.f; r.norm = 0.f; r.dls = -1.f; r.inv_du = 0.f;
+            r.rel0 = r.rel1 = 1.f; r.sigm = r.sig;
         }
         rec[b + k] = r;
         if (k + 1 < n && v[CR_NU] + v[CR_NL] > LEVTHRES) {
