@@ -26,6 +26,34 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
+def measured_traffic(precision):
+    """HBM bytes per k_formal launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (counter values calibrated against a plain copy), or
+    None when no summary for this precision exists."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_c2_{precision}_forward.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[2] == precision:
+            return float(f[3])
+    return None
+
+
+def copy_probe(nbytes):
+    """A plain device-to-device copy of known size: what FETCH_SIZE and
+    WRITE_SIZE read for it calibrates the counters against true bytes."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -37,7 +65,12 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=8,
                     help="CPU baseline traces every n-th ray of the workload")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="run only a plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding of (line, inclination) items is described in DESIGN.md)",
@@ -76,6 +109,7 @@ def main():
     nseg = (nodes - rays) / rays                      # mean segments per ray
     bytes_per_rc = (28.0 if args.precision == "fp32" else 56.0) * nseg
     formal_s = float(np.mean(t_formal)) / 1e3
+    # per launch (one inclination): algorithmic bytes / mean launch duration
     achieved_gbs = bytes_per_rc * nominal / formal_s / 1e9
     flops_per_rc = 40.0 * nseg
     achieved_tf = flops_per_rc * nominal / formal_s / 1e12
@@ -109,10 +143,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.precision) if args.workload == "c2" else None,
             "kernel": "k_formal",
             "bytes_per_ray_channel": bytes_per_rc,
-            "kernel_ms": formal_s * 1e3,
+            "kernel_ms": formal_s * 1e3 / len(items),     # mean per launch (one item)
+            "launches_per_step": len(items),
             "valu_tflops_algorithmic": achieved_tf,
             "valu_frac_of_fp32_peak": achieved_tf / FP32_PEAK_TFLOPS,
         },

@@ -431,9 +431,15 @@ static int build_traj(rl_ctx *c, RaySet &rs) {
     HIPCHK(c, cnt.ensure((size_t)n * 4));
     HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));
     const int blend = rs.ring_ri[1] > 0.0 ? 1 : 0;
+    // upper bound of the nodes of one ray: every R crossing, every cone
+    // crossing and every extra point (telescope.F:3279, 3196, RAYEXPT)
+    const int nmax = 2 * (c->nr + 1) + 2 * c->nh + 40;
+    HIPCHK(c, c->d_cross.ensure((size_t)n * nmax * 12));
+    double *cs = c->d_cross.as<double>();
+    int *ccode = reinterpret_cast<int *>(cs + (size_t)n * nmax);
     rl::launch_cross(c->stream, g, c->d_tan2.as<double>(), n, rs.d_x0.as<double>(),
                      rs.d_z0.as<double>(), rs.theta0, rs.costh0, rs.sinth0, blend, s1, s2, iy,
-                     cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, c->d_err.as<int>());
+                     nmax, cnt.as<int>(), cs, ccode, c->d_err.as<int>());
     HIPCHK(c, hipGetLastError());
     std::vector<int> h(n);
     HIPCHK(c, hipMemcpyAsync(h.data(), cnt.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -443,17 +449,9 @@ static int build_traj(rl_ctx *c, RaySet &rs) {
     rs.nnodes = rs.off[n];
     if (int e = upload(c, rs.d_off, rs.off.data(), (size_t)(n + 1) * 8)) return e;
     HIPCHK(c, rs.d_nodes.ensure((size_t)rs.nnodes * sizeof(GeoNode)));
-    HIPCHK(c, c->d_cross.ensure((size_t)rs.nnodes * 16));
-    double *cs = c->d_cross.as<double>();
-    int *ccode = reinterpret_cast<int *>(cs + rs.nnodes);
-    int *cray = ccode + rs.nnodes;
-    rl::launch_cross(c->stream, g, c->d_tan2.as<double>(), n, rs.d_x0.as<double>(),
-                     rs.d_z0.as<double>(), rs.theta0, rs.costh0, rs.sinth0, blend, s1, s2, iy,
-                     nullptr, rs.d_off.as<long long>(), cs, ccode, cray, c->d_err.as<int>());
-    HIPCHK(c, hipGetLastError());
-    rl::launch_geo(c->stream, g, c->d_vel.as<double>(), c->nh, rs.nnodes,
+    rl::launch_geo(c->stream, g, c->d_vel.as<double>(), c->nh, n, nmax,
                    rs.d_off.as<long long>(), rs.d_x0.as<double>(), rs.d_z0.as<double>(),
-                   rs.costh0, rs.sinth0, cs, ccode, cray, rs.d_nodes.as<GeoNode>(),
+                   rs.costh0, rs.sinth0, cs, ccode, rs.d_nodes.as<GeoNode>(),
                    c->d_err.as<int>());
     HIPCHK(c, hipGetLastError());
     if (int e = check_err(c)) return e;
@@ -543,7 +541,7 @@ extern "C" int rl_trace_lines_forward(rl_ctx *c, int nitems, const rl_item *item
             lc.I0 = 1.47455253991e-47 * (nu0s * nu0s * nu0s) /
                     (std::exp(4.7991598e-11 * nu0s / 2.728) - 1.0);
         const int nrays = rs.nrays;
-        const int rpc = 16, nchunks = (rs.nrings + rpc - 1) / rpc;
+        const int rpc = 1, nchunks = rs.nrings;
         HIPCHK(c, c->d_rec.ensure((size_t)rs.nnodes * rsz));
         HIPCHK(c, c->d_rinfo.ensure((size_t)nrays * sizeof(RayInfo)));
         HIPCHK(c, c->d_image.ensure((size_t)nrays * nfr * 8));

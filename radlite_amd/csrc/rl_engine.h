@@ -59,11 +59,11 @@ struct ItemLaunch {
 
 void launch_cross(hipStream_t st, const GridDev &g, const double *tan2, int nrays,
                   const double *x0, const double *z0, double theta0, double costh0,
-                  double sinth0, int central_blend, double *s1, double *s2, int *iy, int *count,
-                  const long long *off, double *cs, int *ccode, int *cray, int *err);
-void launch_geo(hipStream_t st, const GridDev &g, const double *vel, int nh, long long nnodes,
-                const long long *off, const double *x0, const double *z0, double costh0,
-                double sinth0, const double *cs, const int *ccode, const int *cray,
+                  double sinth0, int central_blend, double *s1, double *s2, int *iy, int nmax,
+                  int *count, double *cs, int *ccode, int *err);
+void launch_geo(hipStream_t st, const GridDev &g, const double *vel, int nh, int nrays,
+                int nmax, const long long *off, const double *x0, const double *z0,
+                double costh0, double sinth0, const double *cs, const int *ccode,
                 GeoNode *nodes, int *err);
 void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int lev_up,
                   int lev_low, double molpg, const double *rho, const double *abund,

@@ -32,31 +32,24 @@ struct CellView {
 };
 
 // Stage 1, one thread per ray: the crossing list (s, surface, index) in
-// the reference's order, with the hole-chord and central-beam flags.
+// the reference's order, with the hole-chord and central-beam flags, into
+// a fixed-size slot per ray (nmax bounds the node count).
 // code = icross | rext << 2 | NF_* << 3 | idx << 5
 __global__ void __launch_bounds__(128)
 k_cross(GridView g, const double *__restrict__ tan2, int nrays,
         const double *__restrict__ x0, const double *__restrict__ z0, double theta0,
         double costh0, double sinth0, int central_blend, double *scr_s1, double *scr_s2,
-        int *scr_iy, int *__restrict__ count, const long long *__restrict__ off,
-        double *__restrict__ cs, int *__restrict__ ccode, int *__restrict__ cray, int *err) {
+        int *scr_iy, int nmax, int *__restrict__ count, double *__restrict__ cs,
+        int *__restrict__ ccode, int *err) {
     int ray = blockIdx.x * blockDim.x + threadIdx.x;
     if (ray >= nrays) return;
     Traj t(g, tan2, x0[ray], z0[ray], theta0, costh0, sinth0, scr_s1 + ray, scr_s2 + ray,
            scr_iy + ray, nrays);
-    if (off == nullptr) {
-        auto sink = [&](int, const Crossing &) {};
-        int n = trajectory(t, sink);
-        if (n < 0) { set_err(err, -n); n = 0; }
-        count[ray] = n;
-        return;
-    }
-    const long long base = off[ray];
-    const int cap = (int)(off[ray + 1] - base);
+    const long long base = (long long)ray * nmax;
     int icr_prev = -1, idx_prev = -1, star_done = 0;
     const bool central = ray == 0 && central_blend;
     auto sink = [&](int k, const Crossing &c) {
-        if (k >= cap) { set_err(err, E_CAP); return; }
+        if (k >= nmax) { set_err(err, E_CAP); return; }
         int flag = 0;
         if (k > 0) {
             // hole chord (telescope.F:4095-4104): two consecutive crossings of R_1
@@ -69,29 +62,28 @@ k_cross(GridView g, const double *__restrict__ tan2, int nrays,
         }
         cs[base + k] = c.s;
         ccode[base + k] = c.icross | (c.rext << 2) | (flag << 3) | (c.idx << 5);
-        cray[base + k] = ray;
         icr_prev = c.icross;
         idx_prev = c.idx;
     };
     int n = trajectory(t, sink);
-    if (n < 0) set_err(err, -n);
-    else if (n != cap) set_err(err, E_CAP);
+    if (n < 0) { set_err(err, -n); n = 0; }
+    if (n > nmax) { set_err(err, E_CAP); n = nmax; }
+    count[ray] = n;
 }
+
+__device__ __forceinline__ void geo_node(GridView g, CellView cv, int k, double x0, double z0, double costh0,
+                         double sinth0, const double *__restrict__ cs,
+                         const int *__restrict__ ccode, GeoNode *__restrict__ nodes, int *err);
 
 // Stage 2, one thread per node: position, cell, interpolation stencil,
 // local direction (telescope.F:3682-3718) and projected velocity
-// (get_line_dust_values + omega_dot_v, line.F:4039-4208, 2650-2662).
-__global__ void __launch_bounds__(256)
-k_geo(GridView g, CellView cv, long long nnodes, const long long *__restrict__ off,
-      const double *__restrict__ x0a, const double *__restrict__ z0a, double costh0,
-      double sinth0, const double *__restrict__ cs, const int *__restrict__ ccode,
-      const int *__restrict__ cray, GeoNode *__restrict__ nodes, int *err) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nnodes) return;
+// (get_line_dust_values + omega_dot_v, line.F:4039-4208, 2650-2662),
+// compacted from the per-ray slots into the CSR node array.
+__device__ __forceinline__ void geo_node(GridView g, CellView cv, int k, double x0, double z0, double costh0,
+                         double sinth0, const double *__restrict__ cs,
+                         const int *__restrict__ ccode, GeoNode *__restrict__ nodes, int *err) {
     const double pi = PICONST, epsplus = 1.0e1 * TELESC_EPS;
     const int nr = g.nr, ny = g.ny;
-    const int ray = cray[k];
-    const double x0 = x0a[ray], z0 = z0a[ray];
     const double sinth02 = sinth0 * sinth0, costh02 = costh0 * costh0;
     const double s = cs[k];
     const int code = ccode[k];
@@ -168,7 +160,7 @@ k_geo(GridView g, CellView cv, long long nnodes, const long long *__restrict__ o
     if (mu > 1.0) set_err(err, 393);
     o.u = 3.335668e-11 * (mu * v[0] + sqrt(1.0 - mu * mu) *
                                          (v[1] * sin(sinphi) + v[2] * cos(sinphi)));
-    if (k == off[ray]) {
+    if (k == 0) {
         o.ds = 0.0;
     } else {
         o.ds = s - cs[k - 1];
@@ -176,6 +168,19 @@ k_geo(GridView g, CellView cv, long long nnodes, const long long *__restrict__ o
     }
     o.flag = icross | (flag << 2);
     nodes[k] = o;
+}
+
+__global__ void __launch_bounds__(256)
+k_geo(GridView g, CellView cv, int nmax, const long long *__restrict__ off,
+      const double *__restrict__ x0a, const double *__restrict__ z0a, double costh0,
+      double sinth0, const double *__restrict__ cs_all, const int *__restrict__ ccode_all,
+      GeoNode *__restrict__ nodes_all, int *err) {
+    const int ray = blockIdx.x;
+    const int k = blockIdx.y * blockDim.x + threadIdx.x;
+    const long long o = off[ray];
+    if (k >= (int)(off[ray + 1] - o)) return;
+    geo_node(g, cv, k, x0a[ray], z0a[ray], costh0, sinth0, cs_all + (long long)ray * nmax,
+             ccode_all + (long long)ray * nmax, nodes_all + o, err);
 }
 
 // ---------------------------------------------------------------------
@@ -504,23 +509,23 @@ __global__ void k_flux_final(int nfr, int nchunks, const double *__restrict__ pa
 // ---------------------------------------------------------------------
 void launch_cross(hipStream_t st, const GridDev &gd, const double *tan2, int nrays,
                   const double *x0, const double *z0, double theta0, double costh0,
-                  double sinth0, int central_blend, double *s1, double *s2, int *iy, int *count,
-                  const long long *off, double *cs, int *ccode, int *cray, int *err) {
+                  double sinth0, int central_blend, double *s1, double *s2, int *iy, int nmax,
+                  int *count, double *cs, int *ccode, int *err) {
     GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar};
     dim3 blk(128), grd((nrays + 127) / 128);
     hipLaunchKernelGGL(k_cross, grd, blk, 0, st, g, tan2, nrays, x0, z0, theta0, costh0,
-                       sinth0, central_blend, s1, s2, iy, count, off, cs, ccode, cray, err);
+                       sinth0, central_blend, s1, s2, iy, nmax, count, cs, ccode, err);
 }
 
-void launch_geo(hipStream_t st, const GridDev &gd, const double *vel, int nh, long long nnodes,
-                const long long *off, const double *x0, const double *z0, double costh0,
-                double sinth0, const double *cs, const int *ccode, const int *cray,
+void launch_geo(hipStream_t st, const GridDev &gd, const double *vel, int nh, int nrays,
+                int nmax, const long long *off, const double *x0, const double *z0,
+                double costh0, double sinth0, const double *cs, const int *ccode,
                 GeoNode *nodes, int *err) {
     GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar};
     CellView cv{nh, vel};
-    dim3 blk(256), grd((unsigned)((nnodes + 255) / 256));
-    hipLaunchKernelGGL(k_geo, grd, blk, 0, st, g, cv, nnodes, off, x0, z0, costh0, sinth0, cs,
-                       ccode, cray, nodes, err);
+    hipLaunchKernelGGL(k_geo, dim3(nrays, (nmax + 255) / 256), dim3(256), 0, st, g, cv, nmax, off,
+                       x0, z0, costh0,
+                       sinth0, cs, ccode, nodes, err);
 }
 
 void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int lev_up,
