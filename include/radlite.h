@@ -107,15 +107,19 @@ int rl_setup_rays_circular(rl_ctx *ctx, int nincl, const double *incl_rad,
 int rl_trace_lines_forward(rl_ctx *ctx, int nitems, const rl_item *items,
                            double *flux, double *image, unsigned *warn_bits);
 
-/* Adjoint of rl_trace_lines_forward for the same items: given
- * dL/dflux (same layout as flux), accumulate dL/d(cell table) for the
- * quantities the trace reads: g_nup / g_nlow [nlines][nr][nth] (upper
- * and lower level number densities), g_jd / g_ad [nlines][nr][nth],
- * g_lw [nr][nth].  Any output may be NULL.  Outputs are overwritten. */
+/* Adjoint of the trace for the same items (every item must use
+ * RL_SKIP_NONE).  Given dL/dflux (same layout as flux), runs the forward
+ * again (flux, optional, receives it) and returns dL/d(the inputs the
+ * trace reads), summed over the items: g_rho, g_abund, g_lw [nr][nth];
+ * g_pops [nr][nth][nlev]; g_jd, g_ad [nr][nth][nlines].  Any output may
+ * be NULL; outputs are overwritten.  The reference has no adjoint: this
+ * differentiates its formal solution (charintline telescope.F:3889,
+ * clever_integrate_element_linedust line.F:4636, qdr_src_2
+ * transfer.F:1498) including the sub-grid point positions. */
 int rl_trace_lines_adjoint(rl_ctx *ctx, int nitems, const rl_item *items,
-                           const double *dL_dflux, double *g_nup,
-                           double *g_nlow, double *g_jd, double *g_ad,
-                           double *g_lw);
+                           const double *dL_dflux, double *flux, double *g_rho,
+                           double *g_abund, double *g_lw, double *g_pops,
+                           double *g_jd, double *g_ad);
 
 /* Wait for all work of the context. */
 int rl_sync(rl_ctx *ctx);

@@ -93,6 +93,10 @@ struct rl_ctx {
     DevBuf d_cells, d_rec, d_rinfo, d_image, d_chan, d_partial, d_flux, d_err, d_ovs,
         d_maser, d_scr, d_cross;
     int cells_line = -1;
+    std::vector<double> h_chan, h_vel;
+    // adjoint buffers
+    DevBuf d_arec, d_ckpt, d_ckoff, d_gflux, d_gnode, d_gcell, d_grho, d_gabund, d_glw,
+        d_gpops, d_gjd, d_gad;
     // stats
     double t_ms[5] = {0, 0, 0, 0, 0};
     long long counts[4] = {0, 0, 0, 0};
@@ -229,13 +233,20 @@ extern "C" int rl_set_medium(rl_ctx *c, const double *rho, const double *abund,
     size_t nc = (size_t)c->nr * c->nh;
     if (int e = upload(c, c->d_rho, rho, nc * 8)) return e;
     if (int e = upload(c, c->d_abund, abund, nc * 8)) return e;
-    if (int e = upload(c, c->d_vel, vel, nc * 24)) return e;
+    const bool same_vel = c->h_vel.size() == nc * 3 &&
+                          std::memcmp(c->h_vel.data(), vel, nc * 24) == 0;
+    if (!same_vel) {
+        c->h_vel.assign(vel, vel + nc * 3);
+        if (int e = upload(c, c->d_vel, vel, nc * 24)) return e;
+    }
     if (int e = upload(c, c->d_lw, lw, nc * 8)) return e;
     c->umass_av = umass_av;
     c->aksmax = aksmax;
     c->have_medium = true;
-    for (auto &s : c->incl) s.traj_ready = false;    // u depends on the velocity
+    if (!same_vel)
+        for (auto &s : c->incl) s.traj_ready = false;    // u depends on the velocity
     c->cells_line = -1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));      // host arrays are only read during the call
     return 0;
 }
 
@@ -244,7 +255,9 @@ extern "C" int rl_set_levels(rl_ctx *c, int nlev, const double *pops) {
     HIPCHK(c, hipSetDevice(c->device));
     c->nlev = nlev;
     c->cells_line = -1;
-    return upload(c, c->d_pops, pops, (size_t)c->nr * c->nh * nlev * 8);
+    if (int e = upload(c, c->d_pops, pops, (size_t)c->nr * c->nh * nlev * 8)) return e;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 extern "C" int rl_set_lines(rl_ctx *c, int nl, const double *nu0, const double *A,
@@ -278,6 +291,7 @@ extern "C" int rl_set_dust_line(rl_ctx *c, const double *jd, const double *ad) {
     if (int e = upload(c, c->d_ad, ad, n)) return e;
     c->have_dust = true;
     c->cells_line = -1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -474,6 +488,73 @@ static double star_at(const rl_ctx *c, double freq) {      // line.F:3821-3843
     return (1.0 - wgt) * s[lo] + wgt * s[lo + 1];
 }
 
+// Per-item set-up shared by the forward and the adjoint: trajectories of
+// the item's inclination, the line's cell table, the channel tables
+// (line_setup_passband, line.F:462-469; star per channel, line.F:3821-3843)
+// and the line constants (line.F:1778-1787, 2301, 4578-4588).
+struct ItemSetup {
+    RaySet *rs = nullptr;
+    rl::LineConst lc;
+};
+
+static int setup_item(rl_ctx *c, const rl_item &it, int ii, ItemSetup &out, hipEvent_t ev0,
+                      hipEvent_t ev1) {
+    if (it.line < 0 || it.line >= c->nlines || it.incl < 0 ||
+        it.incl >= (int)c->incl.size() || it.nfr < 2 || !(it.vmax_kms > 0.0) ||
+        (it.skip_mode != RL_SKIP_REFERENCE && it.skip_mode != RL_SKIP_NONE))
+        return fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii);
+    RaySet &rs = c->incl[it.incl];
+    out.rs = &rs;
+    if (ev0) HIPCHK(c, hipEventRecord(ev0, c->stream));
+    if (int e = build_traj(c, rs)) return e;
+    if (ev1) HIPCHK(c, hipEventRecord(ev1, c->stream));
+    // cell table for this line
+    const double molpg = 1.0 / (c->umass_av * 1.6726e-24);
+    const int ncell = c->nr * c->nh;
+    HIPCHK(c, c->d_cells.ensure((size_t)ncell * sizeof(CellRec)));
+    if (c->cells_line != it.line) {
+        rl::launch_cells(c->stream, ncell, c->nlev, c->nlines, it.line,
+                         c->lev_up[it.line], c->lev_low[it.line], molpg,
+                         c->d_rho.as<double>(), c->d_abund.as<double>(),
+                         c->d_lw.as<double>(), c->d_pops.as<double>(),
+                         c->d_jd.as<double>(), c->d_ad.as<double>(),
+                         c->d_cells.as<CellRec>());
+        c->cells_line = it.line;
+    }
+    // channel tables: line_setup_passband (line.F:462-469)
+    const int nfr = it.nfr;
+    const double nu0s = c->nu0[it.line], nu0 = std::fabs(nu0s);
+    const double passb = 3.33567e-6 * nu0 * it.vmax_kms;
+    const double nu1 = 0.0 - passb, dnu = 2.0 * passb / (nfr - 1.0);
+    // kept in the context: the copy below is asynchronous
+    std::vector<double> &chan = c->h_chan;
+    chan.assign(3 * (size_t)nfr, 0.0);
+    for (int k = 0; k < nfr; k++) {
+        double dn = nu1 + k * dnu;
+        chan[k] = dn;
+        chan[nfr + k] = dn / nu0s;
+        chan[2 * nfr + k] = star_at(c, nu0 + dn);
+    }
+    if (int e = upload(c, c->d_chan, chan.data(), chan.size() * 8)) return e;
+    rl::LineConst &lc = out.lc;
+    lc.nu0 = nu0s;
+    lc.c_aa = 3.33567e-6 * nu0;
+    lc.hknu = 5.27296241956e-28 * nu0s;
+    lc.A = c->A[it.line];
+    lc.Bud = 6.78171833781e46 * lc.A / (nu0s * nu0s * nu0s);      // line.F:1778
+    lc.Bdu = lc.Bud * (c->gu[it.line] / c->gl[it.line]);
+    {
+        double q = c->rstar / rs.ring_ri[1];
+        lc.starfract = q * q;
+        if (rs.ring_ri[1] < c->rstar) return fail(c, 124, "central beam resolves the star");
+    }
+    lc.I0 = 0.0;
+    if (c->outer_bc == 2)
+        lc.I0 = 1.47455253991e-47 * (nu0s * nu0s * nu0s) /
+                (std::exp(4.7991598e-11 * nu0s / 2.728) - 1.0);
+    return 0;
+}
+
 extern "C" int rl_trace_lines_forward(rl_ctx *c, int nitems, const rl_item *items,
                                       double *flux, double *image, unsigned *warn) {
     if (!c || nitems < 0 || (nitems && (!items || !flux)))
@@ -490,56 +571,11 @@ extern "C" int rl_trace_lines_forward(rl_ctx *c, int nitems, const rl_item *item
     size_t fo = 0, io = 0;
     for (int ii = 0; ii < nitems; ii++) {
         const rl_item &it = items[ii];
-        if (it.line < 0 || it.line >= c->nlines || it.incl < 0 ||
-            it.incl >= (int)c->incl.size() || it.nfr < 2 || !(it.vmax_kms > 0.0) ||
-            (it.skip_mode != RL_SKIP_REFERENCE && it.skip_mode != RL_SKIP_NONE))
-            return fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii);
-        RaySet &rs = c->incl[it.incl];
-        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-        if (int e = build_traj(c, rs)) return e;
-        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-        // cell table for this line
-        const double molpg = 1.0 / (c->umass_av * 1.6726e-24);
-        const int ncell = c->nr * c->nh;
-        HIPCHK(c, c->d_cells.ensure((size_t)ncell * sizeof(CellRec)));
-        if (c->cells_line != it.line) {
-            rl::launch_cells(c->stream, ncell, c->nlev, c->nlines, it.line,
-                             c->lev_up[it.line], c->lev_low[it.line], molpg,
-                             c->d_rho.as<double>(), c->d_abund.as<double>(),
-                             c->d_lw.as<double>(), c->d_pops.as<double>(),
-                             c->d_jd.as<double>(), c->d_ad.as<double>(),
-                             c->d_cells.as<CellRec>());
-            c->cells_line = it.line;
-        }
-        // channel tables: line_setup_passband (line.F:462-469)
+        ItemSetup su;
+        if (int e = setup_item(c, it, ii, su, c->ev[0], c->ev[1])) return e;
+        RaySet &rs = *su.rs;
+        const rl::LineConst &lc = su.lc;
         const int nfr = it.nfr;
-        const double nu0s = c->nu0[it.line], nu0 = std::fabs(nu0s);
-        const double passb = 3.33567e-6 * nu0 * it.vmax_kms;
-        const double nu1 = 0.0 - passb, dnu = 2.0 * passb / (nfr - 1.0);
-        std::vector<double> chan(3 * (size_t)nfr);
-        for (int k = 0; k < nfr; k++) {
-            double dn = nu1 + k * dnu;
-            chan[k] = dn;
-            chan[nfr + k] = dn / nu0s;
-            chan[2 * nfr + k] = star_at(c, nu0 + dn);
-        }
-        if (int e = upload(c, c->d_chan, chan.data(), chan.size() * 8)) return e;
-        rl::LineConst lc;
-        lc.nu0 = nu0s;
-        lc.c_aa = 3.33567e-6 * nu0;
-        lc.hknu = 5.27296241956e-28 * nu0s;
-        lc.A = c->A[it.line];
-        lc.Bud = 6.78171833781e46 * lc.A / (nu0s * nu0s * nu0s);      // line.F:1778
-        lc.Bdu = lc.Bud * (c->gu[it.line] / c->gl[it.line]);
-        {
-            double q = c->rstar / rs.ring_ri[1];
-            lc.starfract = q * q;
-            if (rs.ring_ri[1] < c->rstar) return fail(c, 124, "central beam resolves the star");
-        }
-        lc.I0 = 0.0;
-        if (c->outer_bc == 2)
-            lc.I0 = 1.47455253991e-47 * (nu0s * nu0s * nu0s) /
-                    (std::exp(4.7991598e-11 * nu0s / 2.728) - 1.0);
         const int nrays = rs.nrays;
         const int rpc = 1, nchunks = rs.nrings;
         HIPCHK(c, c->d_rec.ensure((size_t)rs.nnodes * rsz));
@@ -607,9 +643,127 @@ extern "C" int rl_trace_lines_forward(rl_ctx *c, int nitems, const rl_item *item
     return 0;
 }
 
-extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int, const rl_item *, const double *,
-                                      double *, double *, double *, double *, double *) {
-    return fail(c, 13, "adjoint not available in this build");
+static int zero_dev(rl_ctx *c, DevBuf &b, size_t bytes) {
+    HIPCHK(c, b.ensure(bytes));
+    HIPCHK(c, hipMemsetAsync(b.p, 0, bytes, c->stream));
+    return 0;
+}
+
+extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *items,
+                                      const double *dL_dflux, double *flux, double *g_rho,
+                                      double *g_abund, double *g_lw, double *g_pops,
+                                      double *g_jd, double *g_ad) {
+    if (!c || nitems < 0 || (nitems && (!items || !dL_dflux)))
+        return fail(c, RL_E_ARG, "rl_trace_lines_adjoint: bad args");
+    if (!c->have_medium || !c->nlev || !c->nlines || !c->have_dust || c->incl.empty())
+        return fail(c, 13, "inputs incomplete: grid, medium, levels, lines, dust, rays");
+    for (int ii = 0; ii < nitems; ii++)
+        if (items[ii].skip_mode != RL_SKIP_NONE)
+            return fail(c, 13, "item %d: the adjoint integrates every channel (RL_SKIP_NONE)", ii);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->d_err.ensure(4));
+    HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));
+    const int fp64 = c->precision == RL_FP64;
+    const size_t tsz = fp64 ? 8 : 4;
+    const int ncell = c->nr * c->nh;
+    if (int e = zero_dev(c, c->d_grho, (size_t)ncell * 8)) return e;
+    if (int e = zero_dev(c, c->d_gabund, (size_t)ncell * 8)) return e;
+    if (int e = zero_dev(c, c->d_glw, (size_t)ncell * 8)) return e;
+    if (int e = zero_dev(c, c->d_gpops, (size_t)ncell * c->nlev * 8)) return e;
+    if (int e = zero_dev(c, c->d_gjd, (size_t)ncell * c->nlines * 8)) return e;
+    if (int e = zero_dev(c, c->d_gad, (size_t)ncell * c->nlines * 8)) return e;
+    for (int i = 0; i < 5; i++) c->t_ms[i] = 0.0;
+    for (int i = 0; i < 4; i++) c->counts[i] = 0;
+    const int K = rl::adjoint_block_len();
+    size_t fo = 0;
+    for (int ii = 0; ii < nitems; ii++) {
+        const rl_item &it = items[ii];
+        ItemSetup su;
+        if (int e = setup_item(c, it, ii, su, c->ev[0], c->ev[1])) return e;
+        RaySet &rs = *su.rs;
+        const int nfr = it.nfr, nrays = rs.nrays;
+        // checkpoint CSR: ceil((n-1)/K) blocks per ray
+        std::vector<long long> ckoff(nrays + 1, 0);
+        for (int r = 0; r < nrays; r++) {
+            const long long n = rs.off[r + 1] - rs.off[r];
+            ckoff[r + 1] = ckoff[r] + (n >= 2 ? (n - 1 + K - 1) / K : 0);
+        }
+        if (int e = upload(c, c->d_ckoff, ckoff.data(), ckoff.size() * 8)) return e;
+        HIPCHK(c, c->d_ckpt.ensure((size_t)ckoff[nrays] * nfr * tsz));
+        HIPCHK(c, c->d_arec.ensure((size_t)rs.nnodes * rl::fast_rec_size(fp64)));
+        HIPCHK(c, c->d_rinfo.ensure((size_t)nrays * sizeof(RayInfo)));
+        HIPCHK(c, c->d_image.ensure((size_t)nrays * nfr * 8));
+        HIPCHK(c, c->d_partial.ensure((size_t)rs.nrings * nfr * 8));
+        HIPCHK(c, c->d_flux.ensure((size_t)nfr * 8));
+        if (int e = upload(c, c->d_gflux, dL_dflux + fo, (size_t)nfr * 8)) return e;
+        if (int e = zero_dev(c, c->d_gnode, (size_t)rs.nnodes * 5 * 8)) return e;
+        if (int e = zero_dev(c, c->d_gcell, (size_t)ncell * 5 * 8)) return e;
+        rl::AdjLaunch a;
+        a.nrays = nrays; a.nfr = nfr; a.nrings = rs.nrings; a.nphi = rs.nphi;
+        a.nnodes = rs.nnodes;
+        a.ncell = ncell; a.nlev = c->nlev; a.nlines = c->nlines; a.line = it.line;
+        a.lev_up = c->lev_up[it.line]; a.lev_low = c->lev_low[it.line];
+        a.molpg = 1.0 / (c->umass_av * 1.6726e-24);
+        a.off = rs.d_off.as<long long>();
+        a.ckoff = c->d_ckoff.as<long long>();
+        a.geo = rs.d_nodes.as<GeoNode>();
+        a.cells = c->d_cells.as<CellRec>();
+        a.rec = c->d_arec.p;
+        a.rinfo = c->d_rinfo.as<RayInfo>();
+        a.ckpt = c->d_ckpt.p;
+        a.dnu = c->d_chan.as<double>();
+        a.velo = a.dnu + nfr;
+        a.istar = a.dnu + 2 * nfr;
+        a.lc = su.lc;
+        a.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
+        a.image = c->d_image.as<double>();
+        a.ring_w = rs.d_ringw.as<double>();
+        a.center_w = rs.center_w;
+        a.nchunks = rs.nrings;
+        a.partial = c->d_partial.as<double>();
+        a.flux = c->d_flux.as<double>();
+        a.d2 = (1.0 * 3.08572e18) * (1.0 * 3.08572e18);
+        a.gflux = c->d_gflux.as<double>();
+        a.gnode = c->d_gnode.as<double>();
+        a.gcell = c->d_gcell.as<double>();
+        a.rho = c->d_rho.as<double>();
+        a.abund = c->d_abund.as<double>();
+        a.pops = c->d_pops.as<double>();
+        a.g_rho = c->d_grho.as<double>();
+        a.g_abund = c->d_gabund.as<double>();
+        a.g_lw = c->d_glw.as<double>();
+        a.g_pops = c->d_gpops.as<double>();
+        a.g_jd = c->d_gjd.as<double>();
+        a.g_ad = c->d_gad.as<double>();
+        if (fp64) rl::launch_adjoint<double>(c->stream, a);
+        else rl::launch_adjoint<float>(c->stream, a);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+        if (flux)
+            HIPCHK(c, hipMemcpyAsync(flux + fo, c->d_flux.p, (size_t)nfr * 8,
+                                     hipMemcpyDeviceToHost, c->stream));
+        if (int e = check_err(c)) return e;
+        float t = 0;
+        (void)hipEventElapsedTime(&t, c->ev[1], c->ev[4]);
+        c->t_ms[2] += t;
+        c->t_ms[4] += t;
+        c->counts[0] += nrays;
+        c->counts[1] += rs.nnodes;
+        c->counts[3] += (long long)nrays * nfr;
+        fo += nfr;
+    }
+    auto down = [&](double *dst, DevBuf &b, size_t n) -> int {
+        if (dst) HIPCHK(c, hipMemcpyAsync(dst, b.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+        return 0;
+    };
+    if (int e = down(g_rho, c->d_grho, ncell)) return e;
+    if (int e = down(g_abund, c->d_gabund, ncell)) return e;
+    if (int e = down(g_lw, c->d_glw, ncell)) return e;
+    if (int e = down(g_pops, c->d_gpops, (size_t)ncell * c->nlev)) return e;
+    if (int e = down(g_jd, c->d_gjd, (size_t)ncell * c->nlines)) return e;
+    if (int e = down(g_ad, c->d_gad, (size_t)ncell * c->nlines)) return e;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 // ---------------------------------------------------------------------

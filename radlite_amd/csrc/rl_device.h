@@ -196,93 +196,125 @@ __device__ inline void clever_elem(const LineK<T> &L, T &I, T ds,
 }
 
 // ---------------------------------------------------------------------
-// FP32 performance formulation.  Everything that does not depend on the
-// channel is precomputed per node / per segment (FastRec), and the
-// quadrature weights of qdr_src_2 are evaluated without the cancellation
-// in e1 = dtau - (1 - exp(-dtau)), which loses all single-precision
-// digits for small dtau.
+// Scaled-record formulation (the FP32 performance mode, and the
+// arithmetic of the adjoint in both precisions).  Everything that does
+// not depend on the channel is precomputed per node / per segment
+// (FastRecT), and the quadrature weights of qdr_src_2 are evaluated
+// without the cancellation in e1 = dtau - (1 - exp(-dtau)), which loses
+// all single-precision digits for small dtau.
 // ---------------------------------------------------------------------
-struct alignas(16) FastRec {
+template <typename T>
+struct alignas(16 * sizeof(T) >= 64 ? 64 : 16) FastRecT {
     // node k: emissivities and opacities are stored divided by a per-node
     // scale sig (the dust opacity where there is dust), so that j/alpha is
     // formed from O(1) numbers even where j and alpha themselves are below
     // the single-precision range.  j = sig*(jdh + jch*phi), alpha =
     // sig*(adh + dch*phi) with phi the line profile.
-    float ds;      // s_k - s_{k-1}
-    float u;       // Omega.v/c
-    float sig;     // scale (may underflow to 0 in float; never used as divisor)
-    float jdh, adh;// dust j, alpha / sig
-    float jch;     // 5.27296241956e-28 nu0 N_up A / sig         (line.F:4578)
-    float dch;     // 5.27296241956e-28 nu0 (N_low B_lu - N_up B_ul) / sig
-    float flag;    // NF_* bits
+    T ds;      // s_k - s_{k-1}
+    T u;       // Omega.v/c
+    T sig;     // scale (may underflow to 0 in float; never used as divisor)
+    T jdh, adh;// dust j, alpha / sig
+    T jch;     // 5.27296241956e-28 nu0 N_up A / sig         (line.F:4578)
+    T dch;     // 5.27296241956e-28 nu0 (N_low B_lu - N_up B_ul) / sig
+    T flag;    // NF_* bits
     // segment (k-1 -> k)
-    float inv_aa;  // 1 / (3.33567e-6 |nu0| lwav)                 (line.F:2301)
-    float norm;    // 0.56419583546 * inv_aa
-    float dls;     // sub-grid line length ds/x, < 0 when no sub-grid
-    float inv_du;  // 1 / (u_k - u_{k-1})
-    float rel0, rel1;  // sig_{k-1}/sig_max, sig_k/sig_max (sub-grid weights)
-    float sigm;    // sig_max = max(sig_{k-1}, sig_k)
-    float pad;
+    T inv_aa;  // 1 / (3.33567e-6 |nu0| lwav)                 (line.F:2301)
+    T norm;    // 0.56419583546 * inv_aa
+    T dls;     // sub-grid line length ds/x, < 0 when no sub-grid
+    T inv_du;  // 1 / (u_k - u_{k-1})
+    T rel0, rel1;  // sig_{k-1}/sig_max, sig_k/sig_max (sub-grid weights)
+    T sigm;    // sig_max = max(sig_{k-1}, sig_k)
+    T pad;
+};
+using FastRec = FastRecT<float>;
+
+template <typename T> struct FOps;
+template <> struct FOps<float> {
+    static __device__ inline float ex(float x) { return __expf(x); }
+    static __device__ inline float fdiv(float a, float b) { return __fdividef(a, b); }
+    static __device__ inline float fmn(float a, float b) { return fminf(a, b); }
+    static __device__ inline float fmx(float a, float b) { return fmaxf(a, b); }
+};
+template <> struct FOps<double> {
+    static __device__ inline double ex(double x) { return exp(x); }
+    static __device__ inline double fdiv(double a, double b) { return a / b; }
+    static __device__ inline double fmn(double a, double b) { return fmin(a, b); }
+    static __device__ inline double fmx(double a, double b) { return fmax(a, b); }
 };
 
-// qdr_src_2 weights: I_out = I*xp + a*S1 + b*S2
-__device__ inline void qdr_weights_f(float x, float &a, float &b, float &xp) {
-    if (x > 1.0e-6f) {
-        if (x < 0.5f) {
+// qdr_src_2 weights: I_out = I*xp + a*S1 + b*S2.  ra = a/x (needed by the
+// adjoint: db/dx = a/x, da/dx = xp - a/x).
+template <typename T>
+__device__ inline void qdr_weights_f(T x, T &a, T &b, T &xp, T &ra) {
+    if (x > T(1.0e-6)) {
+        if (x < T(0.5)) {
             // b = e1/x = sum_{n>=1} (-1)^{n+1} x^n/(n+1)!,  e0 = x (1 - b)
-            b = x * (0.5f - x * (1.0f / 6 - x * (1.0f / 24 - x * (1.0f / 120 -
-                x * (1.0f / 720 - x * (1.0f / 5040 - x * (1.0f / 40320)))))));
-            float e0 = x * (1.0f - b);
-            xp = 1.0f - e0;
+            T p = T(0.5) - x * (T(1) / 6 - x * (T(1) / 24 - x * (T(1) / 120 -
+                  x * (T(1) / 720 - x * (T(1) / 5040 - x * (T(1) / 40320))))));
+            if (sizeof(T) == 8)   // double: series continued to x^13/15!
+                p = T(0.5) - x * (T(1) / 6 - x * (T(1) / 24 - x * (T(1) / 120 -
+                    x * (T(1) / 720 - x * (T(1) / 5040 - x * (T(1) / 40320 -
+                    x * (T(1) / 362880 - x * (T(1) / 3628800 - x * (T(1) / 39916800 -
+                    x * (T(1) / 479001600 - x * (T(1) / 6227020800.0 -
+                    x * (T(1) / 87178291200.0 - x * (T(1) / 1307674368000.0)))))))))))));
+            b = x * p;
+            T e0 = x * (T(1) - b);
+            xp = T(1) - e0;
             a = e0 - b;
+            ra = T(1) - b - p;
         } else {
-            xp = __expf(-x);
-            float e0 = 1.0f - xp;
-            b = __fdividef(x - e0, x);
+            xp = FOps<T>::ex(-x);
+            T e0 = T(1) - xp;
+            b = FOps<T>::fdiv(x - e0, x);
             a = e0 - b;
+            ra = FOps<T>::fdiv(a, x);
         }
     } else {
-        a = 0.5f * x;
-        b = 0.5f * x;
-        xp = 1.0f - x;
+        a = T(0.5) * x;
+        b = T(0.5) * x;
+        xp = T(1) - x;
+        ra = T(0.5);
     }
 }
 
-struct FastLine { float nu0, dnu, uch; };
+template <typename T> struct FastLineT { T nu0, dnu, uch; };
+using FastLine = FastLineT<float>;
 
-struct FPt { float u, sig, jd, ad, jc, dc; };     // one point, scaled values
+template <typename T> struct FPtT { T u, sig, jd, ad, jc, dc; };   // one point, scaled values
+using FPt = FPtT<float>;
 
-struct FastCarry { float jl0, al0, S0; int init; };  // line j, alpha / sig at the start
+template <typename T> struct FastCarryT { T jl0, al0, S0; int init; };  // line j, alpha / sig at the start
 
 // integrate_element_linedust (line.F:4515-4624) on scaled values
-__device__ inline void elem_fast(const FastLine &L, float &I, float ds, const FPt &p0,
-                                 const FPt &p1, float inv_aa, float norm, FastCarry &c,
+template <typename T>
+__device__ inline void elem_fast(const FastLineT<T> &L, T &I, T ds, const FPtT<T> &p0,
+                                 const FPtT<T> &p1, T inv_aa, T norm, FastCarryT<T> &c,
                                  int &ovs) {
     if (c.init) {
-        float z0 = (L.dnu - L.nu0 * p0.u) * inv_aa;
-        float phi0 = norm * __expf(-(z0 * z0));
+        T z0 = (L.dnu - L.nu0 * p0.u) * inv_aa;
+        T phi0 = norm * FOps<T>::ex(-(z0 * z0));
         c.jl0 = p0.jc * phi0;
         c.al0 = p0.dc * phi0;
     }
-    float z1 = (L.dnu - L.nu0 * p1.u) * inv_aa;
-    float phi1 = norm * __expf(-(z1 * z1));
-    float jl1 = p1.jc * phi1, al1 = p1.dc * phi1;
-    float jh0 = p0.jd + c.jl0, jh1 = p1.jd + jl1;      // j / sig
-    float ah0 = p0.ad + c.al0, ah1 = p1.ad + al1;      // alpha / sig
-    float a0 = p0.sig * ah0, a1 = p1.sig * ah1;
-    float dtau = 0.5f * (a0 + a1) * ds;
-    float theomax = 0.5f * (p0.sig * jh0 + p1.sig * jh1) * ds;
-    float wa, wb, xp;
-    qdr_weights_f(dtau, wa, wb, xp);
+    T z1 = (L.dnu - L.nu0 * p1.u) * inv_aa;
+    T phi1 = norm * FOps<T>::ex(-(z1 * z1));
+    T jl1 = p1.jc * phi1, al1 = p1.dc * phi1;
+    T jh0 = p0.jd + c.jl0, jh1 = p1.jd + jl1;      // j / sig
+    T ah0 = p0.ad + c.al0, ah1 = p1.ad + al1;      // alpha / sig
+    T a0 = p0.sig * ah0, a1 = p1.sig * ah1;
+    T dtau = T(0.5) * (a0 + a1) * ds;
+    T theomax = T(0.5) * (p0.sig * jh0 + p1.sig * jh1) * ds;
+    T wa, wb, xp, ra;
+    qdr_weights_f<T>(dtau, wa, wb, xp, ra);
     // source functions (transfer.F:1528-1541) from the scaled values; the
     // start point equals the previous end point unless init is set
-    float q1 = ah1 > 0.f ? jh1 / ah1 : 0.f;
-    float q0 = c.init ? (ah0 > 0.f ? jh0 / ah0 : 0.f) : c.S0;
-    float s1 = ah0 > 0.f ? q0 : (ah1 > 0.f ? q1 : 0.f);
-    float s2 = ah1 > 0.f ? q1 : (ah0 > 0.f ? q0 : 0.f);
-    float q = dtau > 1e-9f ? wa * s1 + wb * s2 : theomax;
-    ovs += q > 2.f * theomax;
-    q = fminf(q, theomax);
+    T q1 = ah1 > T(0) ? jh1 / ah1 : T(0);
+    T q0 = c.init ? (ah0 > T(0) ? jh0 / ah0 : T(0)) : c.S0;
+    T s1 = ah0 > T(0) ? q0 : (ah1 > T(0) ? q1 : T(0));
+    T s2 = ah1 > T(0) ? q1 : (ah0 > T(0) ? q0 : T(0));
+    T q = dtau > T(1e-9f) ? wa * s1 + wb * s2 : theomax;
+    ovs += q > T(2) * theomax;
+    q = FOps<T>::fmn(q, theomax);
     I = I * xp + q;
     c.jl0 = jl1;
     c.al0 = al1;
@@ -290,56 +322,87 @@ __device__ inline void elem_fast(const FastLine &L, float &I, float ds, const FP
     c.init = 0;
 }
 
-// charintline (telescope.F:3889-4312) on FastRec data
-__device__ inline float trace_fast(const FastRec *__restrict__ rec, int n, const FastLine &L,
-                                   float I0, float istar, float starfract, int &ovs,
-                                   int &maser) {
-    float I = I0;
-    FastCarry c{0.f, 0.f, 0.f, 1};
-    const FastRec a = rec[0];
-    FPt p0{a.u, a.sig, a.jdh, a.adh, a.jch, a.dch};
-    for (int k = 1; k < n; k++) {
-        const FastRec b = rec[k];
-        const FPt p1{b.u, b.sig, b.jdh, b.adh, b.jch, b.dch};
-        float ds = b.ds;
-        const int fl = (int)b.flag;
-        if (fl & NF_RESET) { ds = 0.f; c.init = 1; }
-        if (fl & NF_BLEND) { I = (1.f - starfract) * I + starfract * istar; c.init = 1; }
-        bool done = false;
-        if (b.dls >= 0.f) {                  // 6x > 1 (line.F:4715)
-            float s_c = ds * (L.uch - p0.u) * b.inv_du;
-            float sright = s_c + 3.f * b.dls, sleft = s_c - 3.f * b.dls;
-            if (sright > 0.f && sleft < ds) {
-                float lg = (sright - sleft) * (1.f / 30.f);
-                float f0 = -sleft / lg, f1 = (ds - sleft) / lg;
-                int ilo = f0 > 0.f ? max(1, (int)fminf(f0, 40.f)) : 1;
-                int ihi = f1 < 40.f ? min(31, (int)fmaxf(f1, -2.f) + 2) : 31;
-                FPt pp = p0;
-                float sa = 0.f;
-                const float inv_ds = 1.f / ds;
-                for (int i = ilo; i <= ihi; i++) {
-                    float s = sleft + (float)(i - 1) * lg;
-                    if (s > 0.f && s < ds) {
-                        // linear interpolation of the unscaled quantities
-                        // (line.F:4775-4790), done with relative weights
-                        float e = s * inv_ds, ep = 1.f - e;
-                        float w0 = ep * b.rel0, w1 = e * b.rel1, W = w0 + w1;
-                        float iw = 1.f / W;
-                        w0 *= iw; w1 *= iw;
-                        FPt q{ep * p0.u + e * p1.u, b.sigm * W, w0 * p0.jd + w1 * p1.jd,
-                              w0 * p0.ad + w1 * p1.ad, w0 * p0.jc + w1 * p1.jc,
-                              w0 * p0.dc + w1 * p1.dc};
-                        elem_fast(L, I, s - sa, pp, q, b.inv_aa, b.norm, c, ovs);
-                        pp = q;
-                        sa = s;
-                    }
-                }
-                elem_fast(L, I, ds - sa, pp, p1, b.inv_aa, b.norm, c, ovs);
-                done = true;
+// Sub-grid points of segment k (clever_integrate_element_linedust,
+// line.F:4706-4800): s_i = sleft + (i-1) lg for i in [ilo, ihi], used when
+// 0 < s_i < ds.  Returns false when the segment is integrated whole.
+template <typename T>
+struct SubGrid { T sleft, lg; int ilo, ihi; };
+
+template <typename T>
+__device__ inline bool subgrid_of(const FastRecT<T> &b, T ds, T u0, T uch, SubGrid<T> &sg) {
+    if (!(b.dls >= T(0))) return false;            // 6x > 1 (line.F:4715)
+    T s_c = ds * (uch - u0) * b.inv_du;
+    T sright = s_c + T(3) * b.dls, sleft = s_c - T(3) * b.dls;
+    if (!(sright > T(0) && sleft < ds)) return false;
+    T lg = (sright - sleft) * (T(1) / T(30));
+    T f0 = -sleft / lg, f1 = (ds - sleft) / lg;
+    sg.sleft = sleft;
+    sg.lg = lg;
+    sg.ilo = f0 > T(0) ? max(1, (int)FOps<T>::fmn(f0, T(40))) : 1;
+    sg.ihi = f1 < T(40) ? min(31, (int)FOps<T>::fmx(f1, T(-2)) + 2) : 31;
+    return true;
+}
+
+// a sub-point at fraction e of the segment: linear interpolation of the
+// unscaled quantities (line.F:4775-4790), done with relative weights
+template <typename T>
+__device__ inline FPtT<T> subpoint(const FastRecT<T> &b, const FPtT<T> &p0, const FPtT<T> &p1,
+                                   T e) {
+    T ep = T(1) - e;
+    T w0 = ep * b.rel0, w1 = e * b.rel1, W = w0 + w1;
+    T iw = T(1) / W;
+    w0 *= iw; w1 *= iw;
+    return FPtT<T>{ep * p0.u + e * p1.u, b.sigm * W, w0 * p0.jd + w1 * p1.jd,
+                   w0 * p0.ad + w1 * p1.ad, w0 * p0.jc + w1 * p1.jc, w0 * p0.dc + w1 * p1.dc};
+}
+
+template <typename T>
+__device__ inline FPtT<T> point_of(const FastRecT<T> &a) {
+    return FPtT<T>{a.u, a.sig, a.jdh, a.adh, a.jch, a.dch};
+}
+
+// one segment (nodes k-1 -> k) of charintline on FastRecT data
+template <typename T>
+__device__ inline void segment_fast(const FastRecT<T> &b, const FPtT<T> &p0, const FPtT<T> &p1,
+                                    const FastLineT<T> &L, T &I, T istar, T starfract,
+                                    FastCarryT<T> &c, int &ovs) {
+    T ds = b.ds;
+    const int fl = (int)b.flag;
+    if (fl & NF_RESET) { ds = T(0); c.init = 1; }
+    if (fl & NF_BLEND) { I = (T(1) - starfract) * I + starfract * istar; c.init = 1; }
+    SubGrid<T> sg;
+    if (subgrid_of<T>(b, ds, p0.u, L.uch, sg)) {
+        FPtT<T> pp = p0;
+        T sa = T(0);
+        const T inv_ds = T(1) / ds;
+        for (int i = sg.ilo; i <= sg.ihi; i++) {
+            T s = sg.sleft + (T)(i - 1) * sg.lg;
+            if (s > T(0) && s < ds) {
+                FPtT<T> q = subpoint<T>(b, p0, p1, s * inv_ds);
+                elem_fast<T>(L, I, s - sa, pp, q, b.inv_aa, b.norm, c, ovs);
+                pp = q;
+                sa = s;
             }
         }
-        if (!done) elem_fast(L, I, ds, p0, p1, b.inv_aa, b.norm, c, ovs);
-        if (b.sig * c.al0 * ds < -0.01f) maser = 1;
+        elem_fast<T>(L, I, ds - sa, pp, p1, b.inv_aa, b.norm, c, ovs);
+    } else {
+        elem_fast<T>(L, I, ds, p0, p1, b.inv_aa, b.norm, c, ovs);
+    }
+}
+
+// charintline (telescope.F:3889-4312) on FastRecT data
+template <typename T>
+__device__ inline T trace_fast(const FastRecT<T> *__restrict__ rec, int n, const FastLineT<T> &L,
+                               T I0, T istar, T starfract, int &ovs, int &maser) {
+    T I = I0;
+    FastCarryT<T> c{T(0), T(0), T(0), 1};
+    FPtT<T> p0 = point_of<T>(rec[0]);
+    for (int k = 1; k < n; k++) {
+        const FastRecT<T> b = rec[k];
+        const FPtT<T> p1 = point_of<T>(b);
+        segment_fast<T>(b, p0, p1, L, I, istar, starfract, c, ovs);
+        T ds = ((int)b.flag & NF_RESET) ? T(0) : b.ds;
+        if (b.sig * c.al0 * ds < T(-0.01f)) maser = 1;
         p0 = p1;
     }
     return I;

@@ -57,6 +57,34 @@ struct ItemLaunch {
     hipEvent_t ev_nodes, ev_formal;
 };
 
+// One item of the adjoint (scaled records of precision T, all channels).
+struct AdjLaunch {
+    int nrays, nfr, nrings, nphi;
+    long long nnodes;
+    int ncell, nlev, nlines, line, lev_up, lev_low;
+    double molpg;
+    const long long *off, *ckoff;      // node CSR; checkpoint CSR (blocks per ray)
+    const GeoNode *geo;
+    const CellRec *cells;
+    void *rec;                         // FastRecT<T>[nnodes]
+    RayInfo *rinfo;
+    void *ckpt;                        // T[sum blocks][nfr]
+    const double *dnu, *velo, *istar;  // per channel
+    LineConst lc;
+    double aks2;
+    double *image;                     // [nrays][nfr] forward intensities
+    const double *ring_w;
+    double center_w;
+    int nchunks;
+    double *partial, *flux;
+    double d2;
+    const double *gflux;               // device [nfr] dL/dflux
+    double *gnode;                     // [nnodes][5]
+    double *gcell;                     // [ncell][5] (jd, ad, lw, N_up, N_low)
+    const double *rho, *abund, *pops;
+    double *g_rho, *g_abund, *g_lw, *g_pops, *g_jd, *g_ad;   // accumulated over items
+};
+
 void launch_cross(hipStream_t st, const GridDev &g, const double *tan2, int nrays,
                   const double *x0, const double *z0, double theta0, double costh0,
                   double sinth0, int central_blend, double *s1, double *s2, int *iy, int nmax,
@@ -70,5 +98,8 @@ void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int
                   const double *lw, const double *pops, const double *jd, const double *ad,
                   CellRec *out);
 template <typename T> void launch_item(hipStream_t st, const ItemLaunch &it);
+template <typename T> void launch_adjoint(hipStream_t st, const AdjLaunch &a);
+int adjoint_block_len();
+size_t fast_rec_size(int precision_fp64);
 
 }  // namespace rl

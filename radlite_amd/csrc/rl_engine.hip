@@ -1,17 +1,23 @@
 // Kernels of the raytracer and their launchers.
 //
-//   k_traj        one thread per ray: trajectory (FP64), node geometry
-//                 (segment length, projected velocity, interpolation
-//                 stencil), CSR by ray.  Run twice: count, then write.
+//   k_cross       one thread per ray: the crossings of the trajectory
+//                 (make_trajectory_c, telescope.F:2787-3677) in FP64,
+//                 written into per-ray slots
+//   k_geo         one thread per node: position, stencil, local direction
+//                 and projected velocity (telescope.F:3682-3718,
+//                 line.F:2650-2662), compacted into the CSR node array
 //   k_cells       per line: cell table {jd, ad, lw, N_up, N_low}
-//   k_nodes<T>    one wave per ray: interpolated node records for one
-//                 line + the NONREDUNDANT window and continuum channel
+//   k_nodes       one wave per ray: interpolated node records (FP64) +
+//                 the NONREDUNDANT window and continuum channel
+//   k_nodes_fast  the same as scaled records (FP32 mode, adjoint)
 //   k_formal<T>   one lane per (ray, channel): formal solution along the
 //                 ray (charintline, telescope.F:3889-4312)
 //   k_fixup<T>    one thread per ray: the reference's lazy continuum
 //                 copies for skipped channels (telescope.F:583-611)
-//   k_flux        ring-weighted sums of the image into the spectrum
+//   k_flux_*      ring-weighted sums of the image into the spectrum
 //                 (calc_freq_flux_observer, telescope.F:1386-1435)
+//   k_adj_*       adjoint: checkpointed forward, reverse sweep, node and
+//                 cell gradient scatter
 #include "rl_engine.h"
 
 #include <hip/hip_runtime.h>
@@ -19,6 +25,7 @@
 #include <type_traits>
 
 #include "rl_traj.h"
+#include "rl_adjoint.h"
 
 namespace rl {
 
@@ -287,10 +294,12 @@ __device__ inline void scaled_node(const double v[5], const LineConst &lc, doubl
     if (!(sig > 0.0)) sig = 1.0;
 }
 
-// FP32 performance path: channel-independent products precomputed
+// scaled records (FP32 performance path; both precisions of the adjoint):
+// channel-independent products precomputed
+template <typename T>
 __global__ void __launch_bounds__(64)
 k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__restrict__ geo,
-             const CellRec *__restrict__ cells, FastRec *__restrict__ rec,
+             const CellRec *__restrict__ cells, FastRecT<T> *__restrict__ rec,
              RayInfo *__restrict__ rinfo, int nfr, const double *__restrict__ velo,
              double aks2, int skip_mode, LineConst lc) {
     const int ray = blockIdx.x;
@@ -303,18 +312,18 @@ k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__rest
         const GeoNode gn = geo[b + k];
         double v[5];
         node_interp(gn, cells, v);
-        FastRec r;
+        FastRecT<T> r;
         double sig, jc, dc;
         scaled_node(v, lc, sig, jc, dc);
-        r.ds = (float)gn.ds;
-        r.u = (float)gn.u;
-        r.sig = (float)sig;
-        r.jdh = (float)(v[CR_JD] / sig);
-        r.adh = (float)(v[CR_AD] / sig);
-        r.jch = (float)(jc / sig);
-        r.dch = (float)(dc / sig);
-        r.flag = (float)(gn.flag >> 2);
-        r.pad = 0.f;
+        r.ds = (T)gn.ds;
+        r.u = (T)gn.u;
+        r.sig = (T)sig;
+        r.jdh = (T)(v[CR_JD] / sig);
+        r.adh = (T)(v[CR_AD] / sig);
+        r.jch = (T)(jc / sig);
+        r.dch = (T)(dc / sig);
+        r.flag = (T)(gn.flag >> 2);
+        r.pad = T(0);
         if (k > 0) {
             const GeoNode gp = geo[b + k - 1];
             double vp[5], sp, jcp, dcp;
@@ -322,19 +331,19 @@ k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__rest
             scaled_node(vp, lc, sp, jcp, dcp);
             double lwav = 0.5 * (vp[CR_LW] + v[CR_LW]);
             double aa = lc.c_aa * lwav;
-            r.inv_aa = (float)(1.0 / aa);
-            r.norm = (float)(0.56419583546 / aa);
+            r.inv_aa = (T)(1.0 / aa);
+            r.norm = (T)(0.56419583546 / aa);
             double du = gn.u - gp.u;
             double x = fabs(du / (lwav / 2.99792458e5));
-            r.dls = 2.0 * 3.0 * x > 1.0 ? (float)(gn.ds / x) : -1.0f;
-            r.inv_du = (float)(1.0 / du);
+            r.dls = 2.0 * 3.0 * x > 1.0 ? (T)(gn.ds / x) : T(-1);
+            r.inv_du = (T)(1.0 / du);
             double sm = fmax(sp, sig);
-            r.rel0 = (float)(sp / sm);
-            r.rel1 = (float)(sig / sm);
-            r.sigm = (float)sm;
+            r.rel0 = (T)(sp / sm);
+            r.rel1 = (T)(sig / sm);
+            r.sigm = (T)sm;
         } else {
-            r.inv_aa = 0.f; r.norm = 0.f; r.dls = -1.f; r.inv_du = 0.f;
-            r.rel0 = r.rel1 = 1.f; r.sigm = r.sig;
+            r.inv_aa = T(0); r.norm = T(0); r.dls = -T(1); r.inv_du = T(0);
+            r.rel0 = r.rel1 = T(1); r.sigm = r.sig;
         }
         rec[b + k] = r;
         if (k + 1 < n && v[CR_NU] + v[CR_NL] > LEVTHRES) {
@@ -400,7 +409,7 @@ __device__ inline double trace_dispatch<float>(const FastRec *rec, long long b, 
                                                const LineConst &lc, double dnu, double velo,
                                                double istar, float sf, int &ovs, int &maser) {
     FastLine L{(float)lc.nu0, (float)dnu, (float)velo};
-    return (double)trace_fast(rec + b, n, L, (float)lc.I0, (float)istar, sf, ovs, maser);
+    return (double)trace_fast<float>(rec + b, n, L, (float)lc.I0, (float)istar, sf, ovs, maser);
 }
 
 template <typename T>
@@ -545,7 +554,7 @@ void launch_item(hipStream_t st, const ItemLaunch &it) {
                            it.geo, it.cells, rec, it.rinfo, it.nfr, it.velo, it.aks2,
                            it.skip_mode);
     else
-        hipLaunchKernelGGL(k_nodes_fast, dim3(it.nrays), dim3(64), 0, st, it.nrays, it.off,
+        hipLaunchKernelGGL(k_nodes_fast<float>, dim3(it.nrays), dim3(64), 0, st, it.nrays, it.off,
                            it.geo, it.cells, rec, it.rinfo, it.nfr, it.velo, it.aks2,
                            it.skip_mode, it.lc);
     if (it.ev_nodes) (void)hipEventRecord(it.ev_nodes, st);
@@ -564,6 +573,241 @@ void launch_item(hipStream_t st, const ItemLaunch &it) {
     hipLaunchKernelGGL(k_flux_final, dim3((it.nfr + 63) / 64), dim3(64), 0, st, it.nfr,
                        it.nchunks, it.partial, it.d2, it.flux);
 }
+
+// ---------------------------------------------------------------------
+// adjoint
+// ---------------------------------------------------------------------
+// forward pass of the adjoint: the formal solution on scaled records, one
+// lane per (ray, channel), keeping I before every ADJ_K-th segment
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_adj_fwd(int nrays, int nfr, const long long *__restrict__ off,
+          const long long *__restrict__ ckoff, const FastRecT<T> *__restrict__ rec,
+          const double *__restrict__ dnu, const double *__restrict__ velo,
+          const double *__restrict__ istar, LineConst lc, double *__restrict__ image,
+          T *__restrict__ ckpt) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long long)nrays * nfr) return;
+    const int ray = (int)(g / nfr), ch = (int)(g - (long long)ray * nfr);
+    const long long b = off[ray];
+    const int n = (int)(off[ray + 1] - b);
+    if (n < 2) { image[g] = lc.I0; return; }
+    const FastLineT<T> L{(T)lc.nu0, (T)dnu[ch], (T)velo[ch]};
+    const T sf = ray == 0 ? (T)lc.starfract : T(0);
+    const T ist = (T)istar[ch];
+    T I = (T)lc.I0;
+    FastCarryT<T> c{T(0), T(0), T(0), 1};
+    int ovs = 0;
+    T *ck = ckpt + ckoff[ray] * nfr + ch;
+    FPtT<T> p0 = point_of<T>(rec[b]);
+    for (int k = 1; k < n; k++) {
+        if ((k - 1) % ADJ_K == 0) ck[(long long)((k - 1) / ADJ_K) * nfr] = I;
+        const FastRecT<T> r = rec[b + k];
+        const FPtT<T> p1 = point_of<T>(r);
+        segment_fast<T>(r, p0, p1, L, I, ist, sf, c, ovs);
+        p0 = p1;
+    }
+    image[g] = (double)I;
+}
+
+// sum over the wave of five values; lanes 0-4 add them to out[0..4]
+template <typename T>
+__device__ inline void wave_add5(double *out, T a0, T a1, T a2, T a3, T a4) {
+    for (int m = 32; m >= 1; m >>= 1) {
+        a0 += __shfl_xor(a0, m);
+        a1 += __shfl_xor(a1, m);
+        a2 += __shfl_xor(a2, m);
+        a3 += __shfl_xor(a3, m);
+        a4 += __shfl_xor(a4, m);
+    }
+    const int lane = threadIdx.x & 63;
+    if (lane < 5) {
+        T v = lane == 0 ? a0 : lane == 1 ? a1 : lane == 2 ? a2 : lane == 3 ? a3 : a4;
+        if (v != T(0)) atomicAdd(out + lane, (double)v);
+    }
+}
+
+// line values carried into segment k: node k-1 with segment k-1's profile
+template <typename T>
+__device__ inline void carried_in(const FastRecT<T> &a, const FastLineT<T> &L, T &jl, T &al) {
+    const FPtT<T> p = point_of<T>(a);
+    const T z = (L.dnu - L.nu0 * p.u) * a.inv_aa;
+    const T phi = a.norm * FOps<T>::ex(-(z * z));
+    jl = p.jc * phi;
+    al = p.dc * phi;
+}
+
+// reverse pass: one wave per (ray, group of 64 channels); per node the
+// gradients on (jd, ad, jc, dc) / sig and per segment on the mean width,
+// summed over the wave's channels into gnode
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_adj_rev(int nrays, int nfr, int nphi, const long long *__restrict__ off,
+          const long long *__restrict__ ckoff, const FastRecT<T> *__restrict__ rec,
+          const double *__restrict__ dnu, const double *__restrict__ velo,
+          const double *__restrict__ istar, LineConst lc, const double *__restrict__ gflux,
+          const double *__restrict__ ring_w, double center_w, double d2,
+          const T *__restrict__ ckpt, double *__restrict__ gnode) {
+    const int ray = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int ch = blockIdx.y * 64 + lane;
+    const bool active = ch < nfr;
+    const int chc = active ? ch : nfr - 1;
+    const long long b = off[ray];
+    const int n = (int)(off[ray + 1] - b);
+    if (n < 2) return;                       // uniform over the wave
+    const FastLineT<T> L{(T)lc.nu0, (T)dnu[chc], (T)velo[chc]};
+    // flux = (center_w I_0 + sum_rings ring_w mean_phi I) / d2 (telescope.F:1386-1418)
+    const double w = ray == 0 ? center_w : ring_w[(ray - 1) / nphi] / (1.0 * nphi);
+    T lam = active ? (T)(gflux[ch] * w / d2) : T(0);
+    const T sf = ray == 0 ? (T)lc.starfract : T(0);
+    const T ist = (T)istar[chc];
+    const T cai = (T)lc.c_aa;
+    T gjl = T(0), gal = T(0);
+    PGrad<T> cur, prv;
+    pg_zero(cur);
+    pg_zero(prv);
+    T iseg[ADJ_K];
+    T isub[33];
+    const int nseg = n - 1;
+    const int nblk = (nseg + ADJ_K - 1) / ADJ_K;
+    for (int blk = nblk - 1; blk >= 0; blk--) {
+        const int k0 = blk * ADJ_K + 1, k1 = min(k0 + ADJ_K - 1, nseg);
+        // forward over the block from its checkpoint
+        T I = ckpt[(ckoff[ray] + blk) * nfr + chc];
+        FastCarryT<T> c{T(0), T(0), T(0), k0 == 1 ? 1 : 0};
+        if (k0 > 1) {
+            const FastRecT<T> a = rec[b + k0 - 1];
+            carried_in<T>(a, L, c.jl0, c.al0);
+            const T ah = a.adh + c.al0;
+            c.S0 = ah > T(0) ? (a.jdh + c.jl0) / ah : T(0);
+        }
+        int ovs = 0;
+        FPtT<T> p0 = point_of<T>(rec[b + k0 - 1]);
+        for (int k = k0; k <= k1; k++) {
+            iseg[k - k0] = I;
+            const FastRecT<T> r = rec[b + k];
+            const FPtT<T> p1 = point_of<T>(r);
+            if (active) segment_fast<T>(r, p0, p1, L, I, ist, sf, c, ovs);
+            p0 = p1;
+        }
+        // backwards over the block
+        for (int k = k1; k >= k0; k--) {
+            const FastRecT<T> r = rec[b + k];
+            const FastRecT<T> a = rec[b + k - 1];
+            T g_lnaa = T(0), g_dls = T(0);
+            if (active) {
+                T jl0c = T(0), al0c = T(0);
+                if (k > 1) carried_in<T>(a, L, jl0c, al0c);
+                lam = segment_rev<T>(r, point_of<T>(a), point_of<T>(r), L, iseg[k - k0], lam,
+                                     ist, sf, k == 1 ? 1 : 0, jl0c, al0c, gjl, gal, prv, cur,
+                                     g_lnaa, g_dls, isub);
+            }
+            // d lw_mean: phi through inv_aa = 1/(c_aa lw), sub-grid width dls ~ lw
+            const T g_lw = (g_dls * r.dls - g_lnaa) * cai * r.inv_aa;
+            wave_add5<T>(gnode + (b + k) * 5, cur.jd, cur.ad, cur.jc, cur.dc, g_lw);
+            cur = prv;
+            pg_zero(prv);
+        }
+    }
+    wave_add5<T>(gnode + b * 5, cur.jd, cur.ad, cur.jc, cur.dc, T(0));
+}
+
+// node gradients -> cell gradients through the stencil of
+// get_line_dust_values (line.F:4006-4208); scaled -> true values
+__global__ void k_node_grad(long long nnodes, const GeoNode *__restrict__ geo,
+                            const CellRec *__restrict__ cells,
+                            const double *__restrict__ gnode, LineConst lc,
+                            double *__restrict__ gcell) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnodes) return;
+    const GeoNode gn = geo[k];
+    double v[5], sig, jc, dc;
+    node_interp(gn, cells, v);
+    scaled_node(v, lc, sig, jc, dc);
+    const double *g = gnode + k * 5;
+    const double g_jd = g[0] / sig, g_ad = g[1] / sig, g_jc = g[2] / sig, g_dc = g[3] / sig;
+    // segment k's mean width is (lw_{k-1} + lw_k)/2; a ray's first node
+    // carries no segment gradient, so the next node's slot is 0 across rays
+    const double g_lw = 0.5 * (g[4] + (k + 1 < nnodes ? g[9] : 0.0));
+    double gf[5];
+    gf[CR_JD] = g_jd;
+    gf[CR_AD] = g_ad;
+    gf[CR_LW] = g_lw;
+    gf[CR_NU] = lc.hknu * lc.A * g_jc - lc.hknu * lc.Bud * g_dc;   // jc = hknu N_u A
+    gf[CR_NL] = lc.hknu * lc.Bdu * g_dc;                           // dc = hknu (N_l B_lu - N_u B_ul)
+    const double dr = gn.dr, dt = gn.dt;
+    const int icr = gn.flag & 3;
+    double w00, w01 = 0.0, w10 = 0.0, w11 = 0.0;
+    if (icr == 1) { w00 = 1.0 - dt; w01 = dt; }
+    else if (icr == 2) { w00 = 1.0 - dr; w10 = dr; }
+    else {
+        w00 = (1.0 - dr) * (1.0 - dt); w01 = (1.0 - dr) * dt;
+        w10 = dr * (1.0 - dt); w11 = dr * dt;
+    }
+    for (int f = 0; f < 5; f++) {
+        if (gf[f] == 0.0) continue;
+        if (w00 != 0.0) atomicAdd(gcell + (long long)gn.c00 * 5 + f, w00 * gf[f]);
+        if (w01 != 0.0) atomicAdd(gcell + (long long)gn.c01 * 5 + f, w01 * gf[f]);
+        if (w10 != 0.0) atomicAdd(gcell + (long long)gn.c10 * 5 + f, w10 * gf[f]);
+        if (w11 != 0.0) atomicAdd(gcell + (long long)gn.c11 * 5 + f, w11 * gf[f]);
+    }
+}
+
+// cell gradients -> the inputs of the trace: N = pop * abund * rho * m_pg
+// (line.F:3998), dust j/alpha per line, width
+__global__ void k_cell_grad(int ncell, int nlev, int nlines, int line, int lev_up, int lev_low,
+                            double molpg, const double *__restrict__ rho,
+                            const double *__restrict__ abund, const double *__restrict__ pops,
+                            const double *__restrict__ gcell, double *__restrict__ g_rho,
+                            double *__restrict__ g_abund, double *__restrict__ g_lw,
+                            double *__restrict__ g_pops, double *__restrict__ g_jd,
+                            double *__restrict__ g_ad) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const double *g = gcell + (size_t)c * 5;
+    const double gnu = g[CR_NU], gnl = g[CR_NL];
+    const double pu = pops[(size_t)c * nlev + lev_up], pl = pops[(size_t)c * nlev + lev_low];
+    const double ab = abund[c], rh = rho[c];
+    g_jd[(size_t)c * nlines + line] += g[CR_JD];
+    g_ad[(size_t)c * nlines + line] += g[CR_AD];
+    g_lw[c] += g[CR_LW];
+    g_pops[(size_t)c * nlev + lev_up] += gnu * ab * rh * molpg;
+    g_pops[(size_t)c * nlev + lev_low] += gnl * ab * rh * molpg;
+    g_abund[c] += (gnu * pu + gnl * pl) * rh * molpg;
+    g_rho[c] += (gnu * pu + gnl * pl) * ab * molpg;
+}
+
+template <typename T>
+void launch_adjoint(hipStream_t st, const AdjLaunch &a) {
+    auto *rec = reinterpret_cast<FastRecT<T> *>(a.rec);
+    auto *ck = reinterpret_cast<T *>(a.ckpt);
+    hipLaunchKernelGGL(k_nodes_fast<T>, dim3(a.nrays), dim3(64), 0, st, a.nrays, a.off, a.geo,
+                       a.cells, rec, a.rinfo, a.nfr, a.velo, a.aks2, 1, a.lc);
+    const long long total = (long long)a.nrays * a.nfr;
+    hipLaunchKernelGGL(k_adj_fwd<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                       a.nrays, a.nfr, a.off, a.ckoff, rec, a.dnu, a.velo, a.istar, a.lc,
+                       a.image, ck);
+    dim3 gp((a.nfr + 63) / 64, a.nchunks);
+    hipLaunchKernelGGL(k_flux_partial, gp, dim3(64), 0, st, a.nfr, a.nrings, a.nphi, 1, a.image,
+                       a.ring_w, a.center_w, a.partial);
+    hipLaunchKernelGGL(k_flux_final, dim3((a.nfr + 63) / 64), dim3(64), 0, st, a.nfr,
+                       a.nchunks, a.partial, a.d2, a.flux);
+    hipLaunchKernelGGL(k_adj_rev<T>, dim3(a.nrays, (a.nfr + 63) / 64), dim3(64), 0, st,
+                       a.nrays, a.nfr, a.nphi, a.off, a.ckoff, rec, a.dnu, a.velo, a.istar,
+                       a.lc, a.gflux, a.ring_w, a.center_w, a.d2, ck, a.gnode);
+    hipLaunchKernelGGL(k_node_grad, dim3((unsigned)((a.nnodes + 255) / 256)), dim3(256), 0, st,
+                       a.nnodes, a.geo, a.cells, a.gnode, a.lc, a.gcell);
+    hipLaunchKernelGGL(k_cell_grad, dim3((a.ncell + 255) / 256), dim3(256), 0, st, a.ncell,
+                       a.nlev, a.nlines, a.line, a.lev_up, a.lev_low, a.molpg, a.rho, a.abund,
+                       a.pops, a.gcell, a.g_rho, a.g_abund, a.g_lw, a.g_pops, a.g_jd, a.g_ad);
+}
+
+template void launch_adjoint<double>(hipStream_t, const AdjLaunch &);
+template void launch_adjoint<float>(hipStream_t, const AdjLaunch &);
+
+int adjoint_block_len() { return ADJ_K; }
+size_t fast_rec_size(int fp64) { return fp64 ? sizeof(FastRecT<double>) : sizeof(FastRecT<float>); }
 
 template void launch_item<double>(hipStream_t, const ItemLaunch &);
 template void launch_item<float>(hipStream_t, const ItemLaunch &);

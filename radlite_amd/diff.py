@@ -1,0 +1,144 @@
+"""Differentiable line spectra: pyradlite's pre-pass in torch and the
+raytrace as a torch.autograd.Function whose backward is the HIP adjoint.
+
+Chain (all per cell on the half theta grid [nr, nth]):
+
+* LTE populations n_i = g_i exp(-E_i/kT) / Q(T), Q = T/2.77
+  (pyradlite ``_prep_mol_forcore``, radlite.py:1111-1119, with the
+  closed-form partition function of the workloads);
+* line width lw = sqrt(v_turb^2 + 2kT/(m_mol m_H)) in km/s
+  (``_calc_turbulence``, radlite.py:1618-1622);
+* abundance = exp(log_abund);
+* dust emissivity at each line centre j_d = rho_d kappa_abs(nu0)
+  (B_nu(T) + 1e-290) and extinction a_d = rho_d (kappa_abs + kappa_scat)
+  (line_dust_compute_src_alp, line.F:3560-3607; bplanck, setup.F:937-952);
+* the trace: flux = F(rho, abund, lw, pops, jd, ad) through librl, its
+  gradient through rl_trace_lines_adjoint.
+
+PyTorch is the autograd plumbing here; the trace and its adjoint run in
+the HIP library.  Tensors are float64 on the CPU (the C ABI takes host
+arrays).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import engine, synth
+
+
+def lte_populations(T, E_cm, g, q_over=2.77):
+    """n_i [nr, nth, nlev] = g_i exp(-E_i/kT) / (T/q_over); below 1e-99 -> 0."""
+    EK = torch.as_tensor(np.asarray(E_cm) * synth.K_PER_CM, dtype=T.dtype)
+    gg = torch.as_tensor(np.asarray(g, float), dtype=T.dtype)
+    Tc = T[..., None]
+    n = gg * torch.exp(-EK / Tc) / (Tc / q_over)
+    return torch.where(n < 1e-99, torch.zeros_like(n), n)
+
+
+def line_width_kms(T, v_turb_kms, m_mol=28.0):
+    return torch.sqrt(v_turb_kms ** 2 + 2 * synth.KB * T / (m_mol * synth.MH) / 1e10)
+
+
+def kappa_at_lines(cont_nu, kappa, nu0):
+    """kappa on the continuum grid interpolated at each |nu0| the way
+    line_dust_compute_src_alp does (hunt, then linear; 0 outside)."""
+    x = np.asarray(cont_nu, float)
+    out = np.zeros(len(nu0))
+    asc = x[-1] > x[0]
+    for i, f in enumerate(np.abs(nu0)):
+        n = len(x)
+        lo, hi = 0, n + 1
+        while hi - lo != 1:
+            m = (hi + lo) >> 1
+            if (f > x[m - 1]) == asc:
+                lo = m
+            else:
+                hi = m
+        if lo == 0 or lo == n:
+            continue
+        w = (f - x[lo - 1]) / (x[lo] - x[lo - 1])
+        out[i] = w * kappa[lo] + (1.0 - w) * kappa[lo - 1]
+    return out
+
+
+def dust_line(T_dust, rho_dust, kabs_nu0, kscat_nu0, nu0):
+    """(j_d, a_d) [nr, nth, nlines] at the line centres for one dust species."""
+    nu = torch.as_tensor(np.abs(np.asarray(nu0, float)), dtype=T_dust.dtype)
+    ka = torch.as_tensor(kabs_nu0, dtype=T_dust.dtype)
+    ks = torch.as_tensor(kscat_nu0, dtype=T_dust.dtype)
+    Tc = T_dust[..., None]
+    bnu = 1.47455e-47 * nu ** 3 / (torch.exp(4.7989e-11 * nu / Tc) - 1.0) + 1.0e-290
+    rd = rho_dust[..., None]
+    return rd * ka * bnu, rd * (ka + ks)
+
+
+class _Trace(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rho, abund, lw, pops, jd, ad, model):
+        model._upload(rho, abund, lw, pops, jd, ad)
+        fl, _, _ = model.rt.trace(model.items)
+        ctx.model = model
+        ctx.save_for_backward(rho, abund, lw, pops, jd, ad)
+        return torch.from_numpy(np.concatenate(fl))
+
+    @staticmethod
+    def backward(ctx, g):
+        m = ctx.model
+        m._upload(*ctx.saved_tensors)          # the inputs this forward saw
+        g = g.detach().cpu().numpy().astype(np.float64)
+        parts, o = [], 0
+        for it in m.items:
+            parts.append(g[o:o + it[2]])
+            o += it[2]
+        _, gr = m.rt.adjoint(m.items, parts)
+        t = lambda a: torch.from_numpy(a)
+        return (t(gr["g_rho"]), t(gr["g_abund"]), t(gr["g_lw"]), t(gr["g_pops"]),
+                t(gr["g_jd"]), t(gr["g_ad"]), None)
+
+
+class SpectrumModel:
+    """A loaded run (grid, velocity, lines, star, rays) whose spectrum is
+    differentiable with respect to the per-cell fields.
+
+    ``flux(rho, abund, lw, pops, jd, ad)`` returns the concatenated
+    spectra of ``items`` (every item SKIP_NONE) as a torch tensor."""
+
+    def __init__(self, ri, precision="fp64", incl_rad=None, device=0, items=None):
+        self.ri = ri
+        self.rt = engine.Raytracer(device, precision)
+        self.tables = self.rt.load_run(ri)
+        inc = [ri.incl_deg * 0.0174532925199] if incl_rad is None else list(incl_rad)
+        self.rt.setup_rays(inc, ri.nphi, ri.b_extra, ri.dbdr)
+        nfr, vmax = engine.passband(ri.vmax, ri.dv)
+        self.nfr, self.vmax = nfr, vmax
+        self.items = items or [(il, ii, nfr, vmax, engine.SKIP_NONE)
+                               for ii in range(len(inc)) for il in range(len(ri.A))]
+        tb = self.tables
+        self.kabs_nu0 = kappa_at_lines(ri.cont_nu, ri.kappa_abs[0], tb.nu0)
+        self.kscat_nu0 = kappa_at_lines(ri.cont_nu, ri.kappa_scat[0], tb.nu0)
+
+    def _upload(self, rho, abund, lw, pops, jd, ad):
+        n = lambda x: np.ascontiguousarray(x.detach().cpu().numpy(), dtype=np.float64)
+        lwn = n(lw)
+        aks = float(lwn.max())
+        self.rt.set_medium(n(rho), n(abund), self.ri.vel, lwn, self.ri.umass_av, aks)
+        self.rt.set_levels(n(pops))
+        self.rt.set_dust_line(n(jd), n(ad))
+
+    def flux(self, rho, abund, lw, pops, jd, ad):
+        return _Trace.apply(rho, abund, lw, pops, jd, ad, self)
+
+    def flux_from_physics(self, T, log_abund, v_turb_kms, m_mol=28.0):
+        """Spectrum as a function of per-cell temperature (gas = dust),
+        log-abundance and turbulent width, through the pre-pass."""
+        ri = self.ri
+        rho = torch.as_tensor(ri.rho_gas, dtype=torch.float64)
+        pops = lte_populations(T, ri.E_cm, ri.g)
+        lw = line_width_kms(T, v_turb_kms, m_mol)
+        rho_d = torch.as_tensor(ri.rho_dust[0], dtype=torch.float64)
+        jd, ad = dust_line(T, rho_d, self.kabs_nu0, self.kscat_nu0, self.tables.nu0)
+        return self.flux(rho, torch.exp(log_abund), lw, pops, jd, ad)
+
+    def close(self):
+        self.rt.close()

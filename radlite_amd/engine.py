@@ -69,7 +69,7 @@ def load_library(path=LIB_PATH):
         "rl_trace_lines_forward": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
                                              C.POINTER(C.c_uint)]),
         "rl_trace_lines_adjoint": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
-                                             _dp, _dp, _dp, _dp]),
+                                             _dp, _dp, _dp, _dp, _dp, _dp]),
         "rl_sync": (C.c_int, [vp]),
         "rl_last_stats": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong)]),
         "rl_prepare_line_freqs": (C.c_int, [C.c_int, _dp, _ip, _ip, _dp]),
@@ -177,6 +177,7 @@ class Raytracer:
         self.ring_r = None
         self.ring_edge = None
         self.nr = self.nth = 0
+        self.nlev = self.nlines = 0
 
     def close(self):
         if self.ctx:
@@ -208,12 +209,14 @@ class Raytracer:
 
     def set_levels(self, pops):
         p, pp = _d(pops)
+        self.nlev = p.shape[-1]
         self._chk(self.L.rl_set_levels(self.ctx, p.shape[-1], pp), "rl_set_levels")
 
     def set_lines(self, nu0, A, gu, gl, lev_up, lev_low):
         a = [_d(x) for x in (nu0, A, gu, gl)]
         u, up = _i(lev_up)
         l, lp = _i(lev_low)
+        self.nlines = len(a[0][0])
         self._chk(self.L.rl_set_lines(self.ctx, len(a[0][0]), a[0][1], a[1][1], a[2][1],
                                       a[3][1], up, lp), "rl_set_lines")
 
@@ -286,6 +289,32 @@ class Raytracer:
                 im.append(img[io:io + n].reshape(self.nrays[it[1]], it[2]))
                 io += n
         return fl, (im if img is not None else None), warn
+
+    def adjoint(self, items, dL_dflux):
+        """Gradient of L with respect to the inputs the trace reads, given
+        dL/dflux per item (list of arrays, same layout as trace's flux).
+        Every item must use SKIP_NONE.  Returns (list of flux arrays,
+        dict of gradients g_rho, g_abund, g_lw [nr, nth], g_pops
+        [nr, nth, nlev], g_jd, g_ad [nr, nth, nlines])."""
+        arr = (RLItem * len(items))(*[RLItem(*it) for it in items])
+        nf = [it[2] for it in items]
+        g = np.ascontiguousarray(np.concatenate([np.asarray(x, float) for x in dL_dflux]))
+        assert g.size == sum(nf)
+        flux = np.zeros(sum(nf))
+        nr, nth = self.nr, self.nth
+        out = dict(g_rho=np.zeros((nr, nth)), g_abund=np.zeros((nr, nth)),
+                   g_lw=np.zeros((nr, nth)), g_pops=np.zeros((nr, nth, self.nlev)),
+                   g_jd=np.zeros((nr, nth, self.nlines)), g_ad=np.zeros((nr, nth, self.nlines)))
+        p = {k: v.ctypes.data_as(_dp) for k, v in out.items()}
+        rc = self.L.rl_trace_lines_adjoint(self.ctx, len(items), arr, g.ctypes.data_as(_dp),
+                                           flux.ctypes.data_as(_dp), p["g_rho"], p["g_abund"],
+                                           p["g_lw"], p["g_pops"], p["g_jd"], p["g_ad"])
+        self._chk(rc, "rl_trace_lines_adjoint")
+        fl, o = [], 0
+        for n in nf:
+            fl.append(flux[o:o + n])
+            o += n
+        return fl, out
 
     def stats(self):
         t = np.zeros(5)
