@@ -40,6 +40,59 @@ def measured_traffic(precision):
     return None
 
 
+def bench_adjoint(args):
+    """Forward vs forward+adjoint on the 200x100 disk (34,049 rays x 100
+    channels, 45 deg, every channel integrated): one step is the spectrum
+    (rl_trace_lines_forward) and then its gradient for a chi^2 loss
+    (rl_trace_lines_adjoint, which runs its own checkpointed forward and
+    the reverse sweep, down to per-cell gradients)."""
+    from radlite_amd import engine, synth
+    ri, w = synth.workload_inputs("c4", seed=0)
+    nfr, vmax = engine.passband(ri.vmax, ri.dv)
+    rt = engine.Raytracer(0, args.precision)
+    rt.load_run(ri)
+    rt.setup_rays([w["incl"][0] * 0.0174532925199], ri.nphi, ri.b_extra, ri.dbdr)
+    items = [(0, 0, nfr, vmax, engine.SKIP_NONE)]
+    fl, _, _ = rt.trace(items)
+    target = fl[0] * 1.01
+    sigma = 1e-3 * np.abs(target).max()
+
+    def fwd():
+        return rt.trace(items)[0]
+
+    def fwd_adj():
+        f = rt.trace(items)[0][0]
+        g = 2.0 * (f - target) / sigma ** 2
+        return rt.adjoint(items, [g])
+
+    res = {}
+    for name, fn in (("fwd", fwd), ("fwd_adj", fwd_adj)):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        rt.L.rl_sync(rt.ctx)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        rt.L.rl_sync(rt.ctx)
+        res[name] = (time.perf_counter() - t0) / args.steps
+        if name == "fwd_adj":
+            st = rt.stats()
+            res["adj_device_ms"] = st["t_formal_ms"]
+    nominal = rt.nrays[0] * nfr
+    rt.close()
+    return {
+        "workload": f"c4: nr={w['nr']} x nth={w['nth']}, {nominal // nfr} rays x {nfr} ch x 1 incl",
+        "precision": args.precision,
+        "fwd_ms": res["fwd"] * 1e3,
+        "fwd_adj_ms": res["fwd_adj"] * 1e3,
+        "adjoint_call_device_ms": res["adj_device_ms"],
+        "ratio_fwd_adj_over_fwd": res["fwd_adj"] / res["fwd"],
+        "value_fwd": nominal / res["fwd"],
+        "value_fwd_adj": nominal / res["fwd_adj"],
+        "unit": "ray*channel/s",
+    }
+
+
 def copy_probe(nbytes):
     """A plain device-to-device copy of known size: what FETCH_SIZE and
     WRITE_SIZE read for it calibrates the counters against true bytes."""
@@ -65,6 +118,9 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=8,
                     help="CPU baseline traces every n-th ray of the workload")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the forward+adjoint measurement on the 200x100 disk")
+    ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="run only a plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -77,6 +133,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.adjoint_only:
+        print(json.dumps(bench_adjoint(args)))
+        return
     from radlite_amd import engine, synth
     ri, w = synth.workload_inputs(args.workload, seed=0)
     nfr, vmax = engine.passband(ri.vmax, ri.dv)
@@ -153,6 +212,9 @@ def main():
         },
     }
     rt.close()
+
+    if not args.no_adjoint:
+        out["fwd_adjoint"] = bench_adjoint(args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
