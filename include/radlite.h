@@ -121,6 +121,30 @@ int rl_trace_lines_adjoint(rl_ctx *ctx, int nitems, const rl_item *items,
                            double *g_abund, double *g_lw, double *g_pops,
                            double *g_jd, double *g_ad);
 
+/* ---- batching controls (no counterpart in the reference) ---------- */
+/* A call to rl_trace_lines_forward runs its items in batches: every
+ * item of a batch is resident in device memory at once, and each stage
+ * (cell tables, node records, formal solution, flux) is one launch for
+ * the whole batch, the formal solution as a persistent kernel over a
+ * device work counter.  Batches are cut so that their device memory
+ * stays within the budget: bytes > 0 sets it, 0 (the default) uses 60%
+ * of the free device memory. */
+int rl_set_memory_budget(rl_ctx *ctx, double bytes);
+
+enum {
+    RL_OPT_GRAPH = 1,     /* 1 (default): record each batch shape's launches in a
+                             hipGraph and replay it; 0: launch directly */
+    RL_OPT_AUTOTUNE = 2,  /* 1 (default): time the tile mappings once per batch
+                             shape and keep the fastest; 0: default mapping */
+    RL_OPT_TILE = 3       /* fix the tile mapping: 0 = ray-major (ray, channel)
+                             pairs, 16/32/64 = channels per ray in a wave */
+};
+int rl_set_option(rl_ctx *ctx, int option, int value);
+
+/* Batches of the last forward call and the tile mapping / persistent
+ * grid size the last batch used. */
+int rl_last_batch_info(const rl_ctx *ctx, int *batches, int *tile_cw, int *blocks);
+
 /* Wait for all work of the context. */
 int rl_sync(rl_ctx *ctx);
 

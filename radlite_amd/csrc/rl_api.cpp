@@ -10,6 +10,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -94,6 +96,15 @@ struct rl_ctx {
         d_maser, d_scr, d_cross;
     int cells_line = -1;
     std::vector<double> h_chan, h_vel;
+    // batching
+    DevBuf d_items, d_tile0, d_counter;
+    std::vector<rl::ItemDev> h_items;
+    std::vector<long long> h_tile0;
+    size_t mem_budget = 0;            // 0: a share of the free device memory
+    bool use_graph = true, autotune = true;
+    int fixed_cw = -1, last_cw = 0, last_blocks = 0, last_batches = 0;
+    std::map<std::string, std::pair<int, int>> tune;
+    std::map<std::string, hipGraphExec_t> graphs;
     // adjoint buffers
     DevBuf d_arec, d_ckpt, d_ckoff, d_gflux, d_gnode, d_gcell, d_grho, d_gabund, d_glw,
         d_gpops, d_gjd, d_gad;
@@ -156,6 +167,7 @@ extern "C" void rl_destroy(rl_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -555,6 +567,283 @@ static int setup_item(rl_ctx *c, const rl_item &it, int ii, ItemSetup &out, hipE
     return 0;
 }
 
+// line constants and channel tables of one item (line_setup_passband,
+// line.F:462-469; star per channel, line.F:3821-3843; prepare_lines,
+// line.F:1778-1787); chan receives dnu, dnu/nu0, I_star [3*nfr]
+static int item_consts(rl_ctx *c, const rl_item &it, const RaySet &rs, rl::LineConst &lc,
+                       double *chan) {
+    const int nfr = it.nfr;
+    const double nu0s = c->nu0[it.line], nu0 = std::fabs(nu0s);
+    const double passb = 3.33567e-6 * nu0 * it.vmax_kms;
+    const double nu1 = 0.0 - passb, dnu = 2.0 * passb / (nfr - 1.0);
+    for (int k = 0; k < nfr; k++) {
+        double dn = nu1 + k * dnu;
+        chan[k] = dn;
+        chan[nfr + k] = dn / nu0s;
+        chan[2 * nfr + k] = star_at(c, nu0 + dn);
+    }
+    lc.nu0 = nu0s;
+    lc.c_aa = 3.33567e-6 * nu0;
+    lc.hknu = 5.27296241956e-28 * nu0s;
+    lc.A = c->A[it.line];
+    lc.Bud = 6.78171833781e46 * lc.A / (nu0s * nu0s * nu0s);
+    lc.Bdu = lc.Bud * (c->gu[it.line] / c->gl[it.line]);
+    double q = c->rstar / rs.ring_ri[1];
+    lc.starfract = q * q;
+    if (rs.ring_ri[1] < c->rstar) return fail(c, 124, "central beam resolves the star");
+    lc.I0 = 0.0;
+    if (c->outer_bc == 2)
+        lc.I0 = 1.47455253991e-47 * (nu0s * nu0s * nu0s) /
+                (std::exp(4.7991598e-11 * nu0s / 2.728) - 1.0);
+    return 0;
+}
+
+// Device bytes one item needs in a batch (node records, image, ring
+// partial sums, channel tables, flux, ray info).
+static size_t item_bytes(const rl_ctx *c, const rl_item &it, const RaySet &rs) {
+    const size_t rsz = c->precision == RL_FP64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::FastRec);
+    return (size_t)rs.nnodes * rsz + (size_t)rs.nrays * it.nfr * 8 +
+           (size_t)rs.nrings * it.nfr * 8 + (size_t)it.nfr * 32 + (size_t)rs.nrays * 16 + 64;
+}
+
+static long long tiles_of(int cw, int nrays, int nfr) {
+    if (cw == 0) return ((long long)nrays * nfr + 63) / 64;
+    const int rows = 64 / cw;
+    return (long long)((nrays + rows - 1) / rows) * ((nfr + cw - 1) / cw);
+}
+
+// One batch: items [i0, i1) of the call, all resident at once.
+static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *flux,
+                     double *image, unsigned *warn, size_t fo, size_t io) {
+    const int nb = i1 - i0;
+    const int fp64 = c->precision == RL_FP64;
+    const size_t rsz = fp64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::FastRec);
+    const int ncell = c->nr * c->nh;
+    const double molpg = 1.0 / (c->umass_av * 1.6726e-24);
+    // host layout of the batch
+    std::vector<rl::ItemDev> hi(nb);
+    std::vector<int> slot_line;
+    std::vector<double> &chan = c->h_chan;
+    size_t nchan = 0;
+    for (int k = 0; k < nb; k++) nchan += 3 * (size_t)items[i0 + k].nfr;
+    chan.assign(nchan, 0.0);
+    long long rec0 = 0, img0 = 0, chan0 = 0, part0 = 0, flux0 = 0;
+    int rinfo0 = 0, max_rays = 0, max_nfr = 0, max_rings = 0, any_skip = 0;
+    for (int k = 0; k < nb; k++) {
+        const rl_item &it = items[i0 + k];
+        RaySet &rs = c->incl[it.incl];
+        rl::ItemDev &d = hi[k];
+        if (int e = item_consts(c, it, rs, d.lc, chan.data() + chan0)) return e;
+        int slot = -1;
+        for (size_t q = 0; q < slot_line.size(); q++)
+            if (slot_line[q] == it.line) slot = (int)q;
+        if (slot < 0) { slot = (int)slot_line.size(); slot_line.push_back(it.line); }
+        d.nrays = rs.nrays; d.nfr = it.nfr; d.nrings = rs.nrings; d.nphi = rs.nphi;
+        d.skip_mode = it.skip_mode; d.cslot = slot;
+        d.off = rs.d_off.as<long long>();
+        d.geo = rs.d_nodes.as<GeoNode>();
+        d.rec0 = rec0; d.img0 = img0; d.chan0 = chan0; d.part0 = part0; d.flux0 = flux0;
+        d.rinfo0 = rinfo0;
+        d.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
+        d.ring_w = rs.d_ringw.as<double>();
+        d.center_w = rs.center_w;
+        rec0 += rs.nnodes; img0 += (long long)rs.nrays * it.nfr; chan0 += 3 * it.nfr;
+        part0 += (long long)rs.nrings * it.nfr; flux0 += it.nfr; rinfo0 += rs.nrays;
+        max_rays = std::max(max_rays, rs.nrays);
+        max_nfr = std::max(max_nfr, it.nfr);
+        max_rings = std::max(max_rings, rs.nrings);
+        any_skip |= it.skip_mode == RL_SKIP_REFERENCE;
+        c->counts[0] += rs.nrays;
+        c->counts[1] += rs.nnodes;
+        c->counts[3] += (long long)rs.nrays * it.nfr;
+    }
+    // device buffers
+    HIPCHK(c, c->d_rec.ensure((size_t)rec0 * rsz));
+    HIPCHK(c, c->d_image.ensure((size_t)img0 * 8));
+    HIPCHK(c, c->d_partial.ensure((size_t)part0 * 8));
+    HIPCHK(c, c->d_flux.ensure((size_t)flux0 * 8));
+    HIPCHK(c, c->d_rinfo.ensure((size_t)rinfo0 * sizeof(RayInfo)));
+    HIPCHK(c, c->d_cells.ensure((size_t)slot_line.size() * ncell * sizeof(CellRec)));
+    HIPCHK(c, c->d_ovs.ensure((size_t)nb * 8));
+    HIPCHK(c, c->d_maser.ensure((size_t)((nb + 1) / 2) * 8));
+    HIPCHK(c, c->d_counter.ensure(8));
+    HIPCHK(c, c->d_items.ensure((size_t)nb * sizeof(rl::ItemDev)));
+    HIPCHK(c, c->d_tile0.ensure((size_t)(nb + 1) * 8));
+    if (int e = upload(c, c->d_chan, chan.data(), nchan * 8)) return e;
+    c->cells_line = -1;       // d_cells is rewritten below for this batch's lines
+    c->h_items = hi;
+    if (int e = upload(c, c->d_items, c->h_items.data(), (size_t)nb * sizeof(rl::ItemDev))) return e;
+    rl::BatchLaunch L;
+    L.nitems = nb; L.max_rays = max_rays; L.max_nfr = max_nfr; L.max_rings = max_rings;
+    L.ncell = ncell; L.any_skip = any_skip;
+    L.items = c->d_items.as<rl::ItemDev>();
+    L.cells = c->d_cells.as<CellRec>();
+    L.rec = c->d_rec.p;
+    L.rinfo = c->d_rinfo.as<RayInfo>();
+    L.chan = c->d_chan.as<double>();
+    L.image = c->d_image.as<double>();
+    L.partial = c->d_partial.as<double>();
+    L.flux = c->d_flux.as<double>();
+    L.d2 = (1.0 * 3.08572e18) * (1.0 * 3.08572e18);
+    L.tile0 = c->d_tile0.as<long long>();
+    L.ovs = c->d_ovs.as<unsigned long long>();
+    L.counter = c->d_counter.as<unsigned long long>();
+    L.maser = c->d_maser.as<int>();
+    // tile mapping: autotuned once per batch shape (rays per wave, channel
+    // tiling, persistent grid size)
+    int dev_cu = 0;
+    HIPCHK(c, hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device));
+    const int bpc = rl::formal_batch_blocks_per_cu(fp64);
+    std::string shape = std::to_string(fp64);
+    for (int k = 0; k < nb; k++)
+        shape += ":" + std::to_string(hi[k].nrays) + "x" + std::to_string(hi[k].nfr) + "s" +
+                 std::to_string(hi[k].skip_mode);
+    auto set_tiles = [&](int cw) -> int {
+        std::vector<long long> &t0 = c->h_tile0;
+        t0.assign(nb + 1, 0);
+        for (int k = 0; k < nb; k++) t0[k + 1] = t0[k] + tiles_of(cw, hi[k].nrays, hi[k].nfr);
+        L.ntiles = t0[nb];
+        return upload(c, c->d_tile0, t0.data(), (size_t)(nb + 1) * 8);
+    };
+    auto zero_counters = [&]() {
+        rl::launch_zero_u64(c->stream, L.ovs, nb);
+        rl::launch_zero_u64(c->stream, reinterpret_cast<unsigned long long *>(L.maser),
+                            (nb + 1) / 2);
+    };
+    // cell tables of the batch's lines
+    auto cells = [&]() {
+        for (size_t q = 0; q < slot_line.size(); q++) {
+            const int ln = slot_line[q];
+            rl::launch_cells(c->stream, ncell, c->nlev, c->nlines, ln, c->lev_up[ln],
+                             c->lev_low[ln], molpg, c->d_rho.as<double>(),
+                             c->d_abund.as<double>(), c->d_lw.as<double>(),
+                             c->d_pops.as<double>(), c->d_jd.as<double>(),
+                             c->d_ad.as<double>(), c->d_cells.as<CellRec>() + q * ncell);
+        }
+    };
+    int cw = 0, nblk = dev_cu * bpc;
+    auto tuned = c->tune.find(shape);
+    if (c->fixed_cw >= 0) {
+        cw = c->fixed_cw;
+    } else if (tuned != c->tune.end()) {
+        cw = tuned->second.first;
+        nblk = tuned->second.second;
+    } else if (c->autotune) {
+        cells();
+        rl::launch_batch_nodes(c->stream, fp64, L);
+        float best = 1e30f;
+        const int cws[4] = {0, 64, 32, 16};
+        const int mult[2] = {1, 2};
+        for (int ci = 0; ci < 4; ci++) {
+            if (int e = set_tiles(cws[ci])) return e;
+            for (int mi = 0; mi < 2; mi++) {
+                const int blocks = dev_cu * bpc * mult[mi];
+                zero_counters();
+                HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+                rl::launch_batch_formal(c->stream, fp64, L, cws[ci], blocks);
+                HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+                HIPCHK(c, hipEventSynchronize(c->ev[3]));
+                float t = 0;
+                HIPCHK(c, hipEventElapsedTime(&t, c->ev[2], c->ev[3]));
+                if (t < best) { best = t; cw = cws[ci]; nblk = blocks; }
+            }
+        }
+        c->tune[shape] = {cw, nblk};
+    }
+    c->last_cw = cw;
+    c->last_blocks = nblk;
+    if (int e = set_tiles(cw)) return e;
+    HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));
+    // the device stages; recorded once per (shape, buffers) into a graph
+    auto stages = [&](bool timed) -> int {
+        zero_counters();
+        cells();
+        rl::launch_batch_nodes(c->stream, fp64, L);
+        if (timed) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+        rl::launch_batch_formal(c->stream, fp64, L, cw, nblk);
+        rl::launch_batch_finish(c->stream, fp64, L);
+        if (timed) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+        rl::launch_batch_flux(c->stream, L);
+        return 0;
+    };
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    // everything the recorded launches take by value
+    std::string gkey = shape;
+    {
+        char buf[64];
+        auto add = [&](const void *p) { snprintf(buf, sizeof buf, ":%p", p); gkey += buf; };
+        for (const void *p : {c->d_rec.p, c->d_image.p, c->d_partial.p, c->d_flux.p,
+                              c->d_rinfo.p, c->d_cells.p, c->d_ovs.p, c->d_maser.p,
+                              c->d_counter.p, c->d_items.p, c->d_tile0.p, c->d_chan.p,
+                              c->d_rho.p, c->d_abund.p, c->d_lw.p, c->d_pops.p, c->d_jd.p,
+                              c->d_ad.p})
+            add(p);
+        snprintf(buf, sizeof buf, ":%d:%d:%d:%d:%d:%.17g", cw, nblk, c->nlev, c->nlines, ncell,
+                 molpg);
+        gkey += buf;
+        for (int ln : slot_line) {
+            snprintf(buf, sizeof buf, ":%d:%d:%d", ln, c->lev_up[ln], c->lev_low[ln]);
+            gkey += buf;
+        }
+    }
+    if (c->use_graph) {
+        auto g = c->graphs.find(gkey);
+        if (g == c->graphs.end() && c->graphs.size() >= 32) {
+            for (auto &q : c->graphs) (void)hipGraphExecDestroy(q.second);
+            c->graphs.clear();
+        }
+        if (g == c->graphs.end()) {
+            hipGraph_t graph;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int e = stages(false);
+            hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+            if (e) return e;
+            HIPCHK(c, ec);
+            hipGraphExec_t ex;
+            HIPCHK(c, hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+            g = c->graphs.emplace(gkey, ex).first;
+        }
+        HIPCHK(c, hipGraphLaunch(g->second, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    } else {
+        if (int e = stages(true)) return e;
+        HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(flux + fo, c->d_flux.p, (size_t)flux0 * 8, hipMemcpyDeviceToHost,
+                             c->stream));
+    if (image)
+        HIPCHK(c, hipMemcpyAsync(image + io, c->d_image.p, (size_t)img0 * 8,
+                                 hipMemcpyDeviceToHost, c->stream));
+    std::vector<unsigned long long> ovs(nb);
+    std::vector<int> maser(nb + 1);
+    HIPCHK(c, hipMemcpyAsync(ovs.data(), L.ovs, (size_t)nb * 8, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(maser.data(), L.maser, (size_t)nb * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+    if (int e = check_err(c)) return e;
+    unsigned long long ovs_last = 0;
+    for (int k = 0; k < nb; k++) {
+        if (warn)
+            warn[i0 + k] = (maser[k] ? RL_WARN_MASER : 0u) | (ovs[k] ? RL_WARN_OVERSHOOT : 0u);
+        ovs_last = ovs[k];
+    }
+    c->counts[2] = (long long)ovs_last;
+    float t14 = 0, t12 = 0, t23 = 0, t34 = 0;
+    (void)hipEventElapsedTime(&t14, c->ev[1], c->ev[4]);
+    if (c->use_graph) {
+        c->t_ms[2] += t14;           // nodes, formal and flux run as one graph
+    } else {
+        (void)hipEventElapsedTime(&t12, c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&t23, c->ev[2], c->ev[3]);
+        (void)hipEventElapsedTime(&t34, c->ev[3], c->ev[4]);
+        c->t_ms[1] += t12; c->t_ms[2] += t23; c->t_ms[3] += t34;
+    }
+    c->t_ms[4] += t14;
+    return 0;
+}
+
 extern "C" int rl_trace_lines_forward(rl_ctx *c, int nitems, const rl_item *items,
                                       double *flux, double *image, unsigned *warn) {
     if (!c || nitems < 0 || (nitems && (!items || !flux)))
@@ -563,83 +852,80 @@ extern "C" int rl_trace_lines_forward(rl_ctx *c, int nitems, const rl_item *item
         return fail(c, 13, "inputs incomplete: grid, medium, levels, lines, dust, rays");
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, c->d_err.ensure(4));
-    HIPCHK(c, c->d_ovs.ensure(8));
-    HIPCHK(c, c->d_maser.ensure(4));
     for (int i = 0; i < 5; i++) c->t_ms[i] = 0.0;
     for (int i = 0; i < 4; i++) c->counts[i] = 0;
-    const size_t rsz = c->precision == RL_FP64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::FastRec);
-    size_t fo = 0, io = 0;
     for (int ii = 0; ii < nitems; ii++) {
         const rl_item &it = items[ii];
-        ItemSetup su;
-        if (int e = setup_item(c, it, ii, su, c->ev[0], c->ev[1])) return e;
-        RaySet &rs = *su.rs;
-        const rl::LineConst &lc = su.lc;
-        const int nfr = it.nfr;
-        const int nrays = rs.nrays;
-        const int rpc = 1, nchunks = rs.nrings;
-        HIPCHK(c, c->d_rec.ensure((size_t)rs.nnodes * rsz));
-        HIPCHK(c, c->d_rinfo.ensure((size_t)nrays * sizeof(RayInfo)));
-        HIPCHK(c, c->d_image.ensure((size_t)nrays * nfr * 8));
-        HIPCHK(c, c->d_partial.ensure((size_t)nchunks * nfr * 8));
-        HIPCHK(c, c->d_flux.ensure((size_t)nfr * 8));
-        HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));
-        HIPCHK(c, hipMemsetAsync(c->d_ovs.p, 0, 8, c->stream));
-        HIPCHK(c, hipMemsetAsync(c->d_maser.p, 0, 4, c->stream));
-        rl::ItemLaunch L;
-        L.nrays = nrays; L.nfr = nfr; L.nrings = rs.nrings; L.nphi = rs.nphi;
-        L.skip_mode = it.skip_mode;
-        L.off = rs.d_off.as<long long>();
-        L.geo = rs.d_nodes.as<GeoNode>();
-        L.cells = c->d_cells.as<CellRec>();
-        L.rec = c->d_rec.p;
-        L.rinfo = c->d_rinfo.as<RayInfo>();
-        L.dnu = c->d_chan.as<double>();
-        L.velo = L.dnu + nfr;
-        L.istar = L.dnu + 2 * nfr;
-        L.lc = lc;
-        L.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
-        L.image = c->d_image.as<double>();
-        L.ovs = c->d_ovs.as<unsigned long long>();
-        L.maser = c->d_maser.as<int>();
-        L.ring_w = rs.d_ringw.as<double>();
-        L.center_w = rs.center_w;
-        L.nchunks = nchunks;
-        L.rings_per_chunk = rpc;
-        L.partial = c->d_partial.as<double>();
-        L.d2 = (1.0 * 3.08572e18) * (1.0 * 3.08572e18);
-        L.flux = c->d_flux.as<double>();
-        L.ev_nodes = c->ev[2];
-        L.ev_formal = c->ev[3];
-        if (c->precision == RL_FP64) rl::launch_item<double>(c->stream, L);
-        else rl::launch_item<float>(c->stream, L);
-        HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
-        HIPCHK(c, hipMemcpyAsync(flux + fo, c->d_flux.p, (size_t)nfr * 8,
-                                 hipMemcpyDeviceToHost, c->stream));
-        if (image)
-            HIPCHK(c, hipMemcpyAsync(image + io, c->d_image.p, (size_t)nrays * nfr * 8,
-                                     hipMemcpyDeviceToHost, c->stream));
-        unsigned long long ovs = 0;
-        int maser = 0;
-        HIPCHK(c, hipMemcpyAsync(&ovs, c->d_ovs.p, 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(&maser, c->d_maser.p, 4, hipMemcpyDeviceToHost, c->stream));
-        if (int e = check_err(c)) return e;
-        if (warn) warn[ii] = (maser ? RL_WARN_MASER : 0u) | (ovs ? RL_WARN_OVERSHOOT : 0u);
-        float t01 = 0, t12 = 0, t23 = 0, t34 = 0;
-        (void)hipEventElapsedTime(&t01, c->ev[0], c->ev[1]);
-        (void)hipEventElapsedTime(&t12, c->ev[1], c->ev[2]);
-        (void)hipEventElapsedTime(&t23, c->ev[2], c->ev[3]);
-        (void)hipEventElapsedTime(&t34, c->ev[3], c->ev[4]);
-        c->t_ms[0] += t01; c->t_ms[1] += t12; c->t_ms[2] += t23; c->t_ms[3] += t34;
-        c->t_ms[4] += t01 + t12 + t23 + t34;
-        c->counts[0] += nrays;
-        c->counts[1] += rs.nnodes;
-        c->counts[3] += (long long)nrays * nfr;
-        c->counts[2] = (long long)ovs;   // overshoot count of the last item (diagnostic)
-        fo += nfr;
-        io += (size_t)nrays * nfr;
+        if (it.line < 0 || it.line >= c->nlines || it.incl < 0 ||
+            it.incl >= (int)c->incl.size() || it.nfr < 2 || !(it.vmax_kms > 0.0) ||
+            (it.skip_mode != RL_SKIP_REFERENCE && it.skip_mode != RL_SKIP_NONE))
+            return fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii);
     }
+    // trajectories of every inclination used (line-independent, kept)
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    for (int ii = 0; ii < nitems; ii++)
+        if (int e = build_traj(c, c->incl[items[ii].incl])) return e;
+    HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[5]));
+    float tt = 0;
+    (void)hipEventElapsedTime(&tt, c->ev[0], c->ev[5]);
+    c->t_ms[0] = tt;
+    c->t_ms[4] = tt;
+    // batches under the memory budget
+    size_t budget = c->mem_budget;
+    if (budget == 0) {
+        size_t fr = 0, tot = 0;
+        HIPCHK(c, hipMemGetInfo(&fr, &tot));
+        budget = (size_t)(0.6 * (double)fr) + c->d_rec.n + c->d_image.n + c->d_partial.n;
+    }
+    c->last_batches = 0;
+    size_t fo = 0, io = 0;
+    int i0 = 0;
+    while (i0 < nitems) {
+        size_t used = 0;
+        int i1 = i0;
+        while (i1 < nitems) {
+            const size_t b = item_bytes(c, items[i1], c->incl[items[i1].incl]);
+            if (i1 > i0 && (used + b > budget || i1 - i0 >= 65535)) break;   // grid y/z limit
+            used += b;
+            i1++;
+        }
+        if (int e = run_batch(c, items, i0, i1, flux, image, warn, fo, io)) return e;
+        c->last_batches++;
+        for (int k = i0; k < i1; k++) {
+            fo += items[k].nfr;
+            io += (size_t)c->incl[items[k].incl].nrays * items[k].nfr;
+        }
+        i0 = i1;
+    }
+    return 0;
+}
+
+extern "C" int rl_set_memory_budget(rl_ctx *c, double bytes) {
+    if (!c || bytes < 0) return RL_E_ARG;
+    c->mem_budget = (size_t)bytes;
+    return 0;
+}
+
+extern "C" int rl_set_option(rl_ctx *c, int option, int value) {
+    if (!c) return RL_E_ARG;
+    switch (option) {
+        case RL_OPT_GRAPH: c->use_graph = value != 0; return 0;
+        case RL_OPT_AUTOTUNE: c->autotune = value != 0; if (!value) c->tune.clear(); return 0;
+        case RL_OPT_TILE:
+            c->tune.clear();
+            c->autotune = false;
+            c->fixed_cw = value;
+            return 0;
+        default: return fail(c, RL_E_ARG, "rl_set_option: unknown option %d", option);
+    }
+}
+
+extern "C" int rl_last_batch_info(const rl_ctx *c, int *batches, int *cw, int *blocks) {
+    if (!c) return RL_E_ARG;
+    if (batches) *batches = c->last_batches;
+    if (cw) *cw = c->last_cw;
+    if (blocks) *blocks = c->last_blocks;
     return 0;
 }
 

@@ -57,6 +57,46 @@ struct ItemLaunch {
     hipEvent_t ev_nodes, ev_formal;
 };
 
+// One item of a batched forward trace (device array, one per item).
+struct ItemDev {
+    int nrays, nfr, nrings, nphi, skip_mode;
+    int cslot;                 // cell table slot of the item's line in the batch
+    const long long *off;      // node CSR of the item's inclination
+    const GeoNode *geo;
+    long long rec0;            // first node record
+    long long img0;            // first image value ([ray][channel])
+    long long chan0;           // dnu at chan0, velo at chan0+nfr, istar at chan0+2nfr
+    long long part0;           // partial flux [ring][channel]
+    long long flux0;
+    int rinfo0;
+    LineConst lc;
+    double aks2;
+    const double *ring_w;
+    double center_w;
+};
+
+struct BatchLaunch {
+    int nitems, max_rays, max_nfr, max_rings, ncell, any_skip;
+    const ItemDev *items;      // device
+    const CellRec *cells;      // [slots][ncell]
+    void *rec;
+    RayInfo *rinfo;
+    const double *chan;
+    double *image, *partial, *flux;
+    double d2;
+    const long long *tile0;    // device [nitems+1] first tile of each item
+    long long ntiles;
+    unsigned long long *ovs, *counter;
+    int *maser;
+};
+
+void launch_batch_nodes(hipStream_t st, int fp64, const BatchLaunch &b);
+void launch_batch_formal(hipStream_t st, int fp64, const BatchLaunch &b, int cw, int nblocks);
+void launch_batch_finish(hipStream_t st, int fp64, const BatchLaunch &b);
+void launch_batch_flux(hipStream_t st, const BatchLaunch &b);
+void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
+int formal_batch_blocks_per_cu(int fp64);
+
 // One item of the adjoint (scaled records of precision T, all channels).
 struct AdjLaunch {
     int nrays, nfr, nrings, nphi;

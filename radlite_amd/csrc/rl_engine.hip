@@ -253,15 +253,15 @@ __device__ inline void finish_window(int ray, int lane, double umin, double umax
     }
 }
 
-// FP64 parity path: node values as the reference holds them
-__global__ void __launch_bounds__(64)
-k_nodes(int nrays, const long long *__restrict__ off, const GeoNode *__restrict__ geo,
-        const CellRec *__restrict__ cells, NodeRec<double> *__restrict__ rec,
-        RayInfo *__restrict__ rinfo, int nfr, const double *__restrict__ velo,
-        double aks2, int skip_mode) {
-    const int ray = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (ray >= nrays) return;
+// FP64 parity path: node values as the reference holds them (one wave
+// per ray)
+__device__ inline void nodes_ray_exact(int ray, int lane, const long long *__restrict__ off,
+                                       const GeoNode *__restrict__ geo,
+                                       const CellRec *__restrict__ cells,
+                                       NodeRec<double> *__restrict__ rec,
+                                       RayInfo *__restrict__ rinfo, int nfr,
+                                       const double *__restrict__ velo, double aks2,
+                                       int skip_mode) {
     const long long b = off[ray];
     const int n = (int)(off[ray + 1] - b);
     double umin = 1.0e300, umax = -1.0e300;
@@ -283,6 +283,16 @@ k_nodes(int nrays, const long long *__restrict__ off, const GeoNode *__restrict_
     finish_window(ray, lane, umin, umax, nfr, velo, aks2, skip_mode, n, rinfo);
 }
 
+__global__ void __launch_bounds__(64)
+k_nodes(int nrays, const long long *__restrict__ off, const GeoNode *__restrict__ geo,
+        const CellRec *__restrict__ cells, NodeRec<double> *__restrict__ rec,
+        RayInfo *__restrict__ rinfo, int nfr, const double *__restrict__ velo,
+        double aks2, int skip_mode) {
+    if ((int)blockIdx.x >= nrays) return;
+    nodes_ray_exact(blockIdx.x, threadIdx.x, off, geo, cells, rec, rinfo, nfr, velo, aks2,
+                    skip_mode);
+}
+
 // per-node scale for the FP32 records: the dust opacity where there is
 // dust, else the largest line coefficient, else 1
 __device__ inline void scaled_node(const double v[5], const LineConst &lc, double &sig,
@@ -297,14 +307,13 @@ __device__ inline void scaled_node(const double v[5], const LineConst &lc, doubl
 // scaled records (FP32 performance path; both precisions of the adjoint):
 // channel-independent products precomputed
 template <typename T>
-__global__ void __launch_bounds__(64)
-k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__restrict__ geo,
-             const CellRec *__restrict__ cells, FastRecT<T> *__restrict__ rec,
-             RayInfo *__restrict__ rinfo, int nfr, const double *__restrict__ velo,
-             double aks2, int skip_mode, LineConst lc) {
-    const int ray = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (ray >= nrays) return;
+__device__ inline void nodes_ray_fast(int ray, int lane, const long long *__restrict__ off,
+                                      const GeoNode *__restrict__ geo,
+                                      const CellRec *__restrict__ cells,
+                                      FastRecT<T> *__restrict__ rec,
+                                      RayInfo *__restrict__ rinfo, int nfr,
+                                      const double *__restrict__ velo, double aks2,
+                                      int skip_mode, const LineConst &lc) {
     const long long b = off[ray];
     const int n = (int)(off[ray + 1] - b);
     double umin = 1.0e300, umax = -1.0e300;
@@ -352,6 +361,17 @@ k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__rest
         }
     }
     finish_window(ray, lane, umin, umax, nfr, velo, aks2, skip_mode, n, rinfo);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__restrict__ geo,
+             const CellRec *__restrict__ cells, FastRecT<T> *__restrict__ rec,
+             RayInfo *__restrict__ rinfo, int nfr, const double *__restrict__ velo,
+             double aks2, int skip_mode, LineConst lc) {
+    if ((int)blockIdx.x >= nrays) return;
+    nodes_ray_fast<T>(blockIdx.x, threadIdx.x, off, geo, cells, rec, rinfo, nfr, velo, aks2,
+                      skip_mode, lc);
 }
 
 // ---------------------------------------------------------------------
@@ -412,6 +432,28 @@ __device__ inline double trace_dispatch<float>(const FastRec *rec, long long b, 
     return (double)trace_fast<float>(rec + b, n, L, (float)lc.I0, (float)istar, sf, ovs, maser);
 }
 
+// one (ray, channel) of make_image_circular's loop (telescope.F:550-613):
+// integrated, or marked for the lazy-continuum pass when NONREDUNDANT
+// skipping leaves it out of the ray's window
+template <typename T>
+__device__ inline double formal_one(int ray, int ch, const long long *__restrict__ off,
+                                    const typename RecOf<T>::type *__restrict__ rec,
+                                    const RayInfo *__restrict__ rinfo,
+                                    const double *__restrict__ dnu,
+                                    const double *__restrict__ velo,
+                                    const double *__restrict__ istar, const LineConst &lc,
+                                    double aks2, int skip_mode, int &ovs, int &maser) {
+    const RayInfo ri = rinfo[ray];
+    bool doit = true;
+    if (skip_mode == 0 && ray > 0 && ch > 0 && ch != ri.cont) {
+        doit = velo[ch] <= (double)ri.mx + aks2 && velo[ch] >= (double)ri.mn - aks2;
+    }
+    if (!doit) return __longlong_as_double(0x7ff8dead00000000ll);   // filled by k_fixup
+    T sf = ray == 0 ? (T)lc.starfract : T(0);
+    return trace_dispatch<T>(rec, off[ray], ri.n, lc, dnu[ch], velo[ch], istar[ch], sf, ovs,
+                             maser);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_formal(int nrays, int nfr, const long long *__restrict__ off,
@@ -423,20 +465,8 @@ k_formal(int nrays, int nfr, const long long *__restrict__ off,
     int ovs = 0, maser = 0;
     if (g < (long long)nrays * nfr) {
         const int ray = (int)(g / nfr), ch = (int)(g - (long long)ray * nfr);
-        const RayInfo ri = rinfo[ray];
-        bool doit = true;
-        if (skip_mode == 0 && ray > 0 && ch > 0 && ch != ri.cont) {
-            doit = velo[ch] <= (double)ri.mx + aks2 && velo[ch] >= (double)ri.mn - aks2;
-        }
-        double out;
-        if (doit) {
-            T sf = ray == 0 ? (T)lc.starfract : T(0);
-            out = trace_dispatch<T>(rec, off[ray], ri.n, lc, dnu[ch], velo[ch], istar[ch], sf,
-                                    ovs, maser);
-        } else {
-            out = __longlong_as_double(0x7ff8dead00000000ll);   // filled by k_fixup
-        }
-        image[g] = out;
+        image[g] = formal_one<T>(ray, ch, off, rec, rinfo, dnu, velo, istar, lc, aks2,
+                                 skip_mode, ovs, maser);
     }
     // wave-level pre-reduction of the warning counters
     for (int m = 32; m >= 1; m >>= 1) {
@@ -572,6 +602,223 @@ void launch_item(hipStream_t st, const ItemLaunch &it) {
                        it.rings_per_chunk, it.image, it.ring_w, it.center_w, it.partial);
     hipLaunchKernelGGL(k_flux_final, dim3((it.nfr + 63) / 64), dim3(64), 0, st, it.nfr,
                        it.nchunks, it.partial, it.d2, it.flux);
+}
+
+// ---------------------------------------------------------------------
+// batched items: one launch per stage for many (line, inclination)
+// items with uneven channel counts
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_nodes_batch(const ItemDev *__restrict__ items, const CellRec *__restrict__ cells_all,
+              int ncell, void *rec_all, RayInfo *__restrict__ rinfo_all,
+              const double *__restrict__ chan_all) {
+    const ItemDev &it = items[blockIdx.y];
+    const int ray = blockIdx.x;
+    if (ray >= it.nrays) return;
+    const CellRec *cells = cells_all + (size_t)it.cslot * ncell;
+    const double *velo = chan_all + it.chan0 + it.nfr;
+    if constexpr (std::is_same<T, double>::value)
+        nodes_ray_exact(ray, threadIdx.x, it.off, it.geo, cells,
+                        reinterpret_cast<NodeRec<double> *>(rec_all) + it.rec0,
+                        rinfo_all + it.rinfo0, it.nfr, velo, it.aks2, it.skip_mode);
+    else
+        nodes_ray_fast<float>(ray, threadIdx.x, it.off, it.geo, cells,
+                              reinterpret_cast<FastRec *>(rec_all) + it.rec0,
+                              rinfo_all + it.rinfo0, it.nfr, velo, it.aks2, it.skip_mode,
+                              it.lc);
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// Lane -> (ray, channel) of a tile.  cw == 0: the item's (ray, channel)
+// pairs in ray-major order, 64 per tile (a wave may span several rays);
+// cw in {16, 32, 64}: 64/cw rays per wave, cw consecutive channels each.
+__device__ inline bool tile_lane(int cw, long long lt, int lane, int nrays, int nfr, int &ray,
+                                 int &ch) {
+    if (cw == 0) {
+        const long long g = lt * 64 + lane;
+        if (g >= (long long)nrays * nfr) return false;
+        ray = (int)(g / nfr);
+        ch = (int)(g - (long long)ray * nfr);
+        return true;
+    }
+    const int chunks = (nfr + cw - 1) / cw, rows = 64 / cw;
+    const long long rb = lt / chunks;
+    const int cc = (int)(lt - rb * chunks);
+    ray = (int)(rb * rows) + lane / cw;
+    ch = cc * cw + (lane % cw);
+    return ray < nrays && ch < nfr;
+}
+
+// Persistent formal solution over all tiles of all items: each wave takes
+// the next tile from a device counter until every tile is taken, so the
+// grid drains by itself; items of any size share the machine.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_formal_batch(const ItemDev *__restrict__ items, int nitems,
+               const long long *__restrict__ tile0, long long ntiles, int cw,
+               const void *rec_all, const RayInfo *__restrict__ rinfo_all,
+               const double *__restrict__ chan_all, double *__restrict__ image_all,
+               unsigned long long *__restrict__ ovs_all, int *__restrict__ maser_all,
+               unsigned long long *counter) {
+    const auto *rec_base = reinterpret_cast<const typename RecOf<T>::type *>(rec_all);
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        // Every lane executes the atomic (lane 0 adds 1, the others 0) and the
+        // tile index is lane 0's result through readfirstlane, so it is
+        // wave-uniform (an SGPR) and the loop exit is a scalar branch.  A
+        // lane-0-only atomic broadcast with a shuffle leaves the exit
+        // condition divergent to the compiler, which may then iterate the
+        // other lanes without lane 0.
+        const unsigned long long mine = atomicAdd(counter, lane == 0 ? 1ull : 0ull);
+        const unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)(mine & 0xffffffffull));
+        const unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(mine >> 32));
+        const unsigned long long t = ((unsigned long long)hi32 << 32) | lo32;
+        if (t >= (unsigned long long)ntiles) break;
+        // item of the tile: last item with tile0[item] <= t
+        int lo = 0, hi = nitems;
+        while (hi - lo > 1) {
+            const int m = (lo + hi) >> 1;
+            if (tile0[m] <= (long long)t) lo = m; else hi = m;
+        }
+        const ItemDev &it = items[lo];
+        int ray = 0, ch = 0, ovs = 0, maser = 0;
+        if (tile_lane(cw, (long long)t - tile0[lo], lane, it.nrays, it.nfr, ray, ch)) {
+            const double *dnu = chan_all + it.chan0;
+            image_all[it.img0 + (long long)ray * it.nfr + ch] =
+                formal_one<T>(ray, ch, it.off, rec_base + it.rec0, rinfo_all + it.rinfo0, dnu,
+                              dnu + it.nfr, dnu + 2 * it.nfr, it.lc, it.aks2, it.skip_mode, ovs,
+                              maser);
+        }
+        for (int m = 32; m >= 1; m >>= 1) {
+            ovs += __shfl_xor(ovs, m);
+            maser |= __shfl_xor(maser, m);
+        }
+        if (lane == 0) {
+            if (ovs) atomicAdd(ovs_all + lo, (unsigned long long)ovs);
+            if (maser) atomicOr(maser_all + lo, 1);
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_fixup_batch(const ItemDev *__restrict__ items, const void *rec_all,
+                              const RayInfo *__restrict__ rinfo_all,
+                              const double *__restrict__ chan_all, double *__restrict__ image_all,
+                              unsigned long long *__restrict__ ovs_all,
+                              int *__restrict__ maser_all) {
+    const int item = blockIdx.y;
+    const ItemDev &it = items[item];
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it.skip_mode != 0 || ray == 0 || ray >= it.nrays) return;
+    const RayInfo ri = rinfo_all[it.rinfo0 + ray];
+    if (ri.cont < 0) return;
+    const auto *rec = reinterpret_cast<const typename RecOf<T>::type *>(rec_all) + it.rec0;
+    const double *dnu = chan_all + it.chan0, *velo = dnu + it.nfr, *istar = dnu + 2 * it.nfr;
+    double *row = image_all + it.img0 + (size_t)ray * it.nfr;
+    double cont = row[ri.cont];
+    int ovs = 0, maser = 0;
+    for (int ch = ri.cont + 1; ch < it.nfr; ch++) {
+        bool inwin = velo[ch] <= (double)ri.mx + it.aks2 && velo[ch] >= (double)ri.mn - it.aks2;
+        if (inwin) continue;
+        if (cont != 0.0) {
+            row[ch] = cont;
+        } else {
+            row[ch] = trace_dispatch<T>(rec, it.off[ray], ri.n, it.lc, dnu[ch], velo[ch],
+                                        istar[ch], T(0), ovs, maser);
+            cont = row[ch];
+        }
+    }
+    if (ovs) atomicAdd(ovs_all + item, (unsigned long long)ovs);
+    if (maser) atomicOr(maser_all + item, 1);
+}
+
+__global__ void k_flux_partial_batch(const ItemDev *__restrict__ items,
+                                     const double *__restrict__ image_all,
+                                     double *__restrict__ partial_all) {
+    const ItemDev &it = items[blockIdx.z];
+    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ir = blockIdx.y;
+    if (ch >= it.nfr || ir >= it.nrings) return;
+    const double *image = image_all + it.img0;
+    double acc = 0.0;
+    if (ir == 0) acc = it.center_w * image[ch];
+    const double *row = image + (size_t)(1 + ir * it.nphi) * it.nfr + ch;
+    double s = 0.0;
+    for (int ip = 0; ip < it.nphi; ip++) s = s + row[(size_t)ip * it.nfr];
+    s = s / (1.0 * it.nphi);
+    acc = acc + s * it.ring_w[ir];
+    partial_all[it.part0 + (size_t)ir * it.nfr + ch] = acc;
+}
+
+__global__ void k_flux_final_batch(const ItemDev *__restrict__ items,
+                                   const double *__restrict__ partial_all, double d2,
+                                   double *__restrict__ flux_all) {
+    const ItemDev &it = items[blockIdx.y];
+    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ch >= it.nfr) return;
+    double s = 0.0;
+    for (int k = 0; k < it.nrings; k++) s = s + partial_all[it.part0 + (size_t)k * it.nfr + ch];
+    flux_all[it.flux0 + ch] = s / d2;
+}
+
+void launch_batch_nodes(hipStream_t st, int fp64, const BatchLaunch &b) {
+    dim3 g(b.max_rays, b.nitems);
+    if (fp64)
+        hipLaunchKernelGGL(k_nodes_batch<double>, g, dim3(64), 0, st, b.items, b.cells, b.ncell,
+                           b.rec, b.rinfo, b.chan);
+    else
+        hipLaunchKernelGGL(k_nodes_batch<float>, g, dim3(64), 0, st, b.items, b.cells, b.ncell,
+                           b.rec, b.rinfo, b.chan);
+}
+
+void launch_batch_formal(hipStream_t st, int fp64, const BatchLaunch &b, int cw, int nblocks) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, b.counter, 1);
+    if (fp64)
+        hipLaunchKernelGGL(k_formal_batch<double>, dim3(nblocks), dim3(256), 0, st, b.items,
+                           b.nitems, b.tile0, b.ntiles, cw, b.rec, b.rinfo, b.chan, b.image,
+                           b.ovs, b.maser, b.counter);
+    else
+        hipLaunchKernelGGL(k_formal_batch<float>, dim3(nblocks), dim3(256), 0, st, b.items,
+                           b.nitems, b.tile0, b.ntiles, cw, b.rec, b.rinfo, b.chan, b.image,
+                           b.ovs, b.maser, b.counter);
+}
+
+void launch_batch_finish(hipStream_t st, int fp64, const BatchLaunch &b) {
+    if (b.any_skip) {
+        dim3 g((b.max_rays + 127) / 128, b.nitems);
+        if (fp64)
+            hipLaunchKernelGGL(k_fixup_batch<double>, g, dim3(128), 0, st, b.items, b.rec,
+                               b.rinfo, b.chan, b.image, b.ovs, b.maser);
+        else
+            hipLaunchKernelGGL(k_fixup_batch<float>, g, dim3(128), 0, st, b.items, b.rec,
+                               b.rinfo, b.chan, b.image, b.ovs, b.maser);
+    }
+}
+
+void launch_batch_flux(hipStream_t st, const BatchLaunch &b) {
+    dim3 gp((b.max_nfr + 63) / 64, b.max_rings, b.nitems);
+    hipLaunchKernelGGL(k_flux_partial_batch, gp, dim3(64), 0, st, b.items, b.image, b.partial);
+    dim3 gf((b.max_nfr + 63) / 64, b.nitems);
+    hipLaunchKernelGGL(k_flux_final_batch, gf, dim3(64), 0, st, b.items, b.partial, b.d2,
+                       b.flux);
+}
+
+void launch_zero_u64(hipStream_t st, unsigned long long *p, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+}
+
+int formal_batch_blocks_per_cu(int fp64) {
+    int nb = 0;
+    if (fp64)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_formal_batch<double>, 256, 0);
+    else
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_formal_batch<float>, 256, 0);
+    return nb > 0 ? nb : 1;
 }
 
 // ---------------------------------------------------------------------

@@ -28,6 +28,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 RL_FP64, RL_FP32 = 0, 1
+OPT_GRAPH, OPT_AUTOTUNE, OPT_TILE = 1, 2, 3
 SKIP_REFERENCE, SKIP_NONE = 0, 1
 WARN_MASER, WARN_OVERSHOOT = 1, 2
 
@@ -71,6 +72,9 @@ def load_library(path=LIB_PATH):
         "rl_trace_lines_adjoint": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
                                              _dp, _dp, _dp, _dp, _dp, _dp]),
         "rl_sync": (C.c_int, [vp]),
+        "rl_set_memory_budget": (C.c_int, [vp, C.c_double]),
+        "rl_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
+        "rl_last_batch_info": (C.c_int, [vp, _ip, _ip, _ip]),
         "rl_last_stats": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong)]),
         "rl_prepare_line_freqs": (C.c_int, [C.c_int, _dp, _ip, _ip, _dp]),
         "rl_prepare_line_dust": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int,
@@ -315,6 +319,20 @@ class Raytracer:
             fl.append(flux[o:o + n])
             o += n
         return fl, out
+
+    def set_memory_budget(self, nbytes):
+        """Device bytes a batch of items may use (0: 60% of free memory)."""
+        self._chk(self.L.rl_set_memory_budget(self.ctx, float(nbytes)), "rl_set_memory_budget")
+
+    def set_option(self, option, value):
+        self._chk(self.L.rl_set_option(self.ctx, int(option), int(value)), "rl_set_option")
+
+    def batch_info(self):
+        """(batches of the last forward call, tile mapping, persistent blocks)."""
+        v = [C.c_int(0) for _ in range(3)]
+        self._chk(self.L.rl_last_batch_info(self.ctx, *[C.byref(x) for x in v]),
+                  "rl_last_batch_info")
+        return tuple(x.value for x in v)
 
     def stats(self):
         t = np.zeros(5)

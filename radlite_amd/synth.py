@@ -216,3 +216,28 @@ def workload_inputs(name, seed=0, **over):
     ri = run_inputs(disk, lines, pops, nphi=w["nphi"], dbdr=w["dbdr"], b_extra=w["b_extra"],
                     vmax=vmax, dv=dv, incl_deg=w["incl"][0])
     return ri, w
+
+
+def c3_inputs(seed=0, nlines=64, incl=(10.0, 20.0, 30.0, 40.0, 50.0, 60.0, 70.0, 80.0),
+              nr=256, nth=128, nphi=64, vmax=30.0, nfr_range=(20, 400)):
+    """The batched workload: nr x nth disk, ``nlines`` random lines (lambda0
+    in 4.5-5.0 um, log10 A in [0, 2], E_up in 100-6000 K, g = 2J+1, 2J-1
+    for J in 1..40), populations Boltzmann-then-renormalised, one channel
+    count per line drawn from ``nfr_range`` (inclusive) over +-vmax.
+    Returns (RunInputs, items) with items (line, incl index, nfr, vmax)
+    for every line at every inclination."""
+    rng = np.random.default_rng(seed)
+    disk = make_disk(nr, nth, seed=seed, sigma_t=0.1)
+    lines = random_lines(nlines, seed=seed + 1)
+    pops = lte_pops(disk, lines, q_mode="renorm")
+    nfrs = rng.integers(nfr_range[0], nfr_range[1] + 1, size=nlines)
+    v0, dv0 = passband(vmax, int(nfrs[0]))
+    ri = run_inputs(disk, lines, pops, nphi=nphi, dbdr=1, b_extra=-1, vmax=v0, dv=dv0,
+                    incl_deg=incl[0])
+    items = []
+    for ii in range(len(incl)):
+        for il in range(nlines):
+            v, dv = passband(vmax, int(nfrs[il]))
+            n = int(2.0 * v / dv + 1)
+            items.append((il, ii, n, 0.5 * (n - 1) * dv))
+    return ri, items, list(incl)

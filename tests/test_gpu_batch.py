@@ -1,0 +1,157 @@
+"""Batched forward traces: many (line, inclination) items with uneven
+channel counts in one call.
+
+* A batch gives bit-identical spectra and images to one call per item
+  (same device arithmetic per (ray, channel); same flux summation order).
+* Every tile mapping (ray-major pairs, 16/32/64 channels per ray in a
+  wave) and a memory budget that forces one item per batch give the same
+  bits; graph replay gives the same bits and follows changed inputs.
+* The 256x128 disk with 64 random lines (20-400 channels each) at 8
+  inclinations: the batched FP32 trace against batched FP64 on every
+  (line, inclination) item, <= 1e-4 on the line-integrated flux, and
+  FP64 against the oracle on sampled items.
+"""
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import pyorc
+from cases import build_case, input_hash, load_golden
+from radlite_amd import engine, l4io, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _t6():
+    g = load_golden("t6")
+    with tempfile.TemporaryDirectory() as d:
+        build_case("t6", d)
+        assert str(g["input_sha256"]) == input_hash(d)
+        return l4io.read_run_dir(d)
+
+
+INCS = [35.0 * 0.0174532925199, 62.0 * 0.0174532925199, 81.0 * 0.0174532925199]
+
+
+def _items(ri):
+    nfr, vmax = engine.passband(ri.vmax, ri.dv)
+    out = []
+    for ii in range(len(INCS)):
+        for il in range(3):
+            sk = engine.SKIP_REFERENCE if (il + ii) % 2 == 0 else engine.SKIP_NONE
+            out.append((il, ii, nfr - 4 * il + ii, vmax * (1 + 0.1 * il), sk))
+    return out
+
+
+def _rt(ri, precision):
+    rt = engine.Raytracer(0, precision)
+    rt.load_run(ri)
+    rt.setup_rays(INCS, ri.nphi, ri.b_extra, ri.dbdr)
+    return rt
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_equals_single_items(precision):
+    ri = _t6()
+    say(f"[gpu] batch vs single items {precision}")
+    items = _items(ri)
+    rt = _rt(ri, precision)
+    fb, ib, wb = rt.trace(items, want_image=True)
+    for k, it in enumerate(items):
+        f1, i1, w1 = rt.trace([it], want_image=True)
+        assert np.array_equal(f1[0], fb[k]), k
+        assert np.array_equal(i1[0], ib[k]), k
+        assert w1[0] == wb[k], k
+    rt.close()
+
+
+def test_batch_modes_are_bit_identical():
+    ri = _t6()
+    say("[gpu] tile mappings, memory budget, graph replay")
+    items = _items(ri)
+    rt = _rt(ri, "fp32")
+    ref, refi, refw = rt.trace(items, want_image=True)
+    for cw in (0, 16, 32, 64):
+        rt.set_option(engine.OPT_TILE, cw)
+        f, im, w = rt.trace(items, want_image=True)
+        assert rt.batch_info()[1] == cw
+        for k in range(len(items)):
+            assert np.array_equal(f[k], ref[k]) and np.array_equal(im[k], refi[k]), (cw, k)
+        assert np.array_equal(w, refw)
+    rt.set_option(engine.OPT_AUTOTUNE, 1)
+    rt.set_memory_budget(1)                      # one item per batch
+    f, im, w = rt.trace(items, want_image=True)
+    assert rt.batch_info()[0] == len(items)
+    for k in range(len(items)):
+        assert np.array_equal(f[k], ref[k]) and np.array_equal(im[k], refi[k])
+    rt.set_memory_budget(0)
+    # graph replay (second call with the same shape) and a changed input
+    f2, _, _ = rt.trace(items)
+    f3, _, _ = rt.trace(items)
+    for k in range(len(items)):
+        assert np.array_equal(f2[k], ref[k]) and np.array_equal(f3[k], ref[k])
+    pops2 = ri.pops * 1.3
+    rt.set_levels(pops2)
+    fg, _, _ = rt.trace(items)                  # replayed graph reads the new table
+    rt.set_option(engine.OPT_GRAPH, 0)
+    fd, _, _ = rt.trace(items)                  # direct launches
+    for k in range(len(items)):
+        assert np.array_equal(fg[k], fd[k])
+        assert not np.array_equal(fg[k], ref[k])
+    rt.close()
+
+
+def _line_integral(f):
+    return float(np.sum(f - 0.5 * (f[0] + f[-1])))
+
+
+def test_c3_batched_fp32_against_fp64_every_item():
+    say("[gpu] c3: 64 lines x 8 inclinations")
+    ri, items, incl = synth.c3_inputs(0)
+    incs = [x * 0.0174532925199 for x in incl]
+    its = [(il, ii, n, v, engine.SKIP_NONE) for (il, ii, n, v) in items]
+    res = {}
+    for prec in ("fp64", "fp32"):
+        rt = engine.Raytracer(0, prec)
+        rt.load_run(ri)
+        rt.setup_rays(incs, ri.nphi, ri.b_extra, ri.dbdr)
+        assert rt.nrays == [17025] * 8
+        res[prec], _, _ = rt.trace(its)
+        say(f"    {prec}: batches {rt.batch_info()}, stats {rt.stats()['t_total_ms']:.1f} ms")
+        rt.close()
+    worst_li = worst_ch = 0.0
+    for k in range(len(its)):
+        a, b = res["fp32"][k], res["fp64"][k]
+        li = abs(_line_integral(a) - _line_integral(b)) / abs(_line_integral(b))
+        ch = float(np.max(np.abs(a - b) / np.abs(b)))
+        worst_li, worst_ch = max(worst_li, li), max(worst_ch, ch)
+        assert li <= 1e-4, (k, its[k], li)
+        assert ch <= 1e-4, (k, its[k], ch)
+    say(f"    fp32 vs fp64 over {len(its)} items: line integral {worst_li:.2e}, "
+        f"channel {worst_ch:.2e}")
+    # FP64 against the oracle on sampled items: per-ray intensities of every
+    # 13th ray (the spectra need every ray, which the CPU oracle takes
+    # minutes for); per-ray tolerance as in the single-item tests
+    orc = pyorc.Oracle(pyorc.Prepared(ri))
+    rt = engine.Raytracer(0, "fp64")
+    rt.load_run(ri)
+    rt.setup_rays(incs, ri.nphi, ri.b_extra, ri.dbdr)
+    for k in range(0, len(its), 128):
+        il, ii, n, v, _ = its[k]
+        f, im, _ = rt.trace([its[k]], want_image=True)
+        assert np.array_equal(f[0], res["fp64"][k])
+        o = orc.line_spectrum(il, incl_rad=incs[ii], nfr=n, vmax=v, skip_mode=1,
+                              ray_stride=13)
+        rows = np.arange(0, o["nrays"], 13)
+        scale = np.abs(o["image"][rows]).max()
+        rel = np.abs(im[0][rows] - o["image"][rows]) / np.maximum(np.abs(o["image"][rows]),
+                                                                 1e-12 * scale)
+        say(f"    item {k} {its[k][:3]}: sampled-ray max rel vs oracle {rel.max():.2e}")
+        assert rel.max() <= 1e-8, (k, its[k], rel.max())
+    rt.close()
