@@ -93,6 +93,46 @@ def bench_adjoint(args):
     }
 
 
+def bench_batched(args):
+    """The batched workload: 256x128 disk, 64 random lines with 20-400
+    channels each, 8 inclinations (512 items, 17,025 rays each); one step
+    = ray sets and trajectories for the 8 inclinations + every item's
+    spectrum in one rl_trace_lines_forward call (batches sized to HBM,
+    persistent formal kernel, graph replay, autotuned tiles)."""
+    from radlite_amd import engine, synth
+    ri, items, incl = synth.c3_inputs(0)
+    incs = [x * 0.0174532925199 for x in incl]
+    its = [(il, ii, n, v, engine.SKIP_NONE) for (il, ii, n, v) in items]
+    rt = engine.Raytracer(0, args.precision)
+    rt.load_run(ri)
+
+    def step():
+        rt.setup_rays(incs, ri.nphi, ri.b_extra, ri.dbdr)
+        rt.trace(its)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    rt.L.rl_sync(rt.ctx)
+    steps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    rt.L.rl_sync(rt.ctx)
+    dt = (time.perf_counter() - t0) / steps
+    nominal = int(sum(int(rt.nrays[ii]) * n for (_, ii, n, _, _) in its))
+    nb, cw, blocks = rt.batch_info()
+    st = rt.stats()
+    rt.close()
+    return {
+        "workload": f"c3: nr=256 x nth=128, 64 lines x 8 incl, 17025 rays/incl, "
+                    f"{sum(n for (_, _, n, _, _) in its) // 8} channels/incl (20-400 per line)",
+        "precision": args.precision, "items": len(its), "steps": steps,
+        "ms_per_step": dt * 1e3, "value": nominal / dt, "unit": "ray*channel/s",
+        "nominal_ray_channels": nominal, "batches": nb, "tile_cw": cw,
+        "persistent_blocks": blocks, "t_traj_ms": st["t_traj_ms"],
+    }
+
+
 def copy_probe(nbytes):
     """A plain device-to-device copy of known size: what FETCH_SIZE and
     WRITE_SIZE read for it calibrates the counters against true bytes."""
@@ -121,6 +161,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+adjoint measurement on the 200x100 disk")
     ap.add_argument("--adjoint-only", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 64-line x 8-inclination batched measurement")
+    ap.add_argument("--batched-only", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="run only a plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -135,6 +178,9 @@ def main():
 
     if args.adjoint_only:
         print(json.dumps(bench_adjoint(args)))
+        return
+    if args.batched_only:
+        print(json.dumps(bench_batched(args)))
         return
     from radlite_amd import engine, synth
     ri, w = synth.workload_inputs(args.workload, seed=0)
@@ -215,6 +261,8 @@ def main():
 
     if not args.no_adjoint:
         out["fwd_adjoint"] = bench_adjoint(args)
+    if not args.no_batched:
+        out["batched"] = bench_batched(args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

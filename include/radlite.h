@@ -128,21 +128,25 @@ int rl_trace_lines_adjoint(rl_ctx *ctx, int nitems, const rl_item *items,
  * the whole batch, the formal solution as a persistent kernel over a
  * device work counter.  Batches are cut so that their device memory
  * stays within the budget: bytes > 0 sets it, 0 (the default) uses 60%
- * of the free device memory. */
+ * of the device memory free at the context's first trace. */
 int rl_set_memory_budget(rl_ctx *ctx, double bytes);
 
 enum {
     RL_OPT_GRAPH = 1,     /* 1 (default): record each batch shape's launches in a
                              hipGraph and replay it; 0: launch directly */
-    RL_OPT_AUTOTUNE = 2,  /* 1 (default): time the tile mappings once per batch
-                             shape and keep the fastest; 0: default mapping */
-    RL_OPT_TILE = 3       /* fix the tile mapping: 0 = ray-major (ray, channel)
-                             pairs, 16/32/64 = channels per ray in a wave */
+    RL_OPT_AUTOTUNE = 2,  /* 1 (default): time the tile mappings and the grid
+                             (static, or persistent over a work counter) once per
+                             batch shape and keep the fastest; 0: defaults */
+    RL_OPT_TILE = 3,      /* fix the tile mapping (turns the autotuner off):
+                             0 = ray-major (ray, channel) pairs, 16/32/64 =
+                             channels per ray in a wave */
+    RL_OPT_PERSISTENT = 4 /* with a fixed mapping: 1 (default) persistent grid
+                             over the work counter, 0 static grid */
 };
 int rl_set_option(rl_ctx *ctx, int option, int value);
 
-/* Batches of the last forward call and the tile mapping / persistent
- * grid size the last batch used. */
+/* Batches of the last forward call, and the tile mapping and grid the
+ * last batch used (blocks = 0: static grid; > 0: persistent blocks). */
 int rl_last_batch_info(const rl_ctx *ctx, int *batches, int *tile_cw, int *blocks);
 
 /* Wait for all work of the context. */

@@ -101,8 +101,9 @@ struct rl_ctx {
     std::vector<rl::ItemDev> h_items;
     std::vector<long long> h_tile0;
     size_t mem_budget = 0;            // 0: a share of the free device memory
+    size_t auto_budget = 0;
     bool use_graph = true, autotune = true;
-    int fixed_cw = -1, last_cw = 0, last_blocks = 0, last_batches = 0;
+    int fixed_cw = -1, fixed_persistent = 1, last_cw = 0, last_blocks = 0, last_batches = 0;
     std::map<std::string, std::pair<int, int>> tune;
     std::map<std::string, hipGraphExec_t> graphs;
     // adjoint buffers
@@ -725,27 +726,31 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
     auto tuned = c->tune.find(shape);
     if (c->fixed_cw >= 0) {
         cw = c->fixed_cw;
+        nblk = c->fixed_persistent ? dev_cu * bpc : 0;
     } else if (tuned != c->tune.end()) {
         cw = tuned->second.first;
         nblk = tuned->second.second;
     } else if (c->autotune) {
+        // trials on the first tiles of the batch (enough to fill the device
+        // several times): static grid or persistent grid, x tile mapping
         cells();
         rl::launch_batch_nodes(c->stream, fp64, L);
         float best = 1e30f;
         const int cws[4] = {0, 64, 32, 16};
-        const int mult[2] = {1, 2};
+        const int grids[2] = {0, dev_cu * bpc};
         for (int ci = 0; ci < 4; ci++) {
             if (int e = set_tiles(cws[ci])) return e;
-            for (int mi = 0; mi < 2; mi++) {
-                const int blocks = dev_cu * bpc * mult[mi];
+            const long long ntr = std::min<long long>(L.ntiles, 64LL * 4 * dev_cu * bpc);
+            for (int gi = 0; gi < 2; gi++) {
                 zero_counters();
                 HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-                rl::launch_batch_formal(c->stream, fp64, L, cws[ci], blocks);
+                rl::launch_batch_formal(c->stream, fp64, L, cws[ci], grids[gi], ntr);
                 HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
                 HIPCHK(c, hipEventSynchronize(c->ev[3]));
                 float t = 0;
                 HIPCHK(c, hipEventElapsedTime(&t, c->ev[2], c->ev[3]));
-                if (t < best) { best = t; cw = cws[ci]; nblk = blocks; }
+                t = t / (float)ntr;            // per tile (tile counts differ by mapping)
+                if (t < best) { best = t; cw = cws[ci]; nblk = grids[gi]; }
             }
         }
         c->tune[shape] = {cw, nblk};
@@ -760,7 +765,7 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
         cells();
         rl::launch_batch_nodes(c->stream, fp64, L);
         if (timed) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-        rl::launch_batch_formal(c->stream, fp64, L, cw, nblk);
+        rl::launch_batch_formal(c->stream, fp64, L, cw, nblk, L.ntiles);
         rl::launch_batch_finish(c->stream, fp64, L);
         if (timed) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         rl::launch_batch_flux(c->stream, L);
@@ -872,11 +877,18 @@ extern "C" int rl_trace_lines_forward(rl_ctx *c, int nitems, const rl_item *item
     c->t_ms[0] = tt;
     c->t_ms[4] = tt;
     // batches under the memory budget
+    // the default budget is fixed at the context's first trace (60% of the
+    // device memory free then), so that repeated calls cut the same
+    // batches and reuse their tuning and graphs
     size_t budget = c->mem_budget;
     if (budget == 0) {
-        size_t fr = 0, tot = 0;
-        HIPCHK(c, hipMemGetInfo(&fr, &tot));
-        budget = (size_t)(0.6 * (double)fr) + c->d_rec.n + c->d_image.n + c->d_partial.n;
+        if (c->auto_budget == 0) {
+            size_t fr = 0, tot = 0;
+            HIPCHK(c, hipMemGetInfo(&fr, &tot));
+            c->auto_budget = (size_t)(0.6 * (double)fr) + c->d_rec.n + c->d_image.n +
+                             c->d_partial.n;
+        }
+        budget = c->auto_budget;
     }
     c->last_batches = 0;
     size_t fo = 0, io = 0;
@@ -913,9 +925,15 @@ extern "C" int rl_set_option(rl_ctx *c, int option, int value) {
         case RL_OPT_GRAPH: c->use_graph = value != 0; return 0;
         case RL_OPT_AUTOTUNE: c->autotune = value != 0; if (!value) c->tune.clear(); return 0;
         case RL_OPT_TILE:
+            if (value != 0 && value != 16 && value != 32 && value != 64)
+                return fail(c, RL_E_ARG, "rl_set_option: tile mapping must be 0, 16, 32 or 64");
             c->tune.clear();
             c->autotune = false;
             c->fixed_cw = value;
+            return 0;
+        case RL_OPT_PERSISTENT:
+            c->tune.clear();
+            c->fixed_persistent = value != 0;
             return 0;
         default: return fail(c, RL_E_ARG, "rl_set_option: unknown option %d", option);
     }

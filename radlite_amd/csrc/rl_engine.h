@@ -91,7 +91,10 @@ struct BatchLaunch {
 };
 
 void launch_batch_nodes(hipStream_t st, int fp64, const BatchLaunch &b);
-void launch_batch_formal(hipStream_t st, int fp64, const BatchLaunch &b, int cw, int nblocks);
+// nblocks == 0: static grid over the first ntiles tiles; > 0: persistent
+// grid of nblocks blocks taking tiles from the work counter
+void launch_batch_formal(hipStream_t st, int fp64, const BatchLaunch &b, int cw, int nblocks,
+                         long long ntiles);
 void launch_batch_finish(hipStream_t st, int fp64, const BatchLaunch &b);
 void launch_batch_flux(hipStream_t st, const BatchLaunch &b);
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);

@@ -705,6 +705,44 @@ k_formal_batch(const ItemDev *__restrict__ items, int nitems,
     }
 }
 
+// The same tiles on a static grid: wave w of block b takes tile 4b + w
+// (the hardware dispatcher balances blocks); fewer registers than the
+// persistent loop.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_formal_tiles(const ItemDev *__restrict__ items, int nitems,
+               const long long *__restrict__ tile0, long long ntiles, int cw,
+               const void *rec_all, const RayInfo *__restrict__ rinfo_all,
+               const double *__restrict__ chan_all, double *__restrict__ image_all,
+               unsigned long long *__restrict__ ovs_all, int *__restrict__ maser_all) {
+    const auto *rec_base = reinterpret_cast<const typename RecOf<T>::type *>(rec_all);
+    const int lane = threadIdx.x & 63;
+    const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ntiles) return;                       // whole waves
+    int lo = 0, hi = nitems;
+    while (hi - lo > 1) {
+        const int m = (lo + hi) >> 1;
+        if (tile0[m] <= t) lo = m; else hi = m;
+    }
+    const ItemDev &it = items[lo];
+    int ray = 0, ch = 0, ovs = 0, maser = 0;
+    if (tile_lane(cw, t - tile0[lo], lane, it.nrays, it.nfr, ray, ch)) {
+        const double *dnu = chan_all + it.chan0;
+        image_all[it.img0 + (long long)ray * it.nfr + ch] =
+            formal_one<T>(ray, ch, it.off, rec_base + it.rec0, rinfo_all + it.rinfo0, dnu,
+                          dnu + it.nfr, dnu + 2 * it.nfr, it.lc, it.aks2, it.skip_mode, ovs,
+                          maser);
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        ovs += __shfl_xor(ovs, m);
+        maser |= __shfl_xor(maser, m);
+    }
+    if (lane == 0) {
+        if (ovs) atomicAdd(ovs_all + lo, (unsigned long long)ovs);
+        if (maser) atomicOr(maser_all + lo, 1);
+    }
+}
+
 template <typename T>
 __global__ void k_fixup_batch(const ItemDev *__restrict__ items, const void *rec_all,
                               const RayInfo *__restrict__ rinfo_all,
@@ -776,15 +814,28 @@ void launch_batch_nodes(hipStream_t st, int fp64, const BatchLaunch &b) {
                            b.rec, b.rinfo, b.chan);
 }
 
-void launch_batch_formal(hipStream_t st, int fp64, const BatchLaunch &b, int cw, int nblocks) {
+void launch_batch_formal(hipStream_t st, int fp64, const BatchLaunch &b, int cw, int nblocks,
+                         long long ntiles) {
+    if (nblocks == 0) {           // static grid
+        const unsigned g = (unsigned)((ntiles + 3) / 4);
+        if (fp64)
+            hipLaunchKernelGGL(k_formal_tiles<double>, dim3(g), dim3(256), 0, st, b.items,
+                               b.nitems, b.tile0, ntiles, cw, b.rec, b.rinfo, b.chan, b.image,
+                               b.ovs, b.maser);
+        else
+            hipLaunchKernelGGL(k_formal_tiles<float>, dim3(g), dim3(256), 0, st, b.items,
+                               b.nitems, b.tile0, ntiles, cw, b.rec, b.rinfo, b.chan, b.image,
+                               b.ovs, b.maser);
+        return;
+    }
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, b.counter, 1);
     if (fp64)
         hipLaunchKernelGGL(k_formal_batch<double>, dim3(nblocks), dim3(256), 0, st, b.items,
-                           b.nitems, b.tile0, b.ntiles, cw, b.rec, b.rinfo, b.chan, b.image,
+                           b.nitems, b.tile0, ntiles, cw, b.rec, b.rinfo, b.chan, b.image,
                            b.ovs, b.maser, b.counter);
     else
         hipLaunchKernelGGL(k_formal_batch<float>, dim3(nblocks), dim3(256), 0, st, b.items,
-                           b.nitems, b.tile0, b.ntiles, cw, b.rec, b.rinfo, b.chan, b.image,
+                           b.nitems, b.tile0, ntiles, cw, b.rec, b.rinfo, b.chan, b.image,
                            b.ovs, b.maser, b.counter);
 }
 

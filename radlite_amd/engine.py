@@ -28,7 +28,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 RL_FP64, RL_FP32 = 0, 1
-OPT_GRAPH, OPT_AUTOTUNE, OPT_TILE = 1, 2, 3
+OPT_GRAPH, OPT_AUTOTUNE, OPT_TILE, OPT_PERSISTENT = 1, 2, 3, 4
 SKIP_REFERENCE, SKIP_NONE = 0, 1
 WARN_MASER, WARN_OVERSHOOT = 1, 2
 
@@ -265,7 +265,7 @@ class Raytracer:
         rc = self.L.rl_setup_rays_circular(self.ctx, n, ip_, nphi, nrext, dbdr, None, None,
                                            rr.ctypes.data_as(_dp), re.ctypes.data_as(_dp))
         self._chk(rc, "rl_setup_rays_circular")
-        self.nrays, self.nrings = list(nrays), list(nrings)
+        self.nrays, self.nrings = [int(x) for x in nrays], [int(x) for x in nrings]
         self.ring_r, self.ring_edge = rr, re
         return self.nrays
 

@@ -77,13 +77,16 @@ def test_batch_modes_are_bit_identical():
     items = _items(ri)
     rt = _rt(ri, "fp32")
     ref, refi, refw = rt.trace(items, want_image=True)
-    for cw in (0, 16, 32, 64):
-        rt.set_option(engine.OPT_TILE, cw)
-        f, im, w = rt.trace(items, want_image=True)
-        assert rt.batch_info()[1] == cw
-        for k in range(len(items)):
-            assert np.array_equal(f[k], ref[k]) and np.array_equal(im[k], refi[k]), (cw, k)
-        assert np.array_equal(w, refw)
+    for persistent in (1, 0):
+        rt.set_option(engine.OPT_PERSISTENT, persistent)
+        for cw in (0, 16, 32, 64):
+            rt.set_option(engine.OPT_TILE, cw)
+            f, im, w = rt.trace(items, want_image=True)
+            info = rt.batch_info()
+            assert info[1] == cw and (info[2] > 0) == bool(persistent)
+            for k in range(len(items)):
+                assert np.array_equal(f[k], ref[k]) and np.array_equal(im[k], refi[k]), (cw, k)
+            assert np.array_equal(w, refw)
     rt.set_option(engine.OPT_AUTOTUNE, 1)
     rt.set_memory_budget(1)                      # one item per batch
     f, im, w = rt.trace(items, want_image=True)
