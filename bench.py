@@ -26,16 +26,18 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
-def measured_traffic(precision):
-    """HBM bytes per k_formal launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (counter values calibrated against a plain copy), or
-    None when no summary for this precision exists."""
+def measured_traffic(precision, kernel):
+    """HBM bytes per launch of the formal-solution kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counter values calibrated
+    against a plain copy), or None when the summary has no line for this
+    kernel and precision."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_c2_{precision}_forward.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[2] == precision:
+        if (len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel
+                and f[2] == precision):
             return float(f[3])
     return None
 
@@ -189,6 +191,10 @@ def main():
     skip = engine.SKIP_NONE if args.skip == "none" else engine.SKIP_REFERENCE
     rt = engine.Raytracer(0, args.precision)
     rt.load_run(ri)
+    # direct launches, so that the stage events bracket the formal-solution
+    # kernel alone (a replayed graph is timed as a whole); graph replay is
+    # measured in the batched workload
+    rt.set_option(engine.OPT_GRAPH, 0)
     items = [(0, i, nfr, vmax, skip) for i in range(len(incs))]
 
     def step():
@@ -213,7 +219,8 @@ def main():
     rays = stats["rays"]
     nseg = (nodes - rays) / rays                      # mean segments per ray
     bytes_per_rc = (28.0 if args.precision == "fp32" else 56.0) * nseg
-    formal_s = float(np.mean(t_formal)) / 1e3
+    nbatch, tile_cw, tile_blocks = rt.batch_info()
+    formal_s = float(np.mean(t_formal)) / 1e3     # per step: every batch's formal launch
     # per launch (one inclination): algorithmic bytes / mean launch duration
     achieved_gbs = bytes_per_rc * nominal / formal_s / 1e9
     flops_per_rc = 40.0 * nseg
@@ -248,11 +255,15 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.precision) if args.workload == "c2" else None,
-            "kernel": "k_formal",
+            "traffic": (measured_traffic(args.precision, "k_formal_batch" if tile_blocks
+                                         else "k_formal_tiles")
+                        if args.workload == "c2" else None),
+            "kernel": "k_formal_batch" if tile_blocks else "k_formal_tiles",
             "bytes_per_ray_channel": bytes_per_rc,
-            "kernel_ms": formal_s * 1e3 / len(items),     # mean per launch (one item)
-            "launches_per_step": len(items),
+            "kernel_ms": formal_s * 1e3 / nbatch,        # mean per launch (one batch)
+            "launches_per_step": nbatch,
+            "items_per_launch": len(items) // nbatch,
+            "tile_cw": tile_cw, "persistent_blocks": tile_blocks,
             "valu_tflops_algorithmic": achieved_tf,
             "valu_frac_of_fp32_peak": achieved_tf / FP32_PEAK_TFLOPS,
         },

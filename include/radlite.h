@@ -107,6 +107,15 @@ int rl_setup_rays_circular(rl_ctx *ctx, int nincl, const double *incl_rad,
 int rl_trace_lines_forward(rl_ctx *ctx, int nitems, const rl_item *items,
                            double *flux, double *image, unsigned *warn_bits);
 
+/* rl_trace_lines_forward with the integrated-cell mask: cmask (optional,
+ * image layout) receives 1 where a (ray, channel) was integrated and 0
+ * where NONREDUNDANT skipping copied the continuum (imcir_cmask,
+ * telescope.F:550-613, for this call's items only; the reference keeps
+ * the mask of earlier lines, which a caller reproduces by OR-ing). */
+int rl_trace_lines_forward_ex(rl_ctx *ctx, int nitems, const rl_item *items,
+                              double *flux, double *image, unsigned char *cmask,
+                              unsigned *warn_bits);
+
 /* Adjoint of the trace for the same items (every item must use
  * RL_SKIP_NONE).  Given dL/dflux (same layout as flux), runs the forward
  * again (flux, optional, receives it) and returns dL/d(the inputs the

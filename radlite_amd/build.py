@@ -23,6 +23,31 @@ FLAGS = [
 ]
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_HOST = os.path.join(FORTRAN_DIR, "rl_host")
+
+
+def build_fortran_host(verbose=False):
+    """The Fortran host program (ISO_C_BINDING module + driver), linked
+    against librl.so with an rpath to it."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as mod:
+        cmd = [FLANG, "-O2", "-module-dir", mod,
+               os.path.join(FORTRAN_DIR, "radlite_gpu.f90"),
+               os.path.join(FORTRAN_DIR, "rl_host.f90"),
+               "-o", FORTRAN_HOST + ".tmp", "-L" + HERE, "-lrl",
+               "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("flang failed building the Fortran host")
+    os.replace(FORTRAN_HOST + ".tmp", FORTRAN_HOST)
+    return FORTRAN_HOST
+
+
 def build(verbose=False):
     srcs = [os.path.join(HERE, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -40,3 +65,4 @@ def build(verbose=False):
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran_host(verbose=True))

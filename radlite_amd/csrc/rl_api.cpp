@@ -97,7 +97,7 @@ struct rl_ctx {
     int cells_line = -1;
     std::vector<double> h_chan, h_vel;
     // batching
-    DevBuf d_items, d_tile0, d_counter;
+    DevBuf d_items, d_tile0, d_counter, d_cmask;
     std::vector<rl::ItemDev> h_items;
     std::vector<long long> h_tile0;
     size_t mem_budget = 0;            // 0: a share of the free device memory
@@ -615,7 +615,8 @@ static long long tiles_of(int cw, int nrays, int nfr) {
 
 // One batch: items [i0, i1) of the call, all resident at once.
 static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *flux,
-                     double *image, unsigned *warn, size_t fo, size_t io) {
+                     double *image, unsigned char *cmask, unsigned *warn, size_t fo,
+                     size_t io) {
     const int nb = i1 - i0;
     const int fp64 = c->precision == RL_FP64;
     const size_t rsz = fp64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::FastRec);
@@ -670,6 +671,7 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
     HIPCHK(c, c->d_counter.ensure(8));
     HIPCHK(c, c->d_items.ensure((size_t)nb * sizeof(rl::ItemDev)));
     HIPCHK(c, c->d_tile0.ensure((size_t)(nb + 1) * 8));
+    if (cmask) HIPCHK(c, c->d_cmask.ensure((size_t)img0));
     if (int e = upload(c, c->d_chan, chan.data(), nchan * 8)) return e;
     c->cells_line = -1;       // d_cells is rewritten below for this batch's lines
     c->h_items = hi;
@@ -690,6 +692,7 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
     L.ovs = c->d_ovs.as<unsigned long long>();
     L.counter = c->d_counter.as<unsigned long long>();
     L.maser = c->d_maser.as<int>();
+    L.cmask = cmask ? c->d_cmask.as<unsigned char>() : nullptr;
     // tile mapping: autotuned once per batch shape (rays per wave, channel
     // tiling, persistent grid size)
     int dev_cu = 0;
@@ -780,6 +783,7 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
         for (const void *p : {c->d_rec.p, c->d_image.p, c->d_partial.p, c->d_flux.p,
                               c->d_rinfo.p, c->d_cells.p, c->d_ovs.p, c->d_maser.p,
                               c->d_counter.p, c->d_items.p, c->d_tile0.p, c->d_chan.p,
+                              (void *)L.cmask,
                               c->d_rho.p, c->d_abund.p, c->d_lw.p, c->d_pops.p, c->d_jd.p,
                               c->d_ad.p})
             add(p);
@@ -821,6 +825,9 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
     if (image)
         HIPCHK(c, hipMemcpyAsync(image + io, c->d_image.p, (size_t)img0 * 8,
                                  hipMemcpyDeviceToHost, c->stream));
+    if (cmask)
+        HIPCHK(c, hipMemcpyAsync(cmask + io, c->d_cmask.p, (size_t)img0, hipMemcpyDeviceToHost,
+                                 c->stream));
     std::vector<unsigned long long> ovs(nb);
     std::vector<int> maser(nb + 1);
     HIPCHK(c, hipMemcpyAsync(ovs.data(), L.ovs, (size_t)nb * 8, hipMemcpyDeviceToHost,
@@ -851,6 +858,12 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
 
 extern "C" int rl_trace_lines_forward(rl_ctx *c, int nitems, const rl_item *items,
                                       double *flux, double *image, unsigned *warn) {
+    return rl_trace_lines_forward_ex(c, nitems, items, flux, image, nullptr, warn);
+}
+
+extern "C" int rl_trace_lines_forward_ex(rl_ctx *c, int nitems, const rl_item *items,
+                                         double *flux, double *image, unsigned char *cmask,
+                                         unsigned *warn) {
     if (!c || nitems < 0 || (nitems && (!items || !flux)))
         return fail(c, RL_E_ARG, "rl_trace_lines_forward: bad args");
     if (!c->have_medium || !c->nlev || !c->nlines || !c->have_dust || c->incl.empty())
@@ -902,7 +915,7 @@ extern "C" int rl_trace_lines_forward(rl_ctx *c, int nitems, const rl_item *item
             used += b;
             i1++;
         }
-        if (int e = run_batch(c, items, i0, i1, flux, image, warn, fo, io)) return e;
+        if (int e = run_batch(c, items, i0, i1, flux, image, cmask, warn, fo, io)) return e;
         c->last_batches++;
         for (int k = i0; k < i1; k++) {
             fo += items[k].nfr;

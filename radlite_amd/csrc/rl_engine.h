@@ -83,6 +83,7 @@ struct BatchLaunch {
     RayInfo *rinfo;
     const double *chan;
     double *image, *partial, *flux;
+    unsigned char *cmask;      // optional [image layout]: 1 where integrated
     double d2;
     const long long *tile0;    // device [nitems+1] first tile of each item
     long long ntiles;

@@ -432,6 +432,21 @@ __device__ inline double trace_dispatch<float>(const FastRec *rec, long long b, 
     return (double)trace_fast<float>(rec + b, n, L, (float)lc.I0, (float)istar, sf, ovs, maser);
 }
 
+// image placeholder of a channel left out by NONREDUNDANT skipping (a NaN
+// payload no arithmetic produces), replaced by the lazy-continuum pass
+constexpr long long SKIPPED_BITS = 0x7ff8dead00000000ll;
+
+// imcir_cmask (telescope.F:546-564): 1 for channel 1, the central beam and
+// channels inside the ray's window; the channel integrated only to seed
+// the lazy continuum, and its copies, stay 0
+__device__ inline unsigned char window_mask(int ray, int ch, const RayInfo *__restrict__ rinfo,
+                                            const double *__restrict__ velo, double aks2,
+                                            int skip_mode) {
+    if (skip_mode != 0 || ray == 0 || ch == 0) return 1;
+    const RayInfo ri = rinfo[ray];
+    return velo[ch] <= (double)ri.mx + aks2 && velo[ch] >= (double)ri.mn - aks2;
+}
+
 // one (ray, channel) of make_image_circular's loop (telescope.F:550-613):
 // integrated, or marked for the lazy-continuum pass when NONREDUNDANT
 // skipping leaves it out of the ray's window
@@ -448,7 +463,7 @@ __device__ inline double formal_one(int ray, int ch, const long long *__restrict
     if (skip_mode == 0 && ray > 0 && ch > 0 && ch != ri.cont) {
         doit = velo[ch] <= (double)ri.mx + aks2 && velo[ch] >= (double)ri.mn - aks2;
     }
-    if (!doit) return __longlong_as_double(0x7ff8dead00000000ll);   // filled by k_fixup
+    if (!doit) return __longlong_as_double(SKIPPED_BITS);   // filled by k_fixup
     T sf = ray == 0 ? (T)lc.starfract : T(0);
     return trace_dispatch<T>(rec, off[ray], ri.n, lc, dnu[ch], velo[ch], istar[ch], sf, ovs,
                              maser);
@@ -664,6 +679,7 @@ k_formal_batch(const ItemDev *__restrict__ items, int nitems,
                const void *rec_all, const RayInfo *__restrict__ rinfo_all,
                const double *__restrict__ chan_all, double *__restrict__ image_all,
                unsigned long long *__restrict__ ovs_all, int *__restrict__ maser_all,
+               unsigned char *__restrict__ cmask_all,
                unsigned long long *counter) {
     const auto *rec_base = reinterpret_cast<const typename RecOf<T>::type *>(rec_all);
     const int lane = threadIdx.x & 63;
@@ -689,10 +705,15 @@ k_formal_batch(const ItemDev *__restrict__ items, int nitems,
         int ray = 0, ch = 0, ovs = 0, maser = 0;
         if (tile_lane(cw, (long long)t - tile0[lo], lane, it.nrays, it.nfr, ray, ch)) {
             const double *dnu = chan_all + it.chan0;
-            image_all[it.img0 + (long long)ray * it.nfr + ch] =
+            const long long ix = it.img0 + (long long)ray * it.nfr + ch;
+            const double v =
                 formal_one<T>(ray, ch, it.off, rec_base + it.rec0, rinfo_all + it.rinfo0, dnu,
                               dnu + it.nfr, dnu + 2 * it.nfr, it.lc, it.aks2, it.skip_mode, ovs,
                               maser);
+            image_all[ix] = v;
+            if (cmask_all)
+                cmask_all[ix] = window_mask(ray, ch, rinfo_all + it.rinfo0, dnu + it.nfr,
+                                            it.aks2, it.skip_mode);
         }
         for (int m = 32; m >= 1; m >>= 1) {
             ovs += __shfl_xor(ovs, m);
@@ -714,7 +735,8 @@ k_formal_tiles(const ItemDev *__restrict__ items, int nitems,
                const long long *__restrict__ tile0, long long ntiles, int cw,
                const void *rec_all, const RayInfo *__restrict__ rinfo_all,
                const double *__restrict__ chan_all, double *__restrict__ image_all,
-               unsigned long long *__restrict__ ovs_all, int *__restrict__ maser_all) {
+               unsigned long long *__restrict__ ovs_all, int *__restrict__ maser_all,
+               unsigned char *__restrict__ cmask_all) {
     const auto *rec_base = reinterpret_cast<const typename RecOf<T>::type *>(rec_all);
     const int lane = threadIdx.x & 63;
     const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -728,10 +750,15 @@ k_formal_tiles(const ItemDev *__restrict__ items, int nitems,
     int ray = 0, ch = 0, ovs = 0, maser = 0;
     if (tile_lane(cw, t - tile0[lo], lane, it.nrays, it.nfr, ray, ch)) {
         const double *dnu = chan_all + it.chan0;
-        image_all[it.img0 + (long long)ray * it.nfr + ch] =
+        const long long ix = it.img0 + (long long)ray * it.nfr + ch;
+        const double v =
             formal_one<T>(ray, ch, it.off, rec_base + it.rec0, rinfo_all + it.rinfo0, dnu,
                           dnu + it.nfr, dnu + 2 * it.nfr, it.lc, it.aks2, it.skip_mode, ovs,
                           maser);
+        image_all[ix] = v;
+        if (cmask_all)
+            cmask_all[ix] = window_mask(ray, ch, rinfo_all + it.rinfo0, dnu + it.nfr, it.aks2,
+                                        it.skip_mode);
     }
     for (int m = 32; m >= 1; m >>= 1) {
         ovs += __shfl_xor(ovs, m);
@@ -748,7 +775,7 @@ __global__ void k_fixup_batch(const ItemDev *__restrict__ items, const void *rec
                               const RayInfo *__restrict__ rinfo_all,
                               const double *__restrict__ chan_all, double *__restrict__ image_all,
                               unsigned long long *__restrict__ ovs_all,
-                              int *__restrict__ maser_all) {
+                              int *__restrict__ maser_all, unsigned char *__restrict__ cmask_all) {
     const int item = blockIdx.y;
     const ItemDev &it = items[item];
     const int ray = blockIdx.x * blockDim.x + threadIdx.x;
@@ -821,22 +848,22 @@ void launch_batch_formal(hipStream_t st, int fp64, const BatchLaunch &b, int cw,
         if (fp64)
             hipLaunchKernelGGL(k_formal_tiles<double>, dim3(g), dim3(256), 0, st, b.items,
                                b.nitems, b.tile0, ntiles, cw, b.rec, b.rinfo, b.chan, b.image,
-                               b.ovs, b.maser);
+                               b.ovs, b.maser, b.cmask);
         else
             hipLaunchKernelGGL(k_formal_tiles<float>, dim3(g), dim3(256), 0, st, b.items,
                                b.nitems, b.tile0, ntiles, cw, b.rec, b.rinfo, b.chan, b.image,
-                               b.ovs, b.maser);
+                               b.ovs, b.maser, b.cmask);
         return;
     }
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, b.counter, 1);
     if (fp64)
         hipLaunchKernelGGL(k_formal_batch<double>, dim3(nblocks), dim3(256), 0, st, b.items,
                            b.nitems, b.tile0, ntiles, cw, b.rec, b.rinfo, b.chan, b.image,
-                           b.ovs, b.maser, b.counter);
+                           b.ovs, b.maser, b.cmask, b.counter);
     else
         hipLaunchKernelGGL(k_formal_batch<float>, dim3(nblocks), dim3(256), 0, st, b.items,
                            b.nitems, b.tile0, ntiles, cw, b.rec, b.rinfo, b.chan, b.image,
-                           b.ovs, b.maser, b.counter);
+                           b.ovs, b.maser, b.cmask, b.counter);
 }
 
 void launch_batch_finish(hipStream_t st, int fp64, const BatchLaunch &b) {
@@ -844,10 +871,10 @@ void launch_batch_finish(hipStream_t st, int fp64, const BatchLaunch &b) {
         dim3 g((b.max_rays + 127) / 128, b.nitems);
         if (fp64)
             hipLaunchKernelGGL(k_fixup_batch<double>, g, dim3(128), 0, st, b.items, b.rec,
-                               b.rinfo, b.chan, b.image, b.ovs, b.maser);
+                               b.rinfo, b.chan, b.image, b.ovs, b.maser, b.cmask);
         else
             hipLaunchKernelGGL(k_fixup_batch<float>, g, dim3(128), 0, st, b.items, b.rec,
-                               b.rinfo, b.chan, b.image, b.ovs, b.maser);
+                               b.rinfo, b.chan, b.image, b.ovs, b.maser, b.cmask);
     }
 }
 

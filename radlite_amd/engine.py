@@ -69,6 +69,8 @@ def load_library(path=LIB_PATH):
                                              _ip, _ip, _dp, _dp]),
         "rl_trace_lines_forward": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
                                              C.POINTER(C.c_uint)]),
+        "rl_trace_lines_forward_ex": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
+                                                C.POINTER(C.c_ubyte), C.POINTER(C.c_uint)]),
         "rl_trace_lines_adjoint": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
                                              _dp, _dp, _dp, _dp, _dp, _dp]),
         "rl_sync": (C.c_int, [vp]),
@@ -269,30 +271,35 @@ class Raytracer:
         self.ring_r, self.ring_edge = rr, re
         return self.nrays
 
-    def trace(self, items, want_image=False):
+    def trace(self, items, want_image=False, want_cmask=False):
         """items: list of (line, incl, nfr, vmax_kms, skip_mode).
-        Returns (list of flux arrays, list of images or None, warn bits)."""
+        Returns (list of flux arrays, list of images or None, warn bits),
+        and the list of integrated-cell masks [nrays, nfr] when want_cmask."""
         arr = (RLItem * len(items))(*[RLItem(*it) for it in items])
         nf = [it[2] for it in items]
         flux = np.zeros(sum(nf))
-        img = None
-        if want_image:
-            img = np.zeros(sum(self.nrays[it[1]] * it[2] for it in items))
+        nimg = sum(self.nrays[it[1]] * it[2] for it in items)
+        img = np.zeros(nimg) if want_image else None
+        cm = np.zeros(nimg, np.uint8) if want_cmask else None
         warn = np.zeros(len(items), np.uint32)
-        rc = self.L.rl_trace_lines_forward(
+        rc = self.L.rl_trace_lines_forward_ex(
             self.ctx, len(items), arr, flux.ctypes.data_as(_dp),
             img.ctypes.data_as(_dp) if img is not None else None,
+            cm.ctypes.data_as(C.POINTER(C.c_ubyte)) if cm is not None else None,
             warn.ctypes.data_as(C.POINTER(C.c_uint)))
         self._chk(rc, "rl_trace_lines_forward")
-        fl, im, o, io = [], [], 0, 0
+        fl, im, cms, o, io = [], [], [], 0, 0
         for it in items:
             fl.append(flux[o:o + it[2]])
             o += it[2]
+            n = self.nrays[it[1]] * it[2]
             if img is not None:
-                n = self.nrays[it[1]] * it[2]
                 im.append(img[io:io + n].reshape(self.nrays[it[1]], it[2]))
-                io += n
-        return fl, (im if img is not None else None), warn
+            if cm is not None:
+                cms.append(cm[io:io + n].reshape(self.nrays[it[1]], it[2]))
+            io += n
+        out = (fl, (im if img is not None else None), warn)
+        return out + (cms,) if want_cmask else out
 
     def adjoint(self, items, dL_dflux):
         """Gradient of L with respect to the inputs the trace reads, given
@@ -345,10 +352,15 @@ class Raytracer:
 
 
 def run_directory(path, precision="fp64", skip_mode=SKIP_REFERENCE, device=0,
-                  write_output=True):
+                  write_output=True, save_image=False):
     """The RADlite executable's job for a run directory: every requested
     line at the linespectrum.inp inclination, written as
-    linespectrum_<mol>.dat (main.F:1037-1049, telescope.F:1662-1804)."""
+    linespectrum_<mol>.dat (main.F:1037-1049, telescope.F:1662-1804).
+    With save_image, also the circular image cube of each line,
+    lineposvelcirc_<mol>_<line>.dat (the reference's SAVE_IMCIR build,
+    telescope.F:1348-1373, 1595-1621), whose integrated-cell mask keeps
+    the cells of the earlier lines as the reference's never-cleared
+    imcir_cmask does."""
     ri = l4io.read_run_dir(path)
     rt = Raytracer(device=device, precision=precision)
     tb = rt.load_run(ri)
@@ -356,7 +368,24 @@ def run_directory(path, precision="fp64", skip_mode=SKIP_REFERENCE, device=0,
     incl = ri.incl_deg * 0.0174532925199                      # telescope.F:188
     rt.setup_rays([incl], ri.nphi, ri.b_extra, ri.dbdr)
     lines = list(range(ri.iline_start - 1, ri.iline_start - 1 + ri.nlines_run))
-    fl, _, warn = rt.trace([(il, 0, nfr, vmax, skip_mode) for il in lines])
+    items = [(il, 0, nfr, vmax, skip_mode) for il in lines]
+    if save_image:
+        fl, img, warn, cms = rt.trace(items, want_image=True, want_cmask=True)
+        cum = np.zeros_like(cms[0])
+        for k, il in enumerate(lines):
+            cum |= cms[k]
+            iline = il + 1
+            if iline >= 100:          # the reference names files for lines 1-99 only
+                continue
+            nu0s = tb.nu0[il]
+            passb = 3.33567e-6 * abs(nu0s) * vmax
+            dnu = 2.0 * passb / (nfr - 1.0)
+            velo = np.array([((-passb + ch * dnu) / nu0s) * 2.99792458e5 for ch in range(nfr)])
+            l4io.write_image_cube(os.path.join(path, f"lineposvelcirc_{ri.molname}_{iline}.dat"),
+                                  nu0s, ri.nphi, rt.nrings[0], rt.ring_edge,
+                                  rt.ring_r, velo, img[k], cum)
+    else:
+        fl, _, warn = rt.trace(items)
     blocks = []
     for il, f in zip(lines, fl):
         nu0 = abs(tb.nu0[il])

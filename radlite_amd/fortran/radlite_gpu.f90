@@ -103,6 +103,37 @@ module radlite_gpu
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx
         end function
+        ! host pre-pass with the reference's arithmetic (include/radlite.h)
+        integer(c_int) function rl_prepare_line_freqs(nl, e_cm, lev_up, lev_low, nu0) &
+                bind(C, name="rl_prepare_line_freqs")
+            import :: c_int, c_double
+            integer(c_int), value :: nl
+            real(c_double), intent(in) :: e_cm(*)
+            integer(c_int), intent(in) :: lev_up(*), lev_low(*)
+            real(c_double), intent(out) :: nu0(*)
+        end function
+        integer(c_int) function rl_prepare_line_dust(nr, nth_half, nspec, rho_d, t_d, nfreq, &
+                cont_nu, kabs, kscat, nlines, nu0, jd, ad) bind(C, name="rl_prepare_line_dust")
+            import :: c_int, c_double
+            integer(c_int), value :: nr, nth_half, nspec, nfreq, nlines
+            real(c_double), intent(in) :: rho_d(*), t_d(*), cont_nu(*), kabs(*), kscat(*), nu0(*)
+            real(c_double), intent(out) :: jd(*), ad(*)
+        end function
+        integer(c_int) function rl_prepare_star(nfreq, fnu_1pc, rstar, istar) &
+                bind(C, name="rl_prepare_star")
+            import :: c_int, c_double
+            integer(c_int), value :: nfreq
+            real(c_double), intent(in) :: fnu_1pc(*)
+            real(c_double), value :: rstar
+            real(c_double), intent(out) :: istar(*)
+        end function
+        real(c_double) function rl_prepare_aksmax(n, lw_kms, fix_lw) &
+                bind(C, name="rl_prepare_aksmax")
+            import :: c_int, c_double
+            integer(c_int), value :: n
+            real(c_double), intent(in) :: lw_kms(*)
+            real(c_double), value :: fix_lw
+        end function
         integer(c_int) function rl_passband(vmax, dv, nfr, vmax_out) bind(C, name="rl_passband")
             import :: c_int, c_double
             real(c_double), value :: vmax, dv

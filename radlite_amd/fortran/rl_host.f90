@@ -1,0 +1,342 @@
+! Fortran host for the MI355X raytracer: the RADlite executable's line
+! spectrum job for one run directory, with the raytrace done by librl.so
+! through ISO_C_BINDING (module radlite_gpu).
+!
+!   rl_host [run directory] [fp64|fp32]
+!
+! Reads the run directory's L4 files as RADLite's readers do (radlite.inp
+! main.F:535-597 and tools.F:6-19; linespectrum.inp telescope.F:104-238;
+! line.inp line.F:168-279; moldata line.F:1865-1935; levelpop
+! line.F:842-919; fields setup.F:754-911,1453-1485; dust
+! dust.F:92-520 and setup.F:159-250,701-725; star star.F:449-620),
+! prepares the line tables with the library's host pre-pass (the
+! reference's arithmetic), traces every requested line at the
+! linespectrum.inp inclination (calc_line_spectrum, telescope.F:1465) and
+! writes linespectrum_<molecule>.dat (write_line_spectrum,
+! telescope.F:1662-1804).  A library error stops with the reference's
+! stop code.
+program rl_host
+    use iso_c_binding
+    use radlite_gpu
+    implicit none
+    character(len=512) :: dir, arg, molfile, molname, line
+    integer :: nr, nth, nf, nspec, nsp_t, nlev, nlin, i, k, ir, it, u, idum, isp, nsize
+    integer :: nphi, dbdr, b_extra, outer_bc, nlines_run, istart, iprec, ios, nrows
+    real(c_double) :: umass, fix_lw, vmax, dv, incl_deg, vrad, dist, rstar, aks, vmax2, dummy
+    real(c_double), allocatable :: r(:), th(:), rho(:,:), abund(:,:), vel(:,:,:), lw(:,:)
+    real(c_double), allocatable :: rhod(:,:,:), td(:,:,:), cnu(:), kabs(:,:), kscat(:,:)
+    real(c_double), allocatable :: fnu(:), istar(:), pops(:,:,:), E(:), g(:), A(:), nu0(:)
+    real(c_double), allocatable :: gu(:), gl(:), jd(:,:,:), ad(:,:,:), flux(:), work(:)
+    integer(c_int), allocatable :: lup(:), llo(:), warn(:), nrays(:), nrings(:)
+    type(rl_item), allocatable :: items(:)
+    character(len=16), allocatable :: rows(:)
+    type(c_ptr) :: ctx
+    integer(c_int) :: ierr, nfr
+    real(c_double) :: inc(1)
+
+    dir = '.'
+    iprec = RL_FP64
+    if (command_argument_count() >= 1) call get_command_argument(1, dir)
+    if (command_argument_count() >= 2) then
+        call get_command_argument(2, arg)
+        if (trim(arg) == 'fp32') iprec = RL_FP32
+    end if
+
+    ! radlite.inp: first 16 columns of the non-comment lines
+    allocate(rows(200))
+    nrows = 0
+    open(newunit=u, file=path('radlite.inp'), status='old', action='read')
+    do
+        read(u, '(a)', iostat=ios) line
+        if (ios /= 0) exit
+        if (len_trim(line) == 0) cycle
+        if (index('#;%=', line(1:1)) > 0) cycle
+        nrows = nrows + 1
+        rows(nrows) = line(1:16)
+    end do
+    close(u)
+    read(rows(14), *) outer_bc
+    read(rows(18), *) nphi
+    read(rows(19), *) dbdr
+    read(rows(20), *) b_extra
+
+    ! grid and fields (it fastest, then ir)
+    open(newunit=u, file=path('radius.inp'), status='old', action='read')
+    read(u, *) nr
+    allocate(r(nr))
+    read(u, *) r
+    close(u)
+    open(newunit=u, file=path('theta.inp'), status='old', action='read')
+    read(u, *) nth, idum
+    allocate(th(nth))
+    read(u, *) th
+    close(u)
+    allocate(rho(nth, nr), abund(nth, nr), vel(3, nth, nr), lw(nth, nr))
+    open(newunit=u, file=path('density.inp'), status='old', action='read')
+    read(u, *) idum, idum, idum
+    read(u, *) rho
+    close(u)
+    open(newunit=u, file=path('abundance.inp'), status='old', action='read')
+    read(u, *) idum, idum
+    do ir = 1, nr
+        do it = 1, nth
+            read(u, *) abund(it, ir), dummy
+        end do
+    end do
+    close(u)
+    open(newunit=u, file=path('velocity.inp'), status='old', action='read')
+    read(u, *) idum, idum
+    read(u, *) vel
+    close(u)
+    open(newunit=u, file=path('turbulence.inp'), status='old', action='read')
+    read(u, *) idum
+    read(u, *) idum, idum
+    read(u, *) lw
+    close(u)
+
+    ! dust density, temperature, frequencies, opacities
+    open(newunit=u, file=path('dustdens.inp'), status='old', action='read')
+    read(u, *) nspec, idum, idum, idum
+    allocate(rhod(nth, nr, nspec), td(nth, nr, nspec))
+    read(u, *) rhod
+    close(u)
+    open(newunit=u, file=path('dusttemp_final.dat'), status='old', action='read')
+    read(u, *) nsp_t, idum, idum, idum
+    do isp = 1, nsp_t
+        read(u, *) nsize
+        read(u, *) td(:, :, isp)
+    end do
+    close(u)
+    open(newunit=u, file=path('frequency.inp'), status='old', action='read')
+    read(u, *) nf
+    allocate(cnu(nf), kabs(nf, nspec), kscat(nf, nspec), fnu(nf), istar(nf), work(2 * nf))
+    read(u, *) cnu
+    close(u)
+    open(newunit=u, file=path('dustopac.inp'), status='old', action='read')
+    read(u, *) idum
+    read(u, *) nspec
+    read(u, '(a)') line
+    do isp = 1, nspec
+        read(u, *) idum
+        read(u, *) idum
+        read(u, *) k
+        read(u, '(a)') line
+        call read_opac(k, isp)
+    end do
+    close(u)
+    open(newunit=u, file=path('starinfo.inp'), status='old', action='read')
+    read(u, *) idum
+    read(u, *) rstar
+    close(u)
+    open(newunit=u, file=path('starspectrum.inp'), status='old', action='read')
+    read(u, *) idum
+    read(u, *) work
+    close(u)
+    fnu = work(2:2 * nf:2)
+
+    ! line.inp: mean molecular weight and the fixed line width fall-back
+    open(newunit=u, file=path('line.inp'), status='old', action='read')
+    read(u, *) idum
+    read(u, *) idum
+    read(u, *) umass
+    read(u, '(a)') line
+    read(u, '(a)') line
+    read(u, *) idum
+    read(u, *) idum
+    read(u, *) idum
+    read(u, *) dummy
+    read(u, *) idum
+    fix_lw = 0.0d0
+    do
+        read(u, *) k
+        if (k == 0) exit
+        read(u, *) fix_lw
+    end do
+    close(u)
+
+    ! linespectrum.inp
+    open(newunit=u, file=path('linespectrum.inp'), status='old', action='read')
+    read(u, *) idum
+    read(u, *) idum
+    read(u, '(a)') line
+    read(u, *) idum
+    read(u, '(a)') line
+    read(u, *) vmax
+    read(u, *) dv
+    read(u, '(a)') line
+    read(u, *) idum
+    read(u, '(a)') molfile
+    read(u, *) idum
+    read(u, *) dist
+    read(u, *) incl_deg
+    read(u, *) vrad
+    read(u, *) nlines_run
+    read(u, *) istart
+    close(u)
+    molfile = adjustl(molfile)
+    molname = molfile(1:index(molfile, '.dat', back=.true.) - 1)
+
+    ! molecule: levels (I5,F12.4,F7.1), transitions (3I5,E12.3)
+    open(newunit=u, file=path(trim(molfile)), status='old', action='read')
+    do i = 1, 5
+        read(u, '(a)') line
+    end do
+    read(u, '(I6)') nlev
+    read(u, '(a)') line
+    allocate(E(nlev), g(nlev))
+    do i = 1, nlev
+        read(u, '(I5,F12.4,F7.1)') idum, E(i), g(i)
+    end do
+    read(u, '(a)') line
+    read(u, '(I6)') nlin
+    read(u, '(a)') line
+    allocate(lup(nlin), llo(nlin), A(nlin), nu0(nlin), gu(nlin), gl(nlin))
+    do i = 1, nlin
+        read(u, '(3I5,E12.3)') idum, lup(i), llo(i), A(i)
+    end do
+    close(u)
+    lup = lup - 1
+    llo = llo - 1
+    do i = 1, nlin
+        gu(i) = g(lup(i) + 1)
+        gl(i) = g(llo(i) + 1)
+    end do
+
+    ! level populations
+    open(newunit=u, file=path('levelpop_' // trim(molname) // '.dat'), status='old', &
+         action='read')
+    read(u, *) idum, idum, k, idum
+    allocate(pops(k, nth, nr))
+    call skip_values(u, 2 * k)          ! level energies and weights
+    read(u, *) pops
+    close(u)
+    nlev = k
+
+    ! host pre-pass with the reference's arithmetic
+    allocate(jd(nlin, nth, nr), ad(nlin, nth, nr))
+    call chk(rl_prepare_line_freqs(nlin, E, lup, llo, nu0), 'rl_prepare_line_freqs')
+    call chk(rl_prepare_line_dust(nr, nth, nspec, rhod, td, nf, cnu, kabs, kscat, nlin, nu0, &
+                                  jd, ad), 'rl_prepare_line_dust')
+    call chk(rl_prepare_star(nf, fnu, rstar, istar), 'rl_prepare_star')
+    aks = rl_prepare_aksmax(nr * nth, lw, fix_lw)
+    call chk(rl_passband(vmax, dv, nfr, vmax2), 'rl_passband')
+
+    ! the trace
+    ierr = rl_create(ctx, 0, iprec)
+    if (ierr /= 0) then
+        write(*, '(a,i0)') 'rl_create failed: ', ierr
+        stop 900
+    end if
+    call chkc(rl_set_grid(ctx, nr, r, nth, th))
+    call chkc(rl_set_star(ctx, rstar, nf, cnu, istar))
+    call chkc(rl_set_medium(ctx, rho, abund, vel, lw, umass, aks))
+    call chkc(rl_set_levels(ctx, nlev, pops))
+    call chkc(rl_set_lines(ctx, nlin, nu0, A, gu, gl, lup, llo))
+    call chkc(rl_set_dust_line(ctx, jd, ad))
+    call chkc(rl_set_outer_bc(ctx, outer_bc))
+    inc(1) = incl_deg * 0.0174532925199d0
+    allocate(nrays(1), nrings(1))
+    call chkc(rl_setup_rays_circular(ctx, 1, inc, nphi, b_extra, dbdr, nrays, nrings, &
+                                     c_null_ptr, c_null_ptr))
+    allocate(items(nlines_run), flux(nlines_run * nfr), warn(nlines_run))
+    do i = 1, nlines_run
+        items(i) = rl_item(istart - 2 + i, 0, nfr, vmax2, RL_SKIP_REFERENCE)
+    end do
+    call chkc(rl_trace_lines_forward(ctx, nlines_run, items, flux, c_null_ptr, warn))
+    call rl_destroy(ctx)
+    call write_spectrum()
+
+contains
+
+    function path(name) result(p)
+        character(len=*), intent(in) :: name
+        character(len=1024) :: p
+        p = trim(dir) // '/' // name
+    end function
+
+    subroutine read_opac(idx, isp)
+        integer, intent(in) :: idx, isp
+        integer :: v, nfo, nso
+        character(len=32) :: num
+        write(num, '(i0)') idx
+        open(newunit=v, file=path('dustopac_' // trim(num) // '.inp'), status='old', &
+             action='read')
+        read(v, *) nfo, nso
+        read(v, *) kabs(:, isp)
+        read(v, *) kscat(:, isp)
+        close(v)
+    end subroutine
+
+    ! skip n list-directed values (the energy and weight lines of levelpop)
+    subroutine skip_values(v, n)
+        integer, intent(in) :: v, n
+        real(c_double) :: tmp(n)
+        read(v, *) tmp
+    end subroutine
+
+    subroutine chk(ierr, what)
+        integer(c_int), intent(in) :: ierr
+        character(len=*), intent(in) :: what
+        if (ierr /= 0) then
+            write(*, '(a,a,i0)') what, ': stop ', ierr
+            stop 13
+        end if
+    end subroutine
+
+    subroutine chkc(ierr)
+        integer(c_int), intent(in) :: ierr
+        if (ierr /= 0) then
+            write(*, '(a)') trim(rl_error_text(ctx))
+            call rl_destroy(ctx)
+            select case (ierr)
+            case (13)
+                stop 13
+            case (124)
+                stop 124
+            case (749)
+                stop 749
+            case default
+                stop 1
+            end select
+        end if
+    end subroutine
+
+    ! linespectrum_<molecule>.dat as header_line_spectrum and
+    ! write_line_spectrum write it (telescope.F:1662-1804)
+    subroutine write_spectrum()
+        integer :: v, il, inu, ln
+        real(c_double) :: nuf, passb, dnu, freq, velo
+        character(len=200) :: name80, file80
+        name80 = molname
+        file80 = './' // trim(molfile)
+        open(newunit=v, file=path('linespectrum_' // trim(molname) // '.dat'), &
+             status='replace', action='write')
+        write(v, '(i2)') 1
+        write(v, '(i2)') 1
+        write(v, '(a80)') name80
+        write(v, '(a80)') file80
+        write(v, '(i10)') nlines_run, nfr
+        write(v, '(e12.4,e12.4,f7.3)') 1.0d0, vrad, inc(1) * 57.2957795132d0
+        do il = 1, nlines_run
+            ln = istart - 1 + il
+            nuf = nu0(ln)
+            passb = 3.33567d-6 * abs(nuf) * vmax2
+            dnu = 2.0d0 * passb / (nfr - 1.0d0)
+            write(v, *)
+            write(v, '(i5,i5)') lup(ln) + 1, llo(ln) + 1
+            write(v, '(e14.9)') nuf
+            write(v, '(e10.5)') 0.0d0
+            write(v, '(i5)') nfr
+            write(v, *) ' '
+            do inu = nfr, 1, -1
+                freq = nuf + ((0.0d0 - passb) + (inu - 1) * dnu)
+                velo = freq - nuf
+                velo = -2.99792458d5 * velo / nuf
+                velo = velo + vrad
+                write(v, '(E13.6,1X,E13.6)') velo, flux((il - 1) * nfr + inu)
+            end do
+        end do
+        close(v)
+    end subroutine
+
+end program rl_host

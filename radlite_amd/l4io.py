@@ -316,31 +316,40 @@ def read_run_dir(path) -> RunInputs:
 
 
 def write_line_spectrum_file(fname, molname, nfr_max, incl_rad, blocks,
-                             distance_pc=1.0, vrad=0.0, style=1):
-    """Spectrum output with the reference's header and per-line blocks
-    (telescope.F:1662-1804, formats at :1675-1682, :1773-1802):
-    rows ``v  F_nu`` in E13.6 for inu = nfr..1.
+                             distance_pc=1.0, vrad=0.0, style=1, beamsize=0.0):
+    """Spectrum output as the reference writes it (header_line_spectrum /
+    write_line_spectrum, telescope.F:1662-1804): (i2) format version and
+    style; (a80) molecule name and data file (read through the data
+    directory, line.F:1840-1848, hence ./<name>.dat); (i10) reverting over
+    the line count and the channel count (two records); (e12.4,e12.4,f7.3)
+    distance [pc], radial velocity, inclination [deg] (anginf *
+    57.2957795132); per line an empty record, (i5,i5) levels, (e14.9) nu0,
+    (e10.5) beam size, (i5) channels, a ' ' record, and rows
+    (E13.6,1X,E13.6) v, F_nu for inu = nfr..1.
 
     ``blocks`` is a list of dicts with keys lev_up, lev_low, nu0, nu
     (channel frequencies, ascending), flux (same length)."""
     def e(x, w, d):
         return _fortran_e(x, w, d)
     with open(fname, "w") as f:
-        f.write(f"{1:2d}\n{style:2d}\n{molname:<80s}\n{molname + '.dat':<80s}\n")
-        f.write(f"{len(blocks):10d}{nfr_max:10d}\n")
-        f.write(e(distance_pc, 12, 4) + e(vrad, 12, 4) + f"{incl_rad * 57.2957795129:7.3f}\n")
+        f.write(f"{1:2d}\n{style:2d}\n{molname:<80.80s}\n{'./' + molname + '.dat':<80.80s}\n")
+        f.write(f"{len(blocks):10d}\n{nfr_max:10d}\n")
+        f.write(e(distance_pc, 12, 4) + e(vrad, 12, 4) + f"{incl_rad * 57.2957795132:7.3f}\n")
         for b in blocks:
             nu0 = b["nu0"]
             f.write("\n" + f"{b['lev_up']:5d}{b['lev_low']:5d}\n")
-            f.write(e(nu0, 14, 9) + "\n" + e(-1.0, 10, 5) + "\n")
-            f.write(f"{len(b['flux']):5d}\n\n")
+            f.write(e(nu0, 14, 9) + "\n" + e(beamsize, 10, 5) + "\n")
+            f.write(f"{len(b['flux']):5d}\n  \n")
             for k in range(len(b["flux"]) - 1, -1, -1):
-                v = -2.99792458e5 * (b["nu"][k] - nu0) / nu0 + vrad
+                v = b["nu"][k] - nu0
+                v = -2.99792458e5 * v / nu0
+                v = v + vrad
                 f.write(e(v, 13, 6) + " " + e(b["flux"][k], 13, 6) + "\n")
 
 
 def _fortran_e(x: float, w: int, d: int) -> str:
-    """Fortran Ew.d edit descriptor: 0.ddddE+xx, right-justified."""
+    """Fortran Ew.d edit descriptor: 0.ddddE+xx, right-justified; a
+    three-digit exponent drops the E (0.dddd-119)."""
     x = float(x)
     if x == 0.0:
         s = "0." + "0" * d + "E+00"
@@ -348,22 +357,118 @@ def _fortran_e(x: float, w: int, d: int) -> str:
         m, ex = f"{abs(x):.{d - 1}e}".split("e")
         digits = m.replace(".", "")
         ex = int(ex) + 1
-        s = ("-" if x < 0 else "") + "0." + digits + ("E%+03d" % ex)
+        tail = ("E%+03d" % ex) if abs(ex) <= 99 else ("%+04d" % ex)
+        s = ("-" if x < 0 else "") + "0." + digits + tail
+    if len(s) > w:                    # the leading zero is optional
+        s = s.replace("0.", ".", 1)
+    if len(s) > w:
+        return "*" * w
     return s.rjust(w)
 
 
+def _flang_list(x: float) -> str:
+    """A REAL(8) as flang's list-directed output writes it: one leading
+    blank, the shortest round-trip digits, F form (no leading zero,
+    trailing point for integral values) for 0.095 <= |x| < 1e15, else
+    d.dddE+xx."""
+    from decimal import Decimal
+    x = float(x)
+    if x == 0.0:
+        return " 0."
+    d = Decimal(repr(abs(x))).normalize()
+    t = d.as_tuple()
+    D = "".join(str(v) for v in t.digits)
+    k = len(D) + t.exponent            # |x| = 0.D x 10^k
+    ax = abs(x)
+    if 0.095 <= ax < 1e15 - 0.5:
+        if k <= 0:
+            body = "." + "0" * (-k) + D
+        elif k >= len(D):
+            body = D + "0" * (k - len(D)) + "."
+        else:
+            body = D[:k] + "." + D[k:]
+    else:
+        e = k - 1
+        body = D[0] + "." + D[1:] + "E" + ("+" if e >= 0 else "-") + "%02d" % abs(e)
+    return " " + ("-" if x < 0 else "") + body
+
+
+def write_image_cube(fname, nu0, nphi, nrings, ring_edge, ring_r, velo_kms, image, cmask):
+    """Circular image cube lineposvelcirc_<mol>_<line>.dat (SAVE_IMCIR,
+    telescope.F:1595-1621, 1348-1373; read by PRO/read_imcir.pro):
+    nfr; nu0; nphi nr; ring edges ri(1..nr+1) and radii r(1..nr+1) in
+    E12.6; then per channel a blank record, the velocity dnu/nu0*c, a
+    blank record, the central intensity, a blank record and for every
+    phi, every ring (E10.4,2X,i1) intensity and integrated-cell mask.
+
+    ring_edge: [nrings+2] edges ri(0..nr+1); ring_r: [nrings+1] radii
+    r(0..nr) (r(nr+1) is never set by the reference and is written as 0);
+    image, cmask: [nrays, nfr] with ray 0 the central beam and ray
+    1 + (ir-1)*nphi + (ip-1) the ring ir, angle ip."""
+    nfr = image.shape[1]
+    r_ext = list(ring_r[1:nrings + 1]) + [0.0]
+    lines = [" %d" % nfr, _flang_list(nu0), " %d %d" % (nphi, nrings)]
+    lines += [_fortran_e(ring_edge[ir], 12, 6) for ir in range(1, nrings + 2)]
+    lines += [_fortran_e(v, 12, 6) for v in r_ext]
+    rings = image[1:].reshape(nrings, nphi, nfr)
+    masks = cmask[1:].reshape(nrings, nphi, nfr)
+    blank = "        "
+    for ch in range(nfr):
+        lines += [blank, _flang_list(velo_kms[ch]), blank, _flang_list(image[0, ch]), blank]
+        a = rings[:, :, ch].T.ravel()       # phi outer, ring inner
+        m = masks[:, :, ch].T.ravel()
+        lines += [_fortran_e(v, 10, 4) + "  %1d" % f for v, f in zip(a, m)]
+    with open(fname, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def read_image_cube(fname):
+    """Parse a circular image cube the way read_imcir.pro does."""
+    with open(fname) as f:
+        rec = f.read().split("\n")
+    k = 0
+    nfr = int(rec[k]); k += 1
+    nu0 = float(rec[k]); k += 1
+    nphi, nr = (int(v) for v in rec[k].split()); k += 1
+    ri = np.array([_f(rec[k + i]) for i in range(nr + 1)]); k += nr + 1
+    r = np.array([_f(rec[k + i]) for i in range(nr + 1)]); k += nr + 1
+    velo = np.zeros(nfr); center = np.zeros(nfr)
+    img = np.zeros((nfr, nphi, nr)); cm = np.zeros((nfr, nphi, nr), np.int8)
+    for ch in range(nfr):
+        k += 1
+        velo[ch] = _f(rec[k]); k += 2
+        center[ch] = _f(rec[k]); k += 2
+        for ip in range(nphi):
+            for ir in range(nr):
+                a, b = rec[k].rsplit(None, 1)
+                img[ch, ip, ir] = _f(a); cm[ch, ip, ir] = int(b); k += 1
+    return dict(nfr=nfr, nu0=nu0, nphi=nphi, nr=nr, ri=ri, r=r, velo=velo, center=center,
+                image=img, cmask=cm)
+
+
+def _f(s: str) -> float:
+    """A Fortran real field, including E-less three-digit exponents."""
+    s = s.strip()
+    if "E" not in s.upper() and ("-" in s[1:] or "+" in s[1:]):
+        i = max(s.rfind("-"), s.rfind("+"))
+        s = s[:i] + "E" + s[i:]
+    return float(s)
+
+
 def read_line_spectrum_file(fname):
-    """Parse a spectrum file the way pyradlite does (radlite.py:2825-2850)."""
+    """Parse a spectrum file with pyradlite's indices (radlite.py:2825-2850):
+    line count in record 4, channel maximum in record 5, inclination in
+    record 6; per line the channel count 5 records after the block start
+    and the rows from 7 records after it."""
     with open(fname) as f:
         lines = f.read().splitlines()
-    nl = int(lines[4].split()[0])
-    out, k = [], 6
+    nl = int(lines[4])
+    out, iloc = [], 6
     for _ in range(nl):
-        k += 1
-        up, lo = (int(x) for x in lines[k].split()); k += 1
-        nu0 = float(lines[k]); k += 2
-        nfr = int(lines[k]); k += 2
-        rows = np.array([[float(x) for x in lines[k + i].split()] for i in range(nfr)])
-        k += nfr
+        up, lo = (int(x) for x in lines[iloc + 2].split())
+        nu0 = float(lines[iloc + 3])
+        nfr = int(lines[iloc + 5])
+        rows = np.array([[float(x) for x in ln.split()] for ln in lines[iloc + 7:iloc + 7 + nfr]])
         out.append(dict(lev_up=up, lev_low=lo, nu0=nu0, v=rows[:, 0], flux=rows[:, 1]))
+        iloc = iloc + 7 + nfr - 1
     return out

@@ -1,0 +1,95 @@
+"""Text formats of the outputs: Fortran Ew.d fields (including the
+E-less three-digit exponent) and flang's list-directed REAL(8) output,
+pinned against what flang (the reference's compiler here) prints for the
+same values; the image-cube writer's round trip."""
+import numpy as np
+
+from radlite_amd import l4io
+
+# value -> flang list-directed output (write(*,*) x), recorded from flang
+FLANG_LIST = [
+    (1e15, " 1.E+15"), (1.234e16, " 1.234E+16"), (9.99e15, " 9.99E+15"), (0.1, " .1"),
+    (0.01, " 1.E-02"), (2.5e-5, " 2.5E-05"), (-0.5, " -.5"), (1.0, " 1."), (10.0, " 10."),
+    (3e14, " 300000000000000."), (5.27296241956e-28, " 5.27296241956E-28"),
+    (-6.3786e13, " -63786000000000."), (0.099, " .099"), (123.456, " 123.456"),
+    (-1e-3, " -1.E-03"), (0.0123, " 1.23E-02"), (1.5e15, " 1.5E+15"),
+    (123456789012345.6, " 123456789012345.6"), (1e-300, " 1.E-300"), (5e-324, " 5.E-324"),
+    (0.0999999, " .0999999"), (0.09, " 9.E-02"), (0.095, " .095"), (0.0909, " 9.09E-02"),
+    (6.378605546193367e13, " 63786055461933.67"), (-29.99999997, " -29.99999997"),
+    (0.0, " 0."), (1 / 3, " .3333333333333333"), (123456789.0, " 123456789."),
+]
+
+# (value, w, d) -> flang Ew.d
+FLANG_E = [
+    ((3.2717e-120, 10, 4), "0.3272-119"), ((9.99996e-13, 10, 4), "0.1000E-11"),
+    ((1e-100, 12, 6), "0.100000E-99"), ((9.9999996e12, 12, 6), "0.100000E+14"),
+    ((-2.5, 13, 6), "-0.250000E+01"), ((1e-150, 13, 6), " 0.100000-149"),
+    ((1.0, 10, 4), "0.1000E+01"), ((1.495978707e12, 12, 6), "0.149598E+13"),
+]
+
+
+def test_list_directed_matches_flang():
+    for v, ref in FLANG_LIST:
+        assert l4io._flang_list(v) == ref, (v, l4io._flang_list(v), ref)
+
+
+def test_e_edit_matches_flang():
+    for (v, w, d), ref in FLANG_E:
+        assert l4io._fortran_e(v, w, d) == ref, (v, w, d)
+        assert abs(l4io._f(ref) - v) <= 10 ** (1 - d) * abs(v)
+
+
+def test_image_cube_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    nphi, nr, nfr = 4, 3, 5
+    img = 10 ** rng.uniform(-130, -5, size=(1 + nphi * nr, nfr))
+    cm = rng.integers(0, 2, size=img.shape).astype(np.uint8)
+    edges = np.concatenate([[0.0], np.sort(rng.uniform(1e12, 1e14, nr + 1))])
+    radii = np.concatenate([[0.0], np.sort(rng.uniform(1e12, 1e14, nr))])
+    velo = np.linspace(-30, 30, nfr)
+    f = tmp_path / "cube.dat"
+    l4io.write_image_cube(str(f), 6.3786e13, nphi, nr, edges, radii, velo, img, cm)
+    c = l4io.read_image_cube(str(f))
+    assert (c["nfr"], c["nphi"], c["nr"]) == (nfr, nphi, nr)
+    assert np.allclose(c["ri"], edges[1:], rtol=1e-5)
+    assert np.allclose(c["r"][:-1], radii[1:], rtol=1e-5) and c["r"][-1] == 0.0
+    assert np.allclose(c["velo"], velo, rtol=1e-15)
+    assert np.allclose(c["center"], img[0], rtol=1e-15)
+    for ch in range(nfr):
+        ring = img[1:, ch].reshape(nr, nphi).T
+        assert np.allclose(c["image"][ch], ring, rtol=1e-4)
+        assert np.array_equal(c["cmask"][ch], cm[1:, ch].reshape(nr, nphi).T)
+
+
+def _golden_blocks(name):
+    import tempfile
+    from cases import build_case, load_golden
+    g = load_golden(name)
+    with tempfile.TemporaryDirectory() as d:
+        build_case(name, d)
+        ri = l4io.read_run_dir(d)
+    blocks = []
+    for il in range(len(g["flux"])):
+        nu0 = g["nu0"][il]
+        blocks.append(dict(lev_up=int(ri.lev_up[il]), lev_low=int(ri.lev_low[il]), nu0=nu0,
+                           nu=nu0 + g["dnu"][il], flux=g["flux"][il]))
+    return ri, g, blocks
+
+
+import pytest  # noqa: E402
+
+
+@pytest.mark.parametrize("name", ["t1", "t5", "t6", "c1"])
+def test_spectrum_file_is_the_references(name, tmp_path):
+    """The writer on the reference's own spectra reproduces the file the
+    reference wrote for the same run (tests/golden/spec_<case>.dat)."""
+    import os
+    ri, g, blocks = _golden_blocks(name)
+    f = tmp_path / "spec.dat"
+    l4io.write_line_spectrum_file(str(f), ri.molname, len(g["flux"][0]),
+                                  ri.incl_deg * 0.0174532925199, blocks, vrad=ri.vrad)
+    ref = open(os.path.join(os.path.dirname(__file__), "golden", f"spec_{name}.dat")).read()
+    assert f.read_text() == ref
+    parsed = l4io.read_line_spectrum_file(str(f))
+    for b, p in zip(blocks, parsed):
+        assert np.allclose(p["flux"], b["flux"][::-1], rtol=6e-7)
