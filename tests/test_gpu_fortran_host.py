@@ -46,7 +46,7 @@ def test_fortran_host_writes_the_python_hosts_file(name):
         blocks = l4io.read_line_spectrum_file(os.path.join(fdir, fname))
     for il, b in enumerate(blocks):
         ref = g["flux"][il][::-1]              # file rows run from the last channel
-        assert np.allclose(b["flux"], ref, rtol=6e-7, atol=0), (name, il)
+        assert np.allclose(b["flux"], ref, rtol=5.01e-6, atol=0), (name, il)   # E13.6
     # against the file the reference itself wrote for this run: identical
     # except where a flux within 1e-10 of an E13.6 rounding boundary
     # rounds the other way (the FP64 bound of the trace)
@@ -57,4 +57,5 @@ def test_fortran_host_writes_the_python_hosts_file(name):
     for a, b in zip(got, refl):
         if a != b:
             va, vb = [float(x) for x in a.split()], [float(x) for x in b.split()]
-            assert va[0] == vb[0] and abs(va[1] - vb[1]) <= 1.01e-6 * abs(vb[1]), (a, b)
+            # one unit of the sixth digit of the 0.dddddd mantissa
+            assert va[0] == vb[0] and abs(va[1] - vb[1]) <= 1.01e-5 * abs(vb[1]), (a, b)

@@ -92,4 +92,4 @@ def test_spectrum_file_is_the_references(name, tmp_path):
     assert f.read_text() == ref
     parsed = l4io.read_line_spectrum_file(str(f))
     for b, p in zip(blocks, parsed):
-        assert np.allclose(p["flux"], b["flux"][::-1], rtol=6e-7)
+        assert np.allclose(p["flux"], b["flux"][::-1], rtol=5.01e-6, atol=0)   # E13.6
