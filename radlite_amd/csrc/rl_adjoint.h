@@ -60,8 +60,8 @@ __device__ inline T elem_rev(const FastLineT<T> &L, T I_in, T lam, T ds, const F
     const T theomax = T(0.5) * (p0.sig * jh0 + p1.sig * jh1) * ds;
     T wa, wb, xp, ra;
     qdr_weights_f<T>(dtau, wa, wb, xp, ra);
-    const T q1 = ah1 > T(0) ? jh1 / ah1 : T(0);
-    const T q0 = ah0 > T(0) ? jh0 / ah0 : T(0);
+    const T q1 = ah1 > T(0) ? FOps<T>::sdiv(jh1, ah1) : T(0);
+    const T q0 = ah0 > T(0) ? FOps<T>::sdiv(jh0, ah0) : T(0);
     const T s1 = ah0 > T(0) ? q0 : (ah1 > T(0) ? q1 : T(0));
     const T s2 = ah1 > T(0) ? q1 : (ah0 > T(0) ? q0 : T(0));
     const bool thick = dtau > T(1e-9f);
@@ -81,7 +81,7 @@ __device__ inline T elem_rev(const FastLineT<T> &L, T I_in, T lam, T ds, const F
     }
     // weights: db/dx = a/x, da/dx = xp - a/x, dxp/dx = -xp (thin: 1/2, 1/2, -1)
     T g_tau;
-    if (dtau > T(1.0e-6)) g_tau = -g_xp * xp + g_wa * (xp - ra) + g_wb * ra;
+    if (sizeof(T) == 4 || dtau > T(1.0e-6)) g_tau = -g_xp * xp + g_wa * (xp - ra) + g_wb * ra;
     else g_tau = -g_xp + T(0.5) * (g_wa + g_wb);
     // source-function fall-backs (transfer.F:1528-1541)
     T g_q0 = T(0), g_q1 = T(0);
@@ -131,7 +131,7 @@ __device__ inline T subpoint_rev(const FastRecT<T> &b, const FPtT<T> &p0, const 
                                  T e, const PGrad<T> &gq, PGrad<T> &g0, PGrad<T> &g1) {
     const T ep = T(1) - e;
     const T w0 = ep * b.rel0, w1 = e * b.rel1, W = w0 + w1;
-    const T iw = T(1) / W;
+    const T iw = FOps<T>::fdiv(T(1), W);
     const T n0 = w0 * iw, n1 = w1 * iw;
     const T xq[4] = {n0 * p0.jd + n1 * p1.jd, n0 * p0.ad + n1 * p1.ad,
                      n0 * p0.jc + n1 * p1.jc, n0 * p0.dc + n1 * p1.dc};
@@ -170,13 +170,13 @@ __device__ inline T segment_rev(const FastRecT<T> &b, const FPtT<T> &p0, const F
     SubGrid<T> sg;
     if (subgrid_of<T>(b, ds, p0.u, L.uch, sg)) {
         // forward over the sub-steps, keeping I at each element start
-        const T inv_ds = T(1) / ds;
+        const T inv_ds = FOps<T>::fdiv(T(1), ds);
         int m = 0;
         {
             FastCarryT<T> c{jl0c, al0c, T(0), init};
             if (!init) {
                 const T ah = p0.ad + al0c;
-                c.S0 = ah > T(0) ? (p0.jd + jl0c) / ah : T(0);
+                c.S0 = ah > T(0) ? FOps<T>::sdiv(p0.jd + jl0c, ah) : T(0);
             }
             int ovs = 0;
             FPtT<T> pp = p0;

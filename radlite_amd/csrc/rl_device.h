@@ -231,13 +231,27 @@ using FastRec = FastRecT<float>;
 template <typename T> struct FOps;
 template <> struct FOps<float> {
     static __device__ inline float ex(float x) { return __expf(x); }
-    static __device__ inline float fdiv(float a, float b) { return __fdividef(a, b); }
+    // a/b through the hardware reciprocal (1 ulp) for well-scaled b
+    static __device__ inline float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    // fused multiply-add: the single-precision path is not held to the
+    // reference's rounding, so it contracts (the FP64 parity path does not)
+    static __device__ inline float fma(float a, float b, float c) {
+        return __builtin_fmaf(a, b, c);
+    }
+    // a source function j/alpha (b > 0): the reciprocal unless b is so small
+    // that 1/b would overflow or lose the denormal (line wings of dust-free
+    // cells), where the quotient is divided exactly
+    static __device__ inline float sdiv(float a, float b) {
+        return b > 1.0e-30f ? a * __builtin_amdgcn_rcpf(b) : a / b;
+    }
     static __device__ inline float fmn(float a, float b) { return fminf(a, b); }
     static __device__ inline float fmx(float a, float b) { return fmaxf(a, b); }
 };
 template <> struct FOps<double> {
     static __device__ inline double ex(double x) { return exp(x); }
     static __device__ inline double fdiv(double a, double b) { return a / b; }
+    static __device__ inline double sdiv(double a, double b) { return a / b; }
+    static __device__ inline double fma(double a, double b, double c) { return a * b + c; }
     static __device__ inline double fmn(double a, double b) { return fmin(a, b); }
     static __device__ inline double fmx(double a, double b) { return fmax(a, b); }
 };
@@ -246,11 +260,25 @@ template <> struct FOps<double> {
 // adjoint: db/dx = a/x, da/dx = xp - a/x).
 template <typename T>
 __device__ inline void qdr_weights_f(T x, T &a, T &b, T &xp, T &ra) {
-    if (x > T(1.0e-6)) {
+    // single precision: the series also covers dtau <= 1e-6, where it and
+    // the thin-element weights (a = b = dtau/2, xp = 1 - dtau) agree to
+    // O(dtau^2), below float resolution; one branch fewer per element
+    if (sizeof(T) == 4 || x > T(1.0e-6)) {
         if (x < T(0.5)) {
             // b = e1/x = sum_{n>=1} (-1)^{n+1} x^n/(n+1)!,  e0 = x (1 - b)
-            T p = T(0.5) - x * (T(1) / 6 - x * (T(1) / 24 - x * (T(1) / 120 -
-                  x * (T(1) / 720 - x * (T(1) / 5040 - x * (T(1) / 40320))))));
+            T p;
+            if (sizeof(T) == 4) {
+                using O = FOps<T>;
+                p = O::fma(x, T(-1) / 40320, T(1) / 5040);
+                p = O::fma(x, -p, T(1) / 720);
+                p = O::fma(x, -p, T(1) / 120);
+                p = O::fma(x, -p, T(1) / 24);
+                p = O::fma(x, -p, T(1) / 6);
+                p = O::fma(x, -p, T(0.5));
+            } else {
+                p = T(0.5) - x * (T(1) / 6 - x * (T(1) / 24 - x * (T(1) / 120 -
+                    x * (T(1) / 720 - x * (T(1) / 5040 - x * (T(1) / 40320))))));
+            }
             if (sizeof(T) == 8)   // double: series continued to x^13/15!
                 p = T(0.5) - x * (T(1) / 6 - x * (T(1) / 24 - x * (T(1) / 120 -
                     x * (T(1) / 720 - x * (T(1) / 5040 - x * (T(1) / 40320 -
@@ -291,31 +319,32 @@ __device__ inline void elem_fast(const FastLineT<T> &L, T &I, T ds, const FPtT<T
                                  const FPtT<T> &p1, T inv_aa, T norm, FastCarryT<T> &c,
                                  int &ovs) {
     if (c.init) {
-        T z0 = (L.dnu - L.nu0 * p0.u) * inv_aa;
+        T z0 = FOps<T>::fma(-L.nu0, p0.u, L.dnu) * inv_aa;
         T phi0 = norm * FOps<T>::ex(-(z0 * z0));
         c.jl0 = p0.jc * phi0;
         c.al0 = p0.dc * phi0;
     }
-    T z1 = (L.dnu - L.nu0 * p1.u) * inv_aa;
-    T phi1 = norm * FOps<T>::ex(-(z1 * z1));
+    using O = FOps<T>;
+    T z1 = O::fma(-L.nu0, p1.u, L.dnu) * inv_aa;
+    T phi1 = norm * O::ex(-(z1 * z1));
     T jl1 = p1.jc * phi1, al1 = p1.dc * phi1;
     T jh0 = p0.jd + c.jl0, jh1 = p1.jd + jl1;      // j / sig
     T ah0 = p0.ad + c.al0, ah1 = p1.ad + al1;      // alpha / sig
-    T a0 = p0.sig * ah0, a1 = p1.sig * ah1;
-    T dtau = T(0.5) * (a0 + a1) * ds;
-    T theomax = T(0.5) * (p0.sig * jh0 + p1.sig * jh1) * ds;
+    const T hds = T(0.5) * ds;
+    T dtau = O::fma(p1.sig, ah1, p0.sig * ah0) * hds;
+    T theomax = O::fma(p1.sig, jh1, p0.sig * jh0) * hds;
     T wa, wb, xp, ra;
     qdr_weights_f<T>(dtau, wa, wb, xp, ra);
     // source functions (transfer.F:1528-1541) from the scaled values; the
     // start point equals the previous end point unless init is set
-    T q1 = ah1 > T(0) ? jh1 / ah1 : T(0);
-    T q0 = c.init ? (ah0 > T(0) ? jh0 / ah0 : T(0)) : c.S0;
+    T q1 = ah1 > T(0) ? FOps<T>::sdiv(jh1, ah1) : T(0);
+    T q0 = c.init ? (ah0 > T(0) ? FOps<T>::sdiv(jh0, ah0) : T(0)) : c.S0;
     T s1 = ah0 > T(0) ? q0 : (ah1 > T(0) ? q1 : T(0));
     T s2 = ah1 > T(0) ? q1 : (ah0 > T(0) ? q0 : T(0));
-    T q = dtau > T(1e-9f) ? wa * s1 + wb * s2 : theomax;
+    T q = dtau > T(1e-9f) ? O::fma(wa, s1, wb * s2) : theomax;
     ovs += q > T(2) * theomax;
-    q = FOps<T>::fmn(q, theomax);
-    I = I * xp + q;
+    q = O::fmn(q, theomax);
+    I = O::fma(I, xp, q);
     c.jl0 = jl1;
     c.al0 = al1;
     c.S0 = q1;
@@ -335,7 +364,8 @@ __device__ inline bool subgrid_of(const FastRecT<T> &b, T ds, T u0, T uch, SubGr
     T sright = s_c + T(3) * b.dls, sleft = s_c - T(3) * b.dls;
     if (!(sright > T(0) && sleft < ds)) return false;
     T lg = (sright - sleft) * (T(1) / T(30));
-    T f0 = -sleft / lg, f1 = (ds - sleft) / lg;
+    // only bounds of the index range (the exact 0 < s < ds test follows)
+    T f0 = FOps<T>::fdiv(-sleft, lg), f1 = FOps<T>::fdiv(ds - sleft, lg);
     sg.sleft = sleft;
     sg.lg = lg;
     sg.ilo = f0 > T(0) ? max(1, (int)FOps<T>::fmn(f0, T(40))) : 1;
@@ -350,7 +380,7 @@ __device__ inline FPtT<T> subpoint(const FastRecT<T> &b, const FPtT<T> &p0, cons
                                    T e) {
     T ep = T(1) - e;
     T w0 = ep * b.rel0, w1 = e * b.rel1, W = w0 + w1;
-    T iw = T(1) / W;
+    T iw = FOps<T>::fdiv(T(1), W);
     w0 *= iw; w1 *= iw;
     return FPtT<T>{ep * p0.u + e * p1.u, b.sigm * W, w0 * p0.jd + w1 * p1.jd,
                    w0 * p0.ad + w1 * p1.ad, w0 * p0.jc + w1 * p1.jc, w0 * p0.dc + w1 * p1.dc};
@@ -374,7 +404,7 @@ __device__ inline void segment_fast(const FastRecT<T> &b, const FPtT<T> &p0, con
     if (subgrid_of<T>(b, ds, p0.u, L.uch, sg)) {
         FPtT<T> pp = p0;
         T sa = T(0);
-        const T inv_ds = T(1) / ds;
+        const T inv_ds = FOps<T>::fdiv(T(1), ds);
         for (int i = sg.ilo; i <= sg.ihi; i++) {
             T s = sg.sleft + (T)(i - 1) * sg.lg;
             if (s > T(0) && s < ds) {

@@ -1005,7 +1005,7 @@ k_adj_rev(int nrays, int nfr, int nphi, const long long *__restrict__ off,
             const FastRecT<T> a = rec[b + k0 - 1];
             carried_in<T>(a, L, c.jl0, c.al0);
             const T ah = a.adh + c.al0;
-            c.S0 = ah > T(0) ? (a.jdh + c.jl0) / ah : T(0);
+            c.S0 = ah > T(0) ? FOps<T>::sdiv(a.jdh + c.jl0, ah) : T(0);
         }
         int ovs = 0;
         FPtT<T> p0 = point_of<T>(rec[b + k0 - 1]);
