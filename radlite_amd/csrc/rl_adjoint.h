@@ -21,7 +21,7 @@
 
 namespace rl {
 
-constexpr int ADJ_K = 16;        // checkpoint interval in segments
+constexpr int ADJ_K = 16;        // largest checkpoint interval in segments
 
 template <typename T>
 struct PGrad { T u, sig, jd, ad, jc, dc; };
@@ -90,8 +90,16 @@ __device__ inline T elem_rev(const FastLineT<T> &L, T I_in, T lam, T ds, const F
     T g_jh0 = T(0), g_jh1 = T(0), g_ah0 = T(0), g_ah1 = T(0);
     // divided, not multiplied by 1/ah: ah reaches the denormal range in the
     // line wings of dust-free cells, where 1/ah overflows while g_q/ah does not
-    if (ah1 > T(0) && g_q1 != T(0)) { g_jh1 += g_q1 / ah1; g_ah1 -= (g_q1 * q1) / ah1; }
-    if (ah0 > T(0) && g_q0 != T(0)) { g_jh0 += g_q0 / ah0; g_ah0 -= (g_q0 * q0) / ah0; }
+    if (ah1 > T(0) && g_q1 != T(0)) {
+        const T g = FOps<T>::sdiv(g_q1, ah1);
+        g_jh1 += g;
+        g_ah1 -= g * q1;
+    }
+    if (ah0 > T(0) && g_q0 != T(0)) {
+        const T g = FOps<T>::sdiv(g_q0, ah0);
+        g_jh0 += g;
+        g_ah0 -= g * q0;
+    }
     // theomax and dtau
     const T hd = T(0.5) * ds;
     g_jh0 += g_th * hd * p0.sig;

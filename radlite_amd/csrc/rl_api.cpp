@@ -60,6 +60,9 @@ struct RaySet {
     std::vector<long long> off;
     DevBuf d_off, d_nodes;
     DevBuf d_ringw;
+    // node -> cell stencil transpose for the adjoint (built on first use)
+    bool st_ready = false;
+    DevBuf d_st_vals, d_st_w, d_st_off;
     double center_w = 0.0;
 };
 
@@ -104,6 +107,7 @@ struct rl_ctx {
     size_t auto_budget = 0;
     bool use_graph = true, autotune = true;
     int fixed_cw = -1, fixed_persistent = 1, last_cw = 0, last_blocks = 0, last_batches = 0;
+    int last_kblk = 0;
     std::map<std::string, std::pair<int, int>> tune;
     std::map<std::string, hipGraphExec_t> graphs;
     // adjoint buffers
@@ -483,6 +487,39 @@ static int build_traj(rl_ctx *c, RaySet &rs) {
     HIPCHK(c, hipGetLastError());
     if (int e = check_err(c)) return e;
     rs.traj_ready = true;
+    rs.st_ready = false;
+    return 0;
+}
+
+// the stencil transpose of a ray set's nodes (once per trajectories)
+static int build_transpose(rl_ctx *c, RaySet &rs) {
+    if (rs.st_ready) return 0;
+    const int ncell = c->nr * c->nh;
+    const size_t nent = 4 * (size_t)rs.nnodes;
+    if (nent >= (size_t)1 << 31)
+        return fail(c, 13, "adjoint: %lld nodes exceed the stencil index range", rs.nnodes);
+    DevBuf keys_a, keys_b, vals_a, temp;
+    HIPCHK(c, keys_a.ensure(nent * 4));
+    HIPCHK(c, keys_b.ensure(nent * 4));
+    HIPCHK(c, vals_a.ensure(nent * 4));
+    HIPCHK(c, rs.d_st_vals.ensure(nent * 4));
+    HIPCHK(c, rs.d_st_w.ensure(nent * 8));
+    HIPCHK(c, rs.d_st_off.ensure(((size_t)ncell + 1) * 4));
+    size_t tb = 0;
+    if (rl::build_stencil_transpose(c->stream, rs.nnodes, rs.d_nodes.as<GeoNode>(), ncell,
+                                    keys_a.as<int>(), keys_b.as<int>(), vals_a.as<int>(),
+                                    rs.d_st_vals.as<int>(), rs.d_st_w.as<double>(),
+                                    rs.d_st_off.as<int>(), nullptr, &tb))
+        return fail(c, RL_E_HIP, "stencil transpose: sort size query failed");
+    HIPCHK(c, temp.ensure(tb));
+    if (int e = rl::build_stencil_transpose(c->stream, rs.nnodes, rs.d_nodes.as<GeoNode>(), ncell,
+                                            keys_a.as<int>(), keys_b.as<int>(), vals_a.as<int>(),
+                                            rs.d_st_vals.as<int>(), rs.d_st_w.as<double>(),
+                                            rs.d_st_off.as<int>(), temp.p, &tb))
+        return fail(c, RL_E_HIP, "stencil transpose: sort failed (%d)", e);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));      // temporaries are freed on return
+    rs.st_ready = true;
     return 0;
 }
 
@@ -991,7 +1028,13 @@ extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *item
     if (int e = zero_dev(c, c->d_gad, (size_t)ncell * c->nlines * 8)) return e;
     for (int i = 0; i < 5; i++) c->t_ms[i] = 0.0;
     for (int i = 0; i < 4; i++) c->counts[i] = 0;
-    const int K = rl::adjoint_block_len();
+    const int KMAX = rl::adjoint_block_len();
+    size_t budget = c->mem_budget;
+    if (budget == 0) {
+        size_t fr = 0, tot = 0;
+        HIPCHK(c, hipMemGetInfo(&fr, &tot));
+        budget = (size_t)(0.6 * (double)fr) + c->d_ckpt.n;
+    }
     size_t fo = 0;
     for (int ii = 0; ii < nitems; ii++) {
         const rl_item &it = items[ii];
@@ -999,6 +1042,12 @@ extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *item
         if (int e = setup_item(c, it, ii, su, c->ev[0], c->ev[1])) return e;
         RaySet &rs = *su.rs;
         const int nfr = it.nfr, nrays = rs.nrays;
+        // checkpoint interval: I before every segment (no recomputation in
+        // the reverse sweep) when that fits half the memory budget, else the
+        // smallest power of two up to KMAX that does
+        int K = 1;
+        while (K < KMAX && (size_t)rs.nnodes / K * nfr * tsz > budget / 2) K *= 2;
+        c->last_kblk = K;
         // checkpoint CSR: ceil((n-1)/K) blocks per ray
         std::vector<long long> ckoff(nrays + 1, 0);
         for (int r = 0; r < nrays; r++) {
@@ -1014,7 +1063,8 @@ extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *item
         HIPCHK(c, c->d_flux.ensure((size_t)nfr * 8));
         if (int e = upload(c, c->d_gflux, dL_dflux + fo, (size_t)nfr * 8)) return e;
         if (int e = zero_dev(c, c->d_gnode, (size_t)rs.nnodes * 5 * 8)) return e;
-        if (int e = zero_dev(c, c->d_gcell, (size_t)ncell * 5 * 8)) return e;
+        HIPCHK(c, c->d_gcell.ensure((size_t)rs.nnodes * 5 * 8));
+        if (int e = build_transpose(c, rs)) return e;
         rl::AdjLaunch a;
         a.nrays = nrays; a.nfr = nfr; a.nrings = rs.nrings; a.nphi = rs.nphi;
         a.nnodes = rs.nnodes;
@@ -1028,6 +1078,7 @@ extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *item
         a.rec = c->d_arec.p;
         a.rinfo = c->d_rinfo.as<RayInfo>();
         a.ckpt = c->d_ckpt.p;
+        a.kblk = K;
         a.dnu = c->d_chan.as<double>();
         a.velo = a.dnu + nfr;
         a.istar = a.dnu + 2 * nfr;
@@ -1042,7 +1093,10 @@ extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *item
         a.d2 = (1.0 * 3.08572e18) * (1.0 * 3.08572e18);
         a.gflux = c->d_gflux.as<double>();
         a.gnode = c->d_gnode.as<double>();
-        a.gcell = c->d_gcell.as<double>();
+        a.gtrue = c->d_gcell.as<double>();
+        a.st_off = rs.d_st_off.as<int>();
+        a.st_vals = rs.d_st_vals.as<int>();
+        a.st_w = rs.d_st_w.as<double>();
         a.rho = c->d_rho.as<double>();
         a.abund = c->d_abund.as<double>();
         a.pops = c->d_pops.as<double>();

@@ -113,6 +113,7 @@ struct AdjLaunch {
     void *rec;                         // FastRecT<T>[nnodes]
     RayInfo *rinfo;
     void *ckpt;                        // T[sum blocks][nfr]
+    int kblk;                          // checkpoint interval in segments (1..16)
     const double *dnu, *velo, *istar;  // per channel
     LineConst lc;
     double aks2;
@@ -123,8 +124,10 @@ struct AdjLaunch {
     double *partial, *flux;
     double d2;
     const double *gflux;               // device [nfr] dL/dflux
-    double *gnode;                     // [nnodes][5]
-    double *gcell;                     // [ncell][5] (jd, ad, lw, N_up, N_low)
+    double *gnode;                     // [nnodes][5] scaled-unit node gradients
+    double *gtrue;                     // [nnodes][5] (jd, ad, lw, N_up, N_low)
+    const int *st_off, *st_vals;       // stencil transpose: cell -> entries 4k+j
+    const double *st_w;                // entry weights
     const double *rho, *abund, *pops;
     double *g_rho, *g_abund, *g_lw, *g_pops, *g_jd, *g_ad;   // accumulated over items
 };
@@ -144,6 +147,12 @@ void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int
 template <typename T> void launch_item(hipStream_t st, const ItemLaunch &it);
 template <typename T> void launch_adjoint(hipStream_t st, const AdjLaunch &a);
 int adjoint_block_len();
+// Stencil transpose (cell -> node entries) of a node set; with temp ==
+// nullptr only *temp_bytes is set.  keys/vals: 4*nnodes ints each (a, b);
+// w: 4*nnodes doubles; off: ncell+1 ints.
+int build_stencil_transpose(hipStream_t st, long long nnodes, const GeoNode *geo, int ncell,
+                            int *keys_a, int *keys_b, int *vals_a, int *vals_b, double *w,
+                            int *off, void *temp, size_t *temp_bytes);
 size_t fast_rec_size(int precision_fp64);
 
 }  // namespace rl

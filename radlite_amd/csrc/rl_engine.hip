@@ -24,6 +24,8 @@
 
 #include <type_traits>
 
+#include <hipcub/hipcub.hpp>
+
 #include "rl_traj.h"
 #include "rl_adjoint.h"
 
@@ -910,7 +912,7 @@ k_adj_fwd(int nrays, int nfr, const long long *__restrict__ off,
           const long long *__restrict__ ckoff, const FastRecT<T> *__restrict__ rec,
           const double *__restrict__ dnu, const double *__restrict__ velo,
           const double *__restrict__ istar, LineConst lc, double *__restrict__ image,
-          T *__restrict__ ckpt) {
+          T *__restrict__ ckpt, int kblk) {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long long)nrays * nfr) return;
     const int ray = (int)(g / nfr), ch = (int)(g - (long long)ray * nfr);
@@ -926,7 +928,7 @@ k_adj_fwd(int nrays, int nfr, const long long *__restrict__ off,
     T *ck = ckpt + ckoff[ray] * nfr + ch;
     FPtT<T> p0 = point_of<T>(rec[b]);
     for (int k = 1; k < n; k++) {
-        if ((k - 1) % ADJ_K == 0) ck[(long long)((k - 1) / ADJ_K) * nfr] = I;
+        if ((k - 1) % kblk == 0) ck[(long long)((k - 1) / kblk) * nfr] = I;
         const FastRecT<T> r = rec[b + k];
         const FPtT<T> p1 = point_of<T>(r);
         segment_fast<T>(r, p0, p1, L, I, ist, sf, c, ovs);
@@ -952,6 +954,37 @@ __device__ inline void wave_add5(double *out, T a0, T a1, T a2, T a3, T a4) {
     }
 }
 
+// Node gradients of a wave, summed over its lanes in batches of RED_N
+// nodes through LDS: each lane writes its values in row `lane` (41 words,
+// so that the writes of 64 lanes fall in distinct banks), then lane p sums
+// column p (node p/5, quantity p%5) over the 64 rows.
+constexpr int RED_N = 8;
+
+template <typename T>
+struct WaveNodeSum {
+    T (*row)[RED_N * 5 + 1];   // [64][41] in LDS
+    long long *node;           // [RED_N] in LDS
+    int n = 0;
+
+    __device__ void flush(double *gnode) {
+        __syncthreads();
+        const int lane = threadIdx.x;
+        for (int p = lane; p < n * 5; p += 64) {
+            T acc = T(0);
+            for (int i = 0; i < 64; i++) acc += row[i][p];
+            if (acc != T(0)) atomicAdd(gnode + node[p / 5] * 5 + (p % 5), (double)acc);
+        }
+        __syncthreads();
+        n = 0;
+    }
+    __device__ void push(double *gnode, long long nd, T a0, T a1, T a2, T a3, T a4) {
+        T *r = row[threadIdx.x] + n * 5;
+        r[0] = a0; r[1] = a1; r[2] = a2; r[3] = a3; r[4] = a4;
+        if (threadIdx.x == 0) node[n] = nd;
+        if (++n == RED_N) flush(gnode);
+    }
+};
+
 // line values carried into segment k: node k-1 with segment k-1's profile
 template <typename T>
 __device__ inline void carried_in(const FastRecT<T> &a, const FastLineT<T> &L, T &jl, T &al) {
@@ -972,7 +1005,9 @@ k_adj_rev(int nrays, int nfr, int nphi, const long long *__restrict__ off,
           const double *__restrict__ dnu, const double *__restrict__ velo,
           const double *__restrict__ istar, LineConst lc, const double *__restrict__ gflux,
           const double *__restrict__ ring_w, double center_w, double d2,
-          const T *__restrict__ ckpt, double *__restrict__ gnode) {
+          const T *__restrict__ ckpt, double *__restrict__ gnode, int kblk) {
+    __shared__ T red_row[64][RED_N * 5 + 1];
+    __shared__ long long red_node[RED_N];
     const int ray = blockIdx.x;
     const int lane = threadIdx.x;
     const int ch = blockIdx.y * 64 + lane;
@@ -994,10 +1029,11 @@ k_adj_rev(int nrays, int nfr, int nphi, const long long *__restrict__ off,
     pg_zero(prv);
     T iseg[ADJ_K];
     T isub[33];
+    WaveNodeSum<T> red{red_row, red_node};
     const int nseg = n - 1;
-    const int nblk = (nseg + ADJ_K - 1) / ADJ_K;
+    const int nblk = (nseg + kblk - 1) / kblk;
     for (int blk = nblk - 1; blk >= 0; blk--) {
-        const int k0 = blk * ADJ_K + 1, k1 = min(k0 + ADJ_K - 1, nseg);
+        const int k0 = blk * kblk + 1, k1 = min(k0 + kblk - 1, nseg);
         // forward over the block from its checkpoint
         T I = ckpt[(ckoff[ray] + blk) * nfr + chc];
         FastCarryT<T> c{T(0), T(0), T(0), k0 == 1 ? 1 : 0};
@@ -1011,6 +1047,7 @@ k_adj_rev(int nrays, int nfr, int nphi, const long long *__restrict__ off,
         FPtT<T> p0 = point_of<T>(rec[b + k0 - 1]);
         for (int k = k0; k <= k1; k++) {
             iseg[k - k0] = I;
+            if (k == k1) break;                // the block's last segment is only reversed
             const FastRecT<T> r = rec[b + k];
             const FPtT<T> p1 = point_of<T>(r);
             if (active) segment_fast<T>(r, p0, p1, L, I, ist, sf, c, ovs);
@@ -1030,20 +1067,65 @@ k_adj_rev(int nrays, int nfr, int nphi, const long long *__restrict__ off,
             }
             // d lw_mean: phi through inv_aa = 1/(c_aa lw), sub-grid width dls ~ lw
             const T g_lw = (g_dls * r.dls - g_lnaa) * cai * r.inv_aa;
-            wave_add5<T>(gnode + (b + k) * 5, cur.jd, cur.ad, cur.jc, cur.dc, g_lw);
+            red.push(gnode, b + k, cur.jd, cur.ad, cur.jc, cur.dc, g_lw);
             cur = prv;
             pg_zero(prv);
         }
     }
-    wave_add5<T>(gnode + b * 5, cur.jd, cur.ad, cur.jc, cur.dc, T(0));
+    red.push(gnode, b, cur.jd, cur.ad, cur.jc, cur.dc, T(0));
+    red.flush(gnode);
 }
 
-// node gradients -> cell gradients through the stencil of
-// get_line_dust_values (line.F:4006-4208); scaled -> true values
-__global__ void k_node_grad(long long nnodes, const GeoNode *__restrict__ geo,
-                            const CellRec *__restrict__ cells,
-                            const double *__restrict__ gnode, LineConst lc,
-                            double *__restrict__ gcell) {
+// ---- node -> cell transpose of the interpolation stencil (once per ray set)
+__device__ inline void stencil_weights(const GeoNode &gn, double w[4]) {
+    const double dr = gn.dr, dt = gn.dt;
+    const int icr = gn.flag & 3;
+    w[0] = w[1] = w[2] = w[3] = 0.0;
+    if (icr == 1) { w[0] = 1.0 - dt; w[1] = dt; }
+    else if (icr == 2) { w[0] = 1.0 - dr; w[2] = dr; }
+    else {
+        w[0] = (1.0 - dr) * (1.0 - dt); w[1] = (1.0 - dr) * dt;
+        w[2] = dr * (1.0 - dt); w[3] = dr * dt;
+    }
+}
+
+// entry 4k+j: stencil cell j of node k; key ncell for a zero weight
+__global__ void k_stencil_entries(long long nnodes, const GeoNode *__restrict__ geo, int ncell,
+                                  int *__restrict__ keys, int *__restrict__ vals,
+                                  double *__restrict__ w) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnodes) return;
+    const GeoNode gn = geo[k];
+    double wk[4];
+    stencil_weights(gn, wk);
+    const int c[4] = {gn.c00, gn.c01, gn.c10, gn.c11};
+    for (int j = 0; j < 4; j++) {
+        const long long e = 4 * k + j;
+        keys[e] = wk[j] != 0.0 ? c[j] : ncell;
+        vals[e] = (int)e;
+        w[e] = wk[j];
+    }
+}
+
+// off[c] = first sorted entry of cell c (binary search; off[ncell] = end
+// of the entries with a weight)
+__global__ void k_cell_offsets(int ncell, long long nent, const int *__restrict__ keys,
+                               int *__restrict__ off) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncell) return;
+    long long lo = 0, hi = nent;
+    while (lo < hi) {
+        const long long m = (lo + hi) >> 1;
+        if (keys[m] < c) lo = m + 1; else hi = m;
+    }
+    off[c] = (int)lo;
+}
+
+// node gradients in scaled units -> true node values: jd, ad, lw (the
+// mean width of segments k and k+1 split to their nodes), N_up, N_low
+__global__ void k_node_conv(long long nnodes, const GeoNode *__restrict__ geo,
+                            const CellRec *__restrict__ cells, const double *__restrict__ gnode,
+                            LineConst lc, double *__restrict__ gtrue) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nnodes) return;
     const GeoNode gn = geo[k];
@@ -1051,56 +1133,68 @@ __global__ void k_node_grad(long long nnodes, const GeoNode *__restrict__ geo,
     node_interp(gn, cells, v);
     scaled_node(v, lc, sig, jc, dc);
     const double *g = gnode + k * 5;
-    const double g_jd = g[0] / sig, g_ad = g[1] / sig, g_jc = g[2] / sig, g_dc = g[3] / sig;
-    // segment k's mean width is (lw_{k-1} + lw_k)/2; a ray's first node
-    // carries no segment gradient, so the next node's slot is 0 across rays
-    const double g_lw = 0.5 * (g[4] + (k + 1 < nnodes ? g[9] : 0.0));
-    double gf[5];
-    gf[CR_JD] = g_jd;
-    gf[CR_AD] = g_ad;
-    gf[CR_LW] = g_lw;
-    gf[CR_NU] = lc.hknu * lc.A * g_jc - lc.hknu * lc.Bud * g_dc;   // jc = hknu N_u A
-    gf[CR_NL] = lc.hknu * lc.Bdu * g_dc;                           // dc = hknu (N_l B_lu - N_u B_ul)
-    const double dr = gn.dr, dt = gn.dt;
-    const int icr = gn.flag & 3;
-    double w00, w01 = 0.0, w10 = 0.0, w11 = 0.0;
-    if (icr == 1) { w00 = 1.0 - dt; w01 = dt; }
-    else if (icr == 2) { w00 = 1.0 - dr; w10 = dr; }
-    else {
-        w00 = (1.0 - dr) * (1.0 - dt); w01 = (1.0 - dr) * dt;
-        w10 = dr * (1.0 - dt); w11 = dr * dt;
-    }
-    for (int f = 0; f < 5; f++) {
-        if (gf[f] == 0.0) continue;
-        if (w00 != 0.0) atomicAdd(gcell + (long long)gn.c00 * 5 + f, w00 * gf[f]);
-        if (w01 != 0.0) atomicAdd(gcell + (long long)gn.c01 * 5 + f, w01 * gf[f]);
-        if (w10 != 0.0) atomicAdd(gcell + (long long)gn.c10 * 5 + f, w10 * gf[f]);
-        if (w11 != 0.0) atomicAdd(gcell + (long long)gn.c11 * 5 + f, w11 * gf[f]);
-    }
+    const double g_jc = g[2] / sig, g_dc = g[3] / sig;
+    double *o = gtrue + k * 5;
+    o[CR_JD] = g[0] / sig;
+    o[CR_AD] = g[1] / sig;
+    // a ray's first node carries no segment gradient, so the next node's
+    // slot is 0 across rays
+    o[CR_LW] = 0.5 * (g[4] + (k + 1 < nnodes ? g[9] : 0.0));
+    o[CR_NU] = lc.hknu * lc.A * g_jc - lc.hknu * lc.Bud * g_dc;   // jc = hknu N_u A
+    o[CR_NL] = lc.hknu * lc.Bdu * g_dc;                           // dc = hknu (N_l B_lu - N_u B_ul)
 }
 
-// cell gradients -> the inputs of the trace: N = pop * abund * rho * m_pg
-// (line.F:3998), dust j/alpha per line, width
-__global__ void k_cell_grad(int ncell, int nlev, int nlines, int line, int lev_up, int lev_low,
-                            double molpg, const double *__restrict__ rho,
-                            const double *__restrict__ abund, const double *__restrict__ pops,
-                            const double *__restrict__ gcell, double *__restrict__ g_rho,
-                            double *__restrict__ g_abund, double *__restrict__ g_lw,
-                            double *__restrict__ g_pops, double *__restrict__ g_jd,
-                            double *__restrict__ g_ad) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    const double *g = gcell + (size_t)c * 5;
-    const double gnu = g[CR_NU], gnl = g[CR_NL];
+// one wave per cell: the stencil-weighted sum of its nodes' gradients,
+// then N = pop * abund * rho * m_pg (line.F:3998) to the trace's inputs
+__global__ void __launch_bounds__(64)
+k_cell_gather(int ncell, const int *__restrict__ off, const int *__restrict__ vals,
+              const double *__restrict__ w, const double *__restrict__ gtrue, int nlev,
+              int nlines, int line, int lev_up, int lev_low, double molpg,
+              const double *__restrict__ rho, const double *__restrict__ abund,
+              const double *__restrict__ pops, double *__restrict__ g_rho,
+              double *__restrict__ g_abund, double *__restrict__ g_lw,
+              double *__restrict__ g_pops, double *__restrict__ g_jd,
+              double *__restrict__ g_ad) {
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
+    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = off[c] + lane; i < off[c + 1]; i += 64) {
+        const int e = vals[i];
+        const double we = w[e];
+        const double *g = gtrue + (long long)(e >> 2) * 5;
+        for (int f = 0; f < 5; f++) a[f] += we * g[f];
+    }
+    for (int m = 32; m >= 1; m >>= 1)
+        for (int f = 0; f < 5; f++) a[f] += __shfl_xor(a[f], m);
+    if (lane != 0) return;
+    const double gnu = a[CR_NU], gnl = a[CR_NL];
     const double pu = pops[(size_t)c * nlev + lev_up], pl = pops[(size_t)c * nlev + lev_low];
     const double ab = abund[c], rh = rho[c];
-    g_jd[(size_t)c * nlines + line] += g[CR_JD];
-    g_ad[(size_t)c * nlines + line] += g[CR_AD];
-    g_lw[c] += g[CR_LW];
+    g_jd[(size_t)c * nlines + line] += a[CR_JD];
+    g_ad[(size_t)c * nlines + line] += a[CR_AD];
+    g_lw[c] += a[CR_LW];
     g_pops[(size_t)c * nlev + lev_up] += gnu * ab * rh * molpg;
     g_pops[(size_t)c * nlev + lev_low] += gnl * ab * rh * molpg;
     g_abund[c] += (gnu * pu + gnl * pl) * rh * molpg;
     g_rho[c] += (gnu * pu + gnl * pl) * ab * molpg;
+}
+
+int build_stencil_transpose(hipStream_t st, long long nnodes, const GeoNode *geo, int ncell,
+                            int *keys_a, int *keys_b, int *vals_a, int *vals_b, double *w,
+                            int *off, void *temp, size_t *temp_bytes) {
+    const long long nent = 4 * nnodes;
+    if (temp == nullptr) {
+        return (int)hipcub::DeviceRadixSort::SortPairs(nullptr, *temp_bytes, keys_a, keys_b,
+                                                       vals_a, vals_b, (int)nent, 0, 32, st);
+    }
+    hipLaunchKernelGGL(k_stencil_entries, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0,
+                       st, nnodes, geo, ncell, keys_a, vals_a, w);
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, keys_a, keys_b, vals_a,
+                                                      vals_b, (int)nent, 0, 32, st);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(k_cell_offsets, dim3((ncell + 1 + 255) / 256), dim3(256), 0, st, ncell,
+                       nent, keys_b, off);
+    return 0;
 }
 
 template <typename T>
@@ -1112,7 +1206,7 @@ void launch_adjoint(hipStream_t st, const AdjLaunch &a) {
     const long long total = (long long)a.nrays * a.nfr;
     hipLaunchKernelGGL(k_adj_fwd<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
                        a.nrays, a.nfr, a.off, a.ckoff, rec, a.dnu, a.velo, a.istar, a.lc,
-                       a.image, ck);
+                       a.image, ck, a.kblk);
     dim3 gp((a.nfr + 63) / 64, a.nchunks);
     hipLaunchKernelGGL(k_flux_partial, gp, dim3(64), 0, st, a.nfr, a.nrings, a.nphi, 1, a.image,
                        a.ring_w, a.center_w, a.partial);
@@ -1120,12 +1214,13 @@ void launch_adjoint(hipStream_t st, const AdjLaunch &a) {
                        a.nchunks, a.partial, a.d2, a.flux);
     hipLaunchKernelGGL(k_adj_rev<T>, dim3(a.nrays, (a.nfr + 63) / 64), dim3(64), 0, st,
                        a.nrays, a.nfr, a.nphi, a.off, a.ckoff, rec, a.dnu, a.velo, a.istar,
-                       a.lc, a.gflux, a.ring_w, a.center_w, a.d2, ck, a.gnode);
-    hipLaunchKernelGGL(k_node_grad, dim3((unsigned)((a.nnodes + 255) / 256)), dim3(256), 0, st,
-                       a.nnodes, a.geo, a.cells, a.gnode, a.lc, a.gcell);
-    hipLaunchKernelGGL(k_cell_grad, dim3((a.ncell + 255) / 256), dim3(256), 0, st, a.ncell,
-                       a.nlev, a.nlines, a.line, a.lev_up, a.lev_low, a.molpg, a.rho, a.abund,
-                       a.pops, a.gcell, a.g_rho, a.g_abund, a.g_lw, a.g_pops, a.g_jd, a.g_ad);
+                       a.lc, a.gflux, a.ring_w, a.center_w, a.d2, ck, a.gnode, a.kblk);
+    hipLaunchKernelGGL(k_node_conv, dim3((unsigned)((a.nnodes + 255) / 256)), dim3(256), 0, st,
+                       a.nnodes, a.geo, a.cells, a.gnode, a.lc, a.gtrue);
+    hipLaunchKernelGGL(k_cell_gather, dim3(a.ncell), dim3(64), 0, st, a.ncell, a.st_off,
+                       a.st_vals, a.st_w, a.gtrue, a.nlev, a.nlines, a.line, a.lev_up,
+                       a.lev_low, a.molpg, a.rho, a.abund, a.pops, a.g_rho, a.g_abund, a.g_lw,
+                       a.g_pops, a.g_jd, a.g_ad);
 }
 
 template void launch_adjoint<double>(hipStream_t, const AdjLaunch &);
