@@ -44,10 +44,12 @@ def measured_traffic(precision, kernel):
 
 def bench_adjoint(args):
     """Forward vs forward+adjoint on the 200x100 disk (34,049 rays x 100
-    channels, 45 deg, every channel integrated): one step is the spectrum
-    (rl_trace_lines_forward) and then its gradient for a chi^2 loss
-    (rl_trace_lines_adjoint, which runs its own checkpointed forward and
-    the reverse sweep, down to per-cell gradients)."""
+    channels, 45 deg, every channel integrated).  Forward: the spectrum
+    (rl_trace_lines_forward).  Forward+adjoint, one training step: the
+    spectrum with the adjoint's tape kept (rl_trace_lines_forward_tape: I
+    checkpoints) and then the gradient of a chi^2 loss
+    (rl_trace_lines_adjoint: the reverse sweep from the tape, down to
+    per-cell gradients)."""
     from radlite_amd import engine, synth
     ri, w = synth.workload_inputs("c4", seed=0)
     nfr, vmax = engine.passband(ri.vmax, ri.dv)
@@ -63,12 +65,15 @@ def bench_adjoint(args):
         return rt.trace(items)[0]
 
     def fwd_adj():
-        f = rt.trace(items)[0][0]
+        f = rt.trace_tape(items)[0]
         g = 2.0 * (f - target) / sigma ** 2
         return rt.adjoint(items, [g])
 
+    def fwd_tape():
+        return rt.trace_tape(items)
+
     res = {}
-    for name, fn in (("fwd", fwd), ("fwd_adj", fwd_adj)):
+    for name, fn in (("fwd", fwd), ("fwd_tape", fwd_tape), ("fwd_adj", fwd_adj)):
         for _ in range(max(1, args.warmup)):
             fn()
         rt.L.rl_sync(rt.ctx)
@@ -80,14 +85,18 @@ def bench_adjoint(args):
         if name == "fwd_adj":
             st = rt.stats()
             res["adj_device_ms"] = st["t_formal_ms"]
+            res["kblk"], res["used_tape"] = rt.adjoint_info()
     nominal = rt.nrays[0] * nfr
     rt.close()
     return {
         "workload": f"c4: nr={w['nr']} x nth={w['nth']}, {nominal // nfr} rays x {nfr} ch x 1 incl",
         "precision": args.precision,
         "fwd_ms": res["fwd"] * 1e3,
+        "fwd_tape_ms": res["fwd_tape"] * 1e3,
         "fwd_adj_ms": res["fwd_adj"] * 1e3,
         "adjoint_call_device_ms": res["adj_device_ms"],
+        "checkpoint_interval": res["kblk"],
+        "adjoint_from_tape": bool(res["used_tape"]),
         "ratio_fwd_adj_over_fwd": res["fwd_adj"] / res["fwd"],
         "value_fwd": nominal / res["fwd"],
         "value_fwd_adj": nominal / res["fwd_adj"],

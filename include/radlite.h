@@ -117,9 +117,12 @@ int rl_trace_lines_forward_ex(rl_ctx *ctx, int nitems, const rl_item *items,
                               unsigned *warn_bits);
 
 /* Adjoint of the trace for the same items (every item must use
- * RL_SKIP_NONE).  Given dL/dflux (same layout as flux), runs the forward
- * again (flux, optional, receives it) and returns dL/d(the inputs the
- * trace reads), summed over the items: g_rho, g_abund, g_lw [nr][nth];
+ * RL_SKIP_NONE).  Given dL/dflux (same layout as flux), returns dL/d(the
+ * inputs the trace reads) and, in flux (optional), the spectrum.  When
+ * the last rl_trace_lines_forward_tape call was on the same items and no
+ * input has been set since, only the reverse sweep runs, from that tape;
+ * otherwise the checkpointed forward runs first, item by item.  Gradients
+ * are summed over the items: g_rho, g_abund, g_lw [nr][nth];
  * g_pops [nr][nth][nlev]; g_jd, g_ad [nr][nth][nlines].  Any output may
  * be NULL; outputs are overwritten.  The reference has no adjoint: this
  * differentiates its formal solution (charintline telescope.F:3889,
@@ -129,6 +132,22 @@ int rl_trace_lines_adjoint(rl_ctx *ctx, int nitems, const rl_item *items,
                            const double *dL_dflux, double *flux, double *g_rho,
                            double *g_abund, double *g_lw, double *g_pops,
                            double *g_jd, double *g_ad);
+
+/* Forward trace of RL_SKIP_NONE items that keeps, per item, the node
+ * records and I before every K-th segment of every (ray, channel) on the
+ * device (the tape) for a following rl_trace_lines_adjoint: the forward
+ * half of a training step.  flux as in rl_trace_lines_forward.  The tape
+ * replaces the previous one and is dropped by rl_tape_release or
+ * rl_destroy; any rl_set_* or rl_setup_rays_circular call makes it stale.
+ * No counterpart in the reference (its trace has no adjoint). */
+int rl_trace_lines_forward_tape(rl_ctx *ctx, int nitems, const rl_item *items,
+                                double *flux);
+int rl_tape_release(rl_ctx *ctx);
+
+/* Checkpoint interval K (segments) of the last adjoint or tape call, and
+ * whether the last adjoint call ran from the tape (1) or ran its own
+ * forward (0). */
+int rl_last_adjoint_info(const rl_ctx *ctx, int *ckpt_interval, int *used_tape);
 
 /* ---- batching controls (no counterpart in the reference) ---------- */
 /* A call to rl_trace_lines_forward runs its items in batches: every
@@ -149,8 +168,12 @@ enum {
     RL_OPT_TILE = 3,      /* fix the tile mapping (turns the autotuner off):
                              0 = ray-major (ray, channel) pairs, 16/32/64 =
                              channels per ray in a wave */
-    RL_OPT_PERSISTENT = 4 /* with a fixed mapping: 1 (default) persistent grid
+    RL_OPT_PERSISTENT = 4,/* with a fixed mapping: 1 (default) persistent grid
                              over the work counter, 0 static grid */
+    RL_OPT_CKPT = 5       /* adjoint checkpoint interval: 0 (default) chosen
+                             from the memory budget (1 when every segment's I
+                             fits half of it, else up to 16), or fixed to
+                             1, 2, 4, 8 or 16 segments */
 };
 int rl_set_option(rl_ctx *ctx, int option, int value);
 

@@ -49,6 +49,13 @@ struct DevBuf {
     template <typename T> T *as() const { return static_cast<T *>(p); }
 };
 
+// One item's adjoint state: node records, the checkpoint CSR and the I
+// checkpoints.  The tape keeps one per item between
+// rl_trace_lines_forward_tape and rl_trace_lines_adjoint.
+struct AdjItemBufs {
+    DevBuf rec, ckoff, ckpt;
+};
+
 struct RaySet {
     double theta0 = 0.0, costh0 = 1.0, sinth0 = 0.0;
     int nrays = 0, nrings = 0, nphi = 0;
@@ -107,10 +114,18 @@ struct rl_ctx {
     size_t auto_budget = 0;
     bool use_graph = true, autotune = true;
     int fixed_cw = -1, fixed_persistent = 1, last_cw = 0, last_blocks = 0, last_batches = 0;
-    int last_kblk = 0;
+    int last_kblk = 0, fixed_kblk = 0, last_used_tape = 0;
+    // inputs generation: bumped by every setter, so that a tape taken
+    // before a change is not used after it
+    unsigned long long gen = 1;
     std::map<std::string, std::pair<int, int>> tune;
     std::map<std::string, hipGraphExec_t> graphs;
-    // adjoint buffers
+    // adjoint buffers; the tape of rl_trace_lines_forward_tape
+    std::vector<AdjItemBufs> tape;
+    std::vector<rl_item> tape_items;
+    std::vector<double> tape_flux;
+    unsigned long long tape_gen = 0;
+    int tape_kblk = 0;
     DevBuf d_arec, d_ckpt, d_ckoff, d_gflux, d_gnode, d_gcell, d_grho, d_gabund, d_glw,
         d_gpops, d_gjd, d_gad;
     // stats
@@ -199,6 +214,7 @@ extern "C" int rl_last_stats(const rl_ctx *c, double *t, long long *n) {
 // inputs
 // ---------------------------------------------------------------------
 extern "C" int rl_set_grid(rl_ctx *c, int nr, const double *r, int nh, const double *th) {
+    if (c) c->gen++;                         // invalidates a tape
     if (!c || !r || !th || nr < 3 || nh < 2) return fail(c, RL_E_ARG, "rl_set_grid: bad args");
     HIPCHK(c, hipSetDevice(c->device));
     for (int i = 1; i < nr; i++)
@@ -243,6 +259,7 @@ extern "C" int rl_set_grid(rl_ctx *c, int nr, const double *r, int nh, const dou
 extern "C" int rl_set_medium(rl_ctx *c, const double *rho, const double *abund,
                              const double *vel, const double *lw, double umass_av,
                              double aksmax) {
+    if (c) c->gen++;                         // invalidates a tape
     if (!c || !c->nr) return fail(c, RL_E_ARG, "rl_set_medium: set the grid first");
     if (!rho || !abund || !vel || !lw || !(umass_av > 0.0))
         return fail(c, RL_E_ARG, "rl_set_medium: bad args");
@@ -268,6 +285,7 @@ extern "C" int rl_set_medium(rl_ctx *c, const double *rho, const double *abund,
 }
 
 extern "C" int rl_set_levels(rl_ctx *c, int nlev, const double *pops) {
+    if (c) c->gen++;                         // invalidates a tape
     if (!c || !c->nr || nlev < 2 || !pops) return fail(c, RL_E_ARG, "rl_set_levels: bad args");
     HIPCHK(c, hipSetDevice(c->device));
     c->nlev = nlev;
@@ -279,6 +297,7 @@ extern "C" int rl_set_levels(rl_ctx *c, int nlev, const double *pops) {
 
 extern "C" int rl_set_lines(rl_ctx *c, int nl, const double *nu0, const double *A,
                             const double *gu, const double *gl, const int *lu, const int *ll) {
+    if (c) c->gen++;                         // invalidates a tape
     if (!c || nl < 1 || !nu0 || !A || !gu || !gl || !lu || !ll)
         return fail(c, RL_E_ARG, "rl_set_lines: bad args");
     for (int i = 0; i < nl; i++) {                           // line.F:1739-1770
@@ -300,6 +319,7 @@ extern "C" int rl_set_lines(rl_ctx *c, int nl, const double *nu0, const double *
 }
 
 extern "C" int rl_set_dust_line(rl_ctx *c, const double *jd, const double *ad) {
+    if (c) c->gen++;                         // invalidates a tape
     if (!c || !c->nr || !c->nlines || !jd || !ad)
         return fail(c, RL_E_ARG, "rl_set_dust_line: set grid and lines first");
     HIPCHK(c, hipSetDevice(c->device));
@@ -314,6 +334,7 @@ extern "C" int rl_set_dust_line(rl_ctx *c, const double *jd, const double *ad) {
 
 extern "C" int rl_set_star(rl_ctx *c, double rstar, int ncont, const double *nu,
                            const double *istar) {
+    if (c) c->gen++;                         // invalidates a tape
     if (!c || rstar < 0.0 || ncont < 0 || (ncont > 0 && (!nu || !istar)))
         return fail(c, RL_E_ARG, "rl_set_star: bad args");
     if (c->nr && rstar >= c->rc[2]) return fail(c, 13, "star is bigger than inner grid radius");
@@ -325,6 +346,7 @@ extern "C" int rl_set_star(rl_ctx *c, double rstar, int ncont, const double *nu,
 }
 
 extern "C" int rl_set_outer_bc(rl_ctx *c, int type) {
+    if (c) c->gen++;                         // invalidates a tape
     if (!c) return RL_E_ARG;
     if (type != 0 && type != 2)
         return fail(c, 13, "outer boundary type %d is not supported (0 or 2)", type);
@@ -422,6 +444,7 @@ static int build_rayset(rl_ctx *c, RaySet &rs, double anginf, int nphi, int nrex
 extern "C" int rl_setup_rays_circular(rl_ctx *c, int nincl, const double *incl, int nphi,
                                       int nrext, int dbdr, int *n_rays, int *n_rings,
                                       double *ring_r, double *ring_edge) {
+    if (c) c->gen++;                         // invalidates a tape
     if (!c || !c->nr || nincl < 1 || !incl)
         return fail(c, RL_E_ARG, "rl_setup_rays_circular: set the grid first");
     HIPCHK(c, hipSetDevice(c->device));
@@ -985,6 +1008,12 @@ extern "C" int rl_set_option(rl_ctx *c, int option, int value) {
             c->tune.clear();
             c->fixed_persistent = value != 0;
             return 0;
+        case RL_OPT_CKPT:
+            if (value < 0 || value > rl::adjoint_block_len() || (value & (value - 1)))
+                return fail(c, RL_E_ARG, "rl_set_option: checkpoint interval must be 0 or a "
+                                         "power of two up to %d", rl::adjoint_block_len());
+            c->fixed_kblk = value;
+            return 0;
         default: return fail(c, RL_E_ARG, "rl_set_option: unknown option %d", option);
     }
 }
@@ -1003,22 +1032,216 @@ static int zero_dev(rl_ctx *c, DevBuf &b, size_t bytes) {
     return 0;
 }
 
+static size_t adj_budget(rl_ctx *c, size_t held) {
+    if (c->mem_budget) return c->mem_budget;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return held;
+    return (size_t)(0.6 * (double)fr) + held;
+}
+
+// checkpoint interval for a call: a fixed one (RL_OPT_CKPT), else I
+// before every segment (no recomputation in the reverse sweep) when the
+// checkpoints of all the call's items fit half the memory budget, else
+// the smallest power of two up to the maximum that does (the maximum if
+// none does)
+static int pick_kblk(rl_ctx *c, int nitems, const rl_item *items, size_t budget) {
+    const int KMAX = rl::adjoint_block_len();
+    if (c->fixed_kblk) return c->fixed_kblk;
+    const size_t tsz = c->precision == RL_FP64 ? 8 : 4;
+    double need = 0.0;                       // bytes at K = 1
+    for (int ii = 0; ii < nitems; ii++) {
+        const int in = items[ii].incl;
+        if (in < 0 || in >= (int)c->incl.size()) continue;
+        const RaySet &rs = c->incl[in];
+        // before the first trajectory build nnodes is unknown: assume the
+        // per-ray maximum of segments
+        const double nn = rs.traj_ready
+                              ? (double)rs.nnodes
+                              : (double)rs.nrays * (2.0 * c->nr + 2.0 * c->nh + 34.0);
+        need += nn * items[ii].nfr * (double)tsz;
+    }
+    int K = 1;
+    while (K < KMAX && need / K > 0.5 * (double)budget) K *= 2;
+    return K;
+}
+
+static void fill_adj(rl_ctx *c, const rl_item &it, const ItemSetup &su, AdjItemBufs &ab, int K,
+                     rl::AdjLaunch &a) {
+    const RaySet &rs = *su.rs;
+    const int nfr = it.nfr;
+    a.nrays = rs.nrays; a.nfr = nfr; a.nrings = rs.nrings; a.nphi = rs.nphi;
+    a.nnodes = rs.nnodes;
+    a.ncell = c->nr * c->nh; a.nlev = c->nlev; a.nlines = c->nlines; a.line = it.line;
+    a.lev_up = c->lev_up[it.line]; a.lev_low = c->lev_low[it.line];
+    a.molpg = 1.0 / (c->umass_av * 1.6726e-24);
+    a.off = rs.d_off.as<long long>();
+    a.ckoff = ab.ckoff.as<long long>();
+    a.geo = rs.d_nodes.as<GeoNode>();
+    a.cells = c->d_cells.as<CellRec>();
+    a.rec = ab.rec.p;
+    a.rinfo = c->d_rinfo.as<RayInfo>();
+    a.ckpt = ab.ckpt.p;
+    a.kblk = K;
+    a.dnu = c->d_chan.as<double>();
+    a.velo = a.dnu + nfr;
+    a.istar = a.dnu + 2 * nfr;
+    a.lc = su.lc;
+    a.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
+    a.image = c->d_image.as<double>();
+    a.ring_w = rs.d_ringw.as<double>();
+    a.center_w = rs.center_w;
+    a.nchunks = rs.nrings;
+    a.partial = c->d_partial.as<double>();
+    a.flux = c->d_flux.as<double>();
+    a.d2 = (1.0 * 3.08572e18) * (1.0 * 3.08572e18);
+    a.gflux = c->d_gflux.as<double>();
+    a.gnode = c->d_gnode.as<double>();
+    a.gtrue = c->d_gcell.as<double>();
+    a.st_off = rs.d_st_off.as<int>();
+    a.st_vals = rs.d_st_vals.as<int>();
+    a.st_w = rs.d_st_w.as<double>();
+    a.rho = c->d_rho.as<double>();
+    a.abund = c->d_abund.as<double>();
+    a.pops = c->d_pops.as<double>();
+    a.g_rho = c->d_grho.as<double>();
+    a.g_abund = c->d_gabund.as<double>();
+    a.g_lw = c->d_glw.as<double>();
+    a.g_pops = c->d_gpops.as<double>();
+    a.g_jd = c->d_gjd.as<double>();
+    a.g_ad = c->d_gad.as<double>();
+}
+
+// the checkpointed forward of one item: node records, I checkpoints every
+// K segments, the spectrum (copied to flux_host when given)
+static int adj_forward_item(rl_ctx *c, const rl_item &it, int ii, int K, AdjItemBufs &ab,
+                            double *flux_host) {
+    ItemSetup su;
+    if (int e = setup_item(c, it, ii, su, c->ev[0], c->ev[1])) return e;
+    RaySet &rs = *su.rs;
+    const int nfr = it.nfr, nrays = rs.nrays;
+    const size_t tsz = c->precision == RL_FP64 ? 8 : 4;
+    std::vector<long long> ckoff(nrays + 1, 0);       // ceil((n-1)/K) blocks per ray
+    for (int r = 0; r < nrays; r++) {
+        const long long n = rs.off[r + 1] - rs.off[r];
+        ckoff[r + 1] = ckoff[r] + (n >= 2 ? (n - 1 + K - 1) / K : 0);
+    }
+    if (int e = upload(c, ab.ckoff, ckoff.data(), ckoff.size() * 8)) return e;
+    HIPCHK(c, ab.ckpt.ensure((size_t)ckoff[nrays] * nfr * tsz));
+    HIPCHK(c, ab.rec.ensure((size_t)rs.nnodes * rl::fast_rec_size(c->precision == RL_FP64)));
+    HIPCHK(c, c->d_rinfo.ensure((size_t)nrays * sizeof(RayInfo)));
+    HIPCHK(c, c->d_image.ensure((size_t)nrays * nfr * 8));
+    HIPCHK(c, c->d_partial.ensure((size_t)rs.nrings * nfr * 8));
+    HIPCHK(c, c->d_flux.ensure((size_t)nfr * 8));
+    rl::AdjLaunch a;
+    fill_adj(c, it, su, ab, K, a);
+    if (c->precision == RL_FP64) rl::launch_adjoint_fwd<double>(c->stream, a);
+    else rl::launch_adjoint_fwd<float>(c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    if (flux_host)
+        HIPCHK(c, hipMemcpyAsync(flux_host, c->d_flux.p, (size_t)nfr * 8,
+                                 hipMemcpyDeviceToHost, c->stream));
+    c->counts[0] += nrays;
+    c->counts[1] += rs.nnodes;
+    return 0;
+}
+
+// the reverse sweep of one item, accumulating into the context's cell
+// gradients
+static int adj_reverse_item(rl_ctx *c, const rl_item &it, int ii, int K, AdjItemBufs &ab,
+                            const double *gflux_host) {
+    ItemSetup su;
+    if (int e = setup_item(c, it, ii, su, nullptr, nullptr)) return e;
+    RaySet &rs = *su.rs;
+    const int nfr = it.nfr;
+    if (int e = upload(c, c->d_gflux, gflux_host, (size_t)nfr * 8)) return e;
+    if (int e = zero_dev(c, c->d_gnode, (size_t)rs.nnodes * 5 * 8)) return e;
+    HIPCHK(c, c->d_gcell.ensure((size_t)rs.nnodes * 5 * 8));
+    if (int e = build_transpose(c, rs)) return e;
+    rl::AdjLaunch a;
+    fill_adj(c, it, su, ab, K, a);
+    if (c->precision == RL_FP64) rl::launch_adjoint_rev<double>(c->stream, a);
+    else rl::launch_adjoint_rev<float>(c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    c->counts[3] += (long long)rs.nrays * nfr;
+    return 0;
+}
+
+static int adj_check(rl_ctx *c, int nitems, const rl_item *items, const char *who) {
+    if (!c->have_medium || !c->nlev || !c->nlines || !c->have_dust || c->incl.empty())
+        return fail(c, 13, "inputs incomplete: grid, medium, levels, lines, dust, rays");
+    for (int ii = 0; ii < nitems; ii++)
+        if (items[ii].skip_mode != RL_SKIP_NONE)
+            return fail(c, 13, "%s: item %d: the adjoint integrates every channel (RL_SKIP_NONE)",
+                        who, ii);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->d_err.ensure(4));
+    HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));
+    for (int i = 0; i < 5; i++) c->t_ms[i] = 0.0;
+    for (int i = 0; i < 4; i++) c->counts[i] = 0;
+    return 0;
+}
+
+static bool same_items(const std::vector<rl_item> &a, int n, const rl_item *b) {
+    if ((int)a.size() != n) return false;
+    for (int i = 0; i < n; i++)
+        if (a[i].line != b[i].line || a[i].incl != b[i].incl || a[i].nfr != b[i].nfr ||
+            a[i].vmax_kms != b[i].vmax_kms || a[i].skip_mode != b[i].skip_mode)
+            return false;
+    return true;
+}
+
+extern "C" int rl_tape_release(rl_ctx *c) {
+    if (!c) return RL_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->tape.clear();
+    c->tape_items.clear();
+    return 0;
+}
+
+extern "C" int rl_trace_lines_forward_tape(rl_ctx *c, int nitems, const rl_item *items,
+                                           double *flux) {
+    if (!c || nitems < 0 || (nitems && !items))
+        return fail(c, RL_E_ARG, "rl_trace_lines_forward_tape: bad args");
+    if (int e = adj_check(c, nitems, items, "rl_trace_lines_forward_tape")) return e;
+    size_t held = 0;
+    for (auto &t : c->tape) held += t.rec.n + t.ckpt.n + t.ckoff.n;
+    const size_t budget = adj_budget(c, held);
+    c->tape_items.clear();                   // invalid until the forward completes
+    if ((int)c->tape.size() > nitems) c->tape.resize(nitems);
+    while ((int)c->tape.size() < nitems) c->tape.emplace_back();
+    const int K = pick_kblk(c, nitems, items, budget);
+    c->last_kblk = K;
+    HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+    size_t fo = 0;
+    for (int ii = 0; ii < nitems; ii++) fo += items[ii].nfr;
+    c->tape_flux.assign(fo, 0.0);
+    fo = 0;
+    for (int ii = 0; ii < nitems; ii++) {
+        if (int e = adj_forward_item(c, items[ii], ii, K, c->tape[ii], c->tape_flux.data() + fo))
+            return e;
+        fo += items[ii].nfr;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    if (int e = check_err(c)) return e;
+    if (flux) std::copy(c->tape_flux.begin(), c->tape_flux.end(), flux);
+    float t = 0;
+    (void)hipEventElapsedTime(&t, c->ev[5], c->ev[4]);
+    c->t_ms[4] = t;
+    c->tape_items.assign(items, items + nitems);
+    c->tape_gen = c->gen;
+    c->tape_kblk = K;
+    c->last_used_tape = 0;
+    return 0;
+}
+
 extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *items,
                                       const double *dL_dflux, double *flux, double *g_rho,
                                       double *g_abund, double *g_lw, double *g_pops,
                                       double *g_jd, double *g_ad) {
     if (!c || nitems < 0 || (nitems && (!items || !dL_dflux)))
         return fail(c, RL_E_ARG, "rl_trace_lines_adjoint: bad args");
-    if (!c->have_medium || !c->nlev || !c->nlines || !c->have_dust || c->incl.empty())
-        return fail(c, 13, "inputs incomplete: grid, medium, levels, lines, dust, rays");
-    for (int ii = 0; ii < nitems; ii++)
-        if (items[ii].skip_mode != RL_SKIP_NONE)
-            return fail(c, 13, "item %d: the adjoint integrates every channel (RL_SKIP_NONE)", ii);
-    HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, c->d_err.ensure(4));
-    HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));
-    const int fp64 = c->precision == RL_FP64;
-    const size_t tsz = fp64 ? 8 : 4;
+    if (int e = adj_check(c, nitems, items, "rl_trace_lines_adjoint")) return e;
     const int ncell = c->nr * c->nh;
     if (int e = zero_dev(c, c->d_grho, (size_t)ncell * 8)) return e;
     if (int e = zero_dev(c, c->d_gabund, (size_t)ncell * 8)) return e;
@@ -1026,103 +1249,48 @@ extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *item
     if (int e = zero_dev(c, c->d_gpops, (size_t)ncell * c->nlev * 8)) return e;
     if (int e = zero_dev(c, c->d_gjd, (size_t)ncell * c->nlines * 8)) return e;
     if (int e = zero_dev(c, c->d_gad, (size_t)ncell * c->nlines * 8)) return e;
-    for (int i = 0; i < 5; i++) c->t_ms[i] = 0.0;
-    for (int i = 0; i < 4; i++) c->counts[i] = 0;
-    const int KMAX = rl::adjoint_block_len();
-    size_t budget = c->mem_budget;
-    if (budget == 0) {
-        size_t fr = 0, tot = 0;
-        HIPCHK(c, hipMemGetInfo(&fr, &tot));
-        budget = (size_t)(0.6 * (double)fr) + c->d_ckpt.n;
-    }
+    HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+    // the tape of the last rl_trace_lines_forward_tape, when it was taken
+    // on these items with the current inputs
+    const bool use_tape = c->tape_gen == c->gen && same_items(c->tape_items, nitems, items);
+    c->last_used_tape = use_tape ? 1 : 0;
     size_t fo = 0;
-    for (int ii = 0; ii < nitems; ii++) {
-        const rl_item &it = items[ii];
-        ItemSetup su;
-        if (int e = setup_item(c, it, ii, su, c->ev[0], c->ev[1])) return e;
-        RaySet &rs = *su.rs;
-        const int nfr = it.nfr, nrays = rs.nrays;
-        // checkpoint interval: I before every segment (no recomputation in
-        // the reverse sweep) when that fits half the memory budget, else the
-        // smallest power of two up to KMAX that does
-        int K = 1;
-        while (K < KMAX && (size_t)rs.nnodes / K * nfr * tsz > budget / 2) K *= 2;
-        c->last_kblk = K;
-        // checkpoint CSR: ceil((n-1)/K) blocks per ray
-        std::vector<long long> ckoff(nrays + 1, 0);
-        for (int r = 0; r < nrays; r++) {
-            const long long n = rs.off[r + 1] - rs.off[r];
-            ckoff[r + 1] = ckoff[r] + (n >= 2 ? (n - 1 + K - 1) / K : 0);
+    if (use_tape) {
+        c->last_kblk = c->tape_kblk;
+        if (flux) std::copy(c->tape_flux.begin(), c->tape_flux.end(), flux);
+        for (int ii = 0; ii < nitems; ii++) {
+            if (int e = adj_reverse_item(c, items[ii], ii, c->tape_kblk, c->tape[ii],
+                                         dL_dflux + fo))
+                return e;
+            fo += items[ii].nfr;
         }
-        if (int e = upload(c, c->d_ckoff, ckoff.data(), ckoff.size() * 8)) return e;
-        HIPCHK(c, c->d_ckpt.ensure((size_t)ckoff[nrays] * nfr * tsz));
-        HIPCHK(c, c->d_arec.ensure((size_t)rs.nnodes * rl::fast_rec_size(fp64)));
-        HIPCHK(c, c->d_rinfo.ensure((size_t)nrays * sizeof(RayInfo)));
-        HIPCHK(c, c->d_image.ensure((size_t)nrays * nfr * 8));
-        HIPCHK(c, c->d_partial.ensure((size_t)rs.nrings * nfr * 8));
-        HIPCHK(c, c->d_flux.ensure((size_t)nfr * 8));
-        if (int e = upload(c, c->d_gflux, dL_dflux + fo, (size_t)nfr * 8)) return e;
-        if (int e = zero_dev(c, c->d_gnode, (size_t)rs.nnodes * 5 * 8)) return e;
-        HIPCHK(c, c->d_gcell.ensure((size_t)rs.nnodes * 5 * 8));
-        if (int e = build_transpose(c, rs)) return e;
-        rl::AdjLaunch a;
-        a.nrays = nrays; a.nfr = nfr; a.nrings = rs.nrings; a.nphi = rs.nphi;
-        a.nnodes = rs.nnodes;
-        a.ncell = ncell; a.nlev = c->nlev; a.nlines = c->nlines; a.line = it.line;
-        a.lev_up = c->lev_up[it.line]; a.lev_low = c->lev_low[it.line];
-        a.molpg = 1.0 / (c->umass_av * 1.6726e-24);
-        a.off = rs.d_off.as<long long>();
-        a.ckoff = c->d_ckoff.as<long long>();
-        a.geo = rs.d_nodes.as<GeoNode>();
-        a.cells = c->d_cells.as<CellRec>();
-        a.rec = c->d_arec.p;
-        a.rinfo = c->d_rinfo.as<RayInfo>();
-        a.ckpt = c->d_ckpt.p;
-        a.kblk = K;
-        a.dnu = c->d_chan.as<double>();
-        a.velo = a.dnu + nfr;
-        a.istar = a.dnu + 2 * nfr;
-        a.lc = su.lc;
-        a.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
-        a.image = c->d_image.as<double>();
-        a.ring_w = rs.d_ringw.as<double>();
-        a.center_w = rs.center_w;
-        a.nchunks = rs.nrings;
-        a.partial = c->d_partial.as<double>();
-        a.flux = c->d_flux.as<double>();
-        a.d2 = (1.0 * 3.08572e18) * (1.0 * 3.08572e18);
-        a.gflux = c->d_gflux.as<double>();
-        a.gnode = c->d_gnode.as<double>();
-        a.gtrue = c->d_gcell.as<double>();
-        a.st_off = rs.d_st_off.as<int>();
-        a.st_vals = rs.d_st_vals.as<int>();
-        a.st_w = rs.d_st_w.as<double>();
-        a.rho = c->d_rho.as<double>();
-        a.abund = c->d_abund.as<double>();
-        a.pops = c->d_pops.as<double>();
-        a.g_rho = c->d_grho.as<double>();
-        a.g_abund = c->d_gabund.as<double>();
-        a.g_lw = c->d_glw.as<double>();
-        a.g_pops = c->d_gpops.as<double>();
-        a.g_jd = c->d_gjd.as<double>();
-        a.g_ad = c->d_gad.as<double>();
-        if (fp64) rl::launch_adjoint<double>(c->stream, a);
-        else rl::launch_adjoint<float>(c->stream, a);
-        HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
-        if (flux)
-            HIPCHK(c, hipMemcpyAsync(flux + fo, c->d_flux.p, (size_t)nfr * 8,
-                                     hipMemcpyDeviceToHost, c->stream));
-        if (int e = check_err(c)) return e;
-        float t = 0;
-        (void)hipEventElapsedTime(&t, c->ev[1], c->ev[4]);
-        c->t_ms[2] += t;
-        c->t_ms[4] += t;
-        c->counts[0] += nrays;
-        c->counts[1] += rs.nnodes;
-        c->counts[3] += (long long)nrays * nfr;
-        fo += nfr;
+    } else {
+        AdjItemBufs ab;
+        ab.rec = std::move(c->d_arec);
+        ab.ckoff = std::move(c->d_ckoff);
+        ab.ckpt = std::move(c->d_ckpt);
+        c->last_kblk = 0;
+        int err = 0;
+        for (int ii = 0; ii < nitems && !err; ii++) {
+            // items one at a time through the same buffers: the interval
+            // follows each item's own checkpoint size
+            const int Ki = pick_kblk(c, 1, items + ii, adj_budget(c, ab.ckpt.n));
+            err = adj_forward_item(c, items[ii], ii, Ki, ab, flux ? flux + fo : nullptr);
+            if (!err) err = adj_reverse_item(c, items[ii], ii, Ki, ab, dL_dflux + fo);
+            c->last_kblk = std::max(c->last_kblk, Ki);
+            fo += items[ii].nfr;
+        }
+        c->d_arec = std::move(ab.rec);
+        c->d_ckoff = std::move(ab.ckoff);
+        c->d_ckpt = std::move(ab.ckpt);
+        if (err) return err;
     }
+    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    if (int e = check_err(c)) return e;
+    float t = 0;
+    (void)hipEventElapsedTime(&t, c->ev[5], c->ev[4]);
+    c->t_ms[2] = t;
+    c->t_ms[4] = t;
     auto down = [&](double *dst, DevBuf &b, size_t n) -> int {
         if (dst) HIPCHK(c, hipMemcpyAsync(dst, b.p, n * 8, hipMemcpyDeviceToHost, c->stream));
         return 0;
@@ -1134,6 +1302,13 @@ extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *item
     if (int e = down(g_jd, c->d_gjd, (size_t)ncell * c->nlines)) return e;
     if (int e = down(g_ad, c->d_gad, (size_t)ncell * c->nlines)) return e;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int rl_last_adjoint_info(const rl_ctx *c, int *ckpt_interval, int *used_tape) {
+    if (!c) return RL_E_ARG;
+    if (ckpt_interval) *ckpt_interval = c->last_kblk;
+    if (used_tape) *used_tape = c->last_used_tape;
     return 0;
 }
 

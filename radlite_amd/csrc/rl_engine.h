@@ -145,7 +145,8 @@ void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int
                   const double *lw, const double *pops, const double *jd, const double *ad,
                   CellRec *out);
 template <typename T> void launch_item(hipStream_t st, const ItemLaunch &it);
-template <typename T> void launch_adjoint(hipStream_t st, const AdjLaunch &a);
+template <typename T> void launch_adjoint_fwd(hipStream_t st, const AdjLaunch &a);
+template <typename T> void launch_adjoint_rev(hipStream_t st, const AdjLaunch &a);
 int adjoint_block_len();
 // Stencil transpose (cell -> node entries) of a node set; with temp ==
 // nullptr only *temp_bytes is set.  keys/vals: 4*nnodes ints each (a, b);

@@ -1027,49 +1027,63 @@ k_adj_rev(int nrays, int nfr, int nphi, const long long *__restrict__ off,
     PGrad<T> cur, prv;
     pg_zero(cur);
     pg_zero(prv);
-    T iseg[ADJ_K];
     T isub[33];
     WaveNodeSum<T> red{red_row, red_node};
+    // reverse of segment k (nodes k-1 -> k) from I at its start
+    auto reverse = [&](int k, T Is) {
+        const FastRecT<T> r = rec[b + k];
+        const FastRecT<T> a = rec[b + k - 1];
+        T g_lnaa = T(0), g_dls = T(0);
+        if (active) {
+            T jl0c = T(0), al0c = T(0);
+            if (k > 1) carried_in<T>(a, L, jl0c, al0c);
+            lam = segment_rev<T>(r, point_of<T>(a), point_of<T>(r), L, Is, lam, ist, sf,
+                                 k == 1 ? 1 : 0, jl0c, al0c, gjl, gal, prv, cur, g_lnaa, g_dls,
+                                 isub);
+        }
+        // d lw_mean: phi through inv_aa = 1/(c_aa lw), sub-grid width dls ~ lw
+        const T g_lw = (g_dls * r.dls - g_lnaa) * cai * r.inv_aa;
+        red.push(gnode, b + k, cur.jd, cur.ad, cur.jc, cur.dc, g_lw);
+        cur = prv;
+        pg_zero(prv);
+    };
     const int nseg = n - 1;
-    const int nblk = (nseg + kblk - 1) / kblk;
-    for (int blk = nblk - 1; blk >= 0; blk--) {
-        const int k0 = blk * kblk + 1, k1 = min(k0 + kblk - 1, nseg);
-        // forward over the block from its checkpoint
-        T I = ckpt[(ckoff[ray] + blk) * nfr + chc];
-        FastCarryT<T> c{T(0), T(0), T(0), k0 == 1 ? 1 : 0};
-        if (k0 > 1) {
-            const FastRecT<T> a = rec[b + k0 - 1];
-            carried_in<T>(a, L, c.jl0, c.al0);
-            const T ah = a.adh + c.al0;
-            c.S0 = ah > T(0) ? FOps<T>::sdiv(a.jdh + c.jl0, ah) : T(0);
+    const T *ck = ckpt + ckoff[ray] * nfr + chc;
+    if (kblk == 1) {
+        // I before every segment was kept: nothing to recompute; the next
+        // segment's start value is loaded one iteration ahead
+        T Inext = ck[(long long)(nseg - 1) * nfr];
+        for (int k = nseg; k >= 1; k--) {
+            const T Is = Inext;
+            if (k > 1) Inext = ck[(long long)(k - 2) * nfr];
+            reverse(k, Is);
         }
-        int ovs = 0;
-        FPtT<T> p0 = point_of<T>(rec[b + k0 - 1]);
-        for (int k = k0; k <= k1; k++) {
-            iseg[k - k0] = I;
-            if (k == k1) break;                // the block's last segment is only reversed
-            const FastRecT<T> r = rec[b + k];
-            const FPtT<T> p1 = point_of<T>(r);
-            if (active) segment_fast<T>(r, p0, p1, L, I, ist, sf, c, ovs);
-            p0 = p1;
-        }
-        // backwards over the block
-        for (int k = k1; k >= k0; k--) {
-            const FastRecT<T> r = rec[b + k];
-            const FastRecT<T> a = rec[b + k - 1];
-            T g_lnaa = T(0), g_dls = T(0);
-            if (active) {
-                T jl0c = T(0), al0c = T(0);
-                if (k > 1) carried_in<T>(a, L, jl0c, al0c);
-                lam = segment_rev<T>(r, point_of<T>(a), point_of<T>(r), L, iseg[k - k0], lam,
-                                     ist, sf, k == 1 ? 1 : 0, jl0c, al0c, gjl, gal, prv, cur,
-                                     g_lnaa, g_dls, isub);
+    } else {
+        T iseg[ADJ_K];
+        const int nblk = (nseg + kblk - 1) / kblk;
+        for (int blk = nblk - 1; blk >= 0; blk--) {
+            const int k0 = blk * kblk + 1, k1 = min(k0 + kblk - 1, nseg);
+            // forward over the block from its checkpoint
+            T I = ck[(long long)blk * nfr];
+            FastCarryT<T> c{T(0), T(0), T(0), k0 == 1 ? 1 : 0};
+            if (k0 > 1) {
+                const FastRecT<T> a = rec[b + k0 - 1];
+                carried_in<T>(a, L, c.jl0, c.al0);
+                const T ah = a.adh + c.al0;
+                c.S0 = ah > T(0) ? FOps<T>::sdiv(a.jdh + c.jl0, ah) : T(0);
             }
-            // d lw_mean: phi through inv_aa = 1/(c_aa lw), sub-grid width dls ~ lw
-            const T g_lw = (g_dls * r.dls - g_lnaa) * cai * r.inv_aa;
-            red.push(gnode, b + k, cur.jd, cur.ad, cur.jc, cur.dc, g_lw);
-            cur = prv;
-            pg_zero(prv);
+            int ovs = 0;
+            FPtT<T> p0 = point_of<T>(rec[b + k0 - 1]);
+            for (int k = k0; k <= k1; k++) {
+                iseg[k - k0] = I;
+                if (k == k1) break;                // the block's last segment is only reversed
+                const FastRecT<T> r = rec[b + k];
+                const FPtT<T> p1 = point_of<T>(r);
+                if (active) segment_fast<T>(r, p0, p1, L, I, ist, sf, c, ovs);
+                p0 = p1;
+            }
+            // backwards over the block
+            for (int k = k1; k >= k0; k--) reverse(k, iseg[k - k0]);
         }
     }
     red.push(gnode, b, cur.jd, cur.ad, cur.jc, cur.dc, T(0));
@@ -1197,8 +1211,9 @@ int build_stencil_transpose(hipStream_t st, long long nnodes, const GeoNode *geo
     return 0;
 }
 
+// node records, the checkpointed forward and the flux of one item
 template <typename T>
-void launch_adjoint(hipStream_t st, const AdjLaunch &a) {
+void launch_adjoint_fwd(hipStream_t st, const AdjLaunch &a) {
     auto *rec = reinterpret_cast<FastRecT<T> *>(a.rec);
     auto *ck = reinterpret_cast<T *>(a.ckpt);
     hipLaunchKernelGGL(k_nodes_fast<T>, dim3(a.nrays), dim3(64), 0, st, a.nrays, a.off, a.geo,
@@ -1212,6 +1227,14 @@ void launch_adjoint(hipStream_t st, const AdjLaunch &a) {
                        a.ring_w, a.center_w, a.partial);
     hipLaunchKernelGGL(k_flux_final, dim3((a.nfr + 63) / 64), dim3(64), 0, st, a.nfr,
                        a.nchunks, a.partial, a.d2, a.flux);
+}
+
+// reverse sweep of one item (records and checkpoints from
+// launch_adjoint_fwd), node gradients to the cells
+template <typename T>
+void launch_adjoint_rev(hipStream_t st, const AdjLaunch &a) {
+    auto *rec = reinterpret_cast<const FastRecT<T> *>(a.rec);
+    auto *ck = reinterpret_cast<const T *>(a.ckpt);
     hipLaunchKernelGGL(k_adj_rev<T>, dim3(a.nrays, (a.nfr + 63) / 64), dim3(64), 0, st,
                        a.nrays, a.nfr, a.nphi, a.off, a.ckoff, rec, a.dnu, a.velo, a.istar,
                        a.lc, a.gflux, a.ring_w, a.center_w, a.d2, ck, a.gnode, a.kblk);
@@ -1223,8 +1246,10 @@ void launch_adjoint(hipStream_t st, const AdjLaunch &a) {
                        a.g_pops, a.g_jd, a.g_ad);
 }
 
-template void launch_adjoint<double>(hipStream_t, const AdjLaunch &);
-template void launch_adjoint<float>(hipStream_t, const AdjLaunch &);
+template void launch_adjoint_fwd<double>(hipStream_t, const AdjLaunch &);
+template void launch_adjoint_fwd<float>(hipStream_t, const AdjLaunch &);
+template void launch_adjoint_rev<double>(hipStream_t, const AdjLaunch &);
+template void launch_adjoint_rev<float>(hipStream_t, const AdjLaunch &);
 
 int adjoint_block_len() { return ADJ_K; }
 size_t fast_rec_size(int fp64) { return fp64 ? sizeof(FastRecT<double>) : sizeof(FastRecT<float>); }

@@ -77,15 +77,25 @@ class _Trace(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rho, abund, lw, pops, jd, ad, model):
         model._upload(rho, abund, lw, pops, jd, ad)
-        fl, _, _ = model.rt.trace(model.items)
+        if any(ctx.needs_input_grad[:6]):
+            # keep the adjoint's tape (I checkpoints) on the device
+            fl = model.rt.trace_tape(model.items)
+        else:
+            fl, _, _ = model.rt.trace(model.items)
+        model._calls += 1
         ctx.model = model
+        ctx.call = model._calls
         ctx.save_for_backward(rho, abund, lw, pops, jd, ad)
         return torch.from_numpy(np.concatenate(fl))
 
     @staticmethod
     def backward(ctx, g):
         m = ctx.model
-        m._upload(*ctx.saved_tensors)          # the inputs this forward saw
+        if m._calls != ctx.call:
+            # another forward ran since: restore the inputs this forward saw
+            # (the adjoint then recomputes its forward instead of the tape)
+            m._upload(*ctx.saved_tensors)
+            m._calls += 1
         g = g.detach().cpu().numpy().astype(np.float64)
         parts, o = [], 0
         for it in m.items:
@@ -106,6 +116,7 @@ class SpectrumModel:
 
     def __init__(self, ri, precision="fp64", incl_rad=None, device=0, items=None):
         self.ri = ri
+        self._calls = 0
         self.rt = engine.Raytracer(device, precision)
         self.tables = self.rt.load_run(ri)
         inc = [ri.incl_deg * 0.0174532925199] if incl_rad is None else list(incl_rad)

@@ -28,7 +28,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 RL_FP64, RL_FP32 = 0, 1
-OPT_GRAPH, OPT_AUTOTUNE, OPT_TILE, OPT_PERSISTENT = 1, 2, 3, 4
+OPT_GRAPH, OPT_AUTOTUNE, OPT_TILE, OPT_PERSISTENT, OPT_CKPT = 1, 2, 3, 4, 5
 SKIP_REFERENCE, SKIP_NONE = 0, 1
 WARN_MASER, WARN_OVERSHOOT = 1, 2
 
@@ -73,6 +73,9 @@ def load_library(path=LIB_PATH):
                                                 C.POINTER(C.c_ubyte), C.POINTER(C.c_uint)]),
         "rl_trace_lines_adjoint": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
                                              _dp, _dp, _dp, _dp, _dp, _dp]),
+        "rl_trace_lines_forward_tape": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp]),
+        "rl_tape_release": (C.c_int, [vp]),
+        "rl_last_adjoint_info": (C.c_int, [vp, _ip, _ip]),
         "rl_sync": (C.c_int, [vp]),
         "rl_set_memory_budget": (C.c_int, [vp, C.c_double]),
         "rl_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
@@ -300,6 +303,34 @@ class Raytracer:
             io += n
         out = (fl, (im if img is not None else None), warn)
         return out + (cms,) if want_cmask else out
+
+    def trace_tape(self, items):
+        """Forward trace of SKIP_NONE items that keeps the adjoint's tape
+        (node records and I checkpoints) on the device; a following
+        adjoint() on the same items with unchanged inputs runs only the
+        reverse sweep.  Returns the list of flux arrays."""
+        arr = (RLItem * len(items))(*[RLItem(*it) for it in items])
+        nf = [it[2] for it in items]
+        flux = np.zeros(sum(nf))
+        self._chk(self.L.rl_trace_lines_forward_tape(self.ctx, len(items), arr,
+                                                     flux.ctypes.data_as(_dp)),
+                  "rl_trace_lines_forward_tape")
+        fl, o = [], 0
+        for n in nf:
+            fl.append(flux[o:o + n])
+            o += n
+        return fl
+
+    def tape_release(self):
+        self._chk(self.L.rl_tape_release(self.ctx), "rl_tape_release")
+
+    def adjoint_info(self):
+        """(checkpoint interval of the last adjoint/tape call, 1 if the last
+        adjoint ran from the tape)."""
+        v = [C.c_int(0) for _ in range(2)]
+        self._chk(self.L.rl_last_adjoint_info(self.ctx, *[C.byref(x) for x in v]),
+                  "rl_last_adjoint_info")
+        return tuple(x.value for x in v)
 
     def adjoint(self, items, dL_dflux):
         """Gradient of L with respect to the inputs the trace reads, given

@@ -70,6 +70,62 @@ def test_adjoint_forward_pass_matches_forward(name, precision):
     rt.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_tape_and_checkpoint_intervals(precision):
+    """rl_trace_lines_forward_tape + rl_trace_lines_adjoint (reverse sweep
+    only) against the adjoint that runs its own forward; the tape goes
+    stale when an input is set; every checkpoint interval (I recomputed
+    from checkpoints every 2..16 segments) against every segment kept."""
+    ri = _case("t6")
+    say(f"[gpu] tape and checkpoint intervals t6 {precision}")
+    rt, nfr, vmax = _engine(ri, precision)
+    items = [(il, 0, nfr, vmax, engine.SKIP_NONE) for il in range(len(ri.A))]
+    rng = np.random.default_rng(7)
+    gf = [rng.normal(size=nfr) * 1e20 for _ in items]
+    fa, ga = rt.adjoint(items, gf)
+    assert rt.adjoint_info() == (1, 0)            # small case: every segment kept
+    ft = rt.trace_tape(items)
+    fb, gb = rt.adjoint(items, gf)
+    assert rt.adjoint_info() == (1, 1)
+    for a, b, c in zip(fa, fb, ft):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    for k in ga:
+        assert np.array_equal(ga[k], gb[k]), k
+    rt.set_levels(ri.pops)                        # same values, but an input was set
+    fc, gc = rt.adjoint(items, gf)
+    assert rt.adjoint_info()[1] == 0
+    for k in ga:
+        assert np.array_equal(ga[k], gc[k]), k
+    rt.tape_release()
+    for K in (2, 4, 8, 16):
+        rt.set_option(engine.OPT_CKPT, K)
+        fk, gk = rt.adjoint(items, gf)
+        assert rt.adjoint_info() == (K, 0)
+        for a, b in zip(fa, fk):
+            assert np.array_equal(a, b)
+        worst = 0.0
+        for k in ga:
+            d = float(np.abs(gk[k] - ga[k]).max())
+            s = float(np.abs(ga[k]).max())
+            worst = max(worst, d / s if s > 0 else d)
+        say(f"    K={K}: max |g_K - g_1| / max |g_1| = {worst:.2e}")
+        # the recomputed I follows the same arithmetic as the stored one
+        assert worst <= (1e-14 if precision == "fp64" else 1e-6), (K, worst)
+        ft = rt.trace_tape(items)
+        fk2, gk2 = rt.adjoint(items, gf)
+        assert rt.adjoint_info() == (K, 1)
+        for k in ga:
+            assert np.array_equal(gk[k], gk2[k]), (K, k)
+        rt.tape_release()
+    rt.set_option(engine.OPT_CKPT, 0)
+    rt.set_memory_budget(1)                       # nothing fits: the largest interval
+    rt.adjoint(items, gf)
+    assert rt.adjoint_info()[0] == 16
+    with pytest.raises(engine.RadliteError):
+        rt.set_option(engine.OPT_CKPT, 3)
+    rt.close()
+
+
 def test_adjoint_rejects_channel_skipping():
     ri = _case("t1")
     rt, nfr, vmax = _engine(ri, "fp64")
@@ -194,6 +250,7 @@ def test_c4_chi2_gradients_fd_and_fp32():
     vt = vt0.clone().requires_grad_(True)
     L, S0 = _chi2(m64, T, la, vt, Ft, sigma, with_scale=True)
     L.backward()
+    assert m64.rt.adjoint_info()[1] == 1          # the backward ran from the forward's tape
     g64 = {"T": T.grad.numpy().copy(), "la": la.grad.numpy().copy(), "vt": vt.grad.numpy().copy()}
     L0 = float(L)
     say(f"    chi2 = {L0:.6e}")
