@@ -375,15 +375,26 @@ __device__ inline bool subgrid_of(const FastRecT<T> &b, T ds, T u0, T uch, SubGr
 
 // a sub-point at fraction e of the segment: linear interpolation of the
 // unscaled quantities (line.F:4775-4790), done with relative weights
+// (w receives the normalised weights n0, n1 and 1/W, for the adjoint)
+template <typename T> struct SubW { T e, n0, n1, iw; };
+
 template <typename T>
-__device__ inline FPtT<T> subpoint(const FastRecT<T> &b, const FPtT<T> &p0, const FPtT<T> &p1,
-                                   T e) {
+__device__ inline FPtT<T> subpoint_w(const FastRecT<T> &b, const FPtT<T> &p0,
+                                     const FPtT<T> &p1, T e, SubW<T> &w) {
     T ep = T(1) - e;
     T w0 = ep * b.rel0, w1 = e * b.rel1, W = w0 + w1;
     T iw = FOps<T>::fdiv(T(1), W);
     w0 *= iw; w1 *= iw;
+    w = SubW<T>{e, w0, w1, iw};
     return FPtT<T>{ep * p0.u + e * p1.u, b.sigm * W, w0 * p0.jd + w1 * p1.jd,
                    w0 * p0.ad + w1 * p1.ad, w0 * p0.jc + w1 * p1.jc, w0 * p0.dc + w1 * p1.dc};
+}
+
+template <typename T>
+__device__ inline FPtT<T> subpoint(const FastRecT<T> &b, const FPtT<T> &p0, const FPtT<T> &p1,
+                                   T e) {
+    SubW<T> w;
+    return subpoint_w<T>(b, p0, p1, e, w);
 }
 
 template <typename T>

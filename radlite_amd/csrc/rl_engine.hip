@@ -986,13 +986,13 @@ struct WaveNodeSum {
 };
 
 // line values carried into segment k: node k-1 with segment k-1's profile
+// (z, phi: its argument and value, the end-point profile of segment k-1)
 template <typename T>
-__device__ inline void carried_in(const FastRecT<T> &a, const FastLineT<T> &L, T &jl, T &al) {
-    const FPtT<T> p = point_of<T>(a);
-    const T z = (L.dnu - L.nu0 * p.u) * a.inv_aa;
-    const T phi = a.norm * FOps<T>::ex(-(z * z));
-    jl = p.jc * phi;
-    al = p.dc * phi;
+__device__ inline void carried_in(const FastRecT<T> &a, const FastLineT<T> &L, T &jl, T &al,
+                                  T &z, T &phi) {
+    profile_at<T>(L, a.u, a.inv_aa, a.norm, z, phi);
+    jl = a.jch * phi;
+    al = a.dch * phi;
 }
 
 // reverse pass: one wave per (ray, group of 64 channels); per node the
@@ -1027,27 +1027,37 @@ k_adj_rev(int nrays, int nfr, int nphi, const long long *__restrict__ off,
     PGrad<T> cur, prv;
     pg_zero(cur);
     pg_zero(prv);
-    T isub[33];
+    T isub[33];                    // 33 entries: kept in scratch memory, not registers
     WaveNodeSum<T> red{red_row, red_node};
+    const int nseg = n - 1;
+    // profile at the end node of the segment being reversed, with its
+    // constants: computed for the last segment, then carried from the
+    // previous iteration's carried-in values (the same quantity)
+    T zc, phic;
+    {
+        const FastRecT<T> r = rec[b + nseg];
+        profile_at<T>(L, r.u, r.inv_aa, r.norm, zc, phic);
+    }
     // reverse of segment k (nodes k-1 -> k) from I at its start
     auto reverse = [&](int k, T Is) {
         const FastRecT<T> r = rec[b + k];
         const FastRecT<T> a = rec[b + k - 1];
         T g_lnaa = T(0), g_dls = T(0);
-        if (active) {
-            T jl0c = T(0), al0c = T(0);
-            if (k > 1) carried_in<T>(a, L, jl0c, al0c);
-            lam = segment_rev<T>(r, point_of<T>(a), point_of<T>(r), L, Is, lam, ist, sf,
-                                 k == 1 ? 1 : 0, jl0c, al0c, gjl, gal, prv, cur, g_lnaa, g_dls,
-                                 isub);
-        }
+        T jl0c = T(0), al0c = T(0), za = T(0), phia = T(0);
+        // every lane (lanes past the last channel repeat it with lam = 0,
+        // so their gradients are 0)
+        if (k > 1) carried_in<T>(a, L, jl0c, al0c, za, phia);
+        lam = segment_rev<T>(r, point_of<T>(a), point_of<T>(r), L, Is, lam, ist, sf,
+                             k == 1 ? 1 : 0, jl0c, al0c, zc, phic, gjl, gal, prv, cur, g_lnaa,
+                             g_dls, isub);
+        zc = za;
+        phic = phia;
         // d lw_mean: phi through inv_aa = 1/(c_aa lw), sub-grid width dls ~ lw
         const T g_lw = (g_dls * r.dls - g_lnaa) * cai * r.inv_aa;
         red.push(gnode, b + k, cur.jd, cur.ad, cur.jc, cur.dc, g_lw);
         cur = prv;
         pg_zero(prv);
     };
-    const int nseg = n - 1;
     const T *ck = ckpt + ckoff[ray] * nfr + chc;
     if (kblk == 1) {
         // I before every segment was kept: nothing to recompute; the next
@@ -1068,7 +1078,8 @@ k_adj_rev(int nrays, int nfr, int nphi, const long long *__restrict__ off,
             FastCarryT<T> c{T(0), T(0), T(0), k0 == 1 ? 1 : 0};
             if (k0 > 1) {
                 const FastRecT<T> a = rec[b + k0 - 1];
-                carried_in<T>(a, L, c.jl0, c.al0);
+                T z_, phi_;
+                carried_in<T>(a, L, c.jl0, c.al0, z_, phi_);
                 const T ah = a.adh + c.al0;
                 c.S0 = ah > T(0) ? FOps<T>::sdiv(a.jdh + c.jl0, ah) : T(0);
             }
