@@ -85,7 +85,7 @@ def bench_adjoint(args):
         if name == "fwd_adj":
             st = rt.stats()
             res["adj_device_ms"] = st["t_formal_ms"]
-            res["kblk"], res["used_tape"] = rt.adjoint_info()
+            res["kblk"], res["used_tape"], _ = rt.adjoint_info()
     nominal = rt.nrays[0] * nfr
     rt.close()
     return {

@@ -144,19 +144,31 @@ int rl_trace_lines_forward_tape(rl_ctx *ctx, int nitems, const rl_item *items,
                                 double *flux);
 int rl_tape_release(rl_ctx *ctx);
 
-/* Checkpoint interval K (segments) of the last adjoint or tape call, and
+/* Checkpoint interval K (segments) of the last adjoint or tape call,
  * whether the last adjoint call ran from the tape (1) or ran its own
- * forward (0). */
-int rl_last_adjoint_info(const rl_ctx *ctx, int *ckpt_interval, int *used_tape);
+ * forward (0), and the most ray chunks an item of it was cut into (an
+ * item whose buffers do not fit the memory budget is traced and reversed
+ * a range of rays at a time; the tape needs every item whole). */
+int rl_last_adjoint_info(const rl_ctx *ctx, int *ckpt_interval, int *used_tape,
+                         int *chunks);
+
+/* Device memory: out[0] the budget, out[1] bytes held by resident
+ * trajectories (and adjoint transposes), out[2] work bytes of the last
+ * call's largest batch or adjoint chunk (with its image), out[3] free and
+ * out[4] total device bytes now.  The budget covers trajectories and a
+ * call's work buffers; the trajectories of inclinations a batch does not
+ * use are evicted (and rebuilt when needed) to keep within it. */
+int rl_memory_info(rl_ctx *ctx, double *out);
 
 /* ---- batching controls (no counterpart in the reference) ---------- */
 /* A call to rl_trace_lines_forward runs its items in batches: every
  * item of a batch is resident in device memory at once, and each stage
  * (cell tables, node records, formal solution, flux) is one launch for
  * the whole batch, the formal solution as a persistent kernel over a
- * device work counter.  Batches are cut so that their device memory
- * stays within the budget: bytes > 0 sets it, 0 (the default) uses 60%
- * of the device memory free at the context's first trace. */
+ * device work counter.  Batches are cut so that their device memory,
+ * with the trajectories of their inclinations, stays within the budget
+ * (a batch holds at least one item): bytes > 0 sets it, 0 (the default)
+ * uses 60% of the device memory free at the context's first trace. */
 int rl_set_memory_budget(rl_ctx *ctx, double bytes);
 
 enum {

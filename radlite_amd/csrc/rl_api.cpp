@@ -61,8 +61,11 @@ struct RaySet {
     int nrays = 0, nrings = 0, nphi = 0;
     std::vector<double> x0, z0, ring_r, ring_ri;
     DevBuf d_x0, d_z0;
-    // trajectory CSR (built lazily, shared by every line of this inclination)
-    bool traj_ready = false;
+    // trajectory CSR (built lazily, shared by every line of this inclination;
+    // the nodes may be evicted to make room and are then built again)
+    bool counted = false;             // off / nnodes known
+    bool traj_ready = false;          // d_nodes built
+    unsigned long long used = 0;      // last use, for eviction
     long long nnodes = 0;
     std::vector<long long> off;
     DevBuf d_off, d_nodes;
@@ -112,6 +115,10 @@ struct rl_ctx {
     std::vector<long long> h_tile0;
     size_t mem_budget = 0;            // 0: a share of the free device memory
     size_t auto_budget = 0;
+    unsigned long long tick = 0;      // use counter of the ray sets
+    int cross_owner = -1;             // ray set whose crossings d_cross holds
+    size_t last_work = 0;             // work bytes of the last call's largest unit
+    int last_chunks = 0;
     bool use_graph = true, autotune = true;
     int fixed_cw = -1, fixed_persistent = 1, last_cw = 0, last_blocks = 0, last_batches = 0;
     int last_kblk = 0, fixed_kblk = 0, last_used_tape = 0;
@@ -126,6 +133,7 @@ struct rl_ctx {
     std::vector<double> tape_flux;
     unsigned long long tape_gen = 0;
     int tape_kblk = 0;
+    DevBuf d_stc_vals, d_stc_w, d_stc_off;    // a split item's chunk transpose
     DevBuf d_arec, d_ckpt, d_ckoff, d_gflux, d_gnode, d_gcell, d_grho, d_gabund, d_glw,
         d_gpops, d_gjd, d_gad;
     // stats
@@ -252,6 +260,7 @@ extern "C" int rl_set_grid(rl_ctx *c, int nr, const double *r, int nh, const dou
     if (int e = upload(c, c->d_tan2, t2.data(), t2.size() * 8)) return e;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->incl.clear();
+    c->cross_owner = -1;
     c->cells_line = -1;
     return 0;
 }
@@ -341,7 +350,8 @@ extern "C" int rl_set_star(rl_ctx *c, double rstar, int ncont, const double *nu,
     c->rstar = rstar;
     c->cont_nu.assign(nu, nu + ncont);
     c->istar.assign(istar, istar + ncont);
-    for (auto &s : c->incl) s.traj_ready = false;
+    for (auto &s : c->incl) s.traj_ready = s.counted = false;   // R* bounds the crossings
+    c->cross_owner = -1;
     return 0;
 }
 
@@ -437,7 +447,7 @@ static int build_rayset(rl_ctx *c, RaySet &rs, double anginf, int nphi, int nrex
     if (int e = upload(c, rs.d_x0, rs.x0.data(), rs.x0.size() * 8)) return e;
     if (int e = upload(c, rs.d_z0, rs.z0.data(), rs.z0.size() * 8)) return e;
     if (int e = upload(c, rs.d_ringw, w.data(), w.size() * 8)) return e;
-    rs.traj_ready = false;
+    rs.traj_ready = rs.counted = false;
     return 0;
 }
 
@@ -449,6 +459,7 @@ extern "C" int rl_setup_rays_circular(rl_ctx *c, int nincl, const double *incl, 
         return fail(c, RL_E_ARG, "rl_setup_rays_circular: set the grid first");
     HIPCHK(c, hipSetDevice(c->device));
     c->incl.clear();
+    c->cross_owner = -1;
     c->incl.resize(nincl);
     for (int i = 0; i < nincl; i++) {
         if (int e = build_rayset(c, c->incl[i], incl[i], nphi, nrext, dbdr)) return e;
@@ -464,6 +475,10 @@ extern "C" int rl_setup_rays_circular(rl_ctx *c, int nincl, const double *incl, 
 // ---------------------------------------------------------------------
 // trace
 // ---------------------------------------------------------------------
+// upper bound of the nodes of one ray: every R crossing, every cone
+// crossing and every extra point (telescope.F:3279, 3196, RAYEXPT)
+static int traj_nmax(const rl_ctx *c) { return 2 * (c->nr + 1) + 2 * c->nh + 40; }
+
 static int check_err(rl_ctx *c) {
     int e = 0;
     HIPCHK(c, hipMemcpyAsync(&e, c->d_err.p, 4, hipMemcpyDeviceToHost, c->stream));
@@ -472,8 +487,70 @@ static int check_err(rl_ctx *c) {
     return 0;
 }
 
-static int build_traj(rl_ctx *c, RaySet &rs) {
-    if (rs.traj_ready) return 0;
+// device bytes a ray set holds: nodes and, once built, the adjoint's
+// stencil transpose
+static size_t traj_bytes(const rl_ctx *c, const RaySet &rs) {
+    size_t b = (size_t)rs.nnodes * sizeof(GeoNode) + (size_t)(rs.nrays + 1) * 8;
+    if (rs.st_ready) b += (size_t)rs.nnodes * 4 * 12 + ((size_t)c->nr * c->nh + 1) * 4;
+    return b;
+}
+
+// device bytes a ray set holds now
+static size_t held_bytes(const RaySet &rs) {
+    return rs.d_nodes.n + rs.d_st_vals.n + rs.d_st_w.n + rs.d_st_off.n;
+}
+
+static size_t resident_traj(const rl_ctx *c) {
+    size_t b = 0;
+    for (const RaySet &r : c->incl) b += held_bytes(r);
+    return b;
+}
+
+// the memory budget: set by rl_set_memory_budget, else 60% of the device
+// memory free at the context's first trace plus what the context held
+// then (fixed, so that repeated calls plan alike)
+static size_t ctx_budget(rl_ctx *c) {
+    if (c->mem_budget) return c->mem_budget;
+    if (c->auto_budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
+        c->auto_budget = (size_t)(0.6 * (double)fr) + resident_traj(c) + c->d_rec.n +
+                         c->d_image.n + c->d_partial.n + c->d_ckpt.n + c->d_arec.n;
+    }
+    return c->auto_budget;
+}
+
+static void evict_traj(RaySet &rs) {
+    rs.d_nodes = DevBuf();
+    rs.d_st_vals = DevBuf();
+    rs.d_st_w = DevBuf();
+    rs.d_st_off = DevBuf();
+    rs.traj_ready = false;
+    rs.st_ready = false;
+}
+
+// evict the least recently used ray sets other than those in keep until
+// the resident trajectories plus need fit the budget (or none is left)
+static int make_room(rl_ctx *c, size_t need, const std::vector<int> &keep) {
+    const size_t budget = ctx_budget(c);
+    for (;;) {
+        if (resident_traj(c) + need <= budget) return 0;
+        int victim = -1;
+        for (int i = 0; i < (int)c->incl.size(); i++) {
+            const RaySet &r = c->incl[i];
+            if (!held_bytes(r) || std::find(keep.begin(), keep.end(), i) != keep.end()) continue;
+            if (victim < 0 || r.used < c->incl[victim].used) victim = i;
+        }
+        if (victim < 0) return 0;
+        HIPCHK(c, hipStreamSynchronize(c->stream));   // no launch may still read it
+        evict_traj(c->incl[victim]);
+    }
+}
+
+// ray crossings of a ray set (k_cross) into the context's scratch, and
+// the node counts: off / nnodes
+static int cross_traj(rl_ctx *c, int idx) {
+    RaySet &rs = c->incl[idx];
     rl::GridDev g{c->nr, 2 * c->nh, c->d_rc.as<double>(), c->d_tc.as<double>(), c->rstar};
     const int n = rs.nrays;
     const int iyeq = c->nh;
@@ -485,9 +562,7 @@ static int build_traj(rl_ctx *c, RaySet &rs) {
     HIPCHK(c, cnt.ensure((size_t)n * 4));
     HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));
     const int blend = rs.ring_ri[1] > 0.0 ? 1 : 0;
-    // upper bound of the nodes of one ray: every R crossing, every cone
-    // crossing and every extra point (telescope.F:3279, 3196, RAYEXPT)
-    const int nmax = 2 * (c->nr + 1) + 2 * c->nh + 40;
+    const int nmax = traj_nmax(c);
     HIPCHK(c, c->d_cross.ensure((size_t)n * nmax * 12));
     double *cs = c->d_cross.as<double>();
     int *ccode = reinterpret_cast<int *>(cs + (size_t)n * nmax);
@@ -502,8 +577,47 @@ static int build_traj(rl_ctx *c, RaySet &rs) {
     for (int i = 0; i < n; i++) rs.off[i + 1] = rs.off[i] + h[i];
     rs.nnodes = rs.off[n];
     if (int e = upload(c, rs.d_off, rs.off.data(), (size_t)(n + 1) * 8)) return e;
+    rs.counted = true;
+    c->cross_owner = idx;
+    return 0;
+}
+
+// node counts of a ray set (without building its nodes)
+static int count_traj(rl_ctx *c, int idx) {
+    if (c->incl[idx].counted) return 0;
+    if (int e = cross_traj(c, idx)) return e;
+    return 0;
+}
+
+// the scratch of k_cross is large for big ray sets: not kept past a build
+static void release_cross_scratch(rl_ctx *c) {
+    if (c->d_cross.n + c->d_scr.n > ((size_t)1 << 30)) {
+        (void)hipStreamSynchronize(c->stream);
+        c->d_cross = DevBuf();
+        c->d_scr = DevBuf();
+        c->cross_owner = -1;
+    }
+}
+
+// the nodes of a ray set, built if not resident; other ray sets (not in
+// keep) are evicted first when the budget requires
+static int build_traj_keep(rl_ctx *c, int idx, const std::vector<int> &keep) {
+    RaySet &rs = c->incl[idx];
+    rs.used = ++c->tick;
+    if (rs.traj_ready) return 0;
+    if (int e = count_traj(c, idx)) return e;
+    std::vector<int> k2 = keep;
+    k2.push_back(idx);
+    const size_t want = traj_bytes(c, rs), have = held_bytes(rs);
+    if (int e = make_room(c, want > have ? want - have : 0, k2)) return e;
+    if (c->cross_owner != idx)
+        if (int e = cross_traj(c, idx)) return e;
+    rl::GridDev g{c->nr, 2 * c->nh, c->d_rc.as<double>(), c->d_tc.as<double>(), c->rstar};
+    const int nmax = traj_nmax(c);
+    double *cs = c->d_cross.as<double>();
+    int *ccode = reinterpret_cast<int *>(cs + (size_t)rs.nrays * nmax);
     HIPCHK(c, rs.d_nodes.ensure((size_t)rs.nnodes * sizeof(GeoNode)));
-    rl::launch_geo(c->stream, g, c->d_vel.as<double>(), c->nh, n, nmax,
+    rl::launch_geo(c->stream, g, c->d_vel.as<double>(), c->nh, rs.nrays, nmax,
                    rs.d_off.as<long long>(), rs.d_x0.as<double>(), rs.d_z0.as<double>(),
                    rs.costh0, rs.sinth0, cs, ccode, rs.d_nodes.as<GeoNode>(),
                    c->d_err.as<int>());
@@ -511,37 +625,52 @@ static int build_traj(rl_ctx *c, RaySet &rs) {
     if (int e = check_err(c)) return e;
     rs.traj_ready = true;
     rs.st_ready = false;
+    release_cross_scratch(c);
     return 0;
 }
 
+static int build_traj(rl_ctx *c, RaySet &rs) {
+    return build_traj_keep(c, (int)(&rs - c->incl.data()), std::vector<int>());
+}
+
 // the stencil transpose of a ray set's nodes (once per trajectories)
-static int build_transpose(rl_ctx *c, RaySet &rs) {
-    if (rs.st_ready) return 0;
+// the stencil transpose of nodes [n0, n1) of a ray set (cell -> the
+// entries 4k+j, k relative to n0), into vals / w / off
+static int build_transpose_range(rl_ctx *c, RaySet &rs, long long n0, long long n1,
+                                 DevBuf &vals, DevBuf &w, DevBuf &off) {
     const int ncell = c->nr * c->nh;
-    const size_t nent = 4 * (size_t)rs.nnodes;
+    const long long nn = n1 - n0;
+    const size_t nent = 4 * (size_t)nn;
     if (nent >= (size_t)1 << 31)
-        return fail(c, 13, "adjoint: %lld nodes exceed the stencil index range", rs.nnodes);
+        return fail(c, 13, "adjoint: %lld nodes exceed the stencil index range", nn);
     DevBuf keys_a, keys_b, vals_a, temp;
     HIPCHK(c, keys_a.ensure(nent * 4));
     HIPCHK(c, keys_b.ensure(nent * 4));
     HIPCHK(c, vals_a.ensure(nent * 4));
-    HIPCHK(c, rs.d_st_vals.ensure(nent * 4));
-    HIPCHK(c, rs.d_st_w.ensure(nent * 8));
-    HIPCHK(c, rs.d_st_off.ensure(((size_t)ncell + 1) * 4));
+    HIPCHK(c, vals.ensure(nent * 4));
+    HIPCHK(c, w.ensure(nent * 8));
+    HIPCHK(c, off.ensure(((size_t)ncell + 1) * 4));
+    const GeoNode *geo = rs.d_nodes.as<GeoNode>() + n0;
     size_t tb = 0;
-    if (rl::build_stencil_transpose(c->stream, rs.nnodes, rs.d_nodes.as<GeoNode>(), ncell,
-                                    keys_a.as<int>(), keys_b.as<int>(), vals_a.as<int>(),
-                                    rs.d_st_vals.as<int>(), rs.d_st_w.as<double>(),
-                                    rs.d_st_off.as<int>(), nullptr, &tb))
+    if (rl::build_stencil_transpose(c->stream, nn, geo, ncell, keys_a.as<int>(),
+                                    keys_b.as<int>(), vals_a.as<int>(), vals.as<int>(),
+                                    w.as<double>(), off.as<int>(), nullptr, &tb))
         return fail(c, RL_E_HIP, "stencil transpose: sort size query failed");
     HIPCHK(c, temp.ensure(tb));
-    if (int e = rl::build_stencil_transpose(c->stream, rs.nnodes, rs.d_nodes.as<GeoNode>(), ncell,
-                                            keys_a.as<int>(), keys_b.as<int>(), vals_a.as<int>(),
-                                            rs.d_st_vals.as<int>(), rs.d_st_w.as<double>(),
-                                            rs.d_st_off.as<int>(), temp.p, &tb))
+    if (int e = rl::build_stencil_transpose(c->stream, nn, geo, ncell, keys_a.as<int>(),
+                                            keys_b.as<int>(), vals_a.as<int>(), vals.as<int>(),
+                                            w.as<double>(), off.as<int>(), temp.p, &tb))
         return fail(c, RL_E_HIP, "stencil transpose: sort failed (%d)", e);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));      // temporaries are freed on return
+    return 0;
+}
+
+// the whole ray set's transpose, kept with its nodes
+static int build_transpose(rl_ctx *c, RaySet &rs) {
+    if (rs.st_ready) return 0;
+    if (int e = build_transpose_range(c, rs, 0, rs.nnodes, rs.d_st_vals, rs.d_st_w, rs.d_st_off))
+        return e;
     rs.st_ready = true;
     return 0;
 }
@@ -939,49 +1068,76 @@ extern "C" int rl_trace_lines_forward_ex(rl_ctx *c, int nitems, const rl_item *i
             (it.skip_mode != RL_SKIP_REFERENCE && it.skip_mode != RL_SKIP_NONE))
             return fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii);
     }
-    // trajectories of every inclination used (line-independent, kept)
+    // node counts of every inclination used (k_cross); the nodes are built
+    // per batch, and a ray set not used by a batch may be evicted to make
+    // room for it
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     for (int ii = 0; ii < nitems; ii++)
-        if (int e = build_traj(c, c->incl[items[ii].incl])) return e;
+        if (int e = count_traj(c, items[ii].incl)) return e;
+    const size_t budget = ctx_budget(c);
+    // batches: consecutive items whose buffers, with the nodes of their
+    // inclinations, fit the budget (at least one item per batch)
+    std::vector<std::pair<int, int>> cuts;
+    c->last_work = 0;
+    {
+        int i0 = 0;
+        while (i0 < nitems) {
+            size_t used = 0;
+            std::vector<int> incs;
+            int i1 = i0;
+            while (i1 < nitems) {
+                const int in = items[i1].incl;
+                size_t b = item_bytes(c, items[i1], c->incl[in]);
+                if (std::find(incs.begin(), incs.end(), in) == incs.end())
+                    b += traj_bytes(c, c->incl[in]);
+                if (i1 > i0 && (used + b > budget || i1 - i0 >= 65535)) break;   // grid y/z limit
+                used += b;
+                if (std::find(incs.begin(), incs.end(), in) == incs.end()) incs.push_back(in);
+                i1++;
+            }
+            cuts.emplace_back(i0, i1);
+            c->last_work = std::max(c->last_work, used);
+            i0 = i1;
+        }
+    }
     HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev[5]));
     float tt = 0;
     (void)hipEventElapsedTime(&tt, c->ev[0], c->ev[5]);
     c->t_ms[0] = tt;
     c->t_ms[4] = tt;
-    // batches under the memory budget
-    // the default budget is fixed at the context's first trace (60% of the
-    // device memory free then), so that repeated calls cut the same
-    // batches and reuse their tuning and graphs
-    size_t budget = c->mem_budget;
-    if (budget == 0) {
-        if (c->auto_budget == 0) {
-            size_t fr = 0, tot = 0;
-            HIPCHK(c, hipMemGetInfo(&fr, &tot));
-            c->auto_budget = (size_t)(0.6 * (double)fr) + c->d_rec.n + c->d_image.n +
-                             c->d_partial.n;
-        }
-        budget = c->auto_budget;
-    }
     c->last_batches = 0;
     size_t fo = 0, io = 0;
-    int i0 = 0;
-    while (i0 < nitems) {
-        size_t used = 0;
-        int i1 = i0;
-        while (i1 < nitems) {
-            const size_t b = item_bytes(c, items[i1], c->incl[items[i1].incl]);
-            if (i1 > i0 && (used + b > budget || i1 - i0 >= 65535)) break;   // grid y/z limit
-            used += b;
-            i1++;
+    for (const auto &cut : cuts) {
+        const int i0 = cut.first, i1 = cut.second;
+        // this batch's trajectories resident (others evicted if needed)
+        std::vector<int> incs;
+        size_t ibytes = 0;
+        for (int k = i0; k < i1; k++) {
+            const int in = items[k].incl;
+            ibytes += item_bytes(c, items[k], c->incl[in]);
+            if (std::find(incs.begin(), incs.end(), in) == incs.end()) incs.push_back(in);
         }
+        for (int in : incs)
+            if (!c->incl[in].traj_ready) {
+                const size_t want = traj_bytes(c, c->incl[in]), have = held_bytes(c->incl[in]);
+                ibytes += want > have ? want - have : 0;
+            }
+        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        if (int e = make_room(c, ibytes, incs)) return e;
+        for (int in : incs)
+            if (int e = build_traj_keep(c, in, incs)) return e;
+        HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev[5]));
+        (void)hipEventElapsedTime(&tt, c->ev[0], c->ev[5]);
+        c->t_ms[0] += tt;
+        c->t_ms[4] += tt;
         if (int e = run_batch(c, items, i0, i1, flux, image, cmask, warn, fo, io)) return e;
         c->last_batches++;
         for (int k = i0; k < i1; k++) {
             fo += items[k].nfr;
             io += (size_t)c->incl[items[k].incl].nrays * items[k].nfr;
         }
-        i0 = i1;
     }
     return 0;
 }
@@ -1032,49 +1188,100 @@ static int zero_dev(rl_ctx *c, DevBuf &b, size_t bytes) {
     return 0;
 }
 
-static size_t adj_budget(rl_ctx *c, size_t held) {
-    if (c->mem_budget) return c->mem_budget;
-    size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return held;
-    return (size_t)(0.6 * (double)fr) + held;
+// A chunk of an item's rays: rays [r0, r1), nodes [n0, n1).
+struct AdjChunk {
+    int r0, r1;
+    long long n0, n1;
+};
+
+// checkpoint interval and ray chunks of one adjoint item
+struct AdjPlan {
+    int K = 1;
+    std::vector<AdjChunk> chunks;
+    size_t work = 0;                 // bytes of the largest chunk's buffers + the image
+};
+
+// device bytes of a chunk's buffers at interval K: node records, the I
+// checkpoints, node gradients (scaled and true), the checkpoint CSR and,
+// for a chunk of a split item, its own stencil transpose with the sort's
+// scratch
+static size_t chunk_bytes(const rl_ctx *c, const RaySet &rs, int nfr, int K, int r0, int r1,
+                          bool own_transpose) {
+    const int fp64 = c->precision == RL_FP64;
+    const size_t tsz = fp64 ? 8 : 4;
+    const size_t nn = (size_t)(rs.off[r1] - rs.off[r0]), nr = (size_t)(r1 - r0);
+    size_t b = nn * rl::fast_rec_size(fp64);
+    b += (nn / K + nr) * (size_t)nfr * tsz;
+    b += nn * 5 * 8 * 2;
+    b += (nr + 1) * 8;
+    if (own_transpose) b += nn * 4 * 24 + ((size_t)c->nr * c->nh + 1) * 4 + (nn * 4 * 8 >> 3);
+    return b;
 }
 
-// checkpoint interval for a call: a fixed one (RL_OPT_CKPT), else I
-// before every segment (no recomputation in the reverse sweep) when the
-// checkpoints of all the call's items fit half the memory budget, else
-// the smallest power of two up to the maximum that does (the maximum if
-// none does)
-static int pick_kblk(rl_ctx *c, int nitems, const rl_item *items, size_t budget) {
+// The plan of an item: the smallest interval K (or the fixed one) at which
+// the whole item fits the budget beside its trajectories and image, else
+// the largest interval with the rays cut into chunks of about equal node
+// counts that fit (at least 2, at most 64 chunks; at most 2^28 nodes
+// each, within the transpose's 32-bit entry index).
+static int plan_adjoint(rl_ctx *c, const RaySet &rs, int nfr, bool tape, AdjPlan &p) {
     const int KMAX = rl::adjoint_block_len();
-    if (c->fixed_kblk) return c->fixed_kblk;
-    const size_t tsz = c->precision == RL_FP64 ? 8 : 4;
-    double need = 0.0;                       // bytes at K = 1
-    for (int ii = 0; ii < nitems; ii++) {
-        const int in = items[ii].incl;
-        if (in < 0 || in >= (int)c->incl.size()) continue;
-        const RaySet &rs = c->incl[in];
-        // before the first trajectory build nnodes is unknown: assume the
-        // per-ray maximum of segments
-        const double nn = rs.traj_ready
-                              ? (double)rs.nnodes
-                              : (double)rs.nrays * (2.0 * c->nr + 2.0 * c->nh + 34.0);
-        need += nn * items[ii].nfr * (double)tsz;
+    const size_t budget = ctx_budget(c);
+    const size_t fixed = (size_t)rs.nnodes * sizeof(GeoNode) + (size_t)rs.nnodes * 4 * 12 +
+                         (size_t)rs.nrays * nfr * 8 + (size_t)rs.nrings * nfr * 8;
+    const size_t avail = budget > fixed ? budget - fixed : 0;
+    p.chunks.clear();
+    const bool fits_index = 4 * (size_t)rs.nnodes < ((size_t)1 << 31);
+    for (int K = c->fixed_kblk ? c->fixed_kblk : 1; K <= KMAX; K *= 2) {
+        const size_t b = chunk_bytes(c, rs, nfr, K, 0, rs.nrays, false);
+        if (b <= avail && fits_index) {
+            p.K = K;
+            p.chunks.push_back({0, rs.nrays, 0, rs.nnodes});
+            p.work = b + (size_t)rs.nrays * nfr * 8;
+            return 0;
+        }
+        if (c->fixed_kblk) break;
     }
-    int K = 1;
-    while (K < KMAX && need / K > 0.5 * (double)budget) K *= 2;
-    return K;
+    if (tape)
+        return fail(c, 13, "adjoint tape of %lld nodes x %d channels does not fit the memory "
+                           "budget (%.3g GB); rl_trace_lines_adjoint recomputes in ray chunks",
+                    rs.nnodes, nfr, budget * 1e-9);
+    p.K = c->fixed_kblk ? c->fixed_kblk : KMAX;
+    const size_t whole = chunk_bytes(c, rs, nfr, p.K, 0, rs.nrays, true);
+    long long nch = avail ? (long long)((whole + avail - 1) / avail) : 64;
+    nch = std::max(nch, (long long)(rs.nnodes >> 28) + 1);      // <= 2^28 nodes a chunk
+    nch = std::min(std::max(nch, 2ll), 64ll);
+    nch = std::min(nch, (long long)rs.nrays);
+    // chunk j ends at the first ray boundary at or past j/nch of the nodes
+    int r0 = 0;
+    p.work = 0;
+    for (long long j = 1; j <= nch && r0 < rs.nrays; j++) {
+        int r1 = rs.nrays;
+        if (j < nch) {
+            const long long target = rs.nnodes * j / nch;
+            r1 = (int)(std::lower_bound(rs.off.begin() + r0 + 1, rs.off.end(), target) -
+                       rs.off.begin());
+            r1 = std::min(std::max(r1, r0 + 1), rs.nrays);
+        }
+        p.chunks.push_back({r0, r1, rs.off[r0], rs.off[r1]});
+        p.work = std::max(p.work, chunk_bytes(c, rs, nfr, p.K, r0, r1, true));
+        r0 = r1;
+    }
+    p.work += (size_t)rs.nrays * nfr * 8;
+    return 0;
 }
 
 static void fill_adj(rl_ctx *c, const rl_item &it, const ItemSetup &su, AdjItemBufs &ab, int K,
-                     rl::AdjLaunch &a) {
+                     const AdjChunk &ch, rl::AdjLaunch &a) {
     const RaySet &rs = *su.rs;
     const int nfr = it.nfr;
-    a.nrays = rs.nrays; a.nfr = nfr; a.nrings = rs.nrings; a.nphi = rs.nphi;
-    a.nnodes = rs.nnodes;
+    a.nrays = ch.r1 - ch.r0; a.nfr = nfr; a.nrings = rs.nrings; a.nphi = rs.nphi;
+    a.nnodes = ch.n1 - ch.n0;
+    a.ray0 = ch.r0;
+    a.node0 = ch.n0;
     a.ncell = c->nr * c->nh; a.nlev = c->nlev; a.nlines = c->nlines; a.line = it.line;
     a.lev_up = c->lev_up[it.line]; a.lev_low = c->lev_low[it.line];
     a.molpg = 1.0 / (c->umass_av * 1.6726e-24);
-    a.off = rs.d_off.as<long long>();
+    a.off = rs.d_off.as<long long>() + ch.r0;
     a.ckoff = ab.ckoff.as<long long>();
     a.geo = rs.d_nodes.as<GeoNode>();
     a.cells = c->d_cells.as<CellRec>();
@@ -1087,7 +1294,7 @@ static void fill_adj(rl_ctx *c, const rl_item &it, const ItemSetup &su, AdjItemB
     a.istar = a.dnu + 2 * nfr;
     a.lc = su.lc;
     a.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
-    a.image = c->d_image.as<double>();
+    a.image = c->d_image.as<double>() + (size_t)ch.r0 * nfr;
     a.ring_w = rs.d_ringw.as<double>();
     a.center_w = rs.center_w;
     a.nchunks = rs.nrings;
@@ -1097,9 +1304,10 @@ static void fill_adj(rl_ctx *c, const rl_item &it, const ItemSetup &su, AdjItemB
     a.gflux = c->d_gflux.as<double>();
     a.gnode = c->d_gnode.as<double>();
     a.gtrue = c->d_gcell.as<double>();
-    a.st_off = rs.d_st_off.as<int>();
-    a.st_vals = rs.d_st_vals.as<int>();
-    a.st_w = rs.d_st_w.as<double>();
+    const bool whole = ch.r0 == 0 && ch.r1 == rs.nrays;
+    a.st_off = whole ? rs.d_st_off.as<int>() : c->d_stc_off.as<int>();
+    a.st_vals = whole ? rs.d_st_vals.as<int>() : c->d_stc_vals.as<int>();
+    a.st_w = whole ? rs.d_st_w.as<double>() : c->d_stc_w.as<double>();
     a.rho = c->d_rho.as<double>();
     a.abund = c->d_abund.as<double>();
     a.pops = c->d_pops.as<double>();
@@ -1111,58 +1319,79 @@ static void fill_adj(rl_ctx *c, const rl_item &it, const ItemSetup &su, AdjItemB
     a.g_ad = c->d_gad.as<double>();
 }
 
-// the checkpointed forward of one item: node records, I checkpoints every
-// K segments, the spectrum (copied to flux_host when given)
-static int adj_forward_item(rl_ctx *c, const rl_item &it, int ii, int K, AdjItemBufs &ab,
-                            double *flux_host) {
-    ItemSetup su;
+// the item's trajectories resident (other inclinations evicted as the
+// budget requires), its line tables, and the image / flux buffers
+static int adj_setup(rl_ctx *c, const rl_item &it, int ii, ItemSetup &su) {
+    if (it.incl < 0 || it.incl >= (int)c->incl.size())
+        return fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii);
+    if (int e = build_traj_keep(c, it.incl, std::vector<int>())) return e;
     if (int e = setup_item(c, it, ii, su, c->ev[0], c->ev[1])) return e;
-    RaySet &rs = *su.rs;
-    const int nfr = it.nfr, nrays = rs.nrays;
-    const size_t tsz = c->precision == RL_FP64 ? 8 : 4;
-    std::vector<long long> ckoff(nrays + 1, 0);       // ceil((n-1)/K) blocks per ray
-    for (int r = 0; r < nrays; r++) {
-        const long long n = rs.off[r + 1] - rs.off[r];
-        ckoff[r + 1] = ckoff[r] + (n >= 2 ? (n - 1 + K - 1) / K : 0);
-    }
-    if (int e = upload(c, ab.ckoff, ckoff.data(), ckoff.size() * 8)) return e;
-    HIPCHK(c, ab.ckpt.ensure((size_t)ckoff[nrays] * nfr * tsz));
-    HIPCHK(c, ab.rec.ensure((size_t)rs.nnodes * rl::fast_rec_size(c->precision == RL_FP64)));
-    HIPCHK(c, c->d_rinfo.ensure((size_t)nrays * sizeof(RayInfo)));
-    HIPCHK(c, c->d_image.ensure((size_t)nrays * nfr * 8));
-    HIPCHK(c, c->d_partial.ensure((size_t)rs.nrings * nfr * 8));
-    HIPCHK(c, c->d_flux.ensure((size_t)nfr * 8));
-    rl::AdjLaunch a;
-    fill_adj(c, it, su, ab, K, a);
-    if (c->precision == RL_FP64) rl::launch_adjoint_fwd<double>(c->stream, a);
-    else rl::launch_adjoint_fwd<float>(c->stream, a);
-    HIPCHK(c, hipGetLastError());
-    if (flux_host)
-        HIPCHK(c, hipMemcpyAsync(flux_host, c->d_flux.p, (size_t)nfr * 8,
-                                 hipMemcpyDeviceToHost, c->stream));
-    c->counts[0] += nrays;
-    c->counts[1] += rs.nnodes;
+    const RaySet &rs = *su.rs;
+    HIPCHK(c, c->d_rinfo.ensure((size_t)rs.nrays * sizeof(RayInfo)));
+    HIPCHK(c, c->d_image.ensure((size_t)rs.nrays * it.nfr * 8));
+    HIPCHK(c, c->d_partial.ensure((size_t)rs.nrings * it.nfr * 8));
+    HIPCHK(c, c->d_flux.ensure((size_t)it.nfr * 8));
     return 0;
 }
 
-// the reverse sweep of one item, accumulating into the context's cell
-// gradients
-static int adj_reverse_item(rl_ctx *c, const rl_item &it, int ii, int K, AdjItemBufs &ab,
-                            const double *gflux_host) {
-    ItemSetup su;
-    if (int e = setup_item(c, it, ii, su, nullptr, nullptr)) return e;
-    RaySet &rs = *su.rs;
-    const int nfr = it.nfr;
-    if (int e = upload(c, c->d_gflux, gflux_host, (size_t)nfr * 8)) return e;
-    if (int e = zero_dev(c, c->d_gnode, (size_t)rs.nnodes * 5 * 8)) return e;
-    HIPCHK(c, c->d_gcell.ensure((size_t)rs.nnodes * 5 * 8));
-    if (int e = build_transpose(c, rs)) return e;
+// the checkpointed forward of one chunk: node records, I every K segments
+static int adj_forward_chunk(rl_ctx *c, const rl_item &it, const ItemSetup &su, int K,
+                             const AdjChunk &ch, AdjItemBufs &ab) {
+    const RaySet &rs = *su.rs;
+    const int nfr = it.nfr, nr = ch.r1 - ch.r0;
+    const size_t tsz = c->precision == RL_FP64 ? 8 : 4;
+    std::vector<long long> ckoff(nr + 1, 0);          // ceil((n-1)/K) blocks per ray
+    for (int r = 0; r < nr; r++) {
+        const long long n = rs.off[ch.r0 + r + 1] - rs.off[ch.r0 + r];
+        ckoff[r + 1] = ckoff[r] + (n >= 2 ? (n - 1 + K - 1) / K : 0);
+    }
+    if (int e = upload(c, ab.ckoff, ckoff.data(), ckoff.size() * 8)) return e;
+    HIPCHK(c, ab.ckpt.ensure((size_t)ckoff[nr] * nfr * tsz));
+    HIPCHK(c, ab.rec.ensure((size_t)(ch.n1 - ch.n0) * rl::fast_rec_size(c->precision == RL_FP64)));
     rl::AdjLaunch a;
-    fill_adj(c, it, su, ab, K, a);
+    fill_adj(c, it, su, ab, K, ch, a);
+    if (c->precision == RL_FP64) rl::launch_adjoint_fwd<double>(c->stream, a);
+    else rl::launch_adjoint_fwd<float>(c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    c->counts[0] += nr;
+    c->counts[1] += ch.n1 - ch.n0;
+    return 0;
+}
+
+// the spectrum of an item once every chunk is traced
+static int adj_flux(rl_ctx *c, const rl_item &it, const ItemSetup &su, double *flux_host) {
+    AdjItemBufs none;
+    rl::AdjLaunch a;
+    fill_adj(c, it, su, none, 1, AdjChunk{0, su.rs->nrays, 0, su.rs->nnodes}, a);
+    rl::launch_adjoint_flux(c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    if (flux_host)
+        HIPCHK(c, hipMemcpyAsync(flux_host, c->d_flux.p, (size_t)it.nfr * 8,
+                                 hipMemcpyDeviceToHost, c->stream));
+    return 0;
+}
+
+// the reverse sweep of one chunk (its forward's records and checkpoints),
+// accumulating into the context's cell gradients; gflux is on the device
+static int adj_reverse_chunk(rl_ctx *c, const rl_item &it, const ItemSetup &su, int K,
+                             const AdjChunk &ch, AdjItemBufs &ab) {
+    RaySet &rs = *su.rs;
+    const long long nn = ch.n1 - ch.n0;
+    if (int e = zero_dev(c, c->d_gnode, (size_t)nn * 5 * 8)) return e;
+    HIPCHK(c, c->d_gcell.ensure((size_t)nn * 5 * 8));
+    if (ch.r0 == 0 && ch.r1 == rs.nrays) {
+        if (int e = build_transpose(c, rs)) return e;
+    } else {
+        if (int e = build_transpose_range(c, rs, ch.n0, ch.n1, c->d_stc_vals, c->d_stc_w,
+                                          c->d_stc_off))
+            return e;
+    }
+    rl::AdjLaunch a;
+    fill_adj(c, it, su, ab, K, ch, a);
     if (c->precision == RL_FP64) rl::launch_adjoint_rev<double>(c->stream, a);
     else rl::launch_adjoint_rev<float>(c->stream, a);
     HIPCHK(c, hipGetLastError());
-    c->counts[3] += (long long)rs.nrays * nfr;
+    c->counts[3] += (long long)(ch.r1 - ch.r0) * it.nfr;
     return 0;
 }
 
@@ -1204,24 +1433,53 @@ extern "C" int rl_trace_lines_forward_tape(rl_ctx *c, int nitems, const rl_item 
     if (!c || nitems < 0 || (nitems && !items))
         return fail(c, RL_E_ARG, "rl_trace_lines_forward_tape: bad args");
     if (int e = adj_check(c, nitems, items, "rl_trace_lines_forward_tape")) return e;
-    size_t held = 0;
-    for (auto &t : c->tape) held += t.rec.n + t.ckpt.n + t.ckoff.n;
-    const size_t budget = adj_budget(c, held);
     c->tape_items.clear();                   // invalid until the forward completes
-    if ((int)c->tape.size() > nitems) c->tape.resize(nitems);
-    while ((int)c->tape.size() < nitems) c->tape.emplace_back();
-    const int K = pick_kblk(c, nitems, items, budget);
-    c->last_kblk = K;
+    c->tape.clear();                         // its memory is available to the plan
+    c->tape.resize(nitems);
     HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
     size_t fo = 0;
     for (int ii = 0; ii < nitems; ii++) fo += items[ii].nfr;
     c->tape_flux.assign(fo, 0.0);
     fo = 0;
+    c->last_kblk = 0;
+    c->last_chunks = 1;
+    c->last_work = 0;
+    int Kt = 0;
     for (int ii = 0; ii < nitems; ii++) {
-        if (int e = adj_forward_item(c, items[ii], ii, K, c->tape[ii], c->tape_flux.data() + fo))
-            return e;
-        fo += items[ii].nfr;
+        const rl_item &it = items[ii];
+        if (it.incl < 0 || it.incl >= (int)c->incl.size())
+            return fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii);
+        if (int e = count_traj(c, it.incl)) return e;
+        AdjPlan p;
+        if (int e = plan_adjoint(c, c->incl[it.incl], it.nfr, true, p)) return e;
+        // one interval for the whole tape: the largest any item needs
+        Kt = std::max(Kt, p.K);
+        c->last_work = std::max(c->last_work, p.work);
     }
+    {
+        // every item's records and checkpoints are held at once
+        size_t held = 0, fixed = 0;
+        for (int ii = 0; ii < nitems; ii++) {
+            const RaySet &rs = c->incl[items[ii].incl];
+            held += chunk_bytes(c, rs, items[ii].nfr, Kt, 0, rs.nrays, false);
+            fixed = std::max(fixed, (size_t)rs.nnodes * (sizeof(GeoNode) + 48) +
+                                        (size_t)rs.nrays * items[ii].nfr * 8);
+        }
+        if (held + fixed > ctx_budget(c))
+            return fail(c, 13, "adjoint tape of %d items (%.3g GB) does not fit the memory "
+                               "budget (%.3g GB)", nitems, (held + fixed) * 1e-9,
+                        ctx_budget(c) * 1e-9);
+    }
+    for (int ii = 0; ii < nitems; ii++) {
+        const rl_item &it = items[ii];
+        ItemSetup su;
+        if (int e = adj_setup(c, it, ii, su)) return e;
+        const AdjChunk whole{0, su.rs->nrays, 0, su.rs->nnodes};
+        if (int e = adj_forward_chunk(c, it, su, Kt, whole, c->tape[ii])) return e;
+        if (int e = adj_flux(c, it, su, c->tape_flux.data() + fo)) return e;
+        fo += it.nfr;
+    }
+    c->last_kblk = Kt;
     HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     if (int e = check_err(c)) return e;
     if (flux) std::copy(c->tape_flux.begin(), c->tape_flux.end(), flux);
@@ -1230,7 +1488,7 @@ extern "C" int rl_trace_lines_forward_tape(rl_ctx *c, int nitems, const rl_item 
     c->t_ms[4] = t;
     c->tape_items.assign(items, items + nitems);
     c->tape_gen = c->gen;
-    c->tape_kblk = K;
+    c->tape_kblk = Kt;
     c->last_used_tape = 0;
     return 0;
 }
@@ -1257,32 +1515,55 @@ extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *item
     size_t fo = 0;
     if (use_tape) {
         c->last_kblk = c->tape_kblk;
+        c->last_chunks = 1;
         if (flux) std::copy(c->tape_flux.begin(), c->tape_flux.end(), flux);
         for (int ii = 0; ii < nitems; ii++) {
-            if (int e = adj_reverse_item(c, items[ii], ii, c->tape_kblk, c->tape[ii],
-                                         dL_dflux + fo))
-                return e;
-            fo += items[ii].nfr;
+            const rl_item &it = items[ii];
+            ItemSetup su;
+            if (int e = adj_setup(c, it, ii, su)) return e;
+            if (int e = upload(c, c->d_gflux, dL_dflux + fo, (size_t)it.nfr * 8)) return e;
+            const AdjChunk whole{0, su.rs->nrays, 0, su.rs->nnodes};
+            if (int e = adj_reverse_chunk(c, it, su, c->tape_kblk, whole, c->tape[ii])) return e;
+            fo += it.nfr;
         }
     } else {
+        // items one at a time through the same buffers, each by its own plan
         AdjItemBufs ab;
         ab.rec = std::move(c->d_arec);
         ab.ckoff = std::move(c->d_ckoff);
         ab.ckpt = std::move(c->d_ckpt);
         c->last_kblk = 0;
+        c->last_chunks = 0;
+        c->last_work = 0;
         int err = 0;
         for (int ii = 0; ii < nitems && !err; ii++) {
-            // items one at a time through the same buffers: the interval
-            // follows each item's own checkpoint size
-            const int Ki = pick_kblk(c, 1, items + ii, adj_budget(c, ab.ckpt.n));
-            err = adj_forward_item(c, items[ii], ii, Ki, ab, flux ? flux + fo : nullptr);
-            if (!err) err = adj_reverse_item(c, items[ii], ii, Ki, ab, dL_dflux + fo);
-            c->last_kblk = std::max(c->last_kblk, Ki);
-            fo += items[ii].nfr;
+            const rl_item &it = items[ii];
+            ItemSetup su;
+            AdjPlan p;
+            err = it.incl < 0 || it.incl >= (int)c->incl.size()
+                      ? fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii)
+                      : count_traj(c, it.incl);
+            if (!err) err = plan_adjoint(c, c->incl[it.incl], it.nfr, false, p);
+            if (!err) err = adj_setup(c, it, ii, su);
+            if (!err) err = upload(c, c->d_gflux, dL_dflux + fo, (size_t)it.nfr * 8);
+            for (size_t k = 0; k < p.chunks.size() && !err; k++) {
+                err = adj_forward_chunk(c, it, su, p.K, p.chunks[k], ab);
+                if (!err) err = adj_reverse_chunk(c, it, su, p.K, p.chunks[k], ab);
+            }
+            if (!err) err = adj_flux(c, it, su, flux ? flux + fo : nullptr);
+            c->last_kblk = std::max(c->last_kblk, p.K);
+            c->last_chunks = std::max(c->last_chunks, (int)p.chunks.size());
+            c->last_work = std::max(c->last_work, p.work);
+            fo += it.nfr;
         }
         c->d_arec = std::move(ab.rec);
         c->d_ckoff = std::move(ab.ckoff);
         c->d_ckpt = std::move(ab.ckpt);
+        if (c->last_chunks > 1) {             // per-chunk transposes are not kept
+            c->d_stc_vals = DevBuf();
+            c->d_stc_w = DevBuf();
+            c->d_stc_off = DevBuf();
+        }
         if (err) return err;
     }
     HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
@@ -1305,10 +1586,25 @@ extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *item
     return 0;
 }
 
-extern "C" int rl_last_adjoint_info(const rl_ctx *c, int *ckpt_interval, int *used_tape) {
+extern "C" int rl_last_adjoint_info(const rl_ctx *c, int *ckpt_interval, int *used_tape,
+                                    int *chunks) {
     if (!c) return RL_E_ARG;
     if (ckpt_interval) *ckpt_interval = c->last_kblk;
     if (used_tape) *used_tape = c->last_used_tape;
+    if (chunks) *chunks = c->last_chunks;
+    return 0;
+}
+
+extern "C" int rl_memory_info(rl_ctx *c, double *out) {
+    if (!c || !out) return RL_E_ARG;
+    size_t fr = 0, tot = 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemGetInfo(&fr, &tot));
+    out[0] = (double)ctx_budget(c);
+    out[1] = (double)resident_traj(c);
+    out[2] = (double)c->last_work;
+    out[3] = (double)fr;
+    out[4] = (double)tot;
     return 0;
 }
 

@@ -102,9 +102,15 @@ void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
 int formal_batch_blocks_per_cu(int fp64);
 
 // One item of the adjoint (scaled records of precision T, all channels).
+// A chunk is a range of the ray set's rays: nrays, nnodes and every
+// per-node and per-ray buffer are the chunk's own; off points at the
+// chunk's slice of the ray set's node CSR (global node indices), geo at the
+// ray set's nodes; ray0 and node0 are the chunk's first ray and node.
 struct AdjLaunch {
     int nrays, nfr, nrings, nphi;
     long long nnodes;
+    int ray0;
+    long long node0;
     int ncell, nlev, nlines, line, lev_up, lev_low;
     double molpg;
     const long long *off, *ckoff;      // node CSR; checkpoint CSR (blocks per ray)
@@ -117,7 +123,8 @@ struct AdjLaunch {
     const double *dnu, *velo, *istar;  // per channel
     LineConst lc;
     double aks2;
-    double *image;                     // [nrays][nfr] forward intensities
+    double *image;                     // [nrays][nfr] forward intensities (the chunk's rows;
+                                       // the whole item's for launch_adjoint_flux)
     const double *ring_w;
     double center_w;
     int nchunks;
@@ -147,6 +154,7 @@ void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int
 template <typename T> void launch_item(hipStream_t st, const ItemLaunch &it);
 template <typename T> void launch_adjoint_fwd(hipStream_t st, const AdjLaunch &a);
 template <typename T> void launch_adjoint_rev(hipStream_t st, const AdjLaunch &a);
+void launch_adjoint_flux(hipStream_t st, const AdjLaunch &a);
 int adjoint_block_len();
 // Stencil transpose (cell -> node entries) of a node set; with temp ==
 // nullptr only *temp_bytes is set.  keys/vals: 4*nnodes ints each (a, b);

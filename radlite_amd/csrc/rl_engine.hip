@@ -312,7 +312,7 @@ template <typename T>
 __device__ inline void nodes_ray_fast(int ray, int lane, const long long *__restrict__ off,
                                       const GeoNode *__restrict__ geo,
                                       const CellRec *__restrict__ cells,
-                                      FastRecT<T> *__restrict__ rec,
+                                      FastRecT<T> *__restrict__ rec, long long node0,
                                       RayInfo *__restrict__ rinfo, int nfr,
                                       const double *__restrict__ velo, double aks2,
                                       int skip_mode, const LineConst &lc) {
@@ -356,7 +356,7 @@ __device__ inline void nodes_ray_fast(int ray, int lane, const long long *__rest
             r.inv_aa = T(0); r.norm = T(0); r.dls = -T(1); r.inv_du = T(0);
             r.rel0 = r.rel1 = T(1); r.sigm = r.sig;
         }
-        rec[b + k] = r;
+        rec[b - node0 + k] = r;
         if (k + 1 < n && v[CR_NU] + v[CR_NL] > LEVTHRES) {
             umin = fmin(umin, gn.u);
             umax = fmax(umax, gn.u);
@@ -368,12 +368,12 @@ __device__ inline void nodes_ray_fast(int ray, int lane, const long long *__rest
 template <typename T>
 __global__ void __launch_bounds__(64)
 k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__restrict__ geo,
-             const CellRec *__restrict__ cells, FastRecT<T> *__restrict__ rec,
+             const CellRec *__restrict__ cells, FastRecT<T> *__restrict__ rec, long long node0,
              RayInfo *__restrict__ rinfo, int nfr, const double *__restrict__ velo,
              double aks2, int skip_mode, LineConst lc) {
     if ((int)blockIdx.x >= nrays) return;
-    nodes_ray_fast<T>(blockIdx.x, threadIdx.x, off, geo, cells, rec, rinfo, nfr, velo, aks2,
-                      skip_mode, lc);
+    nodes_ray_fast<T>(blockIdx.x, threadIdx.x, off, geo, cells, rec, node0, rinfo, nfr, velo,
+                      aks2, skip_mode, lc);
 }
 
 // ---------------------------------------------------------------------
@@ -602,7 +602,7 @@ void launch_item(hipStream_t st, const ItemLaunch &it) {
                            it.skip_mode);
     else
         hipLaunchKernelGGL(k_nodes_fast<float>, dim3(it.nrays), dim3(64), 0, st, it.nrays, it.off,
-                           it.geo, it.cells, rec, it.rinfo, it.nfr, it.velo, it.aks2,
+                           it.geo, it.cells, rec, 0ll, it.rinfo, it.nfr, it.velo, it.aks2,
                            it.skip_mode, it.lc);
     if (it.ev_nodes) (void)hipEventRecord(it.ev_nodes, st);
     long long total = (long long)it.nrays * it.nfr;
@@ -641,7 +641,7 @@ k_nodes_batch(const ItemDev *__restrict__ items, const CellRec *__restrict__ cel
                         rinfo_all + it.rinfo0, it.nfr, velo, it.aks2, it.skip_mode);
     else
         nodes_ray_fast<float>(ray, threadIdx.x, it.off, it.geo, cells,
-                              reinterpret_cast<FastRec *>(rec_all) + it.rec0,
+                              reinterpret_cast<FastRec *>(rec_all) + it.rec0, 0,
                               rinfo_all + it.rinfo0, it.nfr, velo, it.aks2, it.skip_mode,
                               it.lc);
 }
@@ -905,10 +905,13 @@ int formal_batch_blocks_per_cu(int fp64) {
 // adjoint
 // ---------------------------------------------------------------------
 // forward pass of the adjoint: the formal solution on scaled records, one
-// lane per (ray, channel), keeping I before every ADJ_K-th segment
+// lane per (ray, channel), keeping I before every kblk-th segment.  A call
+// covers rays ray0.. of the ray set (a chunk): off is the chunk's slice of
+// the node CSR (global node indices), rec, ckoff, ckpt and image are the
+// chunk's own, node0 its first node.
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_adj_fwd(int nrays, int nfr, const long long *__restrict__ off,
+k_adj_fwd(int nrays, int nfr, int ray0, long long node0, const long long *__restrict__ off,
           const long long *__restrict__ ckoff, const FastRecT<T> *__restrict__ rec,
           const double *__restrict__ dnu, const double *__restrict__ velo,
           const double *__restrict__ istar, LineConst lc, double *__restrict__ image,
@@ -916,11 +919,11 @@ k_adj_fwd(int nrays, int nfr, const long long *__restrict__ off,
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long long)nrays * nfr) return;
     const int ray = (int)(g / nfr), ch = (int)(g - (long long)ray * nfr);
-    const long long b = off[ray];
-    const int n = (int)(off[ray + 1] - b);
+    const long long b = off[ray] - node0;
+    const int n = (int)(off[ray + 1] - off[ray]);
     if (n < 2) { image[g] = lc.I0; return; }
     const FastLineT<T> L{(T)lc.nu0, (T)dnu[ch], (T)velo[ch]};
-    const T sf = ray == 0 ? (T)lc.starfract : T(0);
+    const T sf = ray0 + ray == 0 ? (T)lc.starfract : T(0);
     const T ist = (T)istar[ch];
     T I = (T)lc.I0;
     FastCarryT<T> c{T(0), T(0), T(0), 1};
@@ -1000,7 +1003,8 @@ __device__ inline void carried_in(const FastRecT<T> &a, const FastLineT<T> &L, T
 // summed over the wave's channels into gnode
 template <typename T>
 __global__ void __launch_bounds__(64)
-k_adj_rev(int nrays, int nfr, int nphi, const long long *__restrict__ off,
+k_adj_rev(int nrays, int nfr, int nphi, int ray0, long long node0,
+          const long long *__restrict__ off,
           const long long *__restrict__ ckoff, const FastRecT<T> *__restrict__ rec,
           const double *__restrict__ dnu, const double *__restrict__ velo,
           const double *__restrict__ istar, LineConst lc, const double *__restrict__ gflux,
@@ -1013,14 +1017,15 @@ k_adj_rev(int nrays, int nfr, int nphi, const long long *__restrict__ off,
     const int ch = blockIdx.y * 64 + lane;
     const bool active = ch < nfr;
     const int chc = active ? ch : nfr - 1;
-    const long long b = off[ray];
-    const int n = (int)(off[ray + 1] - b);
+    const long long b = off[ray] - node0;    // chunk-local node index
+    const int n = (int)(off[ray + 1] - off[ray]);
     if (n < 2) return;                       // uniform over the wave
     const FastLineT<T> L{(T)lc.nu0, (T)dnu[chc], (T)velo[chc]};
     // flux = (center_w I_0 + sum_rings ring_w mean_phi I) / d2 (telescope.F:1386-1418)
-    const double w = ray == 0 ? center_w : ring_w[(ray - 1) / nphi] / (1.0 * nphi);
+    const int rg = ray0 + ray;               // index in the ray set
+    const double w = rg == 0 ? center_w : ring_w[(rg - 1) / nphi] / (1.0 * nphi);
     T lam = active ? (T)(gflux[ch] * w / d2) : T(0);
-    const T sf = ray == 0 ? (T)lc.starfract : T(0);
+    const T sf = rg == 0 ? (T)lc.starfract : T(0);
     const T ist = (T)istar[chc];
     const T cai = (T)lc.c_aa;
     T gjl = T(0), gal = T(0);
@@ -1222,17 +1227,21 @@ int build_stencil_transpose(hipStream_t st, long long nnodes, const GeoNode *geo
     return 0;
 }
 
-// node records, the checkpointed forward and the flux of one item
+// node records and the checkpointed forward of one chunk of an item's rays
 template <typename T>
 void launch_adjoint_fwd(hipStream_t st, const AdjLaunch &a) {
     auto *rec = reinterpret_cast<FastRecT<T> *>(a.rec);
     auto *ck = reinterpret_cast<T *>(a.ckpt);
     hipLaunchKernelGGL(k_nodes_fast<T>, dim3(a.nrays), dim3(64), 0, st, a.nrays, a.off, a.geo,
-                       a.cells, rec, a.rinfo, a.nfr, a.velo, a.aks2, 1, a.lc);
+                       a.cells, rec, a.node0, a.rinfo, a.nfr, a.velo, a.aks2, 1, a.lc);
     const long long total = (long long)a.nrays * a.nfr;
     hipLaunchKernelGGL(k_adj_fwd<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
-                       a.nrays, a.nfr, a.off, a.ckoff, rec, a.dnu, a.velo, a.istar, a.lc,
-                       a.image, ck, a.kblk);
+                       a.nrays, a.nfr, a.ray0, a.node0, a.off, a.ckoff, rec, a.dnu, a.velo,
+                       a.istar, a.lc, a.image, ck, a.kblk);
+}
+
+// the spectrum of an item from its whole image (every chunk traced)
+void launch_adjoint_flux(hipStream_t st, const AdjLaunch &a) {
     dim3 gp((a.nfr + 63) / 64, a.nchunks);
     hipLaunchKernelGGL(k_flux_partial, gp, dim3(64), 0, st, a.nfr, a.nrings, a.nphi, 1, a.image,
                        a.ring_w, a.center_w, a.partial);
@@ -1247,10 +1256,11 @@ void launch_adjoint_rev(hipStream_t st, const AdjLaunch &a) {
     auto *rec = reinterpret_cast<const FastRecT<T> *>(a.rec);
     auto *ck = reinterpret_cast<const T *>(a.ckpt);
     hipLaunchKernelGGL(k_adj_rev<T>, dim3(a.nrays, (a.nfr + 63) / 64), dim3(64), 0, st,
-                       a.nrays, a.nfr, a.nphi, a.off, a.ckoff, rec, a.dnu, a.velo, a.istar,
-                       a.lc, a.gflux, a.ring_w, a.center_w, a.d2, ck, a.gnode, a.kblk);
+                       a.nrays, a.nfr, a.nphi, a.ray0, a.node0, a.off, a.ckoff, rec, a.dnu,
+                       a.velo, a.istar, a.lc, a.gflux, a.ring_w, a.center_w, a.d2, ck, a.gnode,
+                       a.kblk);
     hipLaunchKernelGGL(k_node_conv, dim3((unsigned)((a.nnodes + 255) / 256)), dim3(256), 0, st,
-                       a.nnodes, a.geo, a.cells, a.gnode, a.lc, a.gtrue);
+                       a.nnodes, a.geo + a.node0, a.cells, a.gnode, a.lc, a.gtrue);
     hipLaunchKernelGGL(k_cell_gather, dim3(a.ncell), dim3(64), 0, st, a.ncell, a.st_off,
                        a.st_vals, a.st_w, a.gtrue, a.nlev, a.nlines, a.line, a.lev_up,
                        a.lev_low, a.molpg, a.rho, a.abund, a.pops, a.g_rho, a.g_abund, a.g_lw,

@@ -77,10 +77,16 @@ class _Trace(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rho, abund, lw, pops, jd, ad, model):
         model._upload(rho, abund, lw, pops, jd, ad)
+        fl = None
         if any(ctx.needs_input_grad[:6]):
-            # keep the adjoint's tape (I checkpoints) on the device
-            fl = model.rt.trace_tape(model.items)
-        else:
+            # keep the adjoint's tape (I checkpoints) on the device, unless it
+            # does not fit the memory budget (the adjoint then recomputes)
+            try:
+                fl = model.rt.trace_tape(model.items)
+            except engine.RadliteError as e:
+                if e.code != 13:
+                    raise
+        if fl is None:
             fl, _, _ = model.rt.trace(model.items)
         model._calls += 1
         ctx.model = model

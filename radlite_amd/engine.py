@@ -75,7 +75,8 @@ def load_library(path=LIB_PATH):
                                              _dp, _dp, _dp, _dp, _dp, _dp]),
         "rl_trace_lines_forward_tape": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp]),
         "rl_tape_release": (C.c_int, [vp]),
-        "rl_last_adjoint_info": (C.c_int, [vp, _ip, _ip]),
+        "rl_last_adjoint_info": (C.c_int, [vp, _ip, _ip, _ip]),
+        "rl_memory_info": (C.c_int, [vp, _dp]),
         "rl_sync": (C.c_int, [vp]),
         "rl_set_memory_budget": (C.c_int, [vp, C.c_double]),
         "rl_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
@@ -326,8 +327,8 @@ class Raytracer:
 
     def adjoint_info(self):
         """(checkpoint interval of the last adjoint/tape call, 1 if the last
-        adjoint ran from the tape)."""
-        v = [C.c_int(0) for _ in range(2)]
+        adjoint ran from the tape, most ray chunks of an item)."""
+        v = [C.c_int(0) for _ in range(3)]
         self._chk(self.L.rl_last_adjoint_info(self.ctx, *[C.byref(x) for x in v]),
                   "rl_last_adjoint_info")
         return tuple(x.value for x in v)
@@ -357,6 +358,13 @@ class Raytracer:
             fl.append(flux[o:o + n])
             o += n
         return fl, out
+
+    def memory_info(self):
+        """dict: budget, resident trajectory bytes, work bytes of the last
+        call's largest unit, free and total device bytes."""
+        v = np.zeros(5)
+        self._chk(self.L.rl_memory_info(self.ctx, v.ctypes.data_as(_dp)), "rl_memory_info")
+        return dict(budget=v[0], traj=v[1], work=v[2], free=v[3], total=v[4])
 
     def set_memory_budget(self, nbytes):
         """Device bytes a batch of items may use (0: 60% of free memory)."""

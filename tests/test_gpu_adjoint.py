@@ -83,10 +83,10 @@ def test_tape_and_checkpoint_intervals(precision):
     rng = np.random.default_rng(7)
     gf = [rng.normal(size=nfr) * 1e20 for _ in items]
     fa, ga = rt.adjoint(items, gf)
-    assert rt.adjoint_info() == (1, 0)            # small case: every segment kept
+    assert rt.adjoint_info() == (1, 0, 1)         # small case: every segment kept, one chunk
     ft = rt.trace_tape(items)
     fb, gb = rt.adjoint(items, gf)
-    assert rt.adjoint_info() == (1, 1)
+    assert rt.adjoint_info() == (1, 1, 1)
     for a, b, c in zip(fa, fb, ft):
         assert np.array_equal(a, b) and np.array_equal(a, c)
     for k in ga:
@@ -100,7 +100,7 @@ def test_tape_and_checkpoint_intervals(precision):
     for K in (2, 4, 8, 16):
         rt.set_option(engine.OPT_CKPT, K)
         fk, gk = rt.adjoint(items, gf)
-        assert rt.adjoint_info() == (K, 0)
+        assert rt.adjoint_info() == (K, 0, 1)
         for a, b in zip(fa, fk):
             assert np.array_equal(a, b)
         worst = 0.0
@@ -113,16 +113,82 @@ def test_tape_and_checkpoint_intervals(precision):
         assert worst <= (1e-14 if precision == "fp64" else 1e-6), (K, worst)
         ft = rt.trace_tape(items)
         fk2, gk2 = rt.adjoint(items, gf)
-        assert rt.adjoint_info() == (K, 1)
+        assert rt.adjoint_info() == (K, 1, 1)
         for k in ga:
             assert np.array_equal(gk[k], gk2[k]), (K, k)
         rt.tape_release()
-    rt.set_option(engine.OPT_CKPT, 0)
-    rt.set_memory_budget(1)                       # nothing fits: the largest interval
-    rt.adjoint(items, gf)
-    assert rt.adjoint_info()[0] == 16
     with pytest.raises(engine.RadliteError):
         rt.set_option(engine.OPT_CKPT, 3)
+    rt.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_adjoint_ray_chunks(precision):
+    """An item whose buffers do not fit the memory budget is traced and
+    reversed a range of rays at a time (the largest checkpoint interval,
+    2..64 chunks, each with its own stencil transpose): the spectrum is
+    bit-identical and the cell gradients agree with the whole item at the
+    same interval to the rounding of the per-chunk cell sums; the tape
+    refuses an item that does not fit."""
+    ri = _case("t6")
+    say(f"[gpu] adjoint ray chunks t6 {precision}")
+    rt, nfr, vmax = _engine(ri, precision)
+    items = [(il, 0, nfr, vmax, engine.SKIP_NONE) for il in range(len(ri.A))]
+    rng = np.random.default_rng(8)
+    gf = [rng.normal(size=nfr) * 1e20 for _ in items]
+    rt.set_option(engine.OPT_CKPT, 16)
+    fa, ga = rt.adjoint(items, gf)
+    assert rt.adjoint_info() == (16, 0, 1)
+    mi = rt.memory_info()                         # the whole item's buffers
+    rt.set_option(engine.OPT_CKPT, 0)
+    # no room at all (the most chunks), and room for about a third of the item
+    for budget, lo, hi in ((1, 32, 64), (mi["traj"] + 0.3 * mi["work"], 2, 63)):
+        rt.set_memory_budget(budget)
+        fc, gc = rt.adjoint(items, gf)
+        K, tape, nch = rt.adjoint_info()
+        say(f"    budget {budget:.3g}: K={K}, {nch} chunks")
+        assert K == 16 and tape == 0 and lo <= nch <= hi, (K, tape, nch)
+        for a, b in zip(fa, fc):
+            assert np.array_equal(a, b)
+        for k in ga:
+            scale = np.abs(ga[k]).max()
+            d = np.abs(gc[k] - ga[k]).max()
+            assert d <= 1e-12 * scale, (k, d, scale)
+        with pytest.raises(engine.RadliteError) as e:
+            rt.trace_tape(items)
+        assert e.value.code == 13
+    rt.set_memory_budget(0)
+    rt.close()
+
+
+def test_forward_evicts_trajectories_to_fit():
+    """Two inclinations under a budget that holds one inclination's
+    trajectories and a batch: the second batch evicts the first
+    inclination's nodes (rebuilt when needed again); the spectra are
+    bit-identical to the unconstrained run."""
+    ri = _case("t6")
+    say("[gpu] trajectory eviction t6")
+    rt = engine.Raytracer(0, "fp32")
+    rt.load_run(ri)
+    nfr, vmax = engine.passband(ri.vmax, ri.dv)
+    rt.setup_rays([0.6, 1.1], ri.nphi, ri.b_extra, ri.dbdr)
+    items = [(0, 0, nfr, vmax, engine.SKIP_NONE), (0, 1, nfr, vmax, engine.SKIP_NONE),
+             (0, 0, nfr, vmax, engine.SKIP_REFERENCE)]
+    ref, _, _ = rt.trace(items)
+    both = rt.memory_info()["traj"]
+    work = []
+    for it in items[:2]:                          # one item with its inclination's nodes
+        rt.trace([it])
+        work.append(rt.memory_info()["work"])
+    rt.set_memory_budget(1.02 * max(work))
+    f, _, _ = rt.trace(items)
+    mi = rt.memory_info()
+    say(f"    both {both:.3g} B, budget {mi['budget']:.3g} B, resident {mi['traj']:.3g} B, "
+        f"batches {rt.batch_info()[0]}")
+    assert rt.batch_info()[0] == 3
+    assert mi["traj"] < both
+    for a, b in zip(ref, f):
+        assert np.array_equal(a, b)
     rt.close()
 
 
