@@ -104,6 +104,59 @@ def bench_adjoint(args):
     }
 
 
+def bench_large(args):
+    """The 1024x512 disk with 1000 ro-vibrational-like lines, 300 channels,
+    4 inclinations (529,409 rays each): the memory plan of the whole job
+    (4000 items, forward and adjoint) without running it, then a measured
+    slice: the forward of 2 lines at every inclination (8 items, batches,
+    trajectories built per inclination and evicted to fit), and the
+    adjoint of one item with gradients with respect to the populations
+    (checkpoint interval and ray chunks chosen to fit).  The whole job's
+    time is extrapolated from the per-item times."""
+    from radlite_amd import engine, synth
+    t0 = time.perf_counter()
+    ri, items, incl = synth.c5_inputs(0)
+    incs = [x * 0.0174532925199 for x in incl]
+    its = [(il, ii, n, v, engine.SKIP_NONE) for (il, ii, n, v) in items]
+    rt = engine.Raytracer(0, "fp32")
+    rt.load_run(ri)
+    rt.setup_rays(incs, ri.nphi, ri.b_extra, ri.dbdr)
+    t_setup = time.perf_counter() - t0
+    print(f"[large] inputs and tables: {t_setup:.1f} s", file=sys.stderr, flush=True)
+    plan = rt.plan(its)
+    print(f"[large] plan of {len(its)} items: {plan}", file=sys.stderr, flush=True)
+    sl = [it for it in its if it[0] < 2]
+    res = {"workload": f"c5: nr=1024 x nth=512, {ri.nphi} phi, {rt.nrays[0]} rays x "
+                       f"{its[0][2]} ch, {len(ri.A)} lines x {len(incl)} incl",
+           "precision": "fp32", "plan_full_job": plan, "setup_s": t_setup}
+    for rep in range(2):                          # the first pass tunes and builds
+        t1 = time.perf_counter()
+        rt.trace(sl)
+        dt = time.perf_counter() - t1
+        print(f"[large] forward slice pass {rep}: {dt:.2f} s, batches {rt.batch_info()}",
+              file=sys.stderr, flush=True)
+    nominal = sum(rt.nrays[it[1]] * it[2] for it in sl)
+    res.update(fwd_items=len(sl), fwd_s=dt, fwd_batches=rt.batch_info()[0],
+               fwd_value=nominal / dt, fwd_mem=rt.memory_info())
+    it1 = [its[0]]
+    g = [np.random.default_rng(1).normal(size=it1[0][2])]
+    for rep in range(2):
+        t1 = time.perf_counter()
+        _, gr = rt.adjoint(it1, g, outputs=("g_pops",))
+        dta = time.perf_counter() - t1
+        print(f"[large] adjoint item pass {rep}: {dta:.2f} s, (K, tape, chunks) "
+              f"{rt.adjoint_info()}", file=sys.stderr, flush=True)
+    K, _, nch = rt.adjoint_info()
+    assert np.isfinite(gr["g_pops"]).all()
+    res.update(adj_item_s=dta, adj_ckpt_interval=K, adj_chunks=nch, adj_mem=rt.memory_info(),
+               adj_value=rt.nrays[0] * it1[0][2] / dta,
+               extrapolated_fwd_h=len(its) * dt / len(sl) / 3600,
+               extrapolated_fwd_adj_h=len(its) * (dt / len(sl) + dta) / 3600,
+               unit="ray*channel/s")
+    rt.close()
+    return res
+
+
 def bench_batched(args):
     """The batched workload: 256x128 disk, 64 random lines with 20-400
     channels each, 8 inclinations (512 items, 17,025 rays each); one step
@@ -175,6 +228,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 64-line x 8-inclination batched measurement")
     ap.add_argument("--batched-only", action="store_true")
+    ap.add_argument("--large-only", action="store_true",
+                    help="only the 1024x512-disk, 1000-line plan and measured slice")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="run only a plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -192,6 +247,9 @@ def main():
         return
     if args.batched_only:
         print(json.dumps(bench_batched(args)))
+        return
+    if args.large_only:
+        print(json.dumps(bench_large(args)))
         return
     from radlite_amd import engine, synth
     ri, w = synth.workload_inputs(args.workload, seed=0)

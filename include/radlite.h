@@ -160,6 +160,16 @@ int rl_last_adjoint_info(const rl_ctx *ctx, int *ckpt_interval, int *used_tape,
  * use are evicted (and rebuilt when needed) to keep within it. */
 int rl_memory_info(rl_ctx *ctx, double *out);
 
+/* The plan of a call on these items, without running it (the node counts
+ * of their inclinations are computed, their nodes are not built):
+ * out[0] forward batches, out[1] bytes of the largest batch with the
+ * nodes of its inclinations, out[2] the adjoint's largest checkpoint
+ * interval, out[3] the most ray chunks of an adjoint item, out[4] the
+ * largest adjoint chunk's bytes with its image, out[5] the budget,
+ * out[6] 1 if the adjoint tape of each item alone fits, out[7] the most
+ * trajectory nodes of an inclination. */
+int rl_plan(rl_ctx *ctx, int nitems, const rl_item *items, double *out);
+
 /* ---- batching controls (no counterpart in the reference) ---------- */
 /* A call to rl_trace_lines_forward runs its items in batches: every
  * item of a batch is resident in device memory at once, and each stage

@@ -520,6 +520,27 @@ static size_t ctx_budget(rl_ctx *c) {
     return c->auto_budget;
 }
 
+// The forward's batch buffers and the adjoint's chunk buffers are kept
+// between calls for reuse; one kind is released when the other kind of
+// call starts and it holds more than a tenth of the budget.
+static void release_forward_work(rl_ctx *c) {
+    if ((c->d_rec.n + c->d_cmask.n) * 10 > ctx_budget(c)) {
+        (void)hipStreamSynchronize(c->stream);
+        c->d_rec = DevBuf();
+        c->d_cmask = DevBuf();
+    }
+}
+
+static void release_adjoint_work(rl_ctx *c) {
+    if ((c->d_arec.n + c->d_ckpt.n + c->d_gnode.n + c->d_gcell.n) * 10 > ctx_budget(c)) {
+        (void)hipStreamSynchronize(c->stream);
+        c->d_arec = DevBuf();
+        c->d_ckpt = DevBuf();
+        c->d_gnode = DevBuf();
+        c->d_gcell = DevBuf();
+    }
+}
+
 static void evict_traj(RaySet &rs) {
     rs.d_nodes = DevBuf();
     rs.d_st_vals = DevBuf();
@@ -590,8 +611,9 @@ static int count_traj(rl_ctx *c, int idx) {
 }
 
 // the scratch of k_cross is large for big ray sets: not kept past a build
+// when it holds more than a tenth of the budget
 static void release_cross_scratch(rl_ctx *c) {
-    if (c->d_cross.n + c->d_scr.n > ((size_t)1 << 30)) {
+    if ((c->d_cross.n + c->d_scr.n) * 10 > ctx_budget(c)) {
         (void)hipStreamSynchronize(c->stream);
         c->d_cross = DevBuf();
         c->d_scr = DevBuf();
@@ -800,6 +822,35 @@ static long long tiles_of(int cw, int nrays, int nfr) {
     if (cw == 0) return ((long long)nrays * nfr + 63) / 64;
     const int rows = 64 / cw;
     return (long long)((nrays + rows - 1) / rows) * ((nfr + cw - 1) / cw);
+}
+
+// batches: consecutive items whose buffers, with the nodes of their
+// inclinations, fit the budget (at least one item per batch; node counts
+// known).  Returns the largest batch's bytes.
+static size_t cut_batches(rl_ctx *c, int nitems, const rl_item *items,
+                          std::vector<std::pair<int, int>> &cuts) {
+    const size_t budget = ctx_budget(c);
+    size_t most = 0;
+    int i0 = 0;
+    while (i0 < nitems) {
+        size_t used = 0;
+        std::vector<int> incs;
+        int i1 = i0;
+        while (i1 < nitems) {
+            const int in = items[i1].incl;
+            size_t b = item_bytes(c, items[i1], c->incl[in]);
+            if (std::find(incs.begin(), incs.end(), in) == incs.end())
+                b += traj_bytes(c, c->incl[in]);
+            if (i1 > i0 && (used + b > budget || i1 - i0 >= 65535)) break;   // grid y/z limit
+            used += b;
+            if (std::find(incs.begin(), incs.end(), in) == incs.end()) incs.push_back(in);
+            i1++;
+        }
+        cuts.emplace_back(i0, i1);
+        most = std::max(most, used);
+        i0 = i1;
+    }
+    return most;
 }
 
 // One batch: items [i0, i1) of the call, all resident at once.
@@ -1068,38 +1119,26 @@ extern "C" int rl_trace_lines_forward_ex(rl_ctx *c, int nitems, const rl_item *i
             (it.skip_mode != RL_SKIP_REFERENCE && it.skip_mode != RL_SKIP_NONE))
             return fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii);
     }
-    // node counts of every inclination used (k_cross); the nodes are built
-    // per batch, and a ray set not used by a batch may be evicted to make
+    // node counts of every inclination used: an inclination whose nodes
+    // surely fit (at the per-ray maximum, within half the budget beside what
+    // is resident) is built at once, the others only counted (k_cross) and
+    // built per batch; a ray set not used by a batch may be evicted to make
     // room for it
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    for (int ii = 0; ii < nitems; ii++)
-        if (int e = count_traj(c, items[ii].incl)) return e;
-    const size_t budget = ctx_budget(c);
-    // batches: consecutive items whose buffers, with the nodes of their
-    // inclinations, fit the budget (at least one item per batch)
-    std::vector<std::pair<int, int>> cuts;
-    c->last_work = 0;
-    {
-        int i0 = 0;
-        while (i0 < nitems) {
-            size_t used = 0;
-            std::vector<int> incs;
-            int i1 = i0;
-            while (i1 < nitems) {
-                const int in = items[i1].incl;
-                size_t b = item_bytes(c, items[i1], c->incl[in]);
-                if (std::find(incs.begin(), incs.end(), in) == incs.end())
-                    b += traj_bytes(c, c->incl[in]);
-                if (i1 > i0 && (used + b > budget || i1 - i0 >= 65535)) break;   // grid y/z limit
-                used += b;
-                if (std::find(incs.begin(), incs.end(), in) == incs.end()) incs.push_back(in);
-                i1++;
-            }
-            cuts.emplace_back(i0, i1);
-            c->last_work = std::max(c->last_work, used);
-            i0 = i1;
+    for (int ii = 0; ii < nitems; ii++) {
+        const int in = items[ii].incl;
+        RaySet &rs = c->incl[in];
+        if (rs.counted) continue;
+        const size_t most = (size_t)rs.nrays * traj_nmax(c) * sizeof(GeoNode);
+        if ((resident_traj(c) + most) * 2 <= ctx_budget(c)) {
+            if (int e = build_traj_keep(c, in, std::vector<int>())) return e;
+        } else if (int e = count_traj(c, in)) {
+            return e;
         }
     }
+    std::vector<std::pair<int, int>> cuts;
+    c->last_work = cut_batches(c, nitems, items, cuts);
+    release_adjoint_work(c);
     HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev[5]));
     float tt = 0;
@@ -1219,21 +1258,23 @@ static size_t chunk_bytes(const rl_ctx *c, const RaySet &rs, int nfr, int K, int
 }
 
 // The plan of an item: the smallest interval K (or the fixed one) at which
-// the whole item fits the budget beside its trajectories and image, else
-// the largest interval with the rays cut into chunks of about equal node
-// counts that fit (at least 2, at most 64 chunks; at most 2^28 nodes
-// each, within the transpose's 32-bit entry index).
+// the whole item fits the budget beside its trajectories, their stencil
+// transpose and its image; else the largest interval (or the fixed one)
+// with the rays cut into chunks of about equal node counts that fit beside
+// the trajectories and the image (at least 2, at most 64 chunks; at most
+// 2^28 nodes each, within the transpose's 32-bit entry index).
 static int plan_adjoint(rl_ctx *c, const RaySet &rs, int nfr, bool tape, AdjPlan &p) {
     const int KMAX = rl::adjoint_block_len();
     const size_t budget = ctx_budget(c);
-    const size_t fixed = (size_t)rs.nnodes * sizeof(GeoNode) + (size_t)rs.nnodes * 4 * 12 +
-                         (size_t)rs.nrays * nfr * 8 + (size_t)rs.nrings * nfr * 8;
-    const size_t avail = budget > fixed ? budget - fixed : 0;
+    const size_t nn = (size_t)rs.nnodes;
+    const size_t base = nn * sizeof(GeoNode) + (size_t)(rs.nrays + 1) * 8 +
+                        (size_t)rs.nrays * nfr * 8 + (size_t)rs.nrings * nfr * 8;
+    const size_t transpose = nn * 4 * 12 + ((size_t)c->nr * c->nh + 1) * 4;
     p.chunks.clear();
-    const bool fits_index = 4 * (size_t)rs.nnodes < ((size_t)1 << 31);
+    const bool fits_index = 4 * nn < ((size_t)1 << 31);
     for (int K = c->fixed_kblk ? c->fixed_kblk : 1; K <= KMAX; K *= 2) {
         const size_t b = chunk_bytes(c, rs, nfr, K, 0, rs.nrays, false);
-        if (b <= avail && fits_index) {
+        if (base + transpose + b <= budget && fits_index) {
             p.K = K;
             p.chunks.push_back({0, rs.nrays, 0, rs.nnodes});
             p.work = b + (size_t)rs.nrays * nfr * 8;
@@ -1246,6 +1287,7 @@ static int plan_adjoint(rl_ctx *c, const RaySet &rs, int nfr, bool tape, AdjPlan
                            "budget (%.3g GB); rl_trace_lines_adjoint recomputes in ray chunks",
                     rs.nnodes, nfr, budget * 1e-9);
     p.K = c->fixed_kblk ? c->fixed_kblk : KMAX;
+    const size_t avail = budget > base ? budget - base : 0;
     const size_t whole = chunk_bytes(c, rs, nfr, p.K, 0, rs.nrays, true);
     long long nch = avail ? (long long)((whole + avail - 1) / avail) : 64;
     nch = std::max(nch, (long long)(rs.nnodes >> 28) + 1);      // <= 2^28 nodes a chunk
@@ -1319,12 +1361,15 @@ static void fill_adj(rl_ctx *c, const rl_item &it, const ItemSetup &su, AdjItemB
     a.g_ad = c->d_gad.as<double>();
 }
 
-// the item's trajectories resident (other inclinations evicted as the
-// budget requires), its line tables, and the image / flux buffers
-static int adj_setup(rl_ctx *c, const rl_item &it, int ii, ItemSetup &su) {
+// the item's trajectories resident (other inclinations evicted so that
+// they and work bytes of buffers fit the budget), its line tables, and the
+// image / flux buffers
+static int adj_setup(rl_ctx *c, const rl_item &it, int ii, ItemSetup &su, size_t work) {
     if (it.incl < 0 || it.incl >= (int)c->incl.size())
         return fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii);
-    if (int e = build_traj_keep(c, it.incl, std::vector<int>())) return e;
+    const std::vector<int> keep{it.incl};
+    if (int e = make_room(c, work, keep)) return e;
+    if (int e = build_traj_keep(c, it.incl, keep)) return e;
     if (int e = setup_item(c, it, ii, su, c->ev[0], c->ev[1])) return e;
     const RaySet &rs = *su.rs;
     HIPCHK(c, c->d_rinfo.ensure((size_t)rs.nrays * sizeof(RayInfo)));
@@ -1433,9 +1478,10 @@ extern "C" int rl_trace_lines_forward_tape(rl_ctx *c, int nitems, const rl_item 
     if (!c || nitems < 0 || (nitems && !items))
         return fail(c, RL_E_ARG, "rl_trace_lines_forward_tape: bad args");
     if (int e = adj_check(c, nitems, items, "rl_trace_lines_forward_tape")) return e;
+    release_forward_work(c);
+    release_adjoint_work(c);
     c->tape_items.clear();                   // invalid until the forward completes
-    c->tape.clear();                         // its memory is available to the plan
-    c->tape.resize(nitems);
+    c->tape.resize(nitems);                  // buffers of a previous tape are reused
     HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
     size_t fo = 0;
     for (int ii = 0; ii < nitems; ii++) fo += items[ii].nfr;
@@ -1473,7 +1519,7 @@ extern "C" int rl_trace_lines_forward_tape(rl_ctx *c, int nitems, const rl_item 
     for (int ii = 0; ii < nitems; ii++) {
         const rl_item &it = items[ii];
         ItemSetup su;
-        if (int e = adj_setup(c, it, ii, su)) return e;
+        if (int e = adj_setup(c, it, ii, su, 0)) return e;
         const AdjChunk whole{0, su.rs->nrays, 0, su.rs->nnodes};
         if (int e = adj_forward_chunk(c, it, su, Kt, whole, c->tape[ii])) return e;
         if (int e = adj_flux(c, it, su, c->tape_flux.data() + fo)) return e;
@@ -1500,6 +1546,7 @@ extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *item
     if (!c || nitems < 0 || (nitems && (!items || !dL_dflux)))
         return fail(c, RL_E_ARG, "rl_trace_lines_adjoint: bad args");
     if (int e = adj_check(c, nitems, items, "rl_trace_lines_adjoint")) return e;
+    release_forward_work(c);
     const int ncell = c->nr * c->nh;
     if (int e = zero_dev(c, c->d_grho, (size_t)ncell * 8)) return e;
     if (int e = zero_dev(c, c->d_gabund, (size_t)ncell * 8)) return e;
@@ -1520,7 +1567,7 @@ extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *item
         for (int ii = 0; ii < nitems; ii++) {
             const rl_item &it = items[ii];
             ItemSetup su;
-            if (int e = adj_setup(c, it, ii, su)) return e;
+            if (int e = adj_setup(c, it, ii, su, 0)) return e;
             if (int e = upload(c, c->d_gflux, dL_dflux + fo, (size_t)it.nfr * 8)) return e;
             const AdjChunk whole{0, su.rs->nrays, 0, su.rs->nnodes};
             if (int e = adj_reverse_chunk(c, it, su, c->tape_kblk, whole, c->tape[ii])) return e;
@@ -1544,7 +1591,16 @@ extern "C" int rl_trace_lines_adjoint(rl_ctx *c, int nitems, const rl_item *item
                       ? fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii)
                       : count_traj(c, it.incl);
             if (!err) err = plan_adjoint(c, c->incl[it.incl], it.nfr, false, p);
-            if (!err) err = adj_setup(c, it, ii, su);
+            if (!err && p.chunks.size() > 1 && c->incl[it.incl].st_ready) {
+                // the split item's chunks carry their own transposes
+                RaySet &rs = c->incl[it.incl];
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                rs.d_st_vals = DevBuf();
+                rs.d_st_w = DevBuf();
+                rs.d_st_off = DevBuf();
+                rs.st_ready = false;
+            }
+            if (!err) err = adj_setup(c, it, ii, su, p.work);
             if (!err) err = upload(c, c->d_gflux, dL_dflux + fo, (size_t)it.nfr * 8);
             for (size_t k = 0; k < p.chunks.size() && !err; k++) {
                 err = adj_forward_chunk(c, it, su, p.K, p.chunks[k], ab);
@@ -1592,6 +1648,46 @@ extern "C" int rl_last_adjoint_info(const rl_ctx *c, int *ckpt_interval, int *us
     if (ckpt_interval) *ckpt_interval = c->last_kblk;
     if (used_tape) *used_tape = c->last_used_tape;
     if (chunks) *chunks = c->last_chunks;
+    return 0;
+}
+
+extern "C" int rl_plan(rl_ctx *c, int nitems, const rl_item *items, double *out) {
+    if (!c || nitems < 0 || (nitems && !items) || !out) return RL_E_ARG;
+    if (c->incl.empty()) return fail(c, 13, "inputs incomplete: rays");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->d_err.ensure(4));
+    for (int ii = 0; ii < nitems; ii++) {
+        const rl_item &it = items[ii];
+        if (it.incl < 0 || it.incl >= (int)c->incl.size() || it.nfr < 2)
+            return fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii);
+        if (int e = count_traj(c, it.incl)) return e;
+    }
+    release_cross_scratch(c);
+    std::vector<std::pair<int, int>> cuts;
+    const size_t most = cut_batches(c, nitems, items, cuts);
+    int K = 0, nch = 0, tape_ok = 1;
+    size_t work = 0;
+    const std::string keep_err = c->err;
+    for (int ii = 0; ii < nitems; ii++) {
+        AdjPlan p;
+        if (int e = plan_adjoint(c, c->incl[items[ii].incl], items[ii].nfr, false, p)) return e;
+        K = std::max(K, p.K);
+        nch = std::max(nch, (int)p.chunks.size());
+        work = std::max(work, p.work);
+        AdjPlan q;
+        if (plan_adjoint(c, c->incl[items[ii].incl], items[ii].nfr, true, q)) tape_ok = 0;
+    }
+    c->err = keep_err;
+    long long nodes = 0;
+    for (const RaySet &r : c->incl) nodes = std::max(nodes, r.nnodes);
+    out[0] = (double)cuts.size();
+    out[1] = (double)most;
+    out[2] = (double)K;
+    out[3] = (double)nch;
+    out[4] = (double)work;
+    out[5] = (double)ctx_budget(c);
+    out[6] = (double)tape_ok;
+    out[7] = (double)nodes;
     return 0;
 }
 

@@ -77,6 +77,7 @@ def load_library(path=LIB_PATH):
         "rl_tape_release": (C.c_int, [vp]),
         "rl_last_adjoint_info": (C.c_int, [vp, _ip, _ip, _ip]),
         "rl_memory_info": (C.c_int, [vp, _dp]),
+        "rl_plan": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp]),
         "rl_sync": (C.c_int, [vp]),
         "rl_set_memory_budget": (C.c_int, [vp, C.c_double]),
         "rl_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
@@ -333,22 +334,24 @@ class Raytracer:
                   "rl_last_adjoint_info")
         return tuple(x.value for x in v)
 
-    def adjoint(self, items, dL_dflux):
+    def adjoint(self, items, dL_dflux, outputs=None):
         """Gradient of L with respect to the inputs the trace reads, given
         dL/dflux per item (list of arrays, same layout as trace's flux).
         Every item must use SKIP_NONE.  Returns (list of flux arrays,
         dict of gradients g_rho, g_abund, g_lw [nr, nth], g_pops
-        [nr, nth, nlev], g_jd, g_ad [nr, nth, nlines])."""
+        [nr, nth, nlev], g_jd, g_ad [nr, nth, nlines]); ``outputs`` (names)
+        limits the dict to those gradients."""
         arr = (RLItem * len(items))(*[RLItem(*it) for it in items])
         nf = [it[2] for it in items]
         g = np.ascontiguousarray(np.concatenate([np.asarray(x, float) for x in dL_dflux]))
         assert g.size == sum(nf)
         flux = np.zeros(sum(nf))
         nr, nth = self.nr, self.nth
-        out = dict(g_rho=np.zeros((nr, nth)), g_abund=np.zeros((nr, nth)),
-                   g_lw=np.zeros((nr, nth)), g_pops=np.zeros((nr, nth, self.nlev)),
-                   g_jd=np.zeros((nr, nth, self.nlines)), g_ad=np.zeros((nr, nth, self.nlines)))
-        p = {k: v.ctypes.data_as(_dp) for k, v in out.items()}
+        shapes = dict(g_rho=(nr, nth), g_abund=(nr, nth), g_lw=(nr, nth),
+                      g_pops=(nr, nth, self.nlev), g_jd=(nr, nth, self.nlines),
+                      g_ad=(nr, nth, self.nlines))
+        out = {k: np.zeros(v) for k, v in shapes.items() if outputs is None or k in outputs}
+        p = {k: (out[k].ctypes.data_as(_dp) if k in out else None) for k in shapes}
         rc = self.L.rl_trace_lines_adjoint(self.ctx, len(items), arr, g.ctypes.data_as(_dp),
                                            flux.ctypes.data_as(_dp), p["g_rho"], p["g_abund"],
                                            p["g_lw"], p["g_pops"], p["g_jd"], p["g_ad"])
@@ -365,6 +368,18 @@ class Raytracer:
         v = np.zeros(5)
         self._chk(self.L.rl_memory_info(self.ctx, v.ctypes.data_as(_dp)), "rl_memory_info")
         return dict(budget=v[0], traj=v[1], work=v[2], free=v[3], total=v[4])
+
+    def plan(self, items):
+        """The memory plan of a call on ``items`` without running it: forward
+        batches and the largest batch's bytes, the adjoint's checkpoint
+        interval, chunks per item and chunk bytes, the budget, whether
+        each item's tape fits, the most nodes of an inclination."""
+        arr = (RLItem * len(items))(*[RLItem(*it) for it in items])
+        v = np.zeros(8)
+        self._chk(self.L.rl_plan(self.ctx, len(items), arr, v.ctypes.data_as(_dp)), "rl_plan")
+        return dict(batches=int(v[0]), batch_bytes=v[1], ckpt_interval=int(v[2]),
+                    adjoint_chunks=int(v[3]), chunk_bytes=v[4], budget=v[5],
+                    tape_fits=bool(v[6]), max_nodes=int(v[7]))
 
     def set_memory_budget(self, nbytes):
         """Device bytes a batch of items may use (0: 60% of free memory)."""

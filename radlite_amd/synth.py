@@ -116,6 +116,48 @@ def random_lines(nlines, seed, lam_range=(4.5, 5.0), logA=(0.0, 2.0),
     return LineSet(E_cm=Eq, g=gq, lev_up=np.array(up), lev_low=np.array(lo), A=Aq)
 
 
+def rovib_lines(nlines, seed, nv=6, jmax=60, e_vib_k=1400.0, b_rot_k=2.0, Eup=(100.0, 8000.0),
+                logA=(0.0, 2.0)):
+    """A ro-vibrational-like ladder: levels (v, J) for v < nv, J <= jmax with
+    E = v e_vib + b J(J+1) [K] and g = 2J+1; lines (v', J') -> (v'', J'')
+    with v'' < v', |J' - J''| = 1 and E_up in ``Eup``, ``nlines`` of them
+    drawn without replacement, A log-uniform in 10**logA."""
+    rng = np.random.default_rng(seed)
+    nl = jmax + 1
+    EK = np.array([v * e_vib_k + b_rot_k * J * (J + 1) for v in range(nv) for J in range(nl)])
+    g = np.array([2.0 * J + 1.0 for v in range(nv) for J in range(nl)])
+    cand = []
+    for vu in range(1, nv):
+        for vl in range(vu):
+            for Ju in range(nl):
+                for Jl in (Ju - 1, Ju + 1):
+                    if 0 <= Jl < nl:
+                        iu, il = vu * nl + Ju, vl * nl + Jl
+                        if Eup[0] <= EK[iu] <= Eup[1]:
+                            cand.append((iu, il))
+    if len(cand) < nlines:
+        raise ValueError(f"{len(cand)} candidate lines < {nlines}")
+    pick = rng.choice(len(cand), size=nlines, replace=False)
+    up = np.array([cand[k][0] + 1 for k in pick])
+    lo = np.array([cand[k][1] + 1 for k in pick])
+    A = 10 ** rng.uniform(*logA, size=nlines)
+    Eq, gq, Aq = l4io.quantize_moldata(list(EK / K_PER_CM), list(g), list(A))
+    return LineSet(E_cm=Eq, g=gq, lev_up=up, lev_low=lo, A=Aq)
+
+
+def nonlte_pops(disk: Disk, lines: LineSet, sigma=0.2, seed=0):
+    """Boltzmann populations times lognormal(sigma) per cell and level,
+    renormalised to sum 1 per cell; [nr, nth, nlev]."""
+    rng = np.random.default_rng(seed)
+    T = disk.t_gas[..., None]
+    EK = lines.E_cm[None, None, :] * K_PER_CM
+    n = lines.g[None, None, :] * np.exp(-EK / T)
+    n *= np.exp(rng.normal(0.0, sigma, size=n.shape))
+    n /= n.sum(axis=-1, keepdims=True)
+    n[n < 1e-99] = 0.0
+    return n
+
+
 def lte_pops(disk: Disk, lines: LineSet, q_mode="t_over_2.77"):
     """n_i = g_i exp(-E_i/kT)/Q(T); shape [nr, nth, nlev]."""
     T = disk.t_gas[..., None]
@@ -240,4 +282,23 @@ def c3_inputs(seed=0, nlines=64, incl=(10.0, 20.0, 30.0, 40.0, 50.0, 60.0, 70.0,
             v, dv = passband(vmax, int(nfrs[il]))
             n = int(2.0 * v / dv + 1)
             items.append((il, ii, n, 0.5 * (n - 1) * dv))
+    return ri, items, list(incl)
+
+
+def c5_inputs(seed=0, nlines=1000, incl=(20.0, 40.0, 60.0, 80.0), nr=1024, nth=512, nphi=512,
+              nfr=300, vmax=30.0):
+    """The large workload: nr x nth disk, ``nlines`` ro-vibrational-like
+    lines (J = 0-60 ladders, A = 1-100 s^-1, E_up = 100-8000 K), non-LTE
+    populations (Boltzmann x lognormal(0.2), renormalised), ``nfr``
+    channels per line over +-vmax, nphi azimuths, the given inclinations.
+    Returns (RunInputs, items) with items (line, incl index, nfr, vmax) for
+    every line at every inclination, inclination-major."""
+    disk = make_disk(nr, nth, seed=seed, sigma_t=0.1)
+    lines = rovib_lines(nlines, seed=seed + 1)
+    pops = nonlte_pops(disk, lines, seed=seed + 2)
+    v, dv = passband(vmax, nfr)
+    ri = run_inputs(disk, lines, pops, nphi=nphi, dbdr=1, b_extra=-1, vmax=v, dv=dv,
+                    incl_deg=incl[0])
+    n = int(2.0 * v / dv + 1)
+    items = [(il, ii, n, 0.5 * (n - 1) * dv) for ii in range(len(incl)) for il in range(nlines)]
     return ri, items, list(incl)

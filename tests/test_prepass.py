@@ -51,3 +51,33 @@ def test_prepass_derivatives():
     assert torch.autograd.gradcheck(
         lambda t: diff.dust_line(t, rd, np.array([100.0, 90.0]), np.zeros(2), nu0)[0],
         (T,), eps=1e-4, rtol=1e-5, atol=0)
+
+
+def test_large_workload_inputs_small_instance():
+    """The 1000-line workload's generator on a small grid: ro-vibrational
+    ladder lines (J' - J'' = +-1, v' > v'', E_up in 100-8000 K, A in
+    1-100 s^-1, g = 2J+1), non-LTE populations that sum to 1 per cell and
+    deviate from Boltzmann by the lognormal factor, items inclination-major."""
+    ri, items, incl = synth.c5_inputs(0, nlines=200, nr=32, nth=16, nphi=8, nfr=30)
+    assert len(items) == 200 * 4 and items[0][:2] == (0, 0) and items[200][:2] == (0, 1)
+    assert all(it[2] == 30 for it in items)
+    nl = 61
+    up, lo = ri.lev_up - 1, ri.lev_low - 1
+    vu, ju, vl, jl = up // nl, up % nl, lo // nl, lo % nl
+    assert np.all(vu > vl) and np.all(np.abs(ju - jl) == 1)
+    EK = ri.E_cm * synth.K_PER_CM
+    assert np.all((EK[up] >= 100.0 - 1.0) & (EK[up] <= 8000.0 + 1.0))
+    assert np.all((ri.A >= 1.0 * (1 - 1e-3)) & (ri.A <= 100.0 * (1 + 1e-3)))
+    assert np.allclose(ri.g, 2.0 * (np.arange(len(ri.g)) % nl) + 1.0)
+    assert len(set(zip(up, lo))) == 200                    # drawn without replacement
+    p = ri.pops
+    assert p.shape == (32, 16, 6 * nl) and np.allclose(p.sum(-1), 1.0, rtol=0, atol=1e-12)
+    # the lognormal factor: log(p / Boltzmann) minus its per-cell offset has
+    # the stated spread on the levels Boltzmann does not push to zero
+    T = synth.make_disk(32, 16, seed=0, sigma_t=0.1).t_gas[..., None]
+    b = ri.g[None, None, :] * np.exp(-EK[None, None, :] / T)
+    ok = (b > 1e-30 * b.max(-1, keepdims=True)) & (p > 0)
+    r = np.where(ok, np.log(np.where(ok, p, 1.0) / np.where(ok, b, 1.0)), 0.0)
+    r = r - (r.sum(-1, keepdims=True) / ok.sum(-1, keepdims=True))
+    sd = np.sqrt((r[ok] ** 2).mean())
+    assert 0.18 < sd < 0.22, sd
