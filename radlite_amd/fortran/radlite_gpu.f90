@@ -103,6 +103,29 @@ module radlite_gpu
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx
         end function
+        integer(c_int) function rl_set_memory_budget(ctx, bytes) &
+                bind(C, name="rl_set_memory_budget")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: bytes
+        end function
+        integer(c_int) function rl_set_option(ctx, option, val) bind(C, name="rl_set_option")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: option, val
+        end function
+        integer(c_int) function rl_plan(ctx, nitems, items, out) bind(C, name="rl_plan")
+            import :: c_ptr, c_int, c_double, rl_item
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nitems
+            type(rl_item), intent(in) :: items(*)
+            real(c_double), intent(out) :: out(8)
+        end function
+        integer(c_int) function rl_memory_info(ctx, out) bind(C, name="rl_memory_info")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: out(5)
+        end function
         ! host pre-pass with the reference's arithmetic (include/radlite.h)
         integer(c_int) function rl_prepare_line_freqs(nl, e_cm, lev_up, lev_low, nu0) &
                 bind(C, name="rl_prepare_line_freqs")
