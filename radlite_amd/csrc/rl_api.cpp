@@ -106,7 +106,12 @@ struct rl_ctx {
     std::vector<RaySet> incl;
     // per-item work buffers
     DevBuf d_cells, d_rec, d_rinfo, d_image, d_chan, d_partial, d_flux, d_err, d_ovs,
-        d_maser, d_scr, d_cross;
+        d_maser;
+    // crossing-pass scratch, one slot per ray set of a launch: its
+    // per-ray cone roots and its crossing slots; x_owner = the ray set
+    // whose crossings a slot holds (-1: none)
+    std::vector<DevBuf> x_scr, x_cross;
+    std::vector<int> x_owner;
     int cells_line = -1;
     std::vector<double> h_chan, h_vel;
     // batching
@@ -116,7 +121,6 @@ struct rl_ctx {
     size_t mem_budget = 0;            // 0: a share of the free device memory
     size_t auto_budget = 0;
     unsigned long long tick = 0;      // use counter of the ray sets
-    int cross_owner = -1;             // ray set whose crossings d_cross holds
     size_t last_work = 0;             // work bytes of the last call's largest unit
     int last_chunks = 0;
     bool use_graph = true, autotune = true;
@@ -260,7 +264,7 @@ extern "C" int rl_set_grid(rl_ctx *c, int nr, const double *r, int nh, const dou
     if (int e = upload(c, c->d_tan2, t2.data(), t2.size() * 8)) return e;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->incl.clear();
-    c->cross_owner = -1;
+    c->x_owner.assign(c->x_owner.size(), -1);
     c->cells_line = -1;
     return 0;
 }
@@ -351,7 +355,7 @@ extern "C" int rl_set_star(rl_ctx *c, double rstar, int ncont, const double *nu,
     c->cont_nu.assign(nu, nu + ncont);
     c->istar.assign(istar, istar + ncont);
     for (auto &s : c->incl) s.traj_ready = s.counted = false;   // R* bounds the crossings
-    c->cross_owner = -1;
+    c->x_owner.assign(c->x_owner.size(), -1);
     return 0;
 }
 
@@ -459,7 +463,7 @@ extern "C" int rl_setup_rays_circular(rl_ctx *c, int nincl, const double *incl, 
         return fail(c, RL_E_ARG, "rl_setup_rays_circular: set the grid first");
     HIPCHK(c, hipSetDevice(c->device));
     c->incl.clear();
-    c->cross_owner = -1;
+    c->x_owner.assign(c->x_owner.size(), -1);
     c->incl.resize(nincl);
     for (int i = 0; i < nincl; i++) {
         if (int e = build_rayset(c, c->incl[i], incl[i], nphi, nrext, dbdr)) return e;
@@ -570,38 +574,65 @@ static int make_room(rl_ctx *c, size_t need, const std::vector<int> &keep) {
 
 // ray crossings of a ray set (k_cross) into the context's scratch, and
 // the node counts: off / nnodes
-static int cross_traj(rl_ctx *c, int idx) {
-    RaySet &rs = c->incl[idx];
+// ray crossings (k_cross) of up to CROSS_SETS_MAX ray sets in one launch,
+// each into its own scratch slot, and their node counts: off / nnodes
+static int cross_sets(rl_ctx *c, const std::vector<int> &idxs) {
+    const int ns = (int)idxs.size();
+    if (ns < 1 || ns > rl::CROSS_SETS_MAX) return fail(c, RL_E_ARG, "crossing: %d ray sets", ns);
     rl::GridDev g{c->nr, 2 * c->nh, c->d_rc.as<double>(), c->d_tc.as<double>(), c->rstar};
-    const int n = rs.nrays;
-    const int iyeq = c->nh;
-    HIPCHK(c, c->d_scr.ensure((size_t)(iyeq + 1) * n * 20));
-    double *s1 = c->d_scr.as<double>();
-    double *s2 = s1 + (size_t)(iyeq + 1) * n;
-    int *iy = reinterpret_cast<int *>(s2 + (size_t)(iyeq + 1) * n);
-    DevBuf cnt;
-    HIPCHK(c, cnt.ensure((size_t)n * 4));
+    const int iyeq = c->nh, nmax = traj_nmax(c);
+    if ((int)c->x_scr.size() < ns) {
+        c->x_scr.resize(ns);
+        c->x_cross.resize(ns);
+        c->x_owner.resize(ns, -1);
+    }
+    rl::CrossSets sets{};
+    sets.n = ns;
+    std::vector<DevBuf> cnt(ns);
+    for (int k = 0; k < ns; k++) {
+        RaySet &rs = c->incl[idxs[k]];
+        const int n = rs.nrays;
+        c->x_owner[k] = -1;
+        HIPCHK(c, c->x_scr[k].ensure((size_t)(iyeq + 1) * n * 20));
+        HIPCHK(c, c->x_cross[k].ensure((size_t)n * nmax * 12));
+        HIPCHK(c, cnt[k].ensure((size_t)n * 4));
+        rl::CrossSet &q = sets.s[k];
+        q.nrays = n;
+        q.central_blend = rs.ring_ri[1] > 0.0 ? 1 : 0;
+        q.x0 = rs.d_x0.as<double>();
+        q.z0 = rs.d_z0.as<double>();
+        q.theta0 = rs.theta0; q.costh0 = rs.costh0; q.sinth0 = rs.sinth0;
+        q.s1 = c->x_scr[k].as<double>();
+        q.s2 = q.s1 + (size_t)(iyeq + 1) * n;
+        q.iy = reinterpret_cast<int *>(q.s2 + (size_t)(iyeq + 1) * n);
+        q.count = cnt[k].as<int>();
+        q.cs = c->x_cross[k].as<double>();
+        q.ccode = reinterpret_cast<int *>(q.cs + (size_t)n * nmax);
+    }
     HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));
-    const int blend = rs.ring_ri[1] > 0.0 ? 1 : 0;
-    const int nmax = traj_nmax(c);
-    HIPCHK(c, c->d_cross.ensure((size_t)n * nmax * 12));
-    double *cs = c->d_cross.as<double>();
-    int *ccode = reinterpret_cast<int *>(cs + (size_t)n * nmax);
-    rl::launch_cross(c->stream, g, c->d_tan2.as<double>(), n, rs.d_x0.as<double>(),
-                     rs.d_z0.as<double>(), rs.theta0, rs.costh0, rs.sinth0, blend, s1, s2, iy,
-                     nmax, cnt.as<int>(), cs, ccode, c->d_err.as<int>());
+    rl::launch_cross(c->stream, g, c->d_tan2.as<double>(), sets, nmax, c->d_err.as<int>());
     HIPCHK(c, hipGetLastError());
-    std::vector<int> h(n);
-    HIPCHK(c, hipMemcpyAsync(h.data(), cnt.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<std::vector<int>> h(ns);
+    for (int k = 0; k < ns; k++) {
+        h[k].resize(sets.s[k].nrays);
+        HIPCHK(c, hipMemcpyAsync(h[k].data(), cnt[k].p, (size_t)sets.s[k].nrays * 4,
+                                 hipMemcpyDeviceToHost, c->stream));
+    }
     if (int e = check_err(c)) return e;
-    rs.off.assign(n + 1, 0);
-    for (int i = 0; i < n; i++) rs.off[i + 1] = rs.off[i] + h[i];
-    rs.nnodes = rs.off[n];
-    if (int e = upload(c, rs.d_off, rs.off.data(), (size_t)(n + 1) * 8)) return e;
-    rs.counted = true;
-    c->cross_owner = idx;
+    for (int k = 0; k < ns; k++) {
+        RaySet &rs = c->incl[idxs[k]];
+        const int n = rs.nrays;
+        rs.off.assign(n + 1, 0);
+        for (int i = 0; i < n; i++) rs.off[i + 1] = rs.off[i] + h[k][i];
+        rs.nnodes = rs.off[n];
+        if (int e = upload(c, rs.d_off, rs.off.data(), (size_t)(n + 1) * 8)) return e;
+        rs.counted = true;
+        c->x_owner[k] = idxs[k];
+    }
     return 0;
 }
+
+static int cross_traj(rl_ctx *c, int idx) { return cross_sets(c, std::vector<int>{idx}); }
 
 // node counts of a ray set (without building its nodes)
 static int count_traj(rl_ctx *c, int idx) {
@@ -613,11 +644,13 @@ static int count_traj(rl_ctx *c, int idx) {
 // the scratch of k_cross is large for big ray sets: not kept past a build
 // when it holds more than a tenth of the budget
 static void release_cross_scratch(rl_ctx *c) {
-    if ((c->d_cross.n + c->d_scr.n) * 10 > ctx_budget(c)) {
+    size_t b = 0;
+    for (size_t k = 0; k < c->x_scr.size(); k++) b += c->x_scr[k].n + c->x_cross[k].n;
+    if (b * 10 > ctx_budget(c)) {
         (void)hipStreamSynchronize(c->stream);
-        c->d_cross = DevBuf();
-        c->d_scr = DevBuf();
-        c->cross_owner = -1;
+        c->x_scr.clear();
+        c->x_cross.clear();
+        c->x_owner.clear();
     }
 }
 
@@ -632,11 +665,14 @@ static int build_traj_keep(rl_ctx *c, int idx, const std::vector<int> &keep) {
     k2.push_back(idx);
     const size_t want = traj_bytes(c, rs), have = held_bytes(rs);
     if (int e = make_room(c, want > have ? want - have : 0, k2)) return e;
-    if (c->cross_owner != idx)
+    int slot = (int)(std::find(c->x_owner.begin(), c->x_owner.end(), idx) - c->x_owner.begin());
+    if (slot == (int)c->x_owner.size()) {
         if (int e = cross_traj(c, idx)) return e;
+        slot = 0;
+    }
     rl::GridDev g{c->nr, 2 * c->nh, c->d_rc.as<double>(), c->d_tc.as<double>(), c->rstar};
     const int nmax = traj_nmax(c);
-    double *cs = c->d_cross.as<double>();
+    double *cs = c->x_cross[slot].as<double>();
     int *ccode = reinterpret_cast<int *>(cs + (size_t)rs.nrays * nmax);
     HIPCHK(c, rs.d_nodes.ensure((size_t)rs.nnodes * sizeof(GeoNode)));
     rl::launch_geo(c->stream, g, c->d_vel.as<double>(), c->nh, rs.nrays, nmax,
@@ -647,7 +683,6 @@ static int build_traj_keep(rl_ctx *c, int idx, const std::vector<int> &keep) {
     if (int e = check_err(c)) return e;
     rs.traj_ready = true;
     rs.st_ready = false;
-    release_cross_scratch(c);
     return 0;
 }
 
@@ -1125,16 +1160,31 @@ extern "C" int rl_trace_lines_forward_ex(rl_ctx *c, int nitems, const rl_item *i
     // built per batch; a ray set not used by a batch may be evicted to make
     // room for it
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    for (int ii = 0; ii < nitems; ii++) {
-        const int in = items[ii].incl;
-        RaySet &rs = c->incl[in];
-        if (rs.counted) continue;
-        const size_t most = (size_t)rs.nrays * traj_nmax(c) * sizeof(GeoNode);
-        if ((resident_traj(c) + most) * 2 <= ctx_budget(c)) {
-            if (int e = build_traj_keep(c, in, std::vector<int>())) return e;
-        } else if (int e = count_traj(c, in)) {
-            return e;
+    {
+        std::vector<int> now, later;
+        size_t room = resident_traj(c);
+        for (int ii = 0; ii < nitems; ii++) {
+            const int in = items[ii].incl;
+            const RaySet &rs = c->incl[in];
+            if (rs.counted || std::find(now.begin(), now.end(), in) != now.end() ||
+                std::find(later.begin(), later.end(), in) != later.end())
+                continue;
+            const size_t most = (size_t)rs.nrays * traj_nmax(c) * (sizeof(GeoNode) + 12);
+            if ((room + most) * 2 <= ctx_budget(c) && (int)now.size() < rl::CROSS_SETS_MAX) {
+                now.push_back(in);
+                room += most;
+            } else {
+                later.push_back(in);
+            }
         }
+        if (!now.empty()) {
+            if (int e = cross_sets(c, now)) return e;
+            for (int in : now)
+                if (int e = build_traj_keep(c, in, now)) return e;
+        }
+        for (int in : later)
+            if (int e = count_traj(c, in)) return e;
+        release_cross_scratch(c);
     }
     std::vector<std::pair<int, int>> cuts;
     c->last_work = cut_batches(c, nitems, items, cuts);
@@ -1166,6 +1216,7 @@ extern "C" int rl_trace_lines_forward_ex(rl_ctx *c, int nitems, const rl_item *i
         if (int e = make_room(c, ibytes, incs)) return e;
         for (int in : incs)
             if (int e = build_traj_keep(c, in, incs)) return e;
+        release_cross_scratch(c);
         HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
         HIPCHK(c, hipEventSynchronize(c->ev[5]));
         (void)hipEventElapsedTime(&tt, c->ev[0], c->ev[5]);
@@ -1370,6 +1421,7 @@ static int adj_setup(rl_ctx *c, const rl_item &it, int ii, ItemSetup &su, size_t
     const std::vector<int> keep{it.incl};
     if (int e = make_room(c, work, keep)) return e;
     if (int e = build_traj_keep(c, it.incl, keep)) return e;
+    release_cross_scratch(c);
     if (int e = setup_item(c, it, ii, su, c->ev[0], c->ev[1])) return e;
     const RaySet &rs = *su.rs;
     HIPCHK(c, c->d_rinfo.ensure((size_t)rs.nrays * sizeof(RayInfo)));

@@ -139,10 +139,26 @@ struct AdjLaunch {
     double *g_rho, *g_abund, *g_lw, *g_pops, *g_jd, *g_ad;   // accumulated over items
 };
 
-void launch_cross(hipStream_t st, const GridDev &g, const double *tan2, int nrays,
-                  const double *x0, const double *z0, double theta0, double costh0,
-                  double sinth0, int central_blend, double *s1, double *s2, int *iy, int nmax,
-                  int *count, double *cs, int *ccode, int *err);
+// One ray set's crossing pass: rays, direction, and where its crossings go
+// (per-ray slots of nmax entries) and its scratch.
+struct CrossSet {
+    int nrays, central_blend;
+    const double *x0, *z0;
+    double theta0, costh0, sinth0;
+    double *s1, *s2;
+    int *iy;
+    int *count;
+    double *cs;
+    int *ccode;
+};
+constexpr int CROSS_SETS_MAX = 8;
+struct CrossSets {
+    CrossSet s[CROSS_SETS_MAX];
+    int n;
+};
+
+void launch_cross(hipStream_t st, const GridDev &g, const double *tan2, const CrossSets &sets,
+                  int nmax, int *err);
 void launch_geo(hipStream_t st, const GridDev &g, const double *vel, int nh, int nrays,
                 int nmax, const long long *off, const double *x0, const double *z0,
                 double costh0, double sinth0, const double *cs, const int *ccode,

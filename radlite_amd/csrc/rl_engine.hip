@@ -44,12 +44,22 @@ struct CellView {
 // the reference's order, with the hole-chord and central-beam flags, into
 // a fixed-size slot per ray (nmax bounds the node count).
 // code = icross | rext << 2 | NF_* << 3 | idx << 5
+// Several ray sets (inclinations) go in one launch, blockIdx.y choosing the
+// set: a set has one thread per ray, a long serial chain of FP64 work, so
+// sets launched together keep more waves resident to hide its latency.
 __global__ void __launch_bounds__(128)
-k_cross(GridView g, const double *__restrict__ tan2, int nrays,
-        const double *__restrict__ x0, const double *__restrict__ z0, double theta0,
-        double costh0, double sinth0, int central_blend, double *scr_s1, double *scr_s2,
-        int *scr_iy, int nmax, int *__restrict__ count, double *__restrict__ cs,
-        int *__restrict__ ccode, int *err) {
+k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, int *err) {
+    const CrossSet &q = sets.s[blockIdx.y];
+    const int nrays = q.nrays;
+    const double *__restrict__ x0 = q.x0;
+    const double *__restrict__ z0 = q.z0;
+    const double theta0 = q.theta0, costh0 = q.costh0, sinth0 = q.sinth0;
+    const int central_blend = q.central_blend;
+    double *scr_s1 = q.s1, *scr_s2 = q.s2;
+    int *scr_iy = q.iy;
+    int *__restrict__ count = q.count;
+    double *__restrict__ cs = q.cs;
+    int *__restrict__ ccode = q.ccode;
     int ray = blockIdx.x * blockDim.x + threadIdx.x;
     if (ray >= nrays) return;
     Traj t(g, tan2, x0[ray], z0[ray], theta0, costh0, sinth0, scr_s1 + ray, scr_s2 + ray,
@@ -563,14 +573,13 @@ __global__ void k_flux_final(int nfr, int nchunks, const double *__restrict__ pa
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
-void launch_cross(hipStream_t st, const GridDev &gd, const double *tan2, int nrays,
-                  const double *x0, const double *z0, double theta0, double costh0,
-                  double sinth0, int central_blend, double *s1, double *s2, int *iy, int nmax,
-                  int *count, double *cs, int *ccode, int *err) {
+void launch_cross(hipStream_t st, const GridDev &gd, const double *tan2, const CrossSets &sets,
+                  int nmax, int *err) {
     GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar};
-    dim3 blk(128), grd((nrays + 127) / 128);
-    hipLaunchKernelGGL(k_cross, grd, blk, 0, st, g, tan2, nrays, x0, z0, theta0, costh0,
-                       sinth0, central_blend, s1, s2, iy, nmax, count, cs, ccode, err);
+    int most = 0;
+    for (int i = 0; i < sets.n; i++) most = max(most, sets.s[i].nrays);
+    dim3 blk(128), grd((most + 127) / 128, sets.n);
+    hipLaunchKernelGGL(k_cross, grd, blk, 0, st, g, tan2, sets, nmax, err);
 }
 
 void launch_geo(hipStream_t st, const GridDev &gd, const double *vel, int nh, int nrays,
