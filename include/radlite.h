@@ -212,6 +212,47 @@ int rl_sync(rl_ctx *ctx);
  * c[2] integrated ray*channels, c[3] nominal ray*channels. */
 int rl_last_stats(const rl_ctx *ctx, double *t_ms, long long *counts);
 
+/* ---- device pre-pass (pyradlite's per-cell tables) ---------------- */
+/* The tables the trace reads, computed on the device from per-cell
+ * physical fields, as pyradlite's pre-pass forms them before a run
+ * (_prep_mol_forcore radlite.py:1024-1127, _calc_turbulence
+ * radlite.py:1592-1631, _calc_abundance radlite.py:1481-1531) and
+ * RADLite adds the dust (line_dust_compute_src_alp line.F:3560-3607,
+ * bplanck setup.F:937-952).  No counterpart in the reference as an entry
+ * point: it replaces the host arrays of rl_set_levels, the width and
+ * abundance of rl_set_medium and rl_set_dust_line for callers that fit
+ * the model, and its adjoint carries the trace adjoint's cell gradients
+ * back to the physical fields without leaving the device. */
+/* Level energies E_K [K] and statistical weights g [nlev]. */
+int rl_set_level_data(rl_ctx *ctx, int nlev, const double *E_K, const double *g);
+/* Dust species: density rho_d [nspec][nr][nth] and kabs, kscat at each
+ * line centre [nspec][nlines] (the opacity tables interpolated at nu0 as
+ * line.F:3560-3607 does; 0 outside the table).  Call after rl_set_lines. */
+int rl_set_dust_species(rl_ctx *ctx, int nspec, const double *rho_d,
+                        const double *kabs_nu0, const double *kscat_nu0);
+/* Per cell [nr][nth]: gas temperature T_gas [K], dust temperature
+ * T_dust [nspec][nr][nth] (NULL: T_gas for every species), log-abundance
+ * (NULL: keep the abundance of rl_set_medium), turbulent velocity
+ * v_turb [km/s]; molecular mass m_mol [m_H]; partition function Q = T/q
+ * (q > 0) or populations renormalised to sum 1 (q = 0).  Sets on the
+ * device: n_l = g_l exp(-E_l/T)/Q (values below 1e-99 are 0), lw =
+ * sqrt(v_turb^2 + 2kT/(m_mol m_H)), abundance = exp(log-abundance),
+ * j_d = sum_s rho_s kabs_s B(nu0, T_s), a_d = sum_s rho_s (kabs_s +
+ * kscat_s), and aksmax = max lw. */
+int rl_set_physics(rl_ctx *ctx, const double *T_gas, const double *T_dust,
+                   const double *log_abund, const double *v_turb_kms, double m_mol,
+                   double q_over);
+/* dL/d(T_gas, T_dust, log-abundance, v_turb) [nr][nth] (T_dust: [nspec]
+ * [nr][nth], only when T_dust was given) from the cell gradients of the
+ * last rl_trace_lines_adjoint call, at the fields of the last
+ * rl_set_physics.  Any output may be NULL. */
+int rl_physics_adjoint(rl_ctx *ctx, double *g_T_gas, double *g_T_dust,
+                       double *g_log_abund, double *g_v_turb);
+/* The device tables (populations [nr][nth][nlev], lw, abundance [nr][nth],
+ * j_d, a_d [nr][nth][nlines]) to the host; any may be NULL. */
+int rl_get_tables(rl_ctx *ctx, double *pops, double *lw, double *abund, double *jd,
+                  double *ad);
+
 /* ---- host pre-pass with the reference's arithmetic (R13) ---------- */
 /* nu0 = 1.509160e26 * 1.986468498e-16 * (E_up - E_low)[cm^-1]. */
 int rl_prepare_line_freqs(int nlines, const double *E_cm, const int *lev_up,

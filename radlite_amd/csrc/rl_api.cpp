@@ -140,6 +140,12 @@ struct rl_ctx {
     DevBuf d_stc_vals, d_stc_w, d_stc_off;    // a split item's chunk transpose
     DevBuf d_arec, d_ckpt, d_ckoff, d_gflux, d_gnode, d_gcell, d_grho, d_gabund, d_glw,
         d_gpops, d_gjd, d_gad;
+    // device pre-pass: level data, dust species at the line centres, and the
+    // last physical fields (for its adjoint)
+    int pre_nlev = 0, pre_nspec = 0, pre_dust_from_gas = 0, pre_ready = 0;
+    double pre_q = 0.0, pre_cw = 0.0;
+    DevBuf d_EK, d_g, d_nu0, d_rhod, d_Td, d_kabs, d_kscat, d_T, d_la, d_vt, d_part;
+    DevBuf d_gT, d_gTd, d_gvt, d_gla;
     // stats
     double t_ms[5] = {0, 0, 0, 0, 0};
     long long counts[4] = {0, 0, 0, 0};
@@ -1753,6 +1759,181 @@ extern "C" int rl_memory_info(rl_ctx *c, double *out) {
     out[2] = (double)c->last_work;
     out[3] = (double)fr;
     out[4] = (double)tot;
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// device pre-pass (pyradlite's per-cell tables) and its adjoint
+// ---------------------------------------------------------------------
+extern "C" int rl_set_level_data(rl_ctx *c, int nlev, const double *E_K, const double *g) {
+    if (c) c->gen++;                         // invalidates a tape
+    if (!c || nlev < 2 || !E_K || !g) return fail(c, RL_E_ARG, "rl_set_level_data: bad args");
+    if (c->nlev && nlev != c->nlev)
+        return fail(c, 13, "rl_set_level_data: %d levels, the populations have %d", nlev,
+                    c->nlev);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (int e = upload(c, c->d_EK, E_K, (size_t)nlev * 8)) return e;
+    if (int e = upload(c, c->d_g, g, (size_t)nlev * 8)) return e;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->pre_nlev = nlev;
+    return 0;
+}
+
+extern "C" int rl_set_dust_species(rl_ctx *c, int nspec, const double *rho_d,
+                                   const double *kabs_nu0, const double *kscat_nu0) {
+    if (c) c->gen++;                         // invalidates a tape
+    if (!c || !c->nr || !c->nlines || nspec < 0 || (nspec && (!rho_d || !kabs_nu0 || !kscat_nu0)))
+        return fail(c, RL_E_ARG, "rl_set_dust_species: set the grid and lines first; bad args");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t ncell = (size_t)c->nr * c->nh;
+    if (nspec) {
+        if (int e = upload(c, c->d_rhod, rho_d, (size_t)nspec * ncell * 8)) return e;
+        if (int e = upload(c, c->d_kabs, kabs_nu0, (size_t)nspec * c->nlines * 8)) return e;
+        if (int e = upload(c, c->d_kscat, kscat_nu0, (size_t)nspec * c->nlines * 8)) return e;
+    }
+    std::vector<double> nu(c->nu0);
+    if (int e = upload(c, c->d_nu0, nu.data(), nu.size() * 8)) return e;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->pre_nspec = nspec;
+    return 0;
+}
+
+static void prepass_args(rl_ctx *c, rl::PrepassLaunch &a) {
+    a.ncell = c->nr * c->nh;
+    a.nlev = c->pre_nlev;
+    a.nlines = c->nlines;
+    a.nspec = c->pre_nspec;
+    a.dust_from_gas = c->pre_dust_from_gas;
+    a.T = c->d_T.as<double>();
+    a.la = c->d_la.n ? c->d_la.as<double>() : nullptr;
+    a.vt = c->d_vt.as<double>();
+    a.EK = c->d_EK.as<double>();
+    a.g = c->d_g.as<double>();
+    a.q = c->pre_q;
+    a.cw = c->pre_cw;
+    a.nu0 = c->d_nu0.as<double>();
+    a.rho_d = c->d_rhod.as<double>();
+    a.Td = c->d_Td.as<double>();
+    a.kabs = c->d_kabs.as<double>();
+    a.kscat = c->d_kscat.as<double>();
+    a.pops = c->d_pops.as<double>();
+    a.lw = c->d_lw.as<double>();
+    a.abund = c->d_abund.as<double>();
+    a.jd = c->d_jd.as<double>();
+    a.ad = c->d_ad.as<double>();
+    a.part = c->d_part.as<double>();
+    a.g_pops = c->d_gpops.as<double>();
+    a.g_lw = c->d_glw.as<double>();
+    a.g_abund = c->d_gabund.as<double>();
+    a.g_jd = c->d_gjd.as<double>();
+    a.gT = c->d_gT.as<double>();
+    a.gTd = c->d_gTd.as<double>();
+    a.gvt = c->d_gvt.as<double>();
+    a.gla = c->d_gla.as<double>();
+}
+
+extern "C" int rl_set_physics(rl_ctx *c, const double *T_gas, const double *T_dust,
+                              const double *log_abund, const double *vturb_kms, double m_mol,
+                              double q_over) {
+    if (c) c->gen++;                         // invalidates a tape
+    if (!c || !T_gas || !vturb_kms || !(m_mol > 0.0) || q_over < 0.0)
+        return fail(c, RL_E_ARG, "rl_set_physics: bad args");
+    if (!c->have_medium || !c->nlines || !c->pre_nlev || !c->d_nu0.n)
+        return fail(c, 13, "rl_set_physics: set the medium, lines, level data and dust species "
+                           "first");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t ncell = (size_t)c->nr * c->nh, nsp = (size_t)c->pre_nspec;
+    if (int e = upload(c, c->d_T, T_gas, ncell * 8)) return e;
+    if (int e = upload(c, c->d_vt, vturb_kms, ncell * 8)) return e;
+    if (log_abund) {
+        if (int e = upload(c, c->d_la, log_abund, ncell * 8)) return e;
+    } else {
+        c->d_la = DevBuf();
+    }
+    HIPCHK(c, c->d_Td.ensure(std::max(nsp, (size_t)1) * ncell * 8));
+    c->pre_dust_from_gas = T_dust ? 0 : 1;
+    for (size_t s = 0; s < nsp; s++) {
+        if (T_dust)
+            HIPCHK(c, hipMemcpyAsync(c->d_Td.as<double>() + s * ncell, T_dust + s * ncell,
+                                     ncell * 8, hipMemcpyHostToDevice, c->stream));
+        else
+            HIPCHK(c, hipMemcpyAsync(c->d_Td.as<double>() + s * ncell, c->d_T.p, ncell * 8,
+                                     hipMemcpyDeviceToDevice, c->stream));
+    }
+    c->nlev = c->pre_nlev;
+    HIPCHK(c, c->d_pops.ensure(ncell * c->nlev * 8));
+    HIPCHK(c, c->d_lw.ensure(ncell * 8));
+    HIPCHK(c, c->d_abund.ensure(ncell * 8));
+    HIPCHK(c, c->d_jd.ensure(ncell * c->nlines * 8));
+    HIPCHK(c, c->d_ad.ensure(ncell * c->nlines * 8));
+    const int nblk = (int)((ncell + 255) / 256);
+    HIPCHK(c, c->d_part.ensure((size_t)nblk * 8));
+    c->pre_q = q_over;
+    c->pre_cw = 2.0 * 1.380649e-16 / (m_mol * 1.6726e-24) / 1e10;   // 2k/(m_mol m_H), (km/s)^2/K
+    rl::PrepassLaunch a;
+    prepass_args(c, a);
+    rl::launch_prepass(c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> part(nblk);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->d_part.p, (size_t)nblk * 8, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double mx = -1.0e300;
+    for (double v : part) mx = std::max(mx, v);
+    c->aksmax = mx;                          // line.F:3020-3033 (widths > 0 here)
+    c->have_dust = true;
+    c->pre_ready = 1;
+    c->cells_line = -1;
+    return 0;
+}
+
+extern "C" int rl_physics_adjoint(rl_ctx *c, double *g_T_gas, double *g_T_dust,
+                                  double *g_log_abund, double *g_vturb) {
+    if (!c) return RL_E_ARG;
+    if (!c->pre_ready || !c->d_gpops.n)
+        return fail(c, 13, "rl_physics_adjoint: needs rl_set_physics and then "
+                           "rl_trace_lines_adjoint");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t ncell = (size_t)c->nr * c->nh, nsp = (size_t)c->pre_nspec;
+    HIPCHK(c, c->d_gT.ensure(ncell * 8));
+    HIPCHK(c, c->d_gTd.ensure(std::max(nsp, (size_t)1) * ncell * 8));
+    HIPCHK(c, c->d_gvt.ensure(ncell * 8));
+    HIPCHK(c, c->d_gla.ensure(ncell * 8));
+    rl::PrepassLaunch a;
+    prepass_args(c, a);
+    rl::launch_prepass_adjoint(c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    auto down = [&](double *dst, DevBuf &b, size_t n) -> int {
+        if (dst) HIPCHK(c, hipMemcpyAsync(dst, b.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+        return 0;
+    };
+    if (int e = down(g_T_gas, c->d_gT, ncell)) return e;
+    if (g_T_dust && !c->pre_dust_from_gas)
+        if (int e = down(g_T_dust, c->d_gTd, nsp * ncell)) return e;
+    if (int e = down(g_log_abund, c->d_gla, ncell)) return e;
+    if (int e = down(g_vturb, c->d_gvt, ncell)) return e;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int rl_get_tables(rl_ctx *c, double *pops, double *lw, double *abund, double *jd,
+                             double *ad) {
+    if (!c || !c->nr) return RL_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t ncell = (size_t)c->nr * c->nh;
+    auto down = [&](double *dst, DevBuf &b, size_t n) -> int {
+        if (dst) {
+            if (b.n < n * 8) return fail(c, 13, "rl_get_tables: table not set");
+            HIPCHK(c, hipMemcpyAsync(dst, b.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+        }
+        return 0;
+    };
+    if (int e = down(pops, c->d_pops, ncell * c->nlev)) return e;
+    if (int e = down(lw, c->d_lw, ncell)) return e;
+    if (int e = down(abund, c->d_abund, ncell)) return e;
+    if (int e = down(jd, c->d_jd, ncell * c->nlines)) return e;
+    if (int e = down(ad, c->d_ad, ncell * c->nlines)) return e;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

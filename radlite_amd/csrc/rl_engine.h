@@ -180,4 +180,23 @@ int build_stencil_transpose(hipStream_t st, long long nnodes, const GeoNode *geo
                             int *off, void *temp, size_t *temp_bytes);
 size_t fast_rec_size(int precision_fp64);
 
+// The device pre-pass (rl_prepass.hip): cell tables from per-cell
+// temperature, log-abundance and turbulence, and its adjoint.
+struct PrepassLaunch {
+    int ncell, nlev, nlines, nspec, dust_from_gas;
+    const double *T, *la, *vt;         // [cell]; la may be null (abundance kept)
+    const double *EK, *g;              // level energies [K] and weights [nlev]
+    double q, cw;                      // Q = T/q (q > 0) or renormalised (q = 0); 2k/(m_mol m_H)/1e10
+    const double *nu0;                 // [line]
+    const double *rho_d, *Td;          // [species][cell]
+    const double *kabs, *kscat;        // at the line centres [species][line]
+    double *pops, *lw, *abund, *jd, *ad;
+    double *part;                      // block maxima of lw, (ncell+255)/256
+    // adjoint
+    const double *g_pops, *g_lw, *g_abund, *g_jd;
+    double *gT, *gTd, *gvt, *gla;
+};
+void launch_prepass(hipStream_t st, const PrepassLaunch &a);
+void launch_prepass_adjoint(hipStream_t st, const PrepassLaunch &a);
+
 }  // namespace rl

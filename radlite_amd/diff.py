@@ -17,7 +17,10 @@ Chain (all per cell on the half theta grid [nr, nth]):
 
 PyTorch is the autograd plumbing here; the trace and its adjoint run in
 the HIP library.  Tensors are float64 on the CPU (the C ABI takes host
-arrays).
+arrays).  ``flux_from_physics`` runs the pre-pass in torch and hands the
+tables to the trace; ``flux_from_physics_device`` runs the same pre-pass
+on the device (rl_set_physics) and its adjoint there (rl_physics_adjoint),
+so that only the three per-cell fields and their gradients cross PCIe.
 """
 from __future__ import annotations
 
@@ -113,6 +116,47 @@ class _Trace(torch.autograd.Function):
                 t(gr["g_jd"]), t(gr["g_ad"]), None)
 
 
+class _PhysTrace(torch.autograd.Function):
+    """flux(T, log_abund, v_turb) with the pre-pass and the trace on the
+    device (rl_set_physics, rl_trace_lines_forward_tape); the backward is
+    rl_trace_lines_adjoint followed by rl_physics_adjoint, the cell
+    gradients staying on the device."""
+
+    @staticmethod
+    def forward(ctx, T, log_abund, v_turb, model):
+        model._set_physics(T, log_abund, v_turb)
+        fl = None
+        if any(ctx.needs_input_grad[:3]):
+            try:
+                fl = model.rt.trace_tape(model.items)
+            except engine.RadliteError as e:
+                if e.code != 13:
+                    raise
+        if fl is None:
+            fl, _, _ = model.rt.trace(model.items)
+        model._calls += 1
+        ctx.model = model
+        ctx.call = model._calls
+        ctx.save_for_backward(T, log_abund, v_turb)
+        return torch.from_numpy(np.concatenate(fl))
+
+    @staticmethod
+    def backward(ctx, g):
+        m = ctx.model
+        if m._calls != ctx.call:
+            m._set_physics(*ctx.saved_tensors)
+            m._calls += 1
+        g = g.detach().cpu().numpy().astype(np.float64)
+        parts, o = [], 0
+        for it in m.items:
+            parts.append(g[o:o + it[2]])
+            o += it[2]
+        m.rt.adjoint(m.items, parts, outputs=())
+        gr = m.rt.physics_adjoint()
+        t = lambda a: torch.from_numpy(a)
+        return t(gr["g_T"]), t(gr["g_log_abund"]), t(gr["g_v_turb"]), None
+
+
 class SpectrumModel:
     """A loaded run (grid, velocity, lines, star, rays) whose spectrum is
     differentiable with respect to the per-cell fields.
@@ -123,6 +167,8 @@ class SpectrumModel:
     def __init__(self, ri, precision="fp64", incl_rad=None, device=0, items=None):
         self.ri = ri
         self._calls = 0
+        self._device_physics = False
+        self._m_mol = 28.0
         self.rt = engine.Raytracer(device, precision)
         self.tables = self.rt.load_run(ri)
         inc = [ri.incl_deg * 0.0174532925199] if incl_rad is None else list(incl_rad)
@@ -145,6 +191,30 @@ class SpectrumModel:
 
     def flux(self, rho, abund, lw, pops, jd, ad):
         return _Trace.apply(rho, abund, lw, pops, jd, ad, self)
+
+    def _set_physics(self, T, log_abund, v_turb):
+        n = lambda x: np.ascontiguousarray(x.detach().cpu().numpy(), dtype=np.float64)
+        self.rt.set_physics(n(T), n(v_turb), log_abund=n(log_abund), m_mol=self._m_mol,
+                            q_over=2.77)
+
+    def flux_from_physics_device(self, T, log_abund, v_turb_kms, m_mol=28.0):
+        """flux_from_physics with the pre-pass on the device (and its
+        adjoint): per call only T, log-abundance and v_turb cross to the
+        device and their gradients back; the dust temperature is T, for
+        every dust species of the run."""
+        if not self._device_physics:
+            ri = self.ri
+            self.rt.set_level_data(np.asarray(ri.E_cm, float) * synth.K_PER_CM,
+                                   np.asarray(ri.g, float))
+            ns = ri.rho_dust.shape[0]
+            ka = np.array([kappa_at_lines(ri.cont_nu, ri.kappa_abs[s], self.tables.nu0)
+                           for s in range(ns)])
+            ks = np.array([kappa_at_lines(ri.cont_nu, ri.kappa_scat[s], self.tables.nu0)
+                           for s in range(ns)])
+            self.rt.set_dust_species(ri.rho_dust, ka, ks)
+            self._device_physics = True
+        self._m_mol = m_mol
+        return _PhysTrace.apply(T, log_abund, v_turb_kms, self)
 
     def flux_from_physics(self, T, log_abund, v_turb_kms, m_mol=28.0):
         """Spectrum as a function of per-cell temperature (gas = dust),

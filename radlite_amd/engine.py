@@ -78,6 +78,11 @@ def load_library(path=LIB_PATH):
         "rl_last_adjoint_info": (C.c_int, [vp, _ip, _ip, _ip]),
         "rl_memory_info": (C.c_int, [vp, _dp]),
         "rl_plan": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp]),
+        "rl_set_level_data": (C.c_int, [vp, C.c_int, _dp, _dp]),
+        "rl_set_dust_species": (C.c_int, [vp, C.c_int, _dp, _dp, _dp]),
+        "rl_set_physics": (C.c_int, [vp, _dp, _dp, _dp, _dp, C.c_double, C.c_double]),
+        "rl_physics_adjoint": (C.c_int, [vp, _dp, _dp, _dp, _dp]),
+        "rl_get_tables": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
         "rl_sync": (C.c_int, [vp]),
         "rl_set_memory_budget": (C.c_int, [vp, C.c_double]),
         "rl_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
@@ -188,7 +193,7 @@ class Raytracer:
         self.ring_r = None
         self.ring_edge = None
         self.nr = self.nth = 0
-        self.nlev = self.nlines = 0
+        self.nlev = self.nlines = self.nspec = 0
 
     def close(self):
         if self.ctx:
@@ -368,6 +373,59 @@ class Raytracer:
         v = np.zeros(5)
         self._chk(self.L.rl_memory_info(self.ctx, v.ctypes.data_as(_dp)), "rl_memory_info")
         return dict(budget=v[0], traj=v[1], work=v[2], free=v[3], total=v[4])
+
+    # ---- device pre-pass ---------------------------------------------
+    def set_level_data(self, E_K, g):
+        e, ep = _d(E_K)
+        gg, gp = _d(g)
+        self._chk(self.L.rl_set_level_data(self.ctx, len(e), ep, gp), "rl_set_level_data")
+        self.nlev = len(e)
+
+    def set_dust_species(self, rho_d, kabs_nu0, kscat_nu0):
+        """rho_d [nspec, nr, nth]; kabs/kscat at the line centres [nspec, nlines]."""
+        r, rp = _d(rho_d)
+        a, ap = _d(kabs_nu0)
+        b, bp = _d(kscat_nu0)
+        self._chk(self.L.rl_set_dust_species(self.ctx, r.shape[0], rp, ap, bp),
+                  "rl_set_dust_species")
+        self.nspec = r.shape[0]
+
+    def set_physics(self, T_gas, v_turb_kms, log_abund=None, T_dust=None, m_mol=28.0,
+                    q_over=2.77):
+        t, tp = _d(T_gas)
+        v, vp_ = _d(v_turb_kms)
+        la, lap = (None, None) if log_abund is None else _d(log_abund)
+        td, tdp = (None, None) if T_dust is None else _d(T_dust)
+        self._chk(self.L.rl_set_physics(self.ctx, tp, tdp, lap, vp_, float(m_mol),
+                                        float(q_over)), "rl_set_physics")
+
+    def physics_adjoint(self, want_dust=False):
+        """dict g_T, g_log_abund, g_v_turb [nr, nth] (and g_T_dust when the
+        dust temperature was given) from the last adjoint call."""
+        nr, nth = self.nr, self.nth
+        out = dict(g_T=np.zeros((nr, nth)), g_log_abund=np.zeros((nr, nth)),
+                   g_v_turb=np.zeros((nr, nth)))
+        gd = np.zeros((self.nspec, nr, nth)) if want_dust else None
+        self._chk(self.L.rl_physics_adjoint(
+            self.ctx, out["g_T"].ctypes.data_as(_dp),
+            gd.ctypes.data_as(_dp) if gd is not None else None,
+            out["g_log_abund"].ctypes.data_as(_dp), out["g_v_turb"].ctypes.data_as(_dp)),
+            "rl_physics_adjoint")
+        if gd is not None:
+            out["g_T_dust"] = gd
+        return out
+
+    def get_tables(self):
+        """The device tables: pops [nr, nth, nlev], lw, abund [nr, nth], jd,
+        ad [nr, nth, nlines]."""
+        nr, nth = self.nr, self.nth
+        t = dict(pops=np.zeros((nr, nth, self.nlev)), lw=np.zeros((nr, nth)),
+                 abund=np.zeros((nr, nth)), jd=np.zeros((nr, nth, self.nlines)),
+                 ad=np.zeros((nr, nth, self.nlines)))
+        self._chk(self.L.rl_get_tables(self.ctx, *[t[k].ctypes.data_as(_dp) for k in
+                                                   ("pops", "lw", "abund", "jd", "ad")]),
+                  "rl_get_tables")
+        return t
 
     def plan(self, items):
         """The memory plan of a call on ``items`` without running it: forward

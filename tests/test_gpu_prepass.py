@@ -1,0 +1,132 @@
+"""The device pre-pass (rl_set_physics) and its adjoint (rl_physics_adjoint).
+
+* Tables from per-cell temperature, log-abundance and turbulence on the
+  200x100 disk against the torch pre-pass (diff.lte_populations,
+  line_width_kms, dust_line) and against the host tables of the reference
+  arithmetic (prepare_tables) at the run's own temperatures; the
+  renormalised populations against numpy.
+* The chi^2 fit through the device pre-pass against the same fit through
+  the torch pre-pass: spectrum and gradients with respect to T,
+  log-abundance and turbulence (FP64 and FP32).
+* A temperature given per dust species separately from the gas.
+"""
+import sys
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from radlite_amd import diff, engine, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _problem(precision):
+    ri, w = synth.workload_inputs("c4", seed=0)
+    model = diff.SpectrumModel(ri, precision)
+    T0 = torch.tensor(ri.t_dust[0], dtype=torch.float64)
+    la0 = torch.log(torch.tensor(ri.abund, dtype=torch.float64))
+    vt0 = torch.full_like(T0, 0.1)
+    return ri, model, T0, la0, vt0
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    s = np.abs(b).max()
+    return float(np.abs(a - b).max() / s) if s > 0 else float(np.abs(a).max())
+
+
+def test_device_tables_match_torch_and_host():
+    say("[gpu] device pre-pass tables")
+    ri, m, T0, la0, vt0 = _problem("fp64")
+    rng = np.random.default_rng(3)
+    T = T0 * torch.from_numpy(np.exp(rng.normal(0.0, 0.2, size=T0.shape)))
+    vt = vt0 * torch.from_numpy(np.exp(rng.normal(0.0, 0.3, size=T0.shape)))
+    la = la0 + torch.from_numpy(rng.normal(0.0, 0.5, size=T0.shape))
+    with torch.no_grad():
+        m.flux_from_physics_device(T, la, vt)       # uploads level and dust data
+    t = m.rt.get_tables()
+    pops = diff.lte_populations(T, ri.E_cm, ri.g).numpy()
+    lw = diff.line_width_kms(T, vt).numpy()
+    jd, ad = diff.dust_line(T, torch.as_tensor(ri.rho_dust[0]), m.kabs_nu0, m.kscat_nu0,
+                            m.tables.nu0)
+    errs = dict(pops=_rel(t["pops"], pops), lw=_rel(t["lw"], lw),
+                abund=_rel(t["abund"], np.exp(la.numpy())), jd=_rel(t["jd"], jd.numpy()),
+                ad=_rel(t["ad"], ad.numpy()))
+    say(f"    device vs torch pre-pass, max rel: {errs}")
+    for k, v in errs.items():
+        assert v <= 1e-13, (k, v)
+    # at the run's own temperature and width the dust equals the host tables
+    # of the reference arithmetic (prepare_tables), bit for bit but for exp
+    with torch.no_grad():
+        m.rt.set_physics(ri.t_dust[0], np.full(T0.shape, 0.1), m_mol=28.0)
+    t = m.rt.get_tables()
+    assert _rel(t["jd"], m.tables.jd) <= 1e-14 and _rel(t["ad"], m.tables.ad) <= 1e-15
+    # renormalised populations (q = 0)
+    m.rt.set_physics(T.numpy(), vt.numpy(), q_over=0.0)
+    t = m.rt.get_tables()
+    EK = np.asarray(ri.E_cm) * synth.K_PER_CM
+    n = np.asarray(ri.g)[None, None, :] * np.exp(-EK[None, None, :] / T.numpy()[..., None])
+    n = n / n.sum(-1, keepdims=True)
+    n[n < 1e-99] = 0.0
+    assert _rel(t["pops"], n) <= 1e-13
+    m.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_device_prepass_gradients_match_torch_prepass(precision):
+    say(f"[gpu] chi^2 through the device pre-pass, {precision}")
+    ri, m, T0, la0, vt0 = _problem(precision)
+    rng = np.random.default_rng(4)
+    Tt = T0 * torch.from_numpy(np.exp(rng.normal(0.0, 0.2, size=T0.shape)))
+    with torch.no_grad():
+        Ft = m.flux_from_physics(Tt, la0, vt0)
+    sigma = 1e-3 * float(Ft.abs().max())
+    res = {}
+    for name, fn in (("torch", m.flux_from_physics), ("device", m.flux_from_physics_device)):
+        T = T0.clone().requires_grad_(True)
+        la = la0.clone().requires_grad_(True)
+        vt = vt0.clone().requires_grad_(True)
+        t0 = time.perf_counter()
+        F = fn(T, la, vt)
+        L = (((F - Ft) / sigma) ** 2).sum()
+        L.backward()
+        dt = time.perf_counter() - t0
+        assert m.rt.adjoint_info()[1] == 1             # the backward ran from the tape
+        res[name] = (F.detach().numpy(), T.grad.numpy(), la.grad.numpy(), vt.grad.numpy(), dt)
+    a, b = res["device"], res["torch"]
+    fl = _rel(a[0], b[0])
+    gs = [_rel(a[i], b[i]) for i in (1, 2, 3)]
+    say(f"    spectrum {fl:.2e}; d/dT {gs[0]:.2e}, d/dla {gs[1]:.2e}, d/dvt {gs[2]:.2e}; "
+        f"step {b[4]*1e3:.1f} ms (torch pre-pass) vs {a[4]*1e3:.1f} ms (device)")
+    tol_f, tol_g = (1e-12, 1e-9) if precision == "fp64" else (1e-5, 1e-3)
+    assert fl <= tol_f
+    assert max(gs) <= tol_g, gs
+    m.close()
+
+
+def test_separate_dust_temperature():
+    """T_dust given per species: the gas and dust temperature gradients
+    add up to the gradient with T_dust = T_gas."""
+    say("[gpu] separate dust temperature")
+    ri, m, T0, la0, vt0 = _problem("fp64")
+    with torch.no_grad():
+        m.flux_from_physics_device(T0, la0, vt0)
+    nfr = m.nfr
+    g = [np.random.default_rng(5).normal(size=nfr) * 1e20]
+    m.rt.set_physics(T0.numpy(), vt0.numpy(), log_abund=la0.numpy())
+    m.rt.adjoint(m.items, g, outputs=())
+    both = m.rt.physics_adjoint()
+    m.rt.set_physics(T0.numpy(), vt0.numpy(), log_abund=la0.numpy(),
+                     T_dust=T0.numpy()[None])
+    f2, _ = m.rt.adjoint(m.items, g, outputs=())
+    sep = m.rt.physics_adjoint(want_dust=True)
+    tot = sep["g_T"] + sep["g_T_dust"][0]
+    assert _rel(tot, both["g_T"]) <= 1e-12
+    assert np.abs(sep["g_T_dust"]).max() > 0
+    m.close()
