@@ -71,6 +71,14 @@ def lib():
         L.orc_trajectory.argtypes = [C.POINTER(OrcTraj), C.POINTER(OrcGrid),
                                      C.c_double, C.c_double, C.c_double,
                                      C.c_double, C.c_double]
+        L.orc_setup_rays_rect.argtypes = [C.POINTER(OrcRays), dp, C.c_double, C.c_int, C.c_int,
+                                          C.c_double, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, C.c_int, C.c_double, dp, dp,
+                                          C.POINTER(C.c_int)]
+        L.orc_posvel.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(OrcRays), dp, C.c_int,
+                                 C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.c_double, C.c_int, C.c_int, dp, dp, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_long)]
         L.orc_passband_nfr.restype = C.c_int
         L.orc_passband_nfr.argtypes = [C.c_double, C.c_double, dp]
         _LIB = L
@@ -170,6 +178,42 @@ class Oracle:
         for k in ("icross", "ir", "it"):
             out[k] = np.ctypeslib.as_array(getattr(t, k), shape=(n,)).copy()
         lib().orc_traj_free(C.byref(t))
+        return out
+
+    def posvel(self, iline, pv, incl_rad=None, nfr=None, vmax=None, nthreads=0):
+        """Position-velocity cube of a line (telescope.F:1828-2183): dict
+        with image and tau [ny, nx, nfr], the rays (x0, z0, b) and the
+        pixel sizes.  pv: nx, ny, size_x, size_y, phioff, xoff, yoff,
+        addstar (linespectrum.inp, telescope.F:196-224)."""
+        p = self.p
+        incl_rad = p.incl_rad if incl_rad is None else incl_rad
+        nfr = p.nfr if nfr is None else nfr
+        vmax = p.vmax if vmax is None else vmax
+        nx, ny = int(pv["nx"]), int(pv["ny"])
+        n = nx * ny + (1 if pv["addstar"] > 0 else 0)
+        rays = OrcRays()
+        b = np.zeros(n)
+        spx, spy, unres = C.c_double(0), C.c_double(0), C.c_int(0)
+        rc = lib().orc_setup_rays_rect(C.byref(rays), _p(b), incl_rad, nx, ny, pv["size_x"],
+                                       pv["size_y"], pv["phioff"], pv["xoff"], pv["yoff"],
+                                       int(pv["addstar"]), p.ri.rstar, C.byref(spx),
+                                       C.byref(spy), C.byref(unres))
+        if rc:
+            raise RuntimeError(f"orc_setup_rays_rect: stop code {rc}")
+        img = np.zeros((ny, nx, nfr))
+        tau = np.zeros((ny, nx, nfr))
+        maser, ovs = C.c_int(0), C.c_long(0)
+        rc = lib().orc_posvel(C.cast(C.byref(self.m), C.c_void_p),
+                              C.cast(C.byref(self.g), C.c_void_p), C.byref(rays), _p(b),
+                              nx, ny, spx.value, spy.value, unres.value, iline, vmax, nfr,
+                              nthreads, _p(img), _p(tau), C.byref(maser), C.byref(ovs))
+        out = dict(image=img, tau=tau, b=b, spx=spx.value, spy=spy.value,
+                   starunres=unres.value, maser=maser.value, overshoot=ovs.value,
+                   x0=np.ctypeslib.as_array(rays.x0, shape=(n,)).copy(),
+                   z0=np.ctypeslib.as_array(rays.z0, shape=(n,)).copy(), theta0=rays.theta0)
+        lib().orc_rays_free(C.byref(rays))
+        if rc:
+            raise RuntimeError(f"orc_posvel: stop code {rc}")
         return out
 
     def line_spectrum(self, iline, incl_rad=None, nphi=None, nrext=None, dbdr=None,

@@ -103,3 +103,62 @@ c
 c
       call real_write_line_spectrum(istyle)
       end
+
+c     ------------------------------------------------------------------
+c     Position-velocity cubes: the link step also wraps
+c     calc_write_line_posvel (telescope.F:1828, called per line from
+c     main.F:1071).  After the real routine (which renders the
+c     rectangular image, make_freq_image_rectang telescope.F:2061, and
+c     writes lineposvel_*.dat at 6 digits) this appends, per line, to
+c     probe_posvel.bin:
+c       int32  iline, nfr, nrx, nry, rays_amount, imrec_starunres,
+c              maserflag, overshoot
+c       f64    line_nu0(iline), line_dnu(1:nfr,iline)
+c       f64    imrec_int(1:nfr,1:nrx,1:nry), imrec_tau(1:nfr,1:nrx,1:nry)
+c       f64    rp_x0, rp_z0, rp_b (1:rays_amount), rp_theta0(1)
+c     ------------------------------------------------------------------
+      subroutine probe_calc_write_line_posvel(iline,distance,
+     %     passband,nfr) bind(C, name='__wrap_calc_write_line_posvel_')
+      implicit none
+      integer iline,nfr
+      doubleprecision distance,passband
+c
+#include "common_grid.h"
+#include "common_telescope.h"
+#include "common_lines.h"
+#include "common_radfield.h"
+c
+      interface
+         subroutine real_calc_write_line_posvel(iline,distance,
+     %        passband,nfr)
+     %        bind(C, name='__real_calc_write_line_posvel_')
+         integer iline,nfr
+         doubleprecision distance,passband
+         end subroutine
+      end interface
+c
+      integer hdr(8),i,inu,ix,iy,nf,nray
+c
+      call real_calc_write_line_posvel(iline,distance,passband,nfr)
+      nf   = freq_nr
+      nray = rays_amount
+      hdr(1) = iline
+      hdr(2) = nf
+      hdr(3) = imrec_nrx
+      hdr(4) = imrec_nry
+      hdr(5) = nray
+      hdr(6) = imrec_starunres
+      hdr(7) = maserflag
+      hdr(8) = transerr_overshoot
+      open(unit=77,file='probe_posvel.bin',access='stream',
+     %     form='unformatted',position='append',status='unknown')
+      write(77) (hdr(i),i=1,8)
+      write(77) line_nu0(iline), (line_dnu(inu,iline),inu=1,nf)
+      write(77) (((imrec_int(inu,ix,iy),inu=1,nf),ix=1,imrec_nrx),
+     %     iy=1,imrec_nry)
+      write(77) (((imrec_tau(inu,ix,iy),inu=1,nf),ix=1,imrec_nrx),
+     %     iy=1,imrec_nry)
+      write(77) (rp_x0(i),i=1,nray), (rp_z0(i),i=1,nray),
+     %     (rp_b(i),i=1,nray), rp_theta0(1)
+      close(77)
+      end

@@ -57,3 +57,31 @@ def read_probe_traj(fname):
         out.append(dict(iray=iray, s=d[0], r=d[1], theta=d[2], mu=d[3], phi=d[4],
                         icross=i[0], ir=i[1], it=i[2]))
     return out
+
+
+def read_probe_posvel(fname):
+    """Records of probe_posvel.bin (probe_dump.F, one per line)."""
+    buf = open(fname, "rb").read()
+    pos = 0
+    out = []
+
+    def take(dtype, n):
+        nonlocal pos
+        a = np.frombuffer(buf, dtype=dtype, count=n, offset=pos)
+        pos += a.nbytes
+        return a
+
+    while pos < len(buf):
+        il, nfr, nx, ny, nray, unres, maser, ovs = [int(v) for v in take(np.int32, 8)]
+        nu0 = float(take(np.float64, 1)[0])
+        dnu = take(np.float64, nfr).copy()
+        img = take(np.float64, nfr * nx * ny).reshape(ny, nx, nfr).copy()
+        tau = take(np.float64, nfr * nx * ny).reshape(ny, nx, nfr).copy()
+        x0 = take(np.float64, nray).copy()
+        z0 = take(np.float64, nray).copy()
+        b = take(np.float64, nray).copy()
+        th0 = float(take(np.float64, 1)[0])
+        out.append(dict(iline=il, nfr=nfr, nx=nx, ny=ny, nrays=nray, starunres=unres,
+                        maser=maser, overshoot=ovs, nu0=nu0, dnu=dnu, image=img, tau=tau,
+                        x0=x0, z0=z0, b=b, theta0=th0))
+    return out

@@ -801,6 +801,7 @@ typedef struct {
     double *ds;        /* ds[k] = s_k - s_{k-1}, k>=1                    */
     int *reset;        /* hole chord: ds := 0 and init := 1              */
     int *blend;        /* star blend happens at node k (central beam)   */
+    int *star;         /* I := I_star at node k (rbeam0 = 0, b <= R*)    */
     double starfract;  /* (rstar/rbeam0)^2                              */
     node_t *nv;
 } raycache_t;
@@ -820,6 +821,10 @@ static double charint(const raycache_t *rc_, line_ctx *L,
         if (rc_->reset[k]) { ds = 0.0; c.init = 1; }
         if (rc_->blend[k]) {            /* telescope.F:4187-4193 */
             I = (1.0 - rc_->starfract) * I + rc_->starfract * istar_ch;
+            c.init = 1;
+        }
+        if (rc_->star[k]) {             /* telescope.F:4194-4208 */
+            I = istar_ch;
             c.init = 1;
         }
         clever_elem(L, &I, ds, a->src, b->src, a->alp, b->alp, a->lw, b->lw,
@@ -846,6 +851,7 @@ static int build_cache(const orc_model *m, const orc_grid *g,
     rc_->ds = (double *)calloc(t->n + 1, sizeof(double));
     rc_->reset = (int *)calloc(t->n + 1, sizeof(int));
     rc_->blend = (int *)calloc(t->n + 1, sizeof(int));
+    rc_->star = (int *)calloc(t->n + 1, sizeof(int));
     rc_->nv = (node_t *)calloc(t->n + 1, sizeof(node_t));
     (void)nr;
     for (k = 0; k < t->n; k++) {
@@ -867,6 +873,12 @@ static int build_cache(const orc_model *m, const orc_grid *g,
             rc_->blend[k] = 1;
             istar_done = 1;
         }
+        /* a ray of the rectangular imager (rbeam0 = 0) whose impact
+           parameter is within the star takes the stellar intensity at
+           every node following a node of the innermost radial cell,
+           itself in that cell (telescope.F:4194-4208) */
+        if (ir == 1 && t->ir[k - 1] == 1 && rbeam0 == 0.0 && t->b <= m->rstar)
+            rc_->star[k] = 1;
     }
     if (rbeam0 > 0.0) {
         double q = m->rstar / rbeam0;
@@ -877,7 +889,7 @@ static int build_cache(const orc_model *m, const orc_grid *g,
 
 static void free_cache(raycache_t *c)
 {
-    free(c->ds); free(c->reset); free(c->blend); free(c->nv);
+    free(c->ds); free(c->reset); free(c->blend); free(c->star); free(c->nv);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1146,4 +1158,158 @@ int orc_ray_nodes(const orc_model *m, const orc_grid *g, const orc_rays *rays, i
     orc_traj_free(&t);
     free_cache(&c);
     return e ? -e : n;
+}
+
+/* ------------------------------------------------------------------ */
+/* Rectangular images: setup_rays_rectang (telescope.F:2229-2405) and   */
+/* make_freq_image_rectang (telescope.F:2061-2183) as                   */
+/* calc_write_line_posvel (telescope.F:1828-2031) uses them.            */
+/* ------------------------------------------------------------------ */
+int orc_setup_rays_rect(orc_rays *rays, double *b, double anginf, int nx, int ny,
+                        double size_x, double size_y, double phioff, double xoff,
+                        double yoff, int addstar, double rstar, double *spx_out,
+                        double *spy_out, int *starunres)
+{
+    double szx, szy, spx, spy, theta0, sinth0;
+    int ix, iy, ir = 0, nxh, nyh, n;
+    if (nx % 2 || ny % 2 || nx < 2 || ny < 2) return 13;          /* :2262-2269 */
+    szx = size_x * 0.5;                                            /* :209-212  */
+    szy = size_y * 0.5;
+    spx = szx / (nx * 0.5);
+    spy = szy / (ny * 0.5);
+    if (spx <= 0.0 || spy <= 0.0) return 13;
+    if (anginf < 1.0e-1) anginf = 0.1;                             /* :2294-2303 */
+    theta0 = anginf + 1.0e-4;
+    sinth0 = sin(theta0);
+    n = nx * ny + (addstar > 0 ? 1 : 0);
+    memset(rays, 0, sizeof(*rays));
+    rays->nrays = n;
+    rays->theta0 = theta0;
+    rays->x0 = (double *)calloc(n, sizeof(double));
+    rays->z0 = (double *)calloc(n, sizeof(double));
+    nxh = nx / 2;
+    nyh = ny / 2;
+    for (ix = 1; ix <= nx; ix++) {
+        for (iy = 1; iy <= ny; iy++) {
+            double x_c = (ix - nxh - 0.5) * spx - xoff;
+            double y_c = (iy - nyh - 0.5) * spy - yoff;
+            double r_c = sqrt(x_c * x_c + y_c * y_c), p_c, zh0, zh02, dum, xh0;
+            if (x_c == 0.0) x_c = x_c + 0.001 * spx;
+            p_c = atan(y_c / x_c) - phioff;
+            if (x_c < 0.0) p_c = p_c + 3.14159265359;
+            while (p_c < 0.0) p_c = p_c + 6.28318530718;
+            while (p_c >= 6.28318530718) p_c = p_c - 6.28318530718;
+            zh0 = -sin(p_c) / sinth0;
+            zh02 = zh0 * zh0;
+            dum = 1.0 - zh02 * sinth0 * sinth0;
+            dum = dum + 1e-4;
+            if (dum < 0.0) return 13;
+            xh0 = cos(p_c) > 0.0 ? sqrt(dum) : -sqrt(dum);
+            rays->x0[ir] = r_c * xh0;
+            rays->z0[ir] = r_c * zh0;
+            b[ir] = r_c;
+            ir++;
+        }
+    }
+    *starunres = 0;
+    if (addstar > 0) {                                             /* :2349-2400 */
+        if (xoff != 0.0 || yoff != 0.0) return 13;
+        rays->x0[ir] = 0.0;
+        rays->z0[ir] = 0.0;
+        b[ir] = 0.0;
+        if (spx * spx + spy * spy > rstar * rstar) *starunres = 1;
+    }
+    *spx_out = spx;
+    *spy_out = spy;
+    return 0;
+}
+
+/* I (erg/s/cm2/Hz/sr) and the line-of-sight optical depth per pixel and
+ * channel, image[iy][ix][ch] and tau[iy][ix][ch].  Every channel is
+ * integrated.  The pos-vel path of the reference does not set the star's
+ * passband spectrum (calc_write_line_posvel calls line_setup_passband but
+ * not line_init_passband_starbc, which calc_line_spectrum calls at
+ * telescope.F:1557), so the stellar intensity there is 0. */
+int orc_posvel(const orc_model *m, const orc_grid *g, const orc_rays *rays, const double *b,
+               int nx, int ny, double spx, double spy, int starunres, int il,
+               double width, int nfr, int nthreads, double *image, double *tau,
+               int *maser_out, long *overshoot_out)
+{
+    double nu0 = fabs(m->nu0[il]), passb, nu1, dnu, *dnu_ch, *velo, Bud, Bdu, I_bc = 0.0;
+    double rmax = RC(g, g->nr), *cen = NULL;
+    int k, err = 0, maser = 0, npix = nx * ny;
+    long ovs = 0;
+    if (nfr < 2) return 13;
+    passb = 3.33567e-6 * nu0 * width;                  /* line.F:462-469 */
+    nu1 = 0.0 - passb;
+    dnu = 2.0 * passb / (nfr - 1.0);
+    dnu_ch = (double *)malloc(sizeof(double) * nfr);
+    velo = (double *)malloc(sizeof(double) * nfr);
+    for (k = 0; k < nfr; k++) {
+        dnu_ch[k] = nu1 + k * dnu;
+        velo[k] = dnu_ch[k] / m->nu0[il];
+    }
+    Bud = 6.78171833781e46 * m->A[il] / (m->nu0[il] * m->nu0[il] * m->nu0[il]);
+    Bdu = Bud * (m->gu[il] / m->gl[il]);
+    if (m->outer_bc == 2) {
+        double f = m->nu0[il];
+        I_bc = 1.47455253991e-47 * (f * f * f) / (exp(4.7991598e-11 * f / 2.728) - 1.0);
+    }
+    if (starunres) cen = (double *)calloc(nfr, sizeof(double));
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4) reduction(+:ovs) reduction(|:maser)
+#endif
+    for (int iray = 0; iray < npix + (starunres ? 1 : 0); iray++) {
+        orc_traj t;
+        raycache_t cache;
+        line_ctx L;
+        int e, ch, mz = 0, ix = iray / ny, iy = iray % ny;
+        double *row = iray < npix ? image + ((size_t)iy * nx + ix) * nfr : cen;
+        double *trow = iray < npix ? tau + ((size_t)iy * nx + ix) * nfr : NULL;
+        if (iray < npix && !(b[iray] < 0.999 * rmax)) {   /* outside: black, tau 0 */
+            for (ch = 0; ch < nfr; ch++) { row[ch] = 0.0; trow[ch] = 0.0; }
+            continue;
+        }
+        memset(&t, 0, sizeof t);
+        memset(&cache, 0, sizeof cache);
+        e = orc_trajectory(&t, g, m->rstar, rays->x0[iray], rays->z0[iray], rays->theta0, 0.0);
+        if (!e) e = build_cache(m, g, &t, il, 0.0, &cache);
+        if (e) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+            err = e;
+            orc_traj_free(&t);
+            free_cache(&cache);
+            continue;
+        }
+        L.nu0 = m->nu0[il]; L.A = m->A[il]; L.Bud = Bud; L.Bdu = Bdu;
+        for (ch = 0; ch < nfr; ch++) {
+            L.dnu_ch = dnu_ch[ch];
+            L.u_ch = velo[ch];
+            row[ch] = charint(&cache, &L, 0.0, I_bc, &ovs, &mz);
+            if (trow) trow[ch] = L.tau;
+        }
+        if (mz) maser = 1;
+        orc_traj_free(&t);
+        free_cache(&cache);
+    }
+    if (!err && starunres) {                                       /* :2142-2177 */
+        double srat = (3.14159265 * m->rstar * m->rstar) / (4.0 * spx * spy);
+        double srat1 = 1.0 - srat;
+        int ixc = nx / 2 - 1, iyc = ny / 2 - 1, dx, dy;
+        for (k = 0; k < nfr; k++) {
+            double dummy = cen[k] * srat;
+            for (dy = 0; dy < 2; dy++)
+                for (dx = 0; dx < 2; dx++) {
+                    double *p = image + ((size_t)(iyc + dy) * nx + ixc + dx) * nfr + k;
+                    *p = dummy + srat1 * *p;
+                }
+        }
+    }
+    if (maser_out) *maser_out = maser;
+    if (overshoot_out) *overshoot_out = ovs;
+    free(dnu_ch); free(velo); free(cen);
+    return err;
 }

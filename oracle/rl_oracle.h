@@ -86,6 +86,19 @@ int orc_line_spectrum(const orc_model *m, const orc_grid *g,
                       int *maser, long *overshoot, long *n_integrated);
 
 /* Host table builders with the reference's arithmetic (R13). */
+/* Rectangular image rays (setup_rays_rectang, telescope.F:2229-2405):
+ * pixel rays ix outer, iy inner, then the central ray when addstar;
+ * b[] receives each ray's pixel radius (rp_b).  sizes: the whole image. */
+int orc_setup_rays_rect(orc_rays *rays, double *b, double anginf, int nx, int ny,
+                        double size_x, double size_y, double phioff, double xoff,
+                        double yoff, int addstar, double rstar, double *spx_out,
+                        double *spy_out, int *starunres);
+/* Position-velocity cube of line il (make_freq_image_rectang,
+ * telescope.F:2061-2183): image and tau [ny][nx][nfr]. */
+int orc_posvel(const orc_model *m, const orc_grid *g, const orc_rays *rays, const double *b,
+               int nx, int ny, double spx, double spy, int starunres, int il,
+               double width, int nfr, int nthreads, double *image, double *tau,
+               int *maser_out, long *overshoot_out);
 double orc_bplanck(double temp, double nu);
 void orc_line_consts(int nlev, const double *E_cm, int nlines,
                      const int *lev_up, const int *lev_low,

@@ -52,14 +52,23 @@ def write_radlite_inp(path, nphi, dbdr, b_extra, outer_bc=0, incl_deg=45.0):
 
 
 def write_linespectrum_inp(path, vmax, dv, incl_deg, nlines, ilinestart=1,
-                           molfile="moldata.dat", vrad=0.0):
-    """linespectrum.inp, object format 2 (telescope.F:104-238)."""
+                           molfile="moldata.dat", vrad=0.0, posvel=None):
+    """linespectrum.inp, object format 2 (telescope.F:104-238): a line
+    spectrum, or with ``posvel`` (dict nx, ny, size_x, size_y [cm, the whole
+    image], phioff [rad], xoff, yoff [cm], addstar) a position-velocity cube
+    (linespec_command 2, telescope.F:196-224)."""
     with open(os.path.join(path, "linespectrum.inp"), "w") as f:
         f.write("1\n1\n" + "-" * 63 + "\n2\n" + "-" * 63 + "\n")
         f.write(_fmt(vmax) + "\n" + _fmt(dv) + "\n" + "-" * 65 + "\n")
-        f.write("2\n" + molfile + "\n0\n1.0\n")
+        f.write("2\n" + molfile + "\n" + ("2" if posvel else "0") + "\n1.0\n")
         f.write(_fmt(incl_deg) + "\n" + _fmt(vrad) + "\n")
         f.write(f"{nlines}\n{ilinestart}\n")
+        if posvel:
+            pv = posvel
+            f.write(f"{int(pv['nx'])}\n{int(pv['ny'])}\n1\n")
+            for k in ("size_x", "size_y", "phioff", "xoff", "yoff"):
+                f.write(_fmt(pv[k]) + "\n")
+            f.write(f"{int(pv['addstar'])}\n")
 
 
 def write_line_inp(path, nlines, umass_av=2.3, nfreq=40, width=80.0, fix_lw=0.0):
@@ -285,6 +294,11 @@ def read_run_dir(path) -> RunInputs:
     molfile = ls[9]
     incl_deg, vrad = float(ls[12]), float(ls[13])
     nlines_run, istart = int(ls[14]), int(ls[15])
+    extra = {}
+    if int(ls[10]) == 2:                  # position-velocity cube (telescope.F:196-224)
+        extra["posvel"] = dict(nx=int(ls[16]), ny=int(ls[17]), size_x=float(ls[19]),
+                               size_y=float(ls[20]), phioff=float(ls[21]),
+                               xoff=float(ls[22]), yoff=float(ls[23]), addstar=int(ls[24]))
     molname = molfile[:molfile.rfind(".dat")]
     # moldata (line.F:1865-1935), fixed columns
     with open(P(molfile)) as f:
@@ -312,7 +326,7 @@ def read_run_dir(path) -> RunInputs:
         lev_low=np.array(lo), A=np.array(A), molname=molname, nphi=nphi,
         dbdr=dbdr, b_extra=b_extra, outer_bc=outer_bc, vmax=vmax, dv=dv,
         incl_deg=incl_deg, vrad=vrad, nlines_run=nlines_run,
-        iline_start=istart)
+        iline_start=istart, extra=extra)
 
 
 def write_line_spectrum_file(fname, molname, nfr_max, incl_rad, blocks,

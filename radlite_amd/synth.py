@@ -191,12 +191,12 @@ def passband(vmax_target, nfr):
 
 def write_run_dir(path, disk: Disk, lines: LineSet, pops, *, nphi, dbdr,
                   b_extra, vmax, dv, incl_deg, nlines_run=None, outer_bc=0,
-                  fix_lw=0.0, umass_av=2.3, nfreq=40):
+                  fix_lw=0.0, umass_av=2.3, nfreq=40, posvel=None):
     os.makedirs(path, exist_ok=True)
     nl = len(lines.A)
     l4io.write_radlite_inp(path, nphi, dbdr, b_extra, outer_bc=outer_bc)
     l4io.write_linespectrum_inp(path, vmax, dv, incl_deg,
-                                nl if nlines_run is None else nlines_run)
+                                nl if nlines_run is None else nlines_run, posvel=posvel)
     l4io.write_line_inp(path, nl, umass_av=umass_av, fix_lw=fix_lw)
     l4io.write_moldata(path, "synth", lines.weight, lines.E_cm, lines.g,
                        lines.lev_up, lines.lev_low, lines.A)

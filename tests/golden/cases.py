@@ -34,9 +34,29 @@ CASES = {
                sigma_t=0.01, image_stride=7),
 }
 
+# position-velocity cubes (rectangular images, linespec_command 2,
+# telescope.F:196-224): an unresolved star added to the 4 central pixels;
+# pixels inside the inner hole, some of whose rays pass within R*,
+# rotated and offset; an image larger than the grid (corner rays not
+# traced) with the CMB outer boundary
+POSVEL_CASES = {
+    "p1": dict(nr=10, nth=8, nphi=8, b_extra=-2, dbdr=1, nfr=24, vmax=30.0, incl=35.0, seed=6,
+               nlines=3, posvel=dict(nx=16, ny=16, size_x=40.0 * synth.AU,
+                                     size_y=40.0 * synth.AU, phioff=0.0, xoff=0.0, yoff=0.0,
+                                     addstar=1)),
+    "p2": dict(nr=8, nth=6, nphi=8, b_extra=-2, dbdr=1, nfr=12, vmax=30.0, incl=45.0, seed=1,
+               posvel=dict(nx=12, ny=10, size_x=0.12 * synth.AU, size_y=0.1 * synth.AU,
+                           phioff=0.3, xoff=0.002 * synth.AU, yoff=-0.001 * synth.AU,
+                           addstar=0)),
+    "p3": dict(nr=9, nth=7, nphi=10, b_extra=-2, dbdr=3, nfr=16, vmax=35.0, incl=60.0, seed=5,
+               outer_bc=2, posvel=dict(nx=14, ny=14, size_x=230.0 * synth.AU,
+                                       size_y=200.0 * synth.AU, phioff=-0.7,
+                                       xoff=3.0 * synth.AU, yoff=0.0, addstar=0)),
+}
+
 
 def build_case(name, path):
-    c = dict(CASES[name])
+    c = dict(CASES[name] if name in CASES else POSVEL_CASES[name])
     disk = synth.make_disk(c["nr"], c["nth"], seed=c["seed"], sigma_t=c.get("sigma_t", 0.05),
                            t_scale=c.get("t_scale", 1.0))
     if c.get("nlines", 1) == 1:
@@ -48,7 +68,7 @@ def build_case(name, path):
     vmax, dv = synth.passband(c["vmax"], c["nfr"])
     synth.write_run_dir(path, disk, lines, pops, nphi=c["nphi"], dbdr=c["dbdr"],
                         b_extra=c["b_extra"], vmax=vmax, dv=dv, incl_deg=c["incl"],
-                        outer_bc=c.get("outer_bc", 0))
+                        outer_bc=c.get("outer_bc", 0), posvel=c.get("posvel"))
     return c
 
 

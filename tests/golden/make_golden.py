@@ -30,7 +30,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, HERE)
 
 import refprobe  # noqa: E402
-from cases import CASES, build_case, golden_path, input_hash  # noqa: E402
+from cases import CASES, POSVEL_CASES, build_case, golden_path, input_hash  # noqa: E402
 
 PROBE = os.path.join(ROOT, "oracle", "_ref", "RADlite_probe")
 
@@ -41,11 +41,43 @@ def ray_rows(rec):
     return np.concatenate([rec["image"][0, :1, :], rec["image"][1:].reshape(nrr * nphi, -1)])
 
 
+def posvel_case(name, c, d):
+    """Run the reference on a position-velocity case; keep its full-precision
+    cube (probe_posvel.bin) and its own lineposvel files."""
+    r = subprocess.run([PROBE], cwd=d, capture_output=True, text=True, timeout=3600)
+    if r.returncode != 0 or not os.path.exists(os.path.join(d, "radlite.success")):
+        print(r.stdout[-3000:], r.stderr[-3000:])
+        raise SystemExit(f"reference failed on {name}")
+    recs = refprobe.read_probe_posvel(os.path.join(d, "probe_posvel.bin"))
+    out = dict(
+        input_sha256=np.array(input_hash(d)),
+        nu0=np.array([x["nu0"] for x in recs]),
+        dnu=np.array([x["dnu"] for x in recs]),
+        image=np.array([x["image"] for x in recs]),       # [line][iy][ix][ch]
+        tau=np.array([x["tau"] for x in recs]),
+        starunres=np.array([x["starunres"] for x in recs]),
+        overshoot=np.array([x["overshoot"] for x in recs]),
+        maser=np.array([x["maser"] for x in recs]),
+        x0=recs[0]["x0"], z0=recs[0]["z0"], b=recs[0]["b"], theta0=np.array(recs[0]["theta0"]),
+    )
+    np.savez_compressed(golden_path(name), **out)
+    for f in sorted(os.listdir(d)):
+        if f.startswith("lineposvel_"):
+            with open(os.path.join(d, f), "rb") as src, \
+                    open(os.path.join(HERE, f"posvel_{name}_{f[11:]}"), "wb") as dst:
+                dst.write(src.read())
+    print(name, "posvel", len(recs), "lines, rays", len(recs[0]["x0"]), "->",
+          os.path.getsize(golden_path(name)), "bytes")
+
+
 def main(names):
     for name in names:
-        c = CASES[name]
+        c = CASES[name] if name in CASES else POSVEL_CASES[name]
         with tempfile.TemporaryDirectory() as d:
             build_case(name, d)
+            if "posvel" in c:
+                posvel_case(name, c, d)
+                continue
             nrays = 1 + (abs(c["b_extra"]) + 10 + (c["nr"] - 1) * c["dbdr"]) * c["nphi"]
             stride = c.get("image_stride", 1)
             tr_stride = 1 if nrays < 2000 else 41
@@ -97,4 +129,4 @@ def main(names):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or list(CASES))
+    main(sys.argv[1:] or list(CASES) + list(POSVEL_CASES))

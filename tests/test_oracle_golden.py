@@ -77,3 +77,36 @@ def test_trajectories(case):
             np.testing.assert_array_equal(t[key], g["traj_" + key][sl], err_msg=f"{name} {iray} {key}")
         for key in ("icross", "ir", "it"):
             np.testing.assert_array_equal(t[key], g["traj_" + key][sl], err_msg=f"{name} {iray} {key}")
+
+
+# ---- position-velocity cubes (rectangular images) ----------------------
+from cases import POSVEL_CASES  # noqa: E402
+
+
+@pytest.mark.parametrize("name", sorted(POSVEL_CASES))
+def test_posvel_cube(name):
+    """Rectangular-image rays (setup_rays_rectang) and every pixel's
+    intensity and line-of-sight optical depth at every channel
+    (make_freq_image_rectang) against the reference, with the corner rays
+    outside the grid, the rays within R* and the added unresolved star."""
+    g = load_golden(name)
+    with tempfile.TemporaryDirectory() as d:
+        build_case(name, d)
+        assert str(g["input_sha256"]) == input_hash(d), "generated inputs drifted"
+        ri = l4io.read_run_dir(d)
+    orc = pyorc.Oracle(pyorc.Prepared(ri))
+    pv = ri.extra["posvel"]
+    for il in range(len(g["nu0"])):
+        o = orc.posvel(il, pv)
+        np.testing.assert_array_equal(o["b"], g["b"])
+        np.testing.assert_allclose(o["x0"], g["x0"], rtol=1e-14, atol=0)
+        np.testing.assert_allclose(o["z0"], g["z0"], rtol=1e-14, atol=1e-14 * np.abs(g["z0"]).max())
+        assert o["starunres"] == g["starunres"][il]
+        assert o["overshoot"] == g["overshoot"][il]
+        r = g["image"][il]
+        rel = np.abs(o["image"] - r) / np.maximum(np.abs(r), 1e-12 * np.abs(r).max())
+        assert rel.max() <= 1e-13, (il, rel.max())
+        t = g["tau"][il]
+        assert np.all((t == 0) == (o["tau"] == 0))
+        trel = np.abs(o["tau"] - t) / np.maximum(np.abs(t), 1e-300)
+        assert trel.max() <= 1e-13, (il, trel.max())
