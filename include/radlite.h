@@ -116,6 +116,30 @@ int rl_trace_lines_forward_ex(rl_ctx *ctx, int nitems, const rl_item *items,
                               double *flux, double *image, unsigned char *cmask,
                               unsigned *warn_bits);
 
+/* Rectangular-image ray sets for position-velocity cubes, one per
+ * inclination (setup_rays_rectang, telescope.F:2229-2468; linespec_command
+ * 2 reads nrx, nry, the image size in cm, phioff, xoff, yoff and addstar,
+ * telescope.F:195-256).  nx, ny even; an inclination below 0.1 rad is
+ * taken as 0.1 (telescope.F:2294-2303); addstar > 0 adds the star when it
+ * is smaller than a pixel (then the image must be centred on it).
+ * n_rays (optional): per inclination the rays (nx*ny, +1 with addstar).
+ * Replaces any circular ray sets. */
+int rl_setup_rays_rectang(rl_ctx *ctx, int nincl, const double *incl_rad, int nx,
+                          int ny, double size_x_cm, double size_y_cm, double phioff,
+                          double xoff_cm, double yoff_cm, int addstar, int *n_rays);
+
+/* Position-velocity cubes of (line, inclination) items on rectangular
+ * ray sets (calc_write_line_posvel, telescope.F:1828-2031, with
+ * make_freq_image_rectang, telescope.F:2061-2215): image and tau
+ * (optional) receive per item [ny][nx][nfr] intensities
+ * [erg/s/cm^2/Hz/ster] and optical depths; every channel is integrated
+ * (skip_mode is ignored); pixels outside 0.999 of the grid's outer radius
+ * are 0 (tau 0); an unresolved star is added to the four central pixels.
+ * The reference sets no stellar spectrum on this path, so rays that end
+ * on the star start from zero intensity there. */
+int rl_trace_lines_posvel(rl_ctx *ctx, int nitems, const rl_item *items,
+                          double *image, double *tau, unsigned *warn_bits);
+
 /* Adjoint of the trace for the same items (every item must use
  * RL_SKIP_NONE).  Given dL/dflux (same layout as flux), returns dL/d(the
  * inputs the trace reads) and, in flux (optional), the spectrum.  When

@@ -1161,7 +1161,7 @@ int orc_ray_nodes(const orc_model *m, const orc_grid *g, const orc_rays *rays, i
 }
 
 /* ------------------------------------------------------------------ */
-/* Rectangular images: setup_rays_rectang (telescope.F:2229-2405) and   */
+/* Rectangular images: setup_rays_rectang (telescope.F:2229-2468) and   */
 /* make_freq_image_rectang (telescope.F:2061-2183) as                   */
 /* calc_write_line_posvel (telescope.F:1828-2031) uses them.            */
 /* ------------------------------------------------------------------ */
@@ -1178,7 +1178,7 @@ int orc_setup_rays_rect(orc_rays *rays, double *b, double anginf, int nx, int ny
     spx = szx / (nx * 0.5);
     spy = szy / (ny * 0.5);
     if (spx <= 0.0 || spy <= 0.0) return 13;
-    if (anginf < 1.0e-1) anginf = 0.1;                             /* :2294-2303 */
+    if (anginf < 1.0e-1) anginf = (double)0.1f;     /* :2294-2303, a REAL literal */
     theta0 = anginf + 1.0e-4;
     sinth0 = sin(theta0);
     n = nx * ny + (addstar > 0 ? 1 : 0);
@@ -1295,7 +1295,7 @@ int orc_posvel(const orc_model *m, const orc_grid *g, const orc_rays *rays, cons
         orc_traj_free(&t);
         free_cache(&cache);
     }
-    if (!err && starunres) {                                       /* :2142-2177 */
+    if (!err && starunres) {                                       /* :2153-2200 */
         double srat = (3.14159265 * m->rstar * m->rstar) / (4.0 * spx * spy);
         double srat1 = 1.0 - srat;
         int ixc = nx / 2 - 1, iyc = ny / 2 - 1, dx, dy;

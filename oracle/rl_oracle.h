@@ -86,7 +86,7 @@ int orc_line_spectrum(const orc_model *m, const orc_grid *g,
                       int *maser, long *overshoot, long *n_integrated);
 
 /* Host table builders with the reference's arithmetic (R13). */
-/* Rectangular image rays (setup_rays_rectang, telescope.F:2229-2405):
+/* Rectangular image rays (setup_rays_rectang, telescope.F:2229-2468):
  * pixel rays ix outer, iy inner, then the central ray when addstar;
  * b[] receives each ray's pixel radius (rp_b).  sizes: the whole image. */
 int orc_setup_rays_rect(orc_rays *rays, double *b, double anginf, int nx, int ny,

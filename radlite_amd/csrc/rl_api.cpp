@@ -70,6 +70,15 @@ struct RaySet {
     std::vector<long long> off;
     DevBuf d_off, d_nodes;
     DevBuf d_ringw;
+    // rectangular image (position-velocity cube) ray set: pixel rays ix
+    // outer, iy inner, then the added-star ray (central, or -1)
+    bool rect = false;
+    int nx = 0, ny = 0, starunres = 0, central = -1;
+    double spx = 0.0, spy = 0.0;
+    std::vector<double> bpix;
+    DevBuf d_bpix;
+    std::vector<int> star_rays;        // traced rays with impact parameter <= R*
+    DevBuf d_star_rays;
     // node -> cell stencil transpose for the adjoint (built on first use)
     bool st_ready = false;
     DevBuf d_st_vals, d_st_w, d_st_off;
@@ -482,6 +491,96 @@ extern "C" int rl_setup_rays_circular(rl_ctx *c, int nincl, const double *incl, 
     return 0;
 }
 
+// setup_rays_rectang (telescope.F:2229-2468) for linespec_command 2: the
+// pixel rays of an nx x ny image of the given size (cm), rotated by phioff
+// and offset, ix outer and iy inner, then (addstar) the central ray for
+// the added unresolved star
+static int build_rayset_rect(rl_ctx *c, RaySet &rs, double anginf, int nx, int ny, double size_x,
+                             double size_y, double phioff, double xoff, double yoff,
+                             int addstar) {
+    if (nx < 2 || ny < 2 || nx % 2 || ny % 2) return fail(c, 13, "nrx and nry must be even");
+    const double szx = size_x * 0.5, szy = size_y * 0.5;             // telescope.F:209-212
+    const double spx = szx / (nx * 0.5), spy = szy / (ny * 0.5);
+    if (!(spx > 0.0) || !(spy > 0.0)) return fail(c, 13, "pixel size must be > 0");
+    if (anginf < 1.0e-1) anginf = (double)0.1f;                      // :2294-2303
+    const double theta0 = anginf + 1.0e-4;
+    const double sinth0 = std::sin(theta0);
+    rs = RaySet();
+    rs.rect = true;
+    rs.theta0 = theta0;
+    rs.costh0 = std::cos(theta0);
+    rs.sinth0 = sinth0;
+    rs.nx = nx; rs.ny = ny; rs.spx = spx; rs.spy = spy;
+    const int n = nx * ny + (addstar > 0 ? 1 : 0);
+    rs.nrays = n;
+    rs.x0.assign(n, 0.0);
+    rs.z0.assign(n, 0.0);
+    rs.bpix.assign(n, 0.0);
+    const int nxh = nx / 2, nyh = ny / 2;
+    int ir = 0;
+    for (int ix = 1; ix <= nx; ix++) {
+        for (int iy = 1; iy <= ny; iy++) {
+            double x_c = (ix - nxh - 0.5) * spx - xoff;
+            const double y_c = (iy - nyh - 0.5) * spy - yoff;
+            const double r_c = std::sqrt(x_c * x_c + y_c * y_c);
+            if (x_c == 0.0) x_c = x_c + 0.001 * spx;
+            double p_c = std::atan(y_c / x_c) - phioff;
+            if (x_c < 0.0) p_c = p_c + 3.14159265359;
+            while (p_c < 0.0) p_c = p_c + 6.28318530718;
+            while (p_c >= 6.28318530718) p_c = p_c - 6.28318530718;
+            const double zh0 = -std::sin(p_c) / sinth0;
+            const double zh02 = zh0 * zh0;
+            double dum = 1.0 - zh02 * sinth0 * sinth0;
+            dum = dum + 1e-4;
+            if (dum < 0.0) return fail(c, 13, "setup_rays_rectang: ray direction");
+            const double xh0 = std::cos(p_c) > 0.0 ? std::sqrt(dum) : -std::sqrt(dum);
+            rs.x0[ir] = r_c * xh0;
+            rs.z0[ir] = r_c * zh0;
+            rs.bpix[ir] = r_c;
+            ir++;
+        }
+    }
+    if (addstar > 0) {                                               // :2349-2400
+        if (xoff != 0.0 || yoff != 0.0)
+            return fail(c, 13, "the added star needs a star-centred image (offsets 0)");
+        rs.central = ir;
+        if (spx * spx + spy * spy > c->rstar * c->rstar) rs.starunres = 1;
+    }
+    // rays the imager traces whose impact parameter is within the star
+    const double rmax = c->rc[c->nr + 1], costh02 = rs.costh0 * rs.costh0;
+    for (int i = 0; i < n; i++) {
+        const double bimpact = std::sqrt(rs.x0[i] * rs.x0[i] + rs.z0[i] * rs.z0[i] * (1.0 - costh02));
+        if (rs.bpix[i] < 0.999 * rmax && bimpact <= c->rstar) rs.star_rays.push_back(i);
+    }
+    if (int e = upload(c, rs.d_x0, rs.x0.data(), (size_t)n * 8)) return e;
+    if (int e = upload(c, rs.d_z0, rs.z0.data(), (size_t)n * 8)) return e;
+    if (int e = upload(c, rs.d_bpix, rs.bpix.data(), (size_t)n * 8)) return e;
+    if (!rs.star_rays.empty())
+        if (int e = upload(c, rs.d_star_rays, rs.star_rays.data(), rs.star_rays.size() * 4))
+            return e;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int rl_setup_rays_rectang(rl_ctx *c, int nincl, const double *incl, int nx, int ny,
+                                     double size_x, double size_y, double phioff, double xoff,
+                                     double yoff, int addstar, int *n_rays) {
+    if (c) c->gen++;                         // invalidates a tape
+    if (!c || !c->nr || nincl < 1 || !incl)
+        return fail(c, RL_E_ARG, "rl_setup_rays_rectang: set the grid first");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->incl.clear();
+    c->x_owner.assign(c->x_owner.size(), -1);
+    c->incl.resize(nincl);
+    for (int i = 0; i < nincl; i++) {
+        if (int e = build_rayset_rect(c, c->incl[i], incl[i], nx, ny, size_x, size_y, phioff,
+                                      xoff, yoff, addstar))
+            return e;
+        if (n_rays) n_rays[i] = c->incl[i].nrays;
+    }
+    return 0;
+}
+
 // ---------------------------------------------------------------------
 // trace
 // ---------------------------------------------------------------------
@@ -604,7 +703,9 @@ static int cross_sets(rl_ctx *c, const std::vector<int> &idxs) {
         HIPCHK(c, cnt[k].ensure((size_t)n * 4));
         rl::CrossSet &q = sets.s[k];
         q.nrays = n;
-        q.central_blend = rs.ring_ri[1] > 0.0 ? 1 : 0;
+        q.central_blend = !rs.rect && rs.ring_ri[1] > 0.0 ? 1 : 0;
+        q.bpix = rs.rect ? rs.d_bpix.as<double>() : nullptr;
+        q.bmax = 0.999 * c->rc[c->nr + 1];               // telescope.F:2116-2118
         q.x0 = rs.d_x0.as<double>();
         q.z0 = rs.d_z0.as<double>();
         q.theta0 = rs.theta0; q.costh0 = rs.costh0; q.sinth0 = rs.sinth0;
@@ -685,6 +786,10 @@ static int build_traj_keep(rl_ctx *c, int idx, const std::vector<int> &keep) {
                    rs.d_off.as<long long>(), rs.d_x0.as<double>(), rs.d_z0.as<double>(),
                    rs.costh0, rs.sinth0, cs, ccode, rs.d_nodes.as<GeoNode>(),
                    c->d_err.as<int>());
+    HIPCHK(c, hipGetLastError());
+    if (rs.rect && !rs.star_rays.empty())
+        rl::launch_star_flags(c->stream, (int)rs.star_rays.size(), rs.d_star_rays.as<int>(),
+                              rs.d_off.as<long long>(), rs.d_nodes.as<GeoNode>(), c->nh);
     HIPCHK(c, hipGetLastError());
     if (int e = check_err(c)) return e;
     rs.traj_ready = true;
@@ -798,7 +903,10 @@ static int setup_item(rl_ctx *c, const rl_item &it, int ii, ItemSetup &out, hipE
         double dn = nu1 + k * dnu;
         chan[k] = dn;
         chan[nfr + k] = dn / nu0s;
-        chan[2 * nfr + k] = star_at(c, nu0 + dn);
+        // the pos-vel path of the reference leaves the star's passband
+        // spectrum unset (calc_write_line_posvel calls no
+        // line_init_passband_starbc, cf. telescope.F:1557): 0 there
+        chan[2 * nfr + k] = rs.rect ? 0.0 : star_at(c, nu0 + dn);
     }
     if (int e = upload(c, c->d_chan, chan.data(), chan.size() * 8)) return e;
     rl::LineConst &lc = out.lc;
@@ -808,7 +916,8 @@ static int setup_item(rl_ctx *c, const rl_item &it, int ii, ItemSetup &out, hipE
     lc.A = c->A[it.line];
     lc.Bud = 6.78171833781e46 * lc.A / (nu0s * nu0s * nu0s);      // line.F:1778
     lc.Bdu = lc.Bud * (c->gu[it.line] / c->gl[it.line]);
-    {
+    lc.starfract = 0.0;
+    if (!rs.rect) {
         double q = c->rstar / rs.ring_ri[1];
         lc.starfract = q * q;
         if (rs.ring_ri[1] < c->rstar) return fail(c, 124, "central beam resolves the star");
@@ -1159,6 +1268,9 @@ extern "C" int rl_trace_lines_forward_ex(rl_ctx *c, int nitems, const rl_item *i
             it.incl >= (int)c->incl.size() || it.nfr < 2 || !(it.vmax_kms > 0.0) ||
             (it.skip_mode != RL_SKIP_REFERENCE && it.skip_mode != RL_SKIP_NONE))
             return fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", ii);
+        if (c->incl[it.incl].rect)
+            return fail(c, 13, "item %d: a rectangular-image ray set: use rl_trace_lines_posvel",
+                        ii);
     }
     // node counts of every inclination used: an inclination whose nodes
     // surely fit (at the per-ray maximum, within half the budget beside what
@@ -1234,6 +1346,118 @@ extern "C" int rl_trace_lines_forward_ex(rl_ctx *c, int nitems, const rl_item *i
             fo += items[k].nfr;
             io += (size_t)c->incl[items[k].incl].nrays * items[k].nfr;
         }
+    }
+    return 0;
+}
+
+// Position-velocity cubes (calc_write_line_posvel, telescope.F:1828-2031;
+// make_freq_image_rectang, telescope.F:2061-2215): per item, the
+// intensity and the optical depth of every pixel ray at every channel,
+// every channel integrated (no NONREDUNDANT skipping: the rectangular
+// imager has none), pixels outside 0.999 R_max black with tau 0, and the
+// unresolved star added to the four central pixels.  image / tau are
+// [item][iy][ix][channel].
+extern "C" int rl_trace_lines_posvel(rl_ctx *c, int nitems, const rl_item *items, double *image,
+                                     double *tau, unsigned *warn) {
+    if (!c || nitems < 0 || (nitems && (!items || !image)))
+        return fail(c, RL_E_ARG, "rl_trace_lines_posvel: bad args");
+    if (!c->have_medium || !c->nlev || !c->nlines || !c->have_dust || c->incl.empty())
+        return fail(c, 13, "inputs incomplete: grid, medium, levels, lines, dust, rays");
+    for (int ii = 0; ii < nitems; ii++) {
+        const rl_item &it = items[ii];
+        if (it.incl < 0 || it.incl >= (int)c->incl.size() || !c->incl[it.incl].rect)
+            return fail(c, 13, "item %d: not a rectangular-image ray set "
+                               "(rl_setup_rays_rectang)", ii);
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->d_err.ensure(4));
+    for (int i = 0; i < 5; i++) c->t_ms[i] = 0.0;
+    for (int i = 0; i < 4; i++) c->counts[i] = 0;
+    release_adjoint_work(c);
+    const bool fp64 = c->precision == RL_FP64;
+    const size_t rsz = fp64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::FastRec);
+    size_t io = 0;
+    std::vector<double> hi, ht;
+    for (int ii = 0; ii < nitems; ii++) {
+        rl_item it = items[ii];
+        it.skip_mode = RL_SKIP_NONE;
+        ItemSetup s;
+        if (int e = setup_item(c, it, ii, s, c->ev[0], c->ev[1])) return e;
+        release_cross_scratch(c);
+        RaySet &rs = *s.rs;
+        const int nfr = it.nfr, npix = rs.nx * rs.ny;
+        const size_t ncv = (size_t)rs.nrays * nfr;
+        HIPCHK(c, c->d_rec.ensure((size_t)rs.nnodes * rsz));
+        HIPCHK(c, c->d_rinfo.ensure((size_t)rs.nrays * sizeof(RayInfo)));
+        HIPCHK(c, c->d_image.ensure(ncv * 8));
+        HIPCHK(c, c->d_partial.ensure(ncv * 8));
+        HIPCHK(c, c->d_ovs.ensure(16));
+        unsigned long long *ovs = c->d_ovs.as<unsigned long long>();
+        rl::launch_zero_u64(c->stream, ovs, 2);
+        rl::PosvelLaunch L;
+        L.nrays = rs.nrays; L.nfr = nfr;
+        L.off = rs.d_off.as<long long>();
+        L.geo = rs.d_nodes.as<GeoNode>();
+        L.cells = c->d_cells.as<CellRec>();
+        L.rinfo = c->d_rinfo.as<RayInfo>();
+        L.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
+        L.rec = c->d_rec.p;
+        L.dnu = c->d_chan.as<double>();
+        L.velo = L.dnu + nfr;
+        L.istar = L.dnu + 2 * nfr;
+        L.lc = s.lc;
+        L.image = c->d_image.as<double>();
+        L.tau = c->d_partial.as<double>();
+        L.ovs = ovs;
+        L.maser = reinterpret_cast<int *>(ovs + 1);
+        if (fp64) rl::launch_posvel<double>(c->stream, L);
+        else rl::launch_posvel<float>(c->stream, L);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+        hi.resize(ncv);
+        ht.resize(ncv);
+        unsigned long long flags[2] = {0, 0};
+        HIPCHK(c, hipMemcpyAsync(hi.data(), L.image, ncv * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(ht.data(), L.tau, ncv * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(flags, ovs, 16, hipMemcpyDeviceToHost, c->stream));
+        if (int e = check_err(c)) return e;             // synchronises the stream
+        // pixel rays are ix-major on the device; the cube is iy-major
+        double *oi = image + io, *ot = tau ? tau + io : nullptr;
+        for (int ix = 0; ix < rs.nx; ix++)
+            for (int iy = 0; iy < rs.ny; iy++) {
+                const double *si = &hi[((size_t)ix * rs.ny + iy) * nfr];
+                const double *st = &ht[((size_t)ix * rs.ny + iy) * nfr];
+                const size_t o = ((size_t)iy * rs.nx + ix) * nfr;
+                std::copy(si, si + nfr, oi + o);
+                if (ot) std::copy(st, st + nfr, ot + o);
+            }
+        if (rs.starunres) {                               // telescope.F:2153-2200
+            const double srat = (3.14159265 * c->rstar * c->rstar) / (4.0 * rs.spx * rs.spy);
+            const double srat1 = 1.0 - srat;
+            const double *cen = &hi[(size_t)rs.central * nfr];
+            const int ixc = rs.nx / 2 - 1, iyc = rs.ny / 2 - 1;
+            for (int k = 0; k < nfr; k++) {
+                const double dummy = cen[k] * srat;
+                for (int dy = 0; dy < 2; dy++)
+                    for (int dx = 0; dx < 2; dx++) {
+                        double &p = oi[((size_t)(iyc + dy) * rs.nx + ixc + dx) * nfr + k];
+                        p = dummy + srat1 * p;
+                    }
+            }
+        }
+        if (warn)
+            warn[ii] = ((int)flags[1] ? RL_WARN_MASER : 0u) | (flags[0] ? RL_WARN_OVERSHOOT : 0u);
+        float t01 = 0, t14 = 0;
+        (void)hipEventElapsedTime(&t01, c->ev[0], c->ev[1]);
+        (void)hipEventElapsedTime(&t14, c->ev[1], c->ev[4]);
+        c->t_ms[0] += t01;
+        c->t_ms[2] += t14;
+        c->t_ms[4] += t01 + t14;
+        c->counts[0] += rs.nrays;
+        c->counts[1] += rs.nnodes;
+        c->counts[2] = (long long)flags[0];
+        c->counts[3] += (long long)ncv;
+        io += (size_t)npix * nfr;
     }
     return 0;
 }
@@ -1501,10 +1725,14 @@ static int adj_reverse_chunk(rl_ctx *c, const rl_item &it, const ItemSetup &su, 
 static int adj_check(rl_ctx *c, int nitems, const rl_item *items, const char *who) {
     if (!c->have_medium || !c->nlev || !c->nlines || !c->have_dust || c->incl.empty())
         return fail(c, 13, "inputs incomplete: grid, medium, levels, lines, dust, rays");
-    for (int ii = 0; ii < nitems; ii++)
+    for (int ii = 0; ii < nitems; ii++) {
         if (items[ii].skip_mode != RL_SKIP_NONE)
             return fail(c, 13, "%s: item %d: the adjoint integrates every channel (RL_SKIP_NONE)",
                         who, ii);
+        const int in = items[ii].incl;
+        if (in >= 0 && in < (int)c->incl.size() && c->incl[in].rect)
+            return fail(c, 13, "%s: item %d: the adjoint is for circular images", who, ii);
+    }
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, c->d_err.ensure(4));
     HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));

@@ -21,6 +21,9 @@ constexpr int EXMAX = 40;     // extra points per ray: at most 1+16+1+16
 constexpr double LEVTHRES = 1.0e-3;                 // configure.h:52
 constexpr int NF_RESET = 1;    // hole chord: ds := 0, init := 1 (telescope.F:4095-4104)
 constexpr int NF_BLEND = 2;    // central-beam star blend here (telescope.F:4144-4193)
+constexpr int NF_STAR = 4;     // I := I_star here: a rectangular-image ray (rbeam0 = 0)
+                               // within R*, node and its predecessor in radial cell 1
+                               // (telescope.F:4194-4208)
 
 // Error codes are the reference's stop codes.
 enum : int { E_OK = 0, E_INPUT = 13, E_STAR = 124, E_DSNEG = 749,
@@ -74,7 +77,7 @@ struct LineK {            // per (line, channel) constants
 };
 
 template <typename T>
-struct Carry { T phi0, srcl0, alpl0, S0; int init; };
+struct Carry { T phi0, srcl0, alpl0, S0; int init; T tau = T(0); };   // tau: char_tau
 
 template <typename T>
 struct Pt { T srcd, alpd, u, nu, nl; };     // one integration point
@@ -141,6 +144,7 @@ __device__ inline void integ_elem(const LineK<T> &L, T &I, T ds,
     T alp0 = p0.alpd + c.alpl0, alp1 = p1.alpd + alpl1;
     const bool have = c.init == 0;
     I = qdr<T>(I, src0, alp0, src1, alp1, ds, ovs, c.S0, have);
+    c.tau = c.tau + T(0.5) * (alp0 + alp1) * ds;               // line.F:4598
     c.phi0 = phi1;
     c.srcl0 = srcl1;
     c.alpl0 = alpl1;
@@ -311,7 +315,7 @@ using FastLine = FastLineT<float>;
 template <typename T> struct FPtT { T u, sig, jd, ad, jc, dc; };   // one point, scaled values
 using FPt = FPtT<float>;
 
-template <typename T> struct FastCarryT { T jl0, al0, S0; int init; };  // line j, alpha / sig at the start
+template <typename T> struct FastCarryT { T jl0, al0, S0; int init; T tau = T(0); };  // line j, alpha / sig at the start; optical depth
 
 // integrate_element_linedust (line.F:4515-4624) on scaled values
 template <typename T>
@@ -333,6 +337,7 @@ __device__ inline void elem_fast(const FastLineT<T> &L, T &I, T ds, const FPtT<T
     const T hds = T(0.5) * ds;
     T dtau = O::fma(p1.sig, ah1, p0.sig * ah0) * hds;
     T theomax = O::fma(p1.sig, jh1, p0.sig * jh0) * hds;
+    c.tau += dtau;
     T wa, wb, xp, ra;
     qdr_weights_f<T>(dtau, wa, wb, xp, ra);
     // source functions (transfer.F:1528-1541) from the scaled values; the
@@ -411,6 +416,7 @@ __device__ inline void segment_fast(const FastRecT<T> &b, const FPtT<T> &p0, con
     const int fl = (int)b.flag;
     if (fl & NF_RESET) { ds = T(0); c.init = 1; }
     if (fl & NF_BLEND) { I = (T(1) - starfract) * I + starfract * istar; c.init = 1; }
+    if (fl & NF_STAR) { I = istar; c.init = 1; }
     SubGrid<T> sg;
     if (subgrid_of<T>(b, ds, p0.u, L.uch, sg)) {
         FPtT<T> pp = p0;
@@ -434,7 +440,8 @@ __device__ inline void segment_fast(const FastRecT<T> &b, const FPtT<T> &p0, con
 // charintline (telescope.F:3889-4312) on FastRecT data
 template <typename T>
 __device__ inline T trace_fast(const FastRecT<T> *__restrict__ rec, int n, const FastLineT<T> &L,
-                               T I0, T istar, T starfract, int &ovs, int &maser) {
+                               T I0, T istar, T starfract, int &ovs, int &maser,
+                               T *tau = nullptr) {
     T I = I0;
     FastCarryT<T> c{T(0), T(0), T(0), 1};
     FPtT<T> p0 = point_of<T>(rec[0]);
@@ -446,6 +453,7 @@ __device__ inline T trace_fast(const FastRecT<T> *__restrict__ rec, int n, const
         if (b.sig * c.al0 * ds < T(-0.01f)) maser = 1;
         p0 = p1;
     }
+    if (tau) *tau = c.tau;
     return I;
 }
 

@@ -144,6 +144,8 @@ struct AdjLaunch {
 struct CrossSet {
     int nrays, central_blend;
     const double *x0, *z0;
+    const double *bpix;        // rectangular images: pixel radius per ray, or null
+    double bmax;               // rays with bpix >= bmax are not traced (0 nodes)
     double theta0, costh0, sinth0;
     double *s1, *s2;
     int *iy;
@@ -159,6 +161,29 @@ struct CrossSets {
 
 void launch_cross(hipStream_t st, const GridDev &g, const double *tan2, const CrossSets &sets,
                   int nmax, int *err);
+// NF_STAR on the nodes of the listed rays (impact parameter within R*)
+// that follow a node of radial cell 1 and are in it themselves
+void launch_star_flags(hipStream_t st, int nlisted, const int *rays, const long long *off,
+                       GeoNode *nodes, int nh);
+
+// Position-velocity cube of one item: every (ray, channel) integrated,
+// intensity and line-of-sight optical depth; rays without nodes (outside
+// the grid) give 0 and 0.
+struct PosvelLaunch {
+    int nrays, nfr;
+    const long long *off;
+    const GeoNode *geo;
+    const CellRec *cells;
+    RayInfo *rinfo;
+    double aks2;
+    void *rec;                         // NodeRec<double> (FP64) or FastRec (FP32)
+    const double *dnu, *velo, *istar;  // per channel
+    LineConst lc;
+    double *image, *tau;               // [ray][channel]
+    unsigned long long *ovs;
+    int *maser;
+};
+template <typename T> void launch_posvel(hipStream_t st, const PosvelLaunch &a);
 void launch_geo(hipStream_t st, const GridDev &g, const double *vel, int nh, int nrays,
                 int nmax, const long long *off, const double *x0, const double *z0,
                 double costh0, double sinth0, const double *cs, const int *ccode,

@@ -62,6 +62,10 @@ k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, i
     int *__restrict__ ccode = q.ccode;
     int ray = blockIdx.x * blockDim.x + threadIdx.x;
     if (ray >= nrays) return;
+    if (q.bpix && !(q.bpix[ray] < q.bmax)) {     // telescope.F:2118: outside, not traced
+        count[ray] = 0;
+        return;
+    }
     Traj t(g, tan2, x0[ray], z0[ray], theta0, costh0, sinth0, scr_s1 + ray, scr_s2 + ray,
            scr_iy + ray, nrays);
     const long long base = (long long)ray * nmax;
@@ -391,7 +395,8 @@ k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__rest
 // ---------------------------------------------------------------------
 template <typename T>
 __device__ inline T trace_one(const NodeRec<T> *__restrict__ rec, int n, const LineK<T> &L,
-                              T I0, T istar, T starfract, int &ovs, int &maser) {
+                              T I0, T istar, T starfract, int &ovs, int &maser,
+                              T *tau = nullptr) {
     T I = I0;
     Carry<T> c;
     c.init = 1; c.phi0 = T(0); c.srcl0 = T(0); c.alpl0 = T(0); c.S0 = T(0);
@@ -407,12 +412,17 @@ __device__ inline T trace_one(const NodeRec<T> *__restrict__ rec, int n, const L
             I = (T(1) - starfract) * I + starfract * istar;
             c.init = 1;
         }
+        if (fl & NF_STAR) {
+            I = istar;
+            c.init = 1;
+        }
         Pt<T> pb{b.jd, b.ad, b.u, b.nu, b.nl};
         clever_elem<T>(L, I, ds, pa, pb, lwa, b.lw, c, ovs);
         if (c.alpl0 * ds < T(-0.01)) maser = 1;
         pa = pb;
         lwa = b.lw;
     }
+    if (tau) *tau = c.tau;
     return I;
 }
 
@@ -569,6 +579,99 @@ __global__ void k_flux_final(int nfr, int nchunks, const double *__restrict__ pa
     for (int c = 0; c < nchunks; c++) s = s + partial[(size_t)c * nfr + ch];
     flux[ch] = s / d2;
 }
+
+// ---------------------------------------------------------------------
+// rectangular images (position-velocity cubes)
+// ---------------------------------------------------------------------
+// a ray of the rectangular imager whose impact parameter is within R*
+// takes the stellar intensity at every node that follows a node of the
+// innermost radial cell and is itself in it (charintline with rbeam0 = 0,
+// telescope.F:4194-4208); the radial index of a node is that of its first
+// stencil cell
+__global__ void k_star_flags(int nlisted, const int *__restrict__ rays,
+                             const long long *__restrict__ off, GeoNode *__restrict__ nodes,
+                             int nh) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlisted) return;
+    const int ray = rays[i];
+    const long long b = off[ray], e = off[ray + 1];
+    int ir_prev = b < e ? nodes[b].c00 / nh + 1 : 0;
+    for (long long k = b + 1; k < e; k++) {
+        const int ir = nodes[k].c00 / nh + 1;
+        if (ir == 1 && ir_prev == 1) nodes[k].flag |= NF_STAR << 2;
+        ir_prev = ir;
+    }
+}
+
+// one lane per (ray, channel): the intensity and char_tau of
+// make_freq_image_rectang (telescope.F:2117-2146)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_formal_posvel(int nrays, int nfr, const long long *__restrict__ off,
+                const typename RecOf<T>::type *__restrict__ rec,
+                const double *__restrict__ dnu, const double *__restrict__ velo,
+                const double *__restrict__ istar, LineConst lc, double *__restrict__ image,
+                double *__restrict__ tau, unsigned long long *ovs_out, int *maser_out) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int ovs = 0, maser = 0;
+    if (g < (long long)nrays * nfr) {
+        const int ray = (int)(g / nfr), ch = (int)(g - (long long)ray * nfr);
+        const long long b = off[ray];
+        const int n = (int)(off[ray + 1] - b);
+        double I = 0.0, t = 0.0;
+        if (n > 0) {
+            if constexpr (std::is_same<T, double>::value) {
+                LineK<double> L;
+                L.nu0 = lc.nu0; L.c_aa = lc.c_aa; L.hknu = lc.hknu;
+                L.A = lc.A; L.Bud = lc.Bud; L.Bdu = lc.Bdu;
+                L.dnu = dnu[ch]; L.uch = velo[ch];
+                I = trace_one<double>(rec + b, n, L, lc.I0, istar[ch], 0.0, ovs, maser, &t);
+            } else {
+                FastLine L{(float)lc.nu0, (float)dnu[ch], (float)velo[ch]};
+                float tf = 0.0f;
+                I = (double)trace_fast<float>(rec + b, n, L, (float)lc.I0, (float)istar[ch],
+                                              0.0f, ovs, maser, &tf);
+                t = (double)tf;
+            }
+        }
+        image[g] = I;
+        tau[g] = t;
+    }
+    // wave totals of the warnings
+    for (int m = 32; m >= 1; m >>= 1) {
+        ovs += __shfl_xor(ovs, m);
+        maser |= __shfl_xor(maser, m);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (ovs) atomicAdd(ovs_out, (unsigned long long)ovs);
+        if (maser) atomicOr(maser_out, 1);
+    }
+}
+
+void launch_star_flags(hipStream_t st, int nlisted, const int *rays, const long long *off,
+                       GeoNode *nodes, int nh) {
+    if (nlisted <= 0) return;
+    hipLaunchKernelGGL(k_star_flags, dim3((nlisted + 63) / 64), dim3(64), 0, st, nlisted, rays,
+                       off, nodes, nh);
+}
+
+template <typename T>
+void launch_posvel(hipStream_t st, const PosvelLaunch &a) {
+    auto *rec = reinterpret_cast<typename RecOf<T>::type *>(a.rec);
+    if constexpr (std::is_same<T, double>::value)
+        hipLaunchKernelGGL(k_nodes, dim3(a.nrays), dim3(64), 0, st, a.nrays, a.off, a.geo,
+                           a.cells, rec, a.rinfo, a.nfr, a.velo, a.aks2, 1);
+    else
+        hipLaunchKernelGGL(k_nodes_fast<float>, dim3(a.nrays), dim3(64), 0, st, a.nrays, a.off,
+                           a.geo, a.cells, rec, 0ll, a.rinfo, a.nfr, a.velo, a.aks2, 1, a.lc);
+    const long long total = (long long)a.nrays * a.nfr;
+    hipLaunchKernelGGL(k_formal_posvel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       st, a.nrays, a.nfr, a.off,
+                       reinterpret_cast<const typename RecOf<T>::type *>(a.rec), a.dnu, a.velo,
+                       a.istar, a.lc, a.image, a.tau, a.ovs, a.maser);
+}
+template void launch_posvel<double>(hipStream_t, const PosvelLaunch &);
+template void launch_posvel<float>(hipStream_t, const PosvelLaunch &);
 
 // ---------------------------------------------------------------------
 // launchers

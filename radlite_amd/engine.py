@@ -67,8 +67,13 @@ def load_library(path=LIB_PATH):
         "rl_set_outer_bc": (C.c_int, [vp, C.c_int]),
         "rl_setup_rays_circular": (C.c_int, [vp, C.c_int, _dp, C.c_int, C.c_int, C.c_int,
                                              _ip, _ip, _dp, _dp]),
+        "rl_setup_rays_rectang": (C.c_int, [vp, C.c_int, _dp, C.c_int, C.c_int, C.c_double,
+                                            C.c_double, C.c_double, C.c_double, C.c_double,
+                                            C.c_int, _ip]),
         "rl_trace_lines_forward": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
                                              C.POINTER(C.c_uint)]),
+        "rl_trace_lines_posvel": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
+                                            C.POINTER(C.c_uint)]),
         "rl_trace_lines_forward_ex": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
                                                 C.POINTER(C.c_ubyte), C.POINTER(C.c_uint)]),
         "rl_trace_lines_adjoint": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
@@ -281,6 +286,48 @@ class Raytracer:
         self.ring_r, self.ring_edge = rr, re
         return self.nrays
 
+    def setup_rays_rectang(self, incl_rad, nx, ny, size_x, size_y, phioff=0.0, xoff=0.0,
+                           yoff=0.0, addstar=0):
+        """Rectangular-image (position-velocity) ray sets, one per
+        inclination: nx x ny pixel rays over size_x x size_y cm, rotated by
+        phioff and offset by (xoff, yoff) cm; addstar > 0 adds an
+        unresolved star.  Replaces the circular ray sets."""
+        inc, ip_ = _d(np.atleast_1d(incl_rad))
+        n = len(inc)
+        nrays = np.zeros(n, np.int32)
+        rc = self.L.rl_setup_rays_rectang(self.ctx, n, ip_, int(nx), int(ny), float(size_x),
+                                          float(size_y), float(phioff), float(xoff),
+                                          float(yoff), int(addstar), nrays.ctypes.data_as(_ip))
+        self._chk(rc, "rl_setup_rays_rectang")
+        self.nrays, self.nrings = [int(x) for x in nrays], [0] * n
+        self.pv_shape = (int(ny), int(nx))
+        return self.nrays
+
+    def trace_posvel(self, items, want_tau=True):
+        """Position-velocity cubes of (line, incl, nfr, vmax_kms[, skip])
+        items on rectangular ray sets.  Returns (list of intensity cubes
+        [ny, nx, nfr], list of optical-depth cubes or None, warn bits)."""
+        its = [tuple(it[:4]) + (SKIP_NONE,) for it in items]
+        arr = (RLItem * len(its))(*[RLItem(*it) for it in its])
+        ny, nx = self.pv_shape
+        tot = sum(ny * nx * it[2] for it in its)
+        img = np.zeros(tot)
+        tau = np.zeros(tot) if want_tau else None
+        warn = np.zeros(len(its), np.uint32)
+        rc = self.L.rl_trace_lines_posvel(
+            self.ctx, len(its), arr, img.ctypes.data_as(_dp),
+            tau.ctypes.data_as(_dp) if tau is not None else None,
+            warn.ctypes.data_as(C.POINTER(C.c_uint)))
+        self._chk(rc, "rl_trace_lines_posvel")
+        im, ta, o = [], [], 0
+        for it in its:
+            n = ny * nx * it[2]
+            im.append(img[o:o + n].reshape(ny, nx, it[2]))
+            if tau is not None:
+                ta.append(tau[o:o + n].reshape(ny, nx, it[2]))
+            o += n
+        return im, (ta if tau is not None else None), warn
+
     def trace(self, items, want_image=False, want_cmask=False):
         """items: list of (line, incl, nfr, vmax_kms, skip_mode).
         Returns (list of flux arrays, list of images or None, warn bits),
@@ -474,6 +521,8 @@ def run_directory(path, precision="fp64", skip_mode=SKIP_REFERENCE, device=0,
     the cells of the earlier lines as the reference's never-cleared
     imcir_cmask does."""
     ri = l4io.read_run_dir(path)
+    if ri.extra.get("posvel"):
+        return run_posvel_directory(path, ri, precision, device, write_output)
     rt = Raytracer(device=device, precision=precision)
     tb = rt.load_run(ri)
     nfr, vmax = passband(ri.vmax, ri.dv)
@@ -511,3 +560,37 @@ def run_directory(path, precision="fp64", skip_mode=SKIP_REFERENCE, device=0,
                                       ri.molname, nfr, incl, blocks, vrad=ri.vrad)
     rt.close()
     return blocks, warn
+
+
+def run_posvel_directory(path, ri=None, precision="fp64", device=0, write_output=True):
+    """The RADlite executable's job for a run directory whose
+    linespectrum.inp asks for position-velocity cubes (linespec_command 2,
+    main.F:1062-1075): every requested line's cube on the rectangular
+    image, written as lineposvel_<mol>_<line>.dat (calc_write_line_posvel,
+    telescope.F:1828-2031).  Returns the list of (image, tau) cubes
+    [ny, nx, nfr] and the warn bits."""
+    ri = l4io.read_run_dir(path) if ri is None else ri
+    pv = ri.extra["posvel"]
+    rt = Raytracer(device=device, precision=precision)
+    tb = rt.load_run(ri)
+    nfr, vmax = passband(ri.vmax, ri.dv)
+    incl = ri.incl_deg * 0.0174532925199                      # telescope.F:188
+    rt.setup_rays_rectang([incl], pv["nx"], pv["ny"], pv["size_x"], pv["size_y"],
+                          pv["phioff"], pv["xoff"], pv["yoff"], pv["addstar"])
+    lines = list(range(ri.iline_start - 1, ri.iline_start - 1 + ri.nlines_run))
+    img, tau, warn = rt.trace_posvel([(il, 0, nfr, vmax) for il in lines])
+    rt.close()
+    if write_output:
+        pix = dict(nx=pv["nx"], ny=pv["ny"], spx=(pv["size_x"] * 0.5) / (pv["nx"] * 0.5),
+                   spy=(pv["size_y"] * 0.5) / (pv["ny"] * 0.5), phioff=pv["phioff"],
+                   xoff=pv["xoff"], yoff=pv["yoff"])
+        for k, il in enumerate(lines):
+            nu0s = tb.nu0[il]
+            nu0 = abs(nu0s)
+            passb = 3.33567e-6 * nu0 * vmax
+            dnu = 2.0 * passb / (nfr - 1.0)
+            nu = nu0 + (-passb + np.arange(nfr) * dnu)
+            l4io.write_posvel_file(os.path.join(path, l4io.posvel_file_name(ri.molname, il + 1)),
+                                   ri.molname, ri.lev_up[il], ri.lev_low[il], nu0s, nu, incl,
+                                   pix, img[k], tau[k], vrad=ri.vrad)
+    return list(zip(img, tau)), warn

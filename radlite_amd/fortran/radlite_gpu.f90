@@ -99,6 +99,24 @@ module radlite_gpu
             type(c_ptr), value :: image
             integer(c_int), intent(out) :: warn_bits(*)
         end function
+        integer(c_int) function rl_setup_rays_rectang(ctx, nincl, incl_rad, nx, ny, size_x, &
+                size_y, phioff, xoff, yoff, addstar, n_rays) bind(C, name="rl_setup_rays_rectang")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nincl, nx, ny, addstar
+            real(c_double), intent(in) :: incl_rad(*)
+            real(c_double), value :: size_x, size_y, phioff, xoff, yoff
+            integer(c_int), intent(out) :: n_rays(*)
+        end function
+        integer(c_int) function rl_trace_lines_posvel(ctx, nitems, items, image, tau, warn_bits) &
+                bind(C, name="rl_trace_lines_posvel")
+            import :: c_ptr, c_int, c_double, rl_item
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nitems
+            type(rl_item), intent(in) :: items(*)
+            real(c_double), intent(out) :: image(*), tau(*)
+            integer(c_int), intent(out) :: warn_bits(*)
+        end function
         integer(c_int) function rl_sync(ctx) bind(C, name="rl_sync")
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx

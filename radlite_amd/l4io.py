@@ -380,6 +380,68 @@ def _fortran_e(x: float, w: int, d: int) -> str:
     return s.rjust(w)
 
 
+def _fortran_es(x: float, w: int, d: int) -> str:
+    """Fortran ESw.d edit descriptor: d.ddddE+xx, right-justified; a
+    three-digit exponent drops the E (d.dddd-119)."""
+    m, ex = f"{float(x):.{d}E}".split("E")
+    e = int(ex)
+    s = m + (("E%+03d" % e) if abs(e) <= 99 else ("%+04d" % e))
+    return s.rjust(w) if len(s) <= w else "*" * w
+
+
+def posvel_file_name(molname: str, iline: int) -> str:
+    """lineposvel_<mol>_<line>.dat as calc_write_line_posvel names it
+    (telescope.F:1940-1962, the non-Intel branch): the line number is
+    written (i2) into a 12-character buffer, of which characters 2:2 are
+    taken below line 10 and 2:3 (the second digit and a blank) from 10 to
+    99; line 100 and above stop the reference (code 177)."""
+    if iline < 1 or iline >= 100:
+        raise ValueError("the reference names posvel files for lines 1-99 only")
+    ch = f"{iline:2d}".ljust(12)
+    return f"lineposvel_{molname}_" + (ch[1:2] if iline < 10 else ch[1:3]) + ".dat"
+
+
+def write_posvel_file(fname, molname, lev_up, lev_low, nu0, nu, incl_rad, pix, image, tau,
+                      distance_pc=1.0, vrad=0.0):
+    """Position-velocity cube as calc_write_line_posvel writes it
+    (telescope.F:1966-2019): an empty record; (I2) format version 1;
+    (A80) molecule name and data file; (ES12.4,ES12.4,F7.3) distance
+    [pc], radial velocity, inclination [deg]; (I5,I5) levels; (ES12.4)
+    nu0; (I5) channels; (I5,I5,ES12.4,ES12.4,F7.3,ES12.4,ES12.4) nx, ny,
+    pixel sizes [cm], phioff, xoff, yoff; a ' ' record; the velocities
+    (E10.3, one per record); a ' ' record; then per channel, iy outer
+    and ix inner, (ES12.5,1X,ES12.5) the Rayleigh-Jeans brightness
+    temperature 3.25465503368e36 / nu0^2 * I and the optical depth,
+    closed by a ' ' record.
+
+    nu0: the line's signed rest frequency; nu: channel frequencies [nfr];
+    pix: dict nx, ny, spx, spy (pixel sizes), phioff, xoff, yoff;
+    image, tau: [ny, nx, nfr]."""
+    es = _fortran_es
+    ny, nx, nfr = image.shape
+    out = ["", " 1", f"{molname:<80.80s}", f"{'./' + molname + '.dat':<80.80s}",
+           es(distance_pc, 12, 4) + es(vrad, 12, 4) + f"{incl_rad * 57.2957795132:7.3f}",
+           f"{int(lev_up):5d}{int(lev_low):5d}", es(nu0, 12, 4), f"{nfr:5d}",
+           f"{int(pix['nx']):5d}{int(pix['ny']):5d}" + es(pix["spx"], 12, 4) +
+           es(pix["spy"], 12, 4) + f"{pix['phioff']:7.3f}" + es(pix["xoff"], 12, 4) +
+           es(pix["yoff"], 12, 4), "  "]
+    for k in range(nfr):
+        v = float(nu[k]) - nu0
+        v = -2.99792458e5 * v / nu0
+        v = v + vrad
+        out.append(_fortran_e(v, 10, 3))
+    out.append("  ")
+    temp = 3.25465503368e36 / (nu0 * nu0)
+    for k in range(nfr):
+        tk = temp * image[:, :, k]
+        for iy in range(ny):
+            out.extend(es(a, 12, 5) + " " + es(b, 12, 5)
+                       for a, b in zip(tk[iy].tolist(), tau[iy, :, k].tolist()))
+        out.append("  ")
+    with open(fname, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
 def _flang_list(x: float) -> str:
     """A REAL(8) as flang's list-directed output writes it: one leading
     blank, the shortest round-trip digits, F form (no leading zero,
