@@ -216,6 +216,50 @@ def bench_batched(args):
     }
 
 
+def bench_posvel(args):
+    """A position-velocity cube: the 400x200 disk at 60 deg, 512x512 pixel
+    rays over 2.2 outer radii, 200 channels over +-40 km/s.  One step = the
+    rectangular ray set and its trajectories + the cube (intensity and
+    optical depth) returned to host memory.  Device time (trajectories,
+    node records, formal kernel) is reported beside the whole call, which
+    includes the 2 x 420 MB device-to-host copy."""
+    import numpy as np
+    from radlite_amd import engine, synth
+    ri, w = synth.workload_inputs("c2", seed=0)
+    nfr, vmax = engine.passband(ri.vmax, ri.dv)
+    nx = ny = 512
+    size = 2.2 * float(ri.r[-1])
+    incl = 60.0 * 0.0174532925199
+    rt = engine.Raytracer(0, args.precision)
+    rt.load_run(ri)
+
+    def step():
+        rt.setup_rays_rectang([incl], nx, ny, size, size, 0.0, 0.0, 0.0, 1)
+        img, tau, _ = rt.trace_posvel([(0, 0, nfr, vmax)])
+        return img[0], rt.stats()
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    steps = max(1, min(args.steps, 3))
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        img, st = step()
+        dev_ms.append(st["t_total_ms"])
+    dt = (time.perf_counter() - t0) / steps
+    rt.close()
+    npc = nx * ny * nfr
+    dms = float(np.mean(dev_ms))
+    return {
+        "workload": f"pos-vel cube: nr=400 x nth=200 disk, 60 deg, {nx}x{ny} pixels, "
+                    f"{nfr} channels", "precision": args.precision, "steps": steps,
+        "pixel_channels": npc, "traced_rays": st["rays"], "nodes": st["nodes"],
+        "device_ms": dms, "device_value": npc / (dms * 1e-3),
+        "call_ms": dt * 1e3, "call_value": npc / dt, "unit": "pixel*channel/s",
+        "t_traj_ms": st["t_traj_ms"], "t_trace_ms": st["t_formal_ms"],
+    }
+
+
 def copy_probe(nbytes):
     """A plain device-to-device copy of known size: what FETCH_SIZE and
     WRITE_SIZE read for it calibrates the counters against true bytes."""
@@ -247,6 +291,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 64-line x 8-inclination batched measurement")
     ap.add_argument("--batched-only", action="store_true")
+    ap.add_argument("--no-posvel", action="store_true",
+                    help="skip the position-velocity cube leg")
+    ap.add_argument("--posvel-only", action="store_true")
     ap.add_argument("--large-only", action="store_true",
                     help="only the 1024x512-disk, 1000-line plan and measured slice")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -269,6 +316,9 @@ def main():
         return
     if args.large_only:
         print(json.dumps(bench_large(args)))
+        return
+    if args.posvel_only:
+        print(json.dumps(bench_posvel(args)))
         return
     from radlite_amd import engine, synth
     ri, w = synth.workload_inputs(args.workload, seed=0)
@@ -360,6 +410,8 @@ def main():
         out["fwd_adjoint"] = bench_adjoint(args)
     if not args.no_batched:
         out["batched"] = bench_batched(args)
+    if not args.no_posvel:
+        out["posvel"] = bench_posvel(args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -1377,7 +1377,7 @@ extern "C" int rl_trace_lines_posvel(rl_ctx *c, int nitems, const rl_item *items
     const bool fp64 = c->precision == RL_FP64;
     const size_t rsz = fp64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::FastRec);
     size_t io = 0;
-    std::vector<double> hi, ht;
+    std::vector<double> cen;                      // the added star's ray
     for (int ii = 0; ii < nitems; ii++) {
         rl_item it = items[ii];
         it.skip_mode = RL_SKIP_NONE;
@@ -1387,15 +1387,16 @@ extern "C" int rl_trace_lines_posvel(rl_ctx *c, int nitems, const rl_item *items
         RaySet &rs = *s.rs;
         const int nfr = it.nfr, npix = rs.nx * rs.ny;
         const size_t ncv = (size_t)rs.nrays * nfr;
+        const size_t npv = (size_t)npix * nfr, nextra = (size_t)(rs.nrays - npix) * nfr;
         HIPCHK(c, c->d_rec.ensure((size_t)rs.nnodes * rsz));
         HIPCHK(c, c->d_rinfo.ensure((size_t)rs.nrays * sizeof(RayInfo)));
-        HIPCHK(c, c->d_image.ensure(ncv * 8));
-        HIPCHK(c, c->d_partial.ensure(ncv * 8));
+        HIPCHK(c, c->d_image.ensure((npv + nextra + 1) * 8));
+        HIPCHK(c, c->d_partial.ensure(npv * 8 + 8));
         HIPCHK(c, c->d_ovs.ensure(16));
         unsigned long long *ovs = c->d_ovs.as<unsigned long long>();
         rl::launch_zero_u64(c->stream, ovs, 2);
         rl::PosvelLaunch L;
-        L.nrays = rs.nrays; L.nfr = nfr;
+        L.nrays = rs.nrays; L.nfr = nfr; L.nx = rs.nx; L.ny = rs.ny;
         L.off = rs.d_off.as<long long>();
         L.geo = rs.d_nodes.as<GeoNode>();
         L.cells = c->d_cells.as<CellRec>();
@@ -1407,6 +1408,7 @@ extern "C" int rl_trace_lines_posvel(rl_ctx *c, int nitems, const rl_item *items
         L.istar = L.dnu + 2 * nfr;
         L.lc = s.lc;
         L.image = c->d_image.as<double>();
+        L.extra = L.image + npv;
         L.tau = c->d_partial.as<double>();
         L.ovs = ovs;
         L.maser = reinterpret_cast<int *>(ovs + 1);
@@ -1414,27 +1416,19 @@ extern "C" int rl_trace_lines_posvel(rl_ctx *c, int nitems, const rl_item *items
         else rl::launch_posvel<float>(c->stream, L);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
-        hi.resize(ncv);
-        ht.resize(ncv);
+        double *oi = image + io, *ot = tau ? tau + io : nullptr;
+        cen.assign(nextra ? nextra : 1, 0.0);
         unsigned long long flags[2] = {0, 0};
-        HIPCHK(c, hipMemcpyAsync(hi.data(), L.image, ncv * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(ht.data(), L.tau, ncv * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(oi, L.image, npv * 8, hipMemcpyDeviceToHost, c->stream));
+        if (ot) HIPCHK(c, hipMemcpyAsync(ot, L.tau, npv * 8, hipMemcpyDeviceToHost, c->stream));
+        if (nextra)
+            HIPCHK(c, hipMemcpyAsync(cen.data(), L.extra, nextra * 8, hipMemcpyDeviceToHost,
+                                     c->stream));
         HIPCHK(c, hipMemcpyAsync(flags, ovs, 16, hipMemcpyDeviceToHost, c->stream));
         if (int e = check_err(c)) return e;             // synchronises the stream
-        // pixel rays are ix-major on the device; the cube is iy-major
-        double *oi = image + io, *ot = tau ? tau + io : nullptr;
-        for (int ix = 0; ix < rs.nx; ix++)
-            for (int iy = 0; iy < rs.ny; iy++) {
-                const double *si = &hi[((size_t)ix * rs.ny + iy) * nfr];
-                const double *st = &ht[((size_t)ix * rs.ny + iy) * nfr];
-                const size_t o = ((size_t)iy * rs.nx + ix) * nfr;
-                std::copy(si, si + nfr, oi + o);
-                if (ot) std::copy(st, st + nfr, ot + o);
-            }
         if (rs.starunres) {                               // telescope.F:2153-2200
             const double srat = (3.14159265 * c->rstar * c->rstar) / (4.0 * rs.spx * rs.spy);
             const double srat1 = 1.0 - srat;
-            const double *cen = &hi[(size_t)rs.central * nfr];
             const int ixc = rs.nx / 2 - 1, iyc = rs.ny / 2 - 1;
             for (int k = 0; k < nfr; k++) {
                 const double dummy = cen[k] * srat;

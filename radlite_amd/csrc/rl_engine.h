@@ -170,7 +170,7 @@ void launch_star_flags(hipStream_t st, int nlisted, const int *rays, const long 
 // intensity and line-of-sight optical depth; rays without nodes (outside
 // the grid) give 0 and 0.
 struct PosvelLaunch {
-    int nrays, nfr;
+    int nrays, nfr, nx, ny;            // rays: nx*ny pixels (ix-major), then extra rays
     const long long *off;
     const GeoNode *geo;
     const CellRec *cells;
@@ -179,7 +179,8 @@ struct PosvelLaunch {
     void *rec;                         // NodeRec<double> (FP64) or FastRec (FP32)
     const double *dnu, *velo, *istar;  // per channel
     LineConst lc;
-    double *image, *tau;               // [ray][channel]
+    double *image, *tau;               // [iy][ix][channel]
+    double *extra;                     // [ray - nx*ny][channel] (intensity)
     unsigned long long *ovs;
     int *maser;
 };

@@ -604,14 +604,17 @@ __global__ void k_star_flags(int nlisted, const int *__restrict__ rays,
 }
 
 // one lane per (ray, channel): the intensity and char_tau of
-// make_freq_image_rectang (telescope.F:2117-2146)
+// make_freq_image_rectang (telescope.F:2117-2146).  Pixel rays are ix-major
+// (ray = ix*ny + iy); the cube is written iy-major, [iy][ix][channel], and
+// rays past the nx*ny pixels (the added star's) into extra[channel]
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_formal_posvel(int nrays, int nfr, const long long *__restrict__ off,
+k_formal_posvel(int nrays, int nfr, int nx, int ny, const long long *__restrict__ off,
                 const typename RecOf<T>::type *__restrict__ rec,
                 const double *__restrict__ dnu, const double *__restrict__ velo,
                 const double *__restrict__ istar, LineConst lc, double *__restrict__ image,
-                double *__restrict__ tau, unsigned long long *ovs_out, int *maser_out) {
+                double *__restrict__ tau, double *__restrict__ extra,
+                unsigned long long *ovs_out, int *maser_out) {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int ovs = 0, maser = 0;
     if (g < (long long)nrays * nfr) {
@@ -634,8 +637,15 @@ k_formal_posvel(int nrays, int nfr, const long long *__restrict__ off,
                 t = (double)tf;
             }
         }
-        image[g] = I;
-        tau[g] = t;
+        const int npix = nx * ny;
+        if (ray < npix) {
+            const int ix = ray / ny, iy = ray - ix * ny;
+            const size_t o = ((size_t)iy * nx + ix) * nfr + ch;
+            image[o] = I;
+            tau[o] = t;
+        } else {
+            extra[(size_t)(ray - npix) * nfr + ch] = I;
+        }
     }
     // wave totals of the warnings
     for (int m = 32; m >= 1; m >>= 1) {
@@ -666,9 +676,9 @@ void launch_posvel(hipStream_t st, const PosvelLaunch &a) {
                            a.geo, a.cells, rec, 0ll, a.rinfo, a.nfr, a.velo, a.aks2, 1, a.lc);
     const long long total = (long long)a.nrays * a.nfr;
     hipLaunchKernelGGL(k_formal_posvel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       st, a.nrays, a.nfr, a.off,
+                       st, a.nrays, a.nfr, a.nx, a.ny, a.off,
                        reinterpret_cast<const typename RecOf<T>::type *>(a.rec), a.dnu, a.velo,
-                       a.istar, a.lc, a.image, a.tau, a.ovs, a.maser);
+                       a.istar, a.lc, a.image, a.tau, a.extra, a.ovs, a.maser);
 }
 template void launch_posvel<double>(hipStream_t, const PosvelLaunch &);
 template void launch_posvel<float>(hipStream_t, const PosvelLaunch &);
