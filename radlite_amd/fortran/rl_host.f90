@@ -13,8 +13,11 @@
 ! reference's arithmetic), traces every requested line at the
 ! linespectrum.inp inclination (calc_line_spectrum, telescope.F:1465) and
 ! writes linespectrum_<molecule>.dat (write_line_spectrum,
-! telescope.F:1662-1804).  A library error stops with the reference's
-! stop code.
+! telescope.F:1662-1804); when linespectrum.inp asks for position-velocity
+! cubes (command 2, telescope.F:202-230) it renders them instead on the
+! rectangular image and writes lineposvel_<molecule>_<line>.dat
+! (calc_write_line_posvel, telescope.F:1828-2031).  A library error stops
+! with the reference's stop code.
 program rl_host
     use iso_c_binding
     use radlite_gpu
@@ -22,6 +25,9 @@ program rl_host
     character(len=512) :: dir, arg, molfile, molname, line
     integer :: nr, nth, nf, nspec, nsp_t, nlev, nlin, i, k, ir, it, u, idum, isp, nsize
     integer :: nphi, dbdr, b_extra, outer_bc, nlines_run, istart, iprec, ios, nrows
+    integer :: lcmd, imx, imy, isz, iaddstar
+    real(c_double) :: szx, szy, phioff, xoff, yoff
+    real(c_double), allocatable :: image(:), tau(:)
     real(c_double) :: umass, fix_lw, vmax, dv, incl_deg, vrad, dist, rstar, aks, vmax2, dummy
     real(c_double), allocatable :: r(:), th(:), rho(:,:), abund(:,:), vel(:,:,:), lw(:,:)
     real(c_double), allocatable :: rhod(:,:,:), td(:,:,:), cnu(:), kabs(:,:), kscat(:,:)
@@ -166,12 +172,30 @@ program rl_host
     read(u, '(a)') line
     read(u, *) idum
     read(u, '(a)') molfile
-    read(u, *) idum
+    read(u, *) lcmd
     read(u, *) dist
     read(u, *) incl_deg
     read(u, *) vrad
     read(u, *) nlines_run
     read(u, *) istart
+    if (lcmd == 2) then
+        read(u, *) imx
+        read(u, *) imy
+        read(u, *) isz
+        read(u, *) szx
+        read(u, *) szy
+        read(u, *) phioff
+        read(u, *) xoff
+        read(u, *) yoff
+        read(u, *) iaddstar
+        if (isz /= 1) then
+            write(*, '(a,i0)') 'Dont recognize size spec ', isz
+            stop 13
+        end if
+    else if (lcmd /= 0) then
+        write(*, '(a,i0)') 'Dont recognize command no. ', lcmd
+        stop 13
+    end if
     close(u)
     molfile = adjustl(molfile)
     molname = molfile(1:index(molfile, '.dat', back=.true.) - 1)
@@ -236,6 +260,19 @@ program rl_host
     call chkc(rl_set_outer_bc(ctx, outer_bc))
     inc(1) = incl_deg * 0.0174532925199d0
     allocate(nrays(1), nrings(1))
+    if (lcmd == 2) then
+        call chkc(rl_setup_rays_rectang(ctx, 1, inc, imx, imy, szx, szy, phioff, xoff, yoff, &
+                                        iaddstar, nrays))
+        allocate(items(nlines_run), warn(nlines_run))
+        allocate(image(nfr * imx * imy * nlines_run), tau(nfr * imx * imy * nlines_run))
+        do i = 1, nlines_run
+            items(i) = rl_item(istart - 2 + i, 0, nfr, vmax2, RL_SKIP_NONE)
+        end do
+        call chkc(rl_trace_lines_posvel(ctx, nlines_run, items, image, tau, warn))
+        call rl_destroy(ctx)
+        call write_posvel()
+        stop
+    end if
     call chkc(rl_setup_rays_circular(ctx, 1, inc, nphi, b_extra, dbdr, nrays, nrings, &
                                      c_null_ptr, c_null_ptr))
     allocate(items(nlines_run), flux(nlines_run * nfr), warn(nlines_run))
@@ -337,6 +374,72 @@ contains
             end do
         end do
         close(v)
+    end subroutine
+
+    ! lineposvel_<molecule>_<line>.dat as calc_write_line_posvel writes it
+    ! (telescope.F:1940-2019): the same records and edit descriptors; the
+    ! cube of line il is image((il-1)*nfr*imx*imy + ((iy-1)*imx + ix-1)*nfr
+    ! + inu), imrec_int(inu, ix, iy)'s order
+    subroutine write_posvel()
+        integer :: v, il, ln, inu, ix, iy
+        integer(c_int64_t) :: o
+        real(c_double) :: nuf, passb, dnu, temp, spx, spy
+        real(c_double), allocatable :: velo(:)
+        character(len=12) :: ch
+        character(len=512) :: fname
+        character(len=200) :: name80, file80
+        name80 = molname
+        file80 = './' // trim(molfile)
+        spx = (szx * 0.5d0) / (imx * 0.5d0)
+        spy = (szy * 0.5d0) / (imy * 0.5d0)
+        allocate(velo(nfr))
+        do il = 1, nlines_run
+            ln = istart - 1 + il
+            write(ch, '(i2)') ln
+            if (ln < 10) then
+                fname = 'lineposvel_' // trim(molname) // '_' // ch(2:2) // '.dat'
+            else if (ln < 100) then
+                fname = 'lineposvel_' // trim(molname) // '_' // ch(2:3) // '.dat'
+            else
+                stop 177
+            end if
+            nuf = nu0(ln)
+            passb = 3.33567d-6 * abs(nuf) * vmax2
+            dnu = 2.0d0 * passb / (nfr - 1.0d0)
+            open(newunit=v, file=path(fname(1:len_trim(fname))), status='replace', &
+                 action='write')
+            write(v, *)
+            write(v, '(I2)') 1
+            write(v, '(A80)') name80
+            write(v, '(A80)') file80
+            write(v, '(ES12.4,ES12.4,F7.3)') 1.0d0, vrad, inc(1) * 57.2957795132d0
+            write(v, '(I5,I5)') lup(ln) + 1, llo(ln) + 1
+            write(v, '(ES12.4)') nuf
+            write(v, '(I5)') nfr
+            write(v, '(I5,I5,ES12.4,ES12.4,F7.3,ES12.4,ES12.4)') imx, imy, spx, spy, phioff, &
+                xoff, yoff
+            write(v, *) ' '
+            do inu = 1, nfr
+                velo(inu) = (abs(nuf) + ((0.0d0 - passb) + (inu - 1) * dnu)) - nuf
+                velo(inu) = -2.99792458d5 * velo(inu) / nuf
+                velo(inu) = velo(inu) + vrad
+            end do
+            write(v, '(E10.3,1X)') (velo(inu), inu = 1, nfr)
+            write(v, *) ' '
+            do inu = 1, nfr
+                do iy = 1, imy
+                    do ix = 1, imx
+                        o = int(il - 1, c_int64_t) * nfr * imx * imy + &
+                            (int(iy - 1, c_int64_t) * imx + (ix - 1)) * nfr + inu
+                        temp = 3.25465503368d36 / nuf**2
+                        temp = temp * image(o)
+                        write(v, '(ES12.5,1X,ES12.5,1X)') temp, tau(o)
+                    end do
+                end do
+                write(v, *) ' '
+            end do
+            close(v)
+        end do
     end subroutine
 
 end program rl_host

@@ -59,3 +59,27 @@ def test_fortran_host_writes_the_python_hosts_file(name):
             va, vb = [float(x) for x in a.split()], [float(x) for x in b.split()]
             # one unit of the sixth digit of the 0.dddddd mantissa
             assert va[0] == vb[0] and abs(va[1] - vb[1]) <= 1.01e-5 * abs(vb[1]), (a, b)
+
+
+@pytest.mark.parametrize("name", ["p1", "p2", "p3"])
+def test_fortran_host_writes_the_references_posvel_files(name):
+    """A run directory asking for position-velocity cubes (linespectrum.inp
+    command 2): the Fortran host's lineposvel files, written with the
+    reference's own format statements from the library's FP64 cubes, are
+    the files the reference wrote for the same run, byte for byte."""
+    print(f"[gpu] fortran host posvel {name}", file=sys.stderr, flush=True)
+    assert os.path.exists(build.FORTRAN_HOST), "build the Fortran host first"
+    g = load_golden(name)
+    with tempfile.TemporaryDirectory() as d:
+        build_case(name, d)
+        assert str(g["input_sha256"]) == input_hash(d)
+        r = subprocess.run([build.FORTRAN_HOST, d, "fp64"], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        ri = l4io.read_run_dir(d)
+        for il in range(len(g["nu0"])):
+            iline = ri.iline_start + il
+            got = open(os.path.join(d, l4io.posvel_file_name(ri.molname, iline))).read()
+            ref = open(os.path.join(os.path.dirname(__file__), "golden",
+                                    f"posvel_{name}_{ri.molname}_{iline}.dat")).read()
+            assert got == ref, (name, iline)
