@@ -48,7 +48,21 @@ struct CellView {
 // set: a set has one thread per ray, a long serial chain of FP64 work, so
 // sets launched together keep more waves resident to hide its latency.
 __global__ void __launch_bounds__(128)
-k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, int *err) {
+k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, int *err,
+        int grid_in_lds) {
+    // the radii, cone angles and tan^2 table (a few KB) staged in LDS: the
+    // merge reads them at every step
+    extern __shared__ double lds_grid[];
+    if (grid_in_lds) {
+        const int nrc = g.nr + 4, ntc = g.ny + 4, nt2 = g.ny / 2 + 1;
+        for (int i = threadIdx.x; i < nrc; i += blockDim.x) lds_grid[i] = g.rc[i];
+        for (int i = threadIdx.x; i < ntc; i += blockDim.x) lds_grid[nrc + i] = g.tc[i];
+        for (int i = threadIdx.x; i < nt2; i += blockDim.x) lds_grid[nrc + ntc + i] = tan2[i];
+        __syncthreads();
+        g.rc = lds_grid;
+        g.tc = lds_grid + nrc;
+        tan2 = lds_grid + nrc + ntc;
+    }
     const CrossSet &q = sets.s[blockIdx.y];
     const int nrays = q.nrays;
     const double *__restrict__ x0 = q.x0;
@@ -692,7 +706,9 @@ void launch_cross(hipStream_t st, const GridDev &gd, const double *tan2, const C
     int most = 0;
     for (int i = 0; i < sets.n; i++) most = max(most, sets.s[i].nrays);
     dim3 blk(128), grd((most + 127) / 128, sets.n);
-    hipLaunchKernelGGL(k_cross, grd, blk, 0, st, g, tan2, sets, nmax, err);
+    const size_t lds = (size_t)(gd.nr + 4 + gd.ny + 4 + gd.ny / 2 + 1) * sizeof(double);
+    const int in_lds = lds <= 48 * 1024 ? 1 : 0;
+    hipLaunchKernelGGL(k_cross, grd, blk, in_lds ? lds : 0, st, g, tan2, sets, nmax, err, in_lds);
 }
 
 void launch_geo(hipStream_t st, const GridDev &gd, const double *vel, int nh, int nrays,

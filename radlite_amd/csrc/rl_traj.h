@@ -3,10 +3,15 @@
 // Restates make_trajectory_c (RADLITE/telescope.F:2787-3720) with the
 // reference's root formulas, eps filters and extra-point rule, but
 // without its per-ray arrays: the theta-cone and R-sphere crossing
-// lists are generated on demand from their index (only the cone roots
-// sar1/sar2 and their cone indices are kept, in a strided scratch
-// buffer), the extra points live in a small private list, and the
-// three-way merge streams nodes straight to the caller's sink.
+// lists are generated on demand from their index, the extra points live
+// in a small private list, and the three-way merge streams nodes straight
+// to the caller's sink.  The cones a ray crosses are those with
+// tan^2(Theta) above a threshold (the discriminant is
+// 4 (tan^2 (cos^2 th0 x0^2 + sin^2 th0 z0^2) - sin^2 th0 x0^2)), a
+// contiguous range of cone indices; their roots are then recomputed from
+// the index when the merge reaches them, with the same arithmetic.  Only
+// when rounding breaks that contiguity are the roots kept, in a strided
+// scratch buffer.
 #pragma once
 #include "rl_device.h"
 
@@ -39,6 +44,7 @@ struct Traj {
     double *sar1, *sar2;
     int *iyar;
     int stride;
+    int iy0 = 0;              // > 0: cone is has index iy0 + is - 1 (roots recomputed)
     int isnr, isdblnr, iup, branchA, ith_amount;
     // R spheres
     int ir_min, ir_amount, half;
@@ -64,9 +70,26 @@ struct Traj {
     __device__ int hunt_t(double x) const {
         return hunt_bisect([&](int i) { return g.TH(i); }, g.ny, x);
     }
-    __device__ double S1(int is) const { return sar1[(size_t)is * stride]; }
-    __device__ double S2(int is) const { return sar2[(size_t)is * stride]; }
-    __device__ int IY(int is) const { return iyar[(size_t)is * stride]; }
+    // roots of cone iy, ordered (telescope.F:2968-3000); sd > 0 is known
+    __device__ void cone_roots(int iy, double &s1, double &s2) const {
+        double tanth2 = tan2[iy];
+        double a = tanth2 * costh02 - sinth02;
+        double b = 2.0 * tanth2 * costh0 * z0;
+        double c = tanth2 * z0 * z0 - x0 * x0;
+        double sd = sqrt(b * b - 4.0 * a * c);
+        s1 = (-b - sd) / (2.0 * a);
+        s2 = (-b + sd) / (2.0 * a);
+        if (s1 > s2) { double d = s1; s1 = s2; s2 = d; }
+    }
+    __device__ double S1(int is) const {
+        if (iy0 > 0) { double a, b; cone_roots(iy0 + is - 1, a, b); return a; }
+        return sar1[(size_t)is * stride];
+    }
+    __device__ double S2(int is) const {
+        if (iy0 > 0) { double a, b; cone_roots(iy0 + is - 1, a, b); return b; }
+        return sar2[(size_t)is * stride];
+    }
+    __device__ int IY(int is) const { return iy0 > 0 ? iy0 + is - 1 : iyar[(size_t)is * stride]; }
 
     // th list entry iss (1-based): ordering of telescope.F:3024-3194
     __device__ void th_entry(int iss, double &s, int &itheta) const {
@@ -130,8 +153,9 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
     t.iyeq = ny / 2;
     t.iyend = ny;
 
-    // theta = const cones (telescope.F:2968-3000)
-    int is = 1;
+    // theta = const cones (telescope.F:2968-3000): which have two roots
+    int is = 1, first = 0;
+    bool contiguous = true;
     for (int iy = 1; iy <= t.iyeq; iy++) {
         double tanth2 = t.tan2[iy];
         double a = tanth2 * costh02 - sinth02;
@@ -139,16 +163,34 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
         double c = tanth2 * z0 * z0 - x0 * x0;
         double sd = b * b - 4.0 * a * c;
         if (sd > 0.0) {
-            sd = sqrt(sd);
-            double s1 = (-b - sd) / (2.0 * a), s2 = (-b + sd) / (2.0 * a);
-            if (s1 > s2) { double d = s1; s1 = s2; s2 = d; }
-            t.iyar[(size_t)is * t.stride] = iy;
-            t.sar1[(size_t)is * t.stride] = s1;
-            t.sar2[(size_t)is * t.stride] = s2;
+            if (first == 0) first = iy;
+            else if (iy != first + is - 1) contiguous = false;
             is++;
         }
     }
     t.isnr = is - 1;
+    if (contiguous && t.isnr > 0) {
+        t.iy0 = first;
+    } else {
+        // kept roots (the same arithmetic)
+        is = 1;
+        for (int iy = 1; iy <= t.iyeq; iy++) {
+            double tanth2 = t.tan2[iy];
+            double a = tanth2 * costh02 - sinth02;
+            double b = 2.0 * tanth2 * costh0 * z0;
+            double c = tanth2 * z0 * z0 - x0 * x0;
+            double sd = b * b - 4.0 * a * c;
+            if (sd > 0.0) {
+                sd = sqrt(sd);
+                double s1 = (-b - sd) / (2.0 * a), s2 = (-b + sd) / (2.0 * a);
+                if (s1 > s2) { double d = s1; s1 = s2; s2 = d; }
+                t.iyar[(size_t)is * t.stride] = iy;
+                t.sar1[(size_t)is * t.stride] = s1;
+                t.sar2[(size_t)is * t.stride] = s2;
+                is++;
+            }
+        }
+    }
     t.isdblnr = 0;
     for (is = 1; is <= t.isnr; is++) {
         double pit = 0.5 * pi - g.TH(t.IY(is));
@@ -264,15 +306,16 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
         sprev = s;
     };
     const int tr_total = t.ir_amount + t.ith_amount;
+    double es = nex >= 1 ? ex_s[1] : 1.0e30;       // the next extra point, kept in a register
     for (int iss = 1; iss <= tr_total; iss++) {
         for (;;) {
             double m = ths < rs ? ths : rs;
-            double es = isex <= nex ? ex_s[isex] : 1.0e30;
             if (!(es < m)) break;
             if (ex_r[isex] <= rmaxt && ex_r[isex] >= rmint &&
                 (es - sprev) > eps * ex_r[isex] && es >= sbeg && es <= send)
                 emit(3, es, ex_ir[isex], ex_it[isex]);
             isex++;
+            es = isex <= nex ? ex_s[isex] : 1.0e30;
         }
         if (ths < rs) {
             double r = t.thr(ths);
