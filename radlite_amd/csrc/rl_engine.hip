@@ -180,10 +180,11 @@ __device__ __forceinline__ void geo_node(GridView g, CellView cv, int k, double 
     const double mu = snew / sqrt(bnew * bnew + snew * snew);
     const double q = znew + snew * costh0;
     const double dummy = bnew * sqrt(bnew * bnew + snew * snew - q * q);
-    double sinphi = dummy > 0.0 ? (bnew * bnew * costh0 - znew * snew) / dummy : epsplus;
-    sinphi = x0 < 0.0 ? asin(sinphi) : pi - asin(sinphi);
-    while (sinphi < 0.0) sinphi = sinphi + 2.0 * pi;
-    while (sinphi >= 2.0 * pi) sinphi = sinphi - 2.0 * pi;
+    // phi = asin(x) (x0 < 0) or pi - asin(x), wrapped to [0, 2 pi); only its
+    // sine and cosine are used: sin(phi) = x, cos(phi) = +-sqrt(1 - x^2)
+    const double sx = dummy > 0.0 ? (bnew * bnew * costh0 - znew * snew) / dummy : epsplus;
+    const double cx = sqrt(1.0 - sx * sx);
+    const double sin_phi = sx, cos_phi = x0 < 0.0 ? cx : -cx;
     double v[3];
     const double *V = cv.vel;
     for (int c = 0; c < 3; c++) {
@@ -196,7 +197,7 @@ __device__ __forceinline__ void geo_node(GridView g, CellView cv, int k, double 
     }
     if (mu > 1.0) set_err(err, 393);
     o.u = 3.335668e-11 * (mu * v[0] + sqrt(1.0 - mu * mu) *
-                                         (v[1] * sin(sinphi) + v[2] * cos(sinphi)));
+                                         (v[1] * sin_phi + v[2] * cos_phi));
     if (k == 0) {
         o.ds = 0.0;
     } else {

@@ -81,12 +81,23 @@ struct Traj {
         s2 = (-b + sd) / (2.0 * a);
         if (s1 > s2) { double d = s1; s1 = s2; s2 = d; }
     }
+    // one root of cone iy: the smaller (lo) or the larger; the pair is
+    // ordered by the sign of the leading coefficient, so each is the same
+    // expression cone_roots evaluates
+    __device__ double cone_root(int iy, bool lo) const {
+        double tanth2 = tan2[iy];
+        double a = tanth2 * costh02 - sinth02;
+        double b = 2.0 * tanth2 * costh0 * z0;
+        double c = tanth2 * z0 * z0 - x0 * x0;
+        double sd = sqrt(b * b - 4.0 * a * c);
+        return (lo == (a > 0.0)) ? (-b - sd) / (2.0 * a) : (-b + sd) / (2.0 * a);
+    }
     __device__ double S1(int is) const {
-        if (iy0 > 0) { double a, b; cone_roots(iy0 + is - 1, a, b); return a; }
+        if (iy0 > 0) return cone_root(iy0 + is - 1, true);
         return sar1[(size_t)is * stride];
     }
     __device__ double S2(int is) const {
-        if (iy0 > 0) { double a, b; cone_roots(iy0 + is - 1, a, b); return b; }
+        if (iy0 > 0) return cone_root(iy0 + is - 1, false);
         return sar2[(size_t)is * stride];
     }
     __device__ int IY(int is) const { return iy0 > 0 ? iy0 + is - 1 : iyar[(size_t)is * stride]; }
