@@ -348,13 +348,25 @@ __device__ inline void nodes_ray_fast(int ray, int lane, const long long *__rest
     const long long b = off[ray];
     const int n = (int)(off[ray + 1] - b);
     double umin = 1.0e300, umax = -1.0e300;
-    for (int k = lane; k < n; k += 64) {
-        const GeoNode gn = geo[b + k];
+    // the previous node's width, scale and projected velocity come from
+    // the neighbouring lane (lane 0: the previous chunk's lane 63); every
+    // lane runs every chunk so that the shuffles see all of the wave
+    double lw_c = 0.0, sig_c = 0.0, u_c = 0.0;
+    for (int k0 = 0; k0 < n; k0 += 64) {
+        const int k = k0 + lane;
+        const bool act = k < n;
+        const GeoNode gn = geo[b + (act ? k : n - 1)];
         double v[5];
         node_interp(gn, cells, v);
-        FastRecT<T> r;
         double sig, jc, dc;
         scaled_node(v, lc, sig, jc, dc);
+        double lw_p = __shfl_up(v[CR_LW], 1), sig_p = __shfl_up(sig, 1), u_p = __shfl_up(gn.u, 1);
+        if (lane == 0) { lw_p = lw_c; sig_p = sig_c; u_p = u_c; }
+        lw_c = __shfl(v[CR_LW], 63);
+        sig_c = __shfl(sig, 63);
+        u_c = __shfl(gn.u, 63);
+        if (!act) continue;
+        FastRecT<T> r;
         r.ds = (T)gn.ds;
         r.u = (T)gn.u;
         r.sig = (T)sig;
@@ -365,15 +377,12 @@ __device__ inline void nodes_ray_fast(int ray, int lane, const long long *__rest
         r.flag = (T)(gn.flag >> 2);
         r.pad = T(0);
         if (k > 0) {
-            const GeoNode gp = geo[b + k - 1];
-            double vp[5], sp, jcp, dcp;
-            node_interp(gp, cells, vp);
-            scaled_node(vp, lc, sp, jcp, dcp);
-            double lwav = 0.5 * (vp[CR_LW] + v[CR_LW]);
+            const double sp = sig_p;
+            double lwav = 0.5 * (lw_p + v[CR_LW]);
             double aa = lc.c_aa * lwav;
             r.inv_aa = (T)(1.0 / aa);
             r.norm = (T)(0.56419583546 / aa);
-            double du = gn.u - gp.u;
+            double du = gn.u - u_p;
             double x = fabs(du / (lwav / 2.99792458e5));
             r.dls = 2.0 * 3.0 * x > 1.0 ? (T)(gn.ds / x) : T(-1);
             r.inv_du = (T)(1.0 / du);
