@@ -242,11 +242,13 @@ template <> struct FOps<float> {
     static __device__ inline float fma(float a, float b, float c) {
         return __builtin_fmaf(a, b, c);
     }
-    // a source function j/alpha (b > 0): the reciprocal unless b is so small
-    // that 1/b would overflow or lose the denormal (line wings of dust-free
-    // cells), where the quotient is divided exactly
+    // a source function j/alpha (b > 0) through the reciprocal; where b is
+    // so small that 1/b would overflow or lose the denormal (line wings of
+    // dust-free cells) both are first scaled by 2^64 (exact), without a
+    // branch
     static __device__ inline float sdiv(float a, float b) {
-        return b > 1.0e-30f ? a * __builtin_amdgcn_rcpf(b) : a / b;
+        const float sc = b < 1.0e-30f ? 0x1p64f : 1.0f;
+        return (a * sc) * __builtin_amdgcn_rcpf(b * sc);
     }
     static __device__ inline float fmn(float a, float b) { return fminf(a, b); }
     static __device__ inline float fmx(float a, float b) { return fmaxf(a, b); }
