@@ -317,7 +317,9 @@ using FastLine = FastLineT<float>;
 template <typename T> struct FPtT { T u, sig, jd, ad, jc, dc; };   // one point, scaled values
 using FPt = FPtT<float>;
 
-template <typename T> struct FastCarryT { T jl0, al0, S0; int init; T tau = T(0); };  // line j, alpha / sig at the start; optical depth
+// line j, alpha / sig at the start; S0 the start's source function; the
+// optical depth along the ray, summed with a compensation term (Kahan)
+template <typename T> struct FastCarryT { T jl0, al0, S0; int init; T tau = T(0), tau_c = T(0); };
 
 // integrate_element_linedust (line.F:4515-4624) on scaled values
 template <typename T>
@@ -339,7 +341,11 @@ __device__ inline void elem_fast(const FastLineT<T> &L, T &I, T ds, const FPtT<T
     const T hds = T(0.5) * ds;
     T dtau = O::fma(p1.sig, ah1, p0.sig * ah0) * hds;
     T theomax = O::fma(p1.sig, jh1, p0.sig * jh0) * hds;
-    c.tau += dtau;
+    {   // compensated sum: tau_c keeps the low-order part lost in tau
+        const T y = dtau - c.tau_c, t = c.tau + y;
+        c.tau_c = (t - c.tau) - y;
+        c.tau = t;
+    }
     T wa, wb, xp, ra;
     qdr_weights_f<T>(dtau, wa, wb, xp, ra);
     // source functions (transfer.F:1528-1541) from the scaled values; the
