@@ -422,9 +422,13 @@ __device__ inline void segment_fast(const FastRecT<T> &b, const FPtT<T> &p0, con
                                     FastCarryT<T> &c, int &ovs) {
     T ds = b.ds;
     const int fl = (int)b.flag;
-    if (fl & NF_RESET) { ds = T(0); c.init = 1; }
-    if (fl & NF_BLEND) { I = (T(1) - starfract) * I + starfract * istar; c.init = 1; }
-    if (fl & NF_STAR) { I = istar; c.init = 1; }
+    // flagged nodes are rare: the wave skips their handling unless one of
+    // its lanes has one (a reset segment's ds is already 0 in the record)
+    if (__builtin_amdgcn_ballot_w64(fl != 0)) {
+        if (fl & NF_RESET) { ds = T(0); c.init = 1; }
+        if (fl & NF_BLEND) { I = (T(1) - starfract) * I + starfract * istar; c.init = 1; }
+        if (fl & NF_STAR) { I = istar; c.init = 1; }
+    }
     SubGrid<T> sg;
     if (subgrid_of<T>(b, ds, p0.u, L.uch, sg)) {
         FPtT<T> pp = p0;
@@ -457,8 +461,7 @@ __device__ inline T trace_fast(const FastRecT<T> *__restrict__ rec, int n, const
         const FastRecT<T> b = rec[k];
         const FPtT<T> p1 = point_of<T>(b);
         segment_fast<T>(b, p0, p1, L, I, istar, starfract, c, ovs);
-        T ds = ((int)b.flag & NF_RESET) ? T(0) : b.ds;
-        if (b.sig * c.al0 * ds < T(-0.01f)) maser = 1;
+        if (b.sig * c.al0 * b.ds < T(-0.01f)) maser = 1;     // b.ds is 0 on a reset
         p0 = p1;
     }
     if (tau) *tau = c.tau;

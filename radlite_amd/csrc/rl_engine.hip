@@ -367,7 +367,7 @@ __device__ inline void nodes_ray_fast(int ray, int lane, const long long *__rest
         u_c = __shfl(gn.u, 63);
         if (!act) continue;
         FastRecT<T> r;
-        r.ds = (T)gn.ds;
+        r.ds = ((gn.flag >> 2) & NF_RESET) ? T(0) : (T)gn.ds;   // the hole chord adds nothing
         r.u = (T)gn.u;
         r.sig = (T)sig;
         r.jdh = (T)(v[CR_JD] / sig);
