@@ -26,6 +26,11 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
+# reference trace time / oracle trace time, one thread each, measured in the
+# build container (oracle/time_reference.py on the 100x60 disk)
+REF_OVER_PORT = 2.24
+
+
 def measured_traffic(precision, kernel):
     """HBM bytes per launch of the formal-solution kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counter values calibrated
@@ -431,6 +436,14 @@ def main():
             "sample": f"every {args.cpu_stride}th ray of both inclinations "
                       f"({nrc} ray*channels, {tc:.1f} s), FP64 C restatement, "
                       f"skip={args.skip}",
+            # the reference's own trace is REF_OVER_PORT times slower than
+            # the oracle per core (flang build of the reference, one thread
+            # each, 100x60 disk: profiles/cpu_reference_vs_oracle_c1.txt)
+            "reference_equivalent": {
+                "value": nrc / tc / REF_OVER_PORT, "cores": cores,
+                "ref_over_port_time": REF_OVER_PORT,
+                "gpu_speedup": out["value"] / (nrc / tc / REF_OVER_PORT),
+            },
         }
     print(json.dumps(out))
 

@@ -162,3 +162,33 @@ c
      %     (rp_b(i),i=1,nray), rp_theta0(1)
       close(77)
       end
+
+c     ------------------------------------------------------------------
+c     Timing: the link step also wraps calc_line_spectrum
+c     (telescope.F:1465, the per-line trace, called from main.F:1047);
+c     the wall time of the real routine is appended, per line, to
+c     probe_time.txt as "iline seconds", so that the reference's trace
+c     can be timed apart from its set-up and file output.
+c     ------------------------------------------------------------------
+      subroutine probe_calc_line_spectrum(iline,distance,passband,nfr)
+     %     bind(C, name='__wrap_calc_line_spectrum_')
+      implicit none
+      integer iline,nfr
+      doubleprecision distance,passband
+      integer*8 c0,c1,crate
+      interface
+         subroutine real_calc_line(iline,distance,passband,nfr)
+     %        bind(C, name='__real_calc_line_spectrum_')
+         integer iline,nfr
+         doubleprecision distance,passband
+         end subroutine
+      end interface
+      call system_clock(c0,crate)
+      call real_calc_line(iline,distance,passband,nfr)
+      call system_clock(c1)
+      open(unit=93,file='probe_time.txt',status='unknown',
+     %     position='append')
+      write(93,'(I6,1X,ES24.16)') iline, dble(c1-c0)/dble(crate)
+      close(93)
+      return
+      end
