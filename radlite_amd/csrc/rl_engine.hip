@@ -1149,8 +1149,11 @@ __device__ inline void carried_in(const FastRecT<T> &a, const FastLineT<T> &L, T
 // reverse pass: one wave per (ray, group of 64 channels); per node the
 // gradients on (jd, ad, jc, dc) / sig and per segment on the mean width,
 // summed over the wave's channels into gnode
+// (FP32: held to 128 VGPRs, four waves per SIMD instead of three; it was
+// 130, and the latency of the per-segment record loads and node sums needs
+// the extra wave more than the two registers)
 template <typename T>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 k_adj_rev(int nrays, int nfr, int nphi, int ray0, long long node0,
           const long long *__restrict__ off,
           const long long *__restrict__ ckoff, const FastRecT<T> *__restrict__ rec,
