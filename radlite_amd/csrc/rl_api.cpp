@@ -1142,7 +1142,10 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
                 HIPCHK(c, hipEventSynchronize(c->ev[3]));
                 float t = 0;
                 HIPCHK(c, hipEventElapsedTime(&t, c->ev[2], c->ev[3]));
-                t = t / (float)ntr;            // per tile (tile counts differ by mapping)
+                // the whole batch at this rate: tiles hold 64 lanes in every
+                // mapping, but a channel tiling leaves the lanes past the last
+                // channel idle, so the mappings have different tile counts
+                t = t / (float)ntr * (float)L.ntiles;
                 if (t < best) { best = t; cw = cws[ci]; nblk = grids[gi]; }
             }
         }

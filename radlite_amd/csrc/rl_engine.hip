@@ -822,15 +822,19 @@ __device__ inline bool tile_lane(int cw, long long lt, int lane, int nrays, int 
 // Persistent formal solution over all tiles of all items: each wave takes
 // the next tile from a device counter until every tile is taken, so the
 // grid drains by itself; items of any size share the machine.
+#define RL_FORMAL_BATCH_PARAMS                                                              \
+    const ItemDev *__restrict__ items, int nitems, const long long *__restrict__ tile0,     \
+        long long ntiles, int cw, const void *rec_all,                                      \
+        const RayInfo *__restrict__ rinfo_all, const double *__restrict__ chan_all,         \
+        double *__restrict__ image_all, unsigned long long *__restrict__ ovs_all,            \
+        int *__restrict__ maser_all, unsigned char *__restrict__ cmask_all,                 \
+        unsigned long long *counter
+#define RL_FORMAL_BATCH_ARGS                                                                \
+    items, nitems, tile0, ntiles, cw, rec_all, rinfo_all, chan_all, image_all, ovs_all,     \
+        maser_all, cmask_all, counter
+
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_formal_batch(const ItemDev *__restrict__ items, int nitems,
-               const long long *__restrict__ tile0, long long ntiles, int cw,
-               const void *rec_all, const RayInfo *__restrict__ rinfo_all,
-               const double *__restrict__ chan_all, double *__restrict__ image_all,
-               unsigned long long *__restrict__ ovs_all, int *__restrict__ maser_all,
-               unsigned char *__restrict__ cmask_all,
-               unsigned long long *counter) {
+__device__ __forceinline__ void formal_batch_body(RL_FORMAL_BATCH_PARAMS) {
     const auto *rec_base = reinterpret_cast<const typename RecOf<T>::type *>(rec_all);
     const int lane = threadIdx.x & 63;
     for (;;) {
@@ -874,6 +878,19 @@ k_formal_batch(const ItemDev *__restrict__ items, int nitems,
             if (maser) atomicOr(maser_all + lo, 1);
         }
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_formal_batch(RL_FORMAL_BATCH_PARAMS) {
+    formal_batch_body<T>(RL_FORMAL_BATCH_ARGS);
+}
+// FP32: held to 72 VGPRs (seven waves per SIMD instead of six); the few
+// spilled values cost less than the extra wave gains on this
+// latency-bound loop (the FP64 kernel keeps its registers)
+template <>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
+k_formal_batch<float>(RL_FORMAL_BATCH_PARAMS) {
+    formal_batch_body<float>(RL_FORMAL_BATCH_ARGS);
 }
 
 // The same tiles on a static grid: wave w of block b takes tile 4b + w
