@@ -880,13 +880,16 @@ __device__ __forceinline__ void formal_batch_body(RL_FORMAL_BATCH_PARAMS) {
     }
 }
 
+// FP64 (the primary template; FP32 is specialised below): held to 128
+// VGPRs, four waves per SIMD instead of three (139 registers)
 template <typename T>
-__global__ void __launch_bounds__(256) k_formal_batch(RL_FORMAL_BATCH_PARAMS) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_formal_batch(RL_FORMAL_BATCH_PARAMS) {
     formal_batch_body<T>(RL_FORMAL_BATCH_ARGS);
 }
 // FP32: held to 72 VGPRs (seven waves per SIMD instead of six); the few
 // spilled values cost less than the extra wave gains on this
-// latency-bound loop (the FP64 kernel keeps its registers)
+// latency-bound loop
 template <>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
 k_formal_batch<float>(RL_FORMAL_BATCH_PARAMS) {
