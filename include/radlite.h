@@ -140,6 +140,17 @@ int rl_setup_rays_rectang(rl_ctx *ctx, int nincl, const double *incl_rad, int nx
 int rl_trace_lines_posvel(rl_ctx *ctx, int nitems, const rl_item *items,
                           double *image, double *tau, unsigned *warn_bits);
 
+/* Resampling of a circular image onto a rectangular grid: the
+ * interpolation step of imcir_conv (PRO/imcir_conv.pro:161-181, IDL
+ * triangulate + trigrid).  image: one item's [nrays][nfr] intensities as
+ * rl_trace_lines_forward returns them; per output pixel nv vertex rays
+ * vtx[npix][nv] (-1: none) and weights w[npix][nv] (the barycentric
+ * weights of the pixel in its triangle, times any per-vertex factor); out
+ * receives [nfr][npix].  The reference has no such library routine: this
+ * replaces the IDL post-processing of the lineposvelcirc file. */
+int rl_resample_image(rl_ctx *ctx, int nrays, int nfr, const double *image, int npix,
+                      int nv, const int *vtx, const double *w, double *out);
+
 /* Adjoint of the trace for the same items (every item must use
  * RL_SKIP_NONE).  Given dL/dflux (same layout as flux), returns dL/d(the
  * inputs the trace reads) and, in flux (optional), the spectrum.  When

@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 LIB = os.path.join(HERE, "librl.so")
-SOURCES = ["csrc/rl_engine.hip", "csrc/rl_prepass.hip", "csrc/rl_api.cpp"]
+SOURCES = ["csrc/rl_engine.hip", "csrc/rl_prepass.hip", "csrc/rl_resample.hip",
+           "csrc/rl_api.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
     # FP64 parity mode rounds every a*b+c as two operations, like the

@@ -1353,6 +1353,41 @@ extern "C" int rl_trace_lines_forward_ex(rl_ctx *c, int nitems, const rl_item *i
     return 0;
 }
 
+// Circular image -> rectangular grid (the interpolation of imcir_conv,
+// PRO/imcir_conv.pro:161-181): out[ch][p] = sum_k w[p][k] image[vtx[p][k]][ch],
+// the triangulation weights formed by the caller.  Every vertex index is
+// checked against nrays before the launch.
+extern "C" int rl_resample_image(rl_ctx *c, int nrays, int nfr, const double *image, int npix,
+                                 int nv, const int *vtx, const double *w, double *out) {
+    if (!c || nrays < 1 || nfr < 1 || npix < 0 || nv < 1 || !image || (npix && (!vtx || !w || !out)))
+        return fail(c, RL_E_ARG, "rl_resample_image: bad args");
+    const size_t nvt = (size_t)npix * nv;
+    for (size_t i = 0; i < nvt; i++)
+        if (vtx[i] >= nrays || vtx[i] < -1)
+            return fail(c, RL_E_ARG, "rl_resample_image: vertex %d out of range", vtx[i]);
+    if (npix == 0) return 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    DevBuf d_img, d_vtx, d_w, d_out;
+    if (int e = upload(c, d_img, image, (size_t)nrays * nfr * 8)) return e;
+    if (int e = upload(c, d_vtx, vtx, nvt * 4)) return e;
+    if (int e = upload(c, d_w, w, nvt * 8)) return e;
+    HIPCHK(c, d_out.ensure((size_t)npix * nfr * 8));
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    rl::launch_rect_gather(c->stream, npix, nfr, nv, d_vtx.as<int>(), d_w.as<double>(),
+                           d_img.as<double>(), d_out.as<double>());
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, d_out.p, (size_t)npix * nfr * 8, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));        // the buffers are freed on return
+    float t = 0;
+    (void)hipEventElapsedTime(&t, c->ev[1], c->ev[4]);
+    for (int i = 0; i < 5; i++) c->t_ms[i] = 0.0;
+    c->t_ms[2] = t;
+    c->t_ms[4] = t;
+    return 0;
+}
+
 // Position-velocity cubes (calc_write_line_posvel, telescope.F:1828-2031;
 // make_freq_image_rectang, telescope.F:2061-2215): per item, the
 // intensity and the optical depth of every pixel ray at every channel,

@@ -225,4 +225,10 @@ struct PrepassLaunch {
 void launch_prepass(hipStream_t st, const PrepassLaunch &a);
 void launch_prepass_adjoint(hipStream_t st, const PrepassLaunch &a);
 
+// Resampling of a circular image onto a grid (rl_resample.hip): out[ch][p]
+// = sum_k w[p][k] img[vtx[p][k]][ch] over nv vertices per pixel (vtx < 0:
+// no contribution)
+void launch_rect_gather(hipStream_t st, int npix, int nfr, int nv, const int *vtx,
+                        const double *w, const double *img, double *out);
+
 }  // namespace rl

@@ -74,6 +74,8 @@ def load_library(path=LIB_PATH):
                                              C.POINTER(C.c_uint)]),
         "rl_trace_lines_posvel": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
                                             C.POINTER(C.c_uint)]),
+        "rl_resample_image": (C.c_int, [vp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, _ip, _dp,
+                                        _dp]),
         "rl_trace_lines_forward_ex": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
                                                 C.POINTER(C.c_ubyte), C.POINTER(C.c_uint)]),
         "rl_trace_lines_adjoint": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
@@ -327,6 +329,21 @@ class Raytracer:
                 ta.append(tau[o:o + n].reshape(ny, nx, it[2]))
             o += n
         return im, (ta if tau is not None else None), warn
+
+    def resample_image(self, image, vtx, w):
+        """image [nrays, nfr] resampled with per-pixel vertices vtx [npix, nv]
+        (-1: none) and weights w [npix, nv] on the device.  Returns
+        [nfr, npix]."""
+        img = np.ascontiguousarray(image, dtype=np.float64)
+        vt = np.ascontiguousarray(vtx, dtype=np.int32)
+        ww = np.ascontiguousarray(w, dtype=np.float64)
+        nrays, nfr = img.shape
+        npix, nv = vt.shape
+        out = np.zeros((nfr, npix))
+        self._chk(self.L.rl_resample_image(self.ctx, nrays, nfr, img.ctypes.data_as(_dp), npix, nv,
+                                           vt.ctypes.data_as(_ip), ww.ctypes.data_as(_dp),
+                                           out.ctypes.data_as(_dp)), "rl_resample_image")
+        return out
 
     def trace(self, items, want_image=False, want_cmask=False):
         """items: list of (line, incl, nfr, vmax_kms, skip_mode).

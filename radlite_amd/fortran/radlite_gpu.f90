@@ -117,6 +117,15 @@ module radlite_gpu
             real(c_double), intent(out) :: image(*), tau(*)
             integer(c_int), intent(out) :: warn_bits(*)
         end function
+        integer(c_int) function rl_resample_image(ctx, nrays, nfr, image, npix, nv, vtx, w, out) &
+                bind(C, name="rl_resample_image")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nrays, nfr, npix, nv
+            real(c_double), intent(in) :: image(*), w(*)
+            integer(c_int), intent(in) :: vtx(*)
+            real(c_double), intent(out) :: out(*)
+        end function
         integer(c_int) function rl_sync(ctx) bind(C, name="rl_sync")
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx

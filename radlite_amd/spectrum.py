@@ -394,8 +394,9 @@ def _fits_bintable(cols):
 
 
 def read_fits(fname):
-    """The header cards and binary tables of a FITS file this module wrote:
-    list of (header dict, {column: values} or None)."""
+    """The header cards and data of a FITS file this module (or
+    radlite_amd.imcir) wrote: list of (header dict, {column: values} for a
+    binary table, the array for an image, or None)."""
     with open(fname, "rb") as f:
         raw = f.read()
     out, pos = [], 0
@@ -417,7 +418,16 @@ def read_fits(fname):
                 hdr[card[:8].strip()] = val
         pos += (-pos) % 2880
         table = None
-        if hdr.get("XTENSION") == "BINTABLE":
+        naxis = int(hdr.get("NAXIS", 0))
+        if hdr.get("XTENSION") != "BINTABLE" and naxis > 0:
+            # an image: big-endian, NAXIS1 fastest
+            shape = [int(hdr[f"NAXIS{k}"]) for k in range(naxis, 0, -1)]
+            dt = {-64: ">f8", -32: ">f4", 8: "u1", 16: ">i2", 32: ">i4", 64: ">i8"}[int(hdr["BITPIX"])]
+            nb = int(np.prod(shape)) * np.dtype(dt).itemsize
+            table = np.frombuffer(raw[pos:pos + nb], dtype=dt).reshape(shape).astype(np.float64)
+            pos += nb
+            pos += (-pos) % 2880
+        elif hdr.get("XTENSION") == "BINTABLE":
             nrow, rowb = hdr["NAXIS2"], hdr["NAXIS1"]
             table, off = {}, 0
             for k in range(1, hdr["TFIELDS"] + 1):
