@@ -47,6 +47,19 @@ def measured_traffic(precision, kernel):
     return None
 
 
+def measured_valu_issue(precision):
+    """VALU wave-instructions per CU-cycle of the formal kernel over the FP32
+    wave64 issue peak (2 per CU-cycle), from the committed rocprofv3 SQ
+    counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"sq_counters_c2_{precision}_forward.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if "VALU wave-instructions per CU-cycle" in line:
+            return float(line.split("=")[1].split()[0]) / 2.0
+    return None
+
+
 def bench_adjoint(args):
     """Forward vs forward+adjoint on the 200x100 disk (34,049 rays x 100
     channels, 45 deg, every channel integrated).  Forward: the spectrum
@@ -407,6 +420,9 @@ def main():
             "tile_cw": tile_cw, "persistent_blocks": tile_blocks,
             "valu_tflops_algorithmic": achieved_tf,
             "valu_frac_of_fp32_peak": achieved_tf / FP32_PEAK_TFLOPS,
+            # the kernel is VALU-issue bound: measured issue rate over peak
+            "valu_issue_frac_measured": (measured_valu_issue(args.precision)
+                                         if args.workload == "c2" else None),
         },
     }
     rt.close()
