@@ -160,14 +160,17 @@ __device__ inline void clever_elem(const LineK<T> &L, T &I, T ds,
     const T crit = T(3.0);
     T lw = T(0.5) * (lw0 + lw1);
     T x = fabs(ddiv<T>(p1.u - p0.u, ddiv<T>(lw, T(2.99792458e5))));
+    // the last (or only) element has one call site: a segment integrated
+    // whole keeps a = p0, sa = 0 (ds - 0 = ds exactly) and its node widths
+    Pt<T> a = p0;
+    T sa = T(0), lwa = lw0, lwb = lw1;
     if (T(2.0) * crit * x > T(1.0)) {
         T s_c = ddiv<T>(ds * (L.uch - p0.u), p1.u - p0.u);
         T dls = ddiv<T>(ds, x);
         T sright = s_c + crit * dls, sleft = s_c - crit * dls;
         if (sright > T(0) && sleft < ds) {
             T lg_ds = ddiv<T>(sright - sleft, T(31 - 1.0));
-            Pt<T> a = p0;
-            T sa = T(0);
+            lwa = lwb = lw;
             // Only sub-points with 0 < s < ds are used; visiting the index
             // range that can contain them (with one index of margin on
             // each side) and keeping the exact test gives the same points.
@@ -192,11 +195,9 @@ __device__ inline void clever_elem(const LineK<T> &L, T &I, T ds,
                     sa = s;
                 }
             }
-            integ_elem<T>(L, I, ds - sa, a, p1, lw, lw, c, ovs);
-            return;
         }
     }
-    integ_elem<T>(L, I, ds, p0, p1, lw0, lw1, c, ovs);
+    integ_elem<T>(L, I, ds - sa, a, p1, lwa, lwb, c, ovs);
 }
 
 // ---------------------------------------------------------------------
@@ -429,10 +430,14 @@ __device__ inline void segment_fast(const FastRecT<T> &b, const FPtT<T> &p0, con
         if (fl & NF_BLEND) { I = (T(1) - starfract) * I + starfract * istar; c.init = 1; }
         if (fl & NF_STAR) { I = istar; c.init = 1; }
     }
+    // one call for the last (or only) element of every segment: a segment
+    // integrated whole has sa = 0 and ds - 0 = ds exactly; the single call
+    // site keeps the loop's carried values in one set of registers (two
+    // call sites cost register copies at their merge and a spill)
     SubGrid<T> sg;
+    FPtT<T> pp = p0;
+    T sa = T(0);
     if (subgrid_of<T>(b, ds, p0.u, L.uch, sg)) {
-        FPtT<T> pp = p0;
-        T sa = T(0);
         const T inv_ds = FOps<T>::fdiv(T(1), ds);
         for (int i = sg.ilo; i <= sg.ihi; i++) {
             T s = sg.sleft + (T)(i - 1) * sg.lg;
@@ -443,10 +448,8 @@ __device__ inline void segment_fast(const FastRecT<T> &b, const FPtT<T> &p0, con
                 sa = s;
             }
         }
-        elem_fast<T>(L, I, ds - sa, pp, p1, b.inv_aa, b.norm, c, ovs);
-    } else {
-        elem_fast<T>(L, I, ds, p0, p1, b.inv_aa, b.norm, c, ovs);
     }
+    elem_fast<T>(L, I, ds - sa, pp, p1, b.inv_aa, b.norm, c, ovs);
 }
 
 // charintline (telescope.F:3889-4312) on FastRecT data
