@@ -887,18 +887,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_formal_batch(RL_FORMAL_BATCH_PARAMS) {
     formal_batch_body<T>(RL_FORMAL_BATCH_ARGS);
 }
-// FP32: held to 72 VGPRs (seven waves per SIMD instead of six); the few
-// spilled values cost less than the extra wave gains on this
-// latency-bound loop
+// FP32: held to 64 VGPRs (eight waves per SIMD; 71 registers unconstrained,
+// seven waves); the spilled values are per-tile state, reloaded once per
+// tile outside the segment loop, so the extra wave costs nothing there
 template <>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 k_formal_batch<float>(RL_FORMAL_BATCH_PARAMS) {
     formal_batch_body<float>(RL_FORMAL_BATCH_ARGS);
 }
 
 // The same tiles on a static grid: wave w of block b takes tile 4b + w
-// (the hardware dispatcher balances blocks); fewer registers than the
-// persistent loop.
+// (the hardware dispatcher balances blocks); no tile counter and no tile
+// loop.
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_formal_tiles(const ItemDev *__restrict__ items, int nitems,
