@@ -228,11 +228,9 @@ __device__ inline T segment_rev(const FastRecT<T> &b, const FPtT<T> &p0, const F
     }
     {
         // forward over the sub-steps, keeping I at each element start
-        FastCarryT<T> c{jl0c, al0c, T(0), init};
-        if (!init) {
-            const T ah = p0.ad + al0c;
-            c.S0 = ah > T(0) ? FOps<T>::sdiv(p0.jd + jl0c, ah) : T(0);
-        }
+        FastCarryT<T> c;
+        c.init = init;
+        if (!init) carry_start<T>(p0, jl0c, al0c, c);
         int ovs = 0;
         FPtT<T> pp = p0;
         T sa = T(0);

@@ -318,30 +318,57 @@ using FastLine = FastLineT<float>;
 template <typename T> struct FPtT { T u, sig, jd, ad, jc, dc; };   // one point, scaled values
 using FPt = FPtT<float>;
 
-// line j, alpha / sig at the start; S0 the start's source function; the
-// optical depth along the ray, summed with a compensation term (Kahan)
-template <typename T> struct FastCarryT { T jl0, al0, S0; int init; T tau = T(0), tau_c = T(0); };
+// What an element needs of its start point, carried from the element that
+// ended there (the start point of an element is the end point of the
+// previous one unless init is set): sa0 = sig*alpha/sig and sj0 = sig*j/sig
+// there (the products the optical depth and the bracket add), ah0 =
+// alpha/sig (its sign picks the source functions), S0 its source
+// function, al0 its line alpha/sig (the maser test); u0 the velocity of
+// the segment's start node (the sub-grid test); the optical depth along
+// the ray, summed with a compensation term (Kahan).  Carrying these
+// instead of the start point keeps the loop from copying six values per
+// segment; the point is read again from its record where more of it is
+// needed (init, sub-grid).
+template <typename T> struct FastCarryT {
+    T sa0 = T(0), sj0 = T(0), ah0 = T(0), S0 = T(0), al0 = T(0);
+    int init = 1;
+    T u0 = T(0), tau = T(0), tau_c = T(0);
+};
 
-// integrate_element_linedust (line.F:4515-4624) on scaled values
+// the carry of a start point p0 with line values jl0, al0 (/ sig)
 template <typename T>
-__device__ inline void elem_fast(const FastLineT<T> &L, T &I, T ds, const FPtT<T> &p0,
+__device__ inline void carry_start(const FPtT<T> &p0, T jl0, T al0, FastCarryT<T> &c) {
+    const T jh0 = p0.jd + jl0, ah0 = p0.ad + al0;
+    c.sa0 = p0.sig * ah0;
+    c.sj0 = p0.sig * jh0;
+    c.ah0 = ah0;
+    c.S0 = ah0 > T(0) ? FOps<T>::sdiv(jh0, ah0) : T(0);
+    c.al0 = al0;
+}
+
+// integrate_element_linedust (line.F:4515-4624) on scaled values.  The
+// start point is taken from the carry; start() supplies it (an FPtT) only
+// when init is set.
+template <typename T, typename Start>
+__device__ inline void elem_core(const FastLineT<T> &L, T &I, T ds, const Start &start,
                                  const FPtT<T> &p1, T inv_aa, T norm, FastCarryT<T> &c,
                                  int &ovs) {
-    if (c.init) {
-        T z0 = FOps<T>::fma(-L.nu0, p0.u, L.dnu) * inv_aa;
-        T phi0 = norm * FOps<T>::ex(-(z0 * z0));
-        c.jl0 = p0.jc * phi0;
-        c.al0 = p0.dc * phi0;
-    }
     using O = FOps<T>;
+    if (c.init) {
+        const FPtT<T> p0 = start();
+        T z0 = O::fma(-L.nu0, p0.u, L.dnu) * inv_aa;
+        T phi0 = norm * O::ex(-(z0 * z0));
+        carry_start<T>(p0, p0.jc * phi0, p0.dc * phi0, c);
+    }
     T z1 = O::fma(-L.nu0, p1.u, L.dnu) * inv_aa;
     T phi1 = norm * O::ex(-(z1 * z1));
     T jl1 = p1.jc * phi1, al1 = p1.dc * phi1;
-    T jh0 = p0.jd + c.jl0, jh1 = p1.jd + jl1;      // j / sig
-    T ah0 = p0.ad + c.al0, ah1 = p1.ad + al1;      // alpha / sig
+    T jh1 = p1.jd + jl1;                          // j / sig
+    T ah1 = p1.ad + al1;                          // alpha / sig
+    const T ah0 = c.ah0;
     const T hds = T(0.5) * ds;
-    T dtau = O::fma(p1.sig, ah1, p0.sig * ah0) * hds;
-    T theomax = O::fma(p1.sig, jh1, p0.sig * jh0) * hds;
+    T dtau = O::fma(p1.sig, ah1, c.sa0) * hds;
+    T theomax = O::fma(p1.sig, jh1, c.sj0) * hds;
     {   // compensated sum: tau_c keeps the low-order part lost in tau
         const T y = dtau - c.tau_c, t = c.tau + y;
         c.tau_c = (t - c.tau) - y;
@@ -349,20 +376,28 @@ __device__ inline void elem_fast(const FastLineT<T> &L, T &I, T ds, const FPtT<T
     }
     T wa, wb, xp, ra;
     qdr_weights_f<T>(dtau, wa, wb, xp, ra);
-    // source functions (transfer.F:1528-1541) from the scaled values; the
-    // start point equals the previous end point unless init is set
+    // source functions (transfer.F:1528-1541) from the scaled values
     T q1 = ah1 > T(0) ? FOps<T>::sdiv(jh1, ah1) : T(0);
-    T q0 = c.init ? (ah0 > T(0) ? FOps<T>::sdiv(jh0, ah0) : T(0)) : c.S0;
+    T q0 = c.S0;
     T s1 = ah0 > T(0) ? q0 : (ah1 > T(0) ? q1 : T(0));
     T s2 = ah1 > T(0) ? q1 : (ah0 > T(0) ? q0 : T(0));
     T q = dtau > T(1e-9f) ? O::fma(wa, s1, wb * s2) : theomax;
     ovs += q > T(2) * theomax;
     q = O::fmn(q, theomax);
     I = O::fma(I, xp, q);
-    c.jl0 = jl1;
-    c.al0 = al1;
+    c.sa0 = p1.sig * ah1;
+    c.sj0 = p1.sig * jh1;
+    c.ah0 = ah1;
     c.S0 = q1;
+    c.al0 = al1;
     c.init = 0;
+}
+
+template <typename T>
+__device__ inline void elem_fast(const FastLineT<T> &L, T &I, T ds, const FPtT<T> &p0,
+                                 const FPtT<T> &p1, T inv_aa, T norm, FastCarryT<T> &c,
+                                 int &ovs) {
+    elem_core<T>(L, I, ds, [&]() { return p0; }, p1, inv_aa, norm, c, ovs);
 }
 
 // Sub-grid points of segment k (clever_integrate_element_linedust,
@@ -416,11 +451,13 @@ __device__ inline FPtT<T> point_of(const FastRecT<T> &a) {
     return FPtT<T>{a.u, a.sig, a.jdh, a.adh, a.jch, a.dch};
 }
 
-// one segment (nodes k-1 -> k) of charintline on FastRecT data
+// one segment (nodes k-1 -> k) of charintline on FastRecT data; prev is
+// the record of node k-1, read only for a flagged or sub-gridded segment
+// (the carry holds what a whole segment needs of its start)
 template <typename T>
-__device__ inline void segment_fast(const FastRecT<T> &b, const FPtT<T> &p0, const FPtT<T> &p1,
-                                    const FastLineT<T> &L, T &I, T istar, T starfract,
-                                    FastCarryT<T> &c, int &ovs) {
+__device__ inline void segment_fast(const FastRecT<T> &b, const FastRecT<T> *prev,
+                                    const FPtT<T> &p1, const FastLineT<T> &L, T &I, T istar,
+                                    T starfract, FastCarryT<T> &c, int &ovs) {
     T ds = b.ds;
     const int fl = (int)b.flag;
     // flagged nodes are rare: the wave skips their handling unless one of
@@ -431,13 +468,14 @@ __device__ inline void segment_fast(const FastRecT<T> &b, const FPtT<T> &p0, con
         if (fl & NF_STAR) { I = istar; c.init = 1; }
     }
     // one call for the last (or only) element of every segment: a segment
-    // integrated whole has sa = 0 and ds - 0 = ds exactly; the single call
-    // site keeps the loop's carried values in one set of registers (two
-    // call sites cost register copies at their merge and a spill)
+    // integrated whole has sa = 0 and ds - 0 = ds exactly, and its start
+    // point is node k-1; after a sub-element init is 0, so the start point
+    // is not read
     SubGrid<T> sg;
-    FPtT<T> pp = p0;
     T sa = T(0);
-    if (subgrid_of<T>(b, ds, p0.u, L.uch, sg)) {
+    if (subgrid_of<T>(b, ds, c.u0, L.uch, sg)) {
+        const FPtT<T> p0 = point_of<T>(*prev);
+        FPtT<T> pp = p0;
         const T inv_ds = FOps<T>::fdiv(T(1), ds);
         for (int i = sg.ilo; i <= sg.ihi; i++) {
             T s = sg.sleft + (T)(i - 1) * sg.lg;
@@ -449,7 +487,9 @@ __device__ inline void segment_fast(const FastRecT<T> &b, const FPtT<T> &p0, con
             }
         }
     }
-    elem_fast<T>(L, I, ds - sa, pp, p1, b.inv_aa, b.norm, c, ovs);
+    elem_core<T>(L, I, ds - sa, [&]() { return point_of<T>(*prev); }, p1, b.inv_aa, b.norm, c,
+                 ovs);
+    c.u0 = p1.u;
 }
 
 // charintline (telescope.F:3889-4312) on FastRecT data
@@ -458,14 +498,12 @@ __device__ inline T trace_fast(const FastRecT<T> *__restrict__ rec, int n, const
                                T I0, T istar, T starfract, int &ovs, int &maser,
                                T *tau = nullptr) {
     T I = I0;
-    FastCarryT<T> c{T(0), T(0), T(0), 1};
-    FPtT<T> p0 = point_of<T>(rec[0]);
+    FastCarryT<T> c;
+    c.u0 = rec[0].u;
     for (int k = 1; k < n; k++) {
         const FastRecT<T> b = rec[k];
-        const FPtT<T> p1 = point_of<T>(b);
-        segment_fast<T>(b, p0, p1, L, I, istar, starfract, c, ovs);
+        segment_fast<T>(b, rec + k - 1, point_of<T>(b), L, I, istar, starfract, c, ovs);
         if (b.sig * c.al0 * b.ds < T(-0.01f)) maser = 1;     // b.ds is 0 on a reset
-        p0 = p1;
     }
     if (tau) *tau = c.tau;
     return I;

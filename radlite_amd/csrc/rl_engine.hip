@@ -1094,16 +1094,14 @@ k_adj_fwd(int nrays, int nfr, int ray0, long long node0, const long long *__rest
     const T sf = ray0 + ray == 0 ? (T)lc.starfract : T(0);
     const T ist = (T)istar[ch];
     T I = (T)lc.I0;
-    FastCarryT<T> c{T(0), T(0), T(0), 1};
+    FastCarryT<T> c;
+    c.u0 = rec[b].u;
     int ovs = 0;
     T *ck = ckpt + ckoff[ray] * nfr + ch;
-    FPtT<T> p0 = point_of<T>(rec[b]);
     for (int k = 1; k < n; k++) {
         if ((k - 1) % kblk == 0) ck[(long long)((k - 1) / kblk) * nfr] = I;
         const FastRecT<T> r = rec[b + k];
-        const FPtT<T> p1 = point_of<T>(r);
-        segment_fast<T>(r, p0, p1, L, I, ist, sf, c, ovs);
-        p0 = p1;
+        segment_fast<T>(r, rec + b + k - 1, point_of<T>(r), L, I, ist, sf, c, ovs);
     }
     image[g] = (double)I;
 }
@@ -1251,23 +1249,22 @@ k_adj_rev(int nrays, int nfr, int nphi, int ray0, long long node0,
             const int k0 = blk * kblk + 1, k1 = min(k0 + kblk - 1, nseg);
             // forward over the block from its checkpoint
             T I = ck[(long long)blk * nfr];
-            FastCarryT<T> c{T(0), T(0), T(0), k0 == 1 ? 1 : 0};
+            FastCarryT<T> c;
+            c.init = k0 == 1 ? 1 : 0;
             if (k0 > 1) {
                 const FastRecT<T> a = rec[b + k0 - 1];
-                T z_, phi_;
-                carried_in<T>(a, L, c.jl0, c.al0, z_, phi_);
-                const T ah = a.adh + c.al0;
-                c.S0 = ah > T(0) ? FOps<T>::sdiv(a.jdh + c.jl0, ah) : T(0);
+                T jl0, al0, z_, phi_;
+                carried_in<T>(a, L, jl0, al0, z_, phi_);
+                carry_start<T>(point_of<T>(a), jl0, al0, c);
             }
+            c.u0 = rec[b + k0 - 1].u;
             int ovs = 0;
-            FPtT<T> p0 = point_of<T>(rec[b + k0 - 1]);
             for (int k = k0; k <= k1; k++) {
                 iseg[k - k0] = I;
                 if (k == k1) break;                // the block's last segment is only reversed
                 const FastRecT<T> r = rec[b + k];
-                const FPtT<T> p1 = point_of<T>(r);
-                if (active) segment_fast<T>(r, p0, p1, L, I, ist, sf, c, ovs);
-                p0 = p1;
+                if (active)
+                    segment_fast<T>(r, rec + b + k - 1, point_of<T>(r), L, I, ist, sf, c, ovs);
             }
             // backwards over the block
             for (int k = k1; k >= k0; k--) reverse(k, iseg[k - k0]);
