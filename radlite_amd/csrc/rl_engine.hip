@@ -1097,9 +1097,18 @@ k_adj_fwd(int nrays, int nfr, int ray0, long long node0, const long long *__rest
     FastCarryT<T> c;
     c.u0 = rec[b].u;
     int ovs = 0;
+    // I before segments 1, 1 + kblk, 1 + 2 kblk, ... into consecutive slots
+    // (a countdown: a remainder by the run-time kblk costs an emulated
+    // division per segment)
     T *ck = ckpt + ckoff[ray] * nfr + ch;
+    int until = 0;                           // segments until the next checkpoint
     for (int k = 1; k < n; k++) {
-        if ((k - 1) % kblk == 0) ck[(long long)((k - 1) / kblk) * nfr] = I;
+        if (until == 0) {
+            *ck = I;
+            ck += nfr;
+            until = kblk;
+        }
+        until--;
         const FastRecT<T> r = rec[b + k];
         segment_fast<T>(r, rec + b + k - 1, point_of<T>(r), L, I, ist, sf, c, ovs);
     }
