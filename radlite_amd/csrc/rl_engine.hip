@@ -1133,14 +1133,18 @@ __device__ inline void wave_add5(double *out, T a0, T a1, T a2, T a3, T a4) {
 }
 
 // Node gradients of a wave, summed over its lanes in batches of RED_N
-// nodes through LDS: each lane writes its values in row `lane` (41 words,
-// so that the writes of 64 lanes fall in distinct banks), then lane p sums
-// column p (node p/5, quantity p%5) over the 64 rows.
-constexpr int RED_N = 8;
+// nodes through LDS: each lane writes its values in row `lane` (RED_W
+// words, odd, so that the writes of 64 lanes fall in distinct banks), then
+// lane p sums column p (node p/5, quantity p%5) over the 64 rows.  Seven
+// nodes: 64 x 37 x 4 B = 9.3 KB per wave, so that the sixteen waves per CU
+// that the reverse sweep's registers allow also fit in LDS (eight nodes,
+// 10.5 KB, left room for fifteen: 13.7 -> 12.6 ms on the 200x100 disk).
+constexpr int RED_N = 7;
+constexpr int RED_W = RED_N * 5 + 2;
 
 template <typename T>
 struct WaveNodeSum {
-    T (*row)[RED_N * 5 + 1];   // [64][41] in LDS
+    T (*row)[RED_W];           // [64][RED_W] in LDS
     long long *node;           // [RED_N] in LDS
     int n = 0;
 
@@ -1188,7 +1192,7 @@ k_adj_rev(int nrays, int nfr, int nphi, int ray0, long long node0,
           const double *__restrict__ istar, LineConst lc, const double *__restrict__ gflux,
           const double *__restrict__ ring_w, double center_w, double d2,
           const T *__restrict__ ckpt, double *__restrict__ gnode, int kblk) {
-    __shared__ T red_row[64][RED_N * 5 + 1];
+    __shared__ T red_row[64][RED_W];
     __shared__ long long red_node[RED_N];
     const int ray = blockIdx.x;
     const int lane = threadIdx.x;
