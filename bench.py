@@ -24,11 +24,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
 
 
-# reference trace time / oracle trace time, one thread each, measured in the
-# build container (oracle/time_reference.py on the 100x60 disk)
-REF_OVER_PORT = 2.24
 
 
 def measured_traffic(precision, kernel):
@@ -91,19 +89,26 @@ def bench_adjoint(args):
         return rt.trace_tape(items)
 
     res = {}
-    for name, fn in (("fwd", fwd), ("fwd_tape", fwd_tape), ("fwd_adj", fwd_adj)):
-        for _ in range(max(1, args.warmup)):
-            fn()
-        rt.L.rl_sync(rt.ctx)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            fn()
-        rt.L.rl_sync(rt.ctx)
-        res[name] = (time.perf_counter() - t0) / args.steps
-        if name == "fwd_adj":
-            st = rt.stats()
-            res["adj_device_ms"] = st["t_formal_ms"]
-            res["kblk"], res["used_tape"], _ = rt.adjoint_info()
+    # checkpoint interval chosen by memory (every segment kept when it fits),
+    # then fixed at 16 segments, the interval the 200x100 workload states
+    for ck, sfx in ((0, ""), (16, "_k16")):
+        rt.set_option(engine.OPT_CKPT, ck)
+        for name, fn in (("fwd", fwd), ("fwd_tape", fwd_tape), ("fwd_adj", fwd_adj)):
+            if ck and name == "fwd":
+                continue
+            for _ in range(max(1, args.warmup)):
+                fn()
+            rt.L.rl_sync(rt.ctx)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                fn()
+            rt.L.rl_sync(rt.ctx)
+            res[name + sfx] = (time.perf_counter() - t0) / args.steps
+            if name == "fwd_adj":
+                st = rt.stats()
+                res["adj_device_ms" + sfx] = st["t_formal_ms"]
+                res["kblk" + sfx], res["used_tape" + sfx], _ = rt.adjoint_info()
+    rt.set_option(engine.OPT_CKPT, 0)
     nominal = rt.nrays[0] * nfr
     rt.close()
     return {
@@ -118,7 +123,71 @@ def bench_adjoint(args):
         "ratio_fwd_adj_over_fwd": res["fwd_adj"] / res["fwd"],
         "value_fwd": nominal / res["fwd"],
         "value_fwd_adj": nominal / res["fwd_adj"],
+        "k16": {
+            "checkpoint_interval": res["kblk_k16"],
+            "adjoint_from_tape": bool(res["used_tape_k16"]),
+            "fwd_tape_ms": res["fwd_tape_k16"] * 1e3,
+            "fwd_adj_ms": res["fwd_adj_k16"] * 1e3,
+            "adjoint_call_device_ms": res["adj_device_ms_k16"],
+            "ratio_fwd_adj_over_fwd": res["fwd_adj_k16"] / res["fwd"],
+            "value_fwd_adj": nominal / res["fwd_adj_k16"],
+        },
         "unit": "ray*channel/s",
+    }
+
+
+def bench_c5_slice(args):
+    """A measured slice of the large workload: the 1024x512 disk (529,409
+    rays per inclination, 300 channels, non-LTE populations of the
+    366-level ro-vibrational ladder) with the first 2 of its 1000 lines at
+    all 4 inclinations, FP32.  Forward: the 8 items in one
+    rl_trace_lines_forward call (batches cut to the memory budget,
+    trajectories of other inclinations evicted and rebuilt as the whole
+    job would).  Adjoint: one item's gradient with respect to the
+    populations (checkpoint interval and ray chunks chosen to fit)."""
+    from radlite_amd import engine, synth
+    t0 = time.perf_counter()
+    ri, items, incl = synth.c5_inputs(0, keep_lines=2)
+    incs = [x * 0.0174532925199 for x in incl]
+    its = [(il, ii, n, v, engine.SKIP_NONE) for (il, ii, n, v) in items]
+    rt = engine.Raytracer(0, "fp32")
+    rt.load_run(ri)
+    rt.setup_rays(incs, ri.nphi, ri.b_extra, ri.dbdr)
+    t_setup = time.perf_counter() - t0
+    print(f"[c5] inputs, tables and ray sets: {t_setup:.1f} s", file=sys.stderr, flush=True)
+    fwd = []
+    for rep in range(2):                          # the first pass tunes and builds
+        t1 = time.perf_counter()
+        rt.trace(its)
+        fwd.append(time.perf_counter() - t1)
+        st = rt.stats()
+        print(f"[c5] forward pass {rep}: {fwd[-1]:.2f} s, batches {rt.batch_info()}",
+              file=sys.stderr, flush=True)
+    nominal = sum(int(rt.nrays[it[1]]) * it[2] for it in its)
+    mem = rt.memory_info()
+    g = [np.random.default_rng(1).normal(size=its[0][2])]
+    adj = []
+    for rep in range(2):
+        t1 = time.perf_counter()
+        _, gr = rt.adjoint([its[0]], g, outputs=("g_pops",))
+        adj.append(time.perf_counter() - t1)
+        print(f"[c5] adjoint item pass {rep}: {adj[-1]:.2f} s, (K, tape, chunks) "
+              f"{rt.adjoint_info()}", file=sys.stderr, flush=True)
+    K, _, nch = rt.adjoint_info()
+    assert np.isfinite(gr["g_pops"]).all()
+    nb = rt.batch_info()[0]
+    nrays0 = int(rt.nrays[0])
+    rt.close()
+    return {
+        "workload": f"c5 slice: nr=1024 x nth=512, {int(its[0][2])} ch, 2 of 1000 lines x "
+                    f"{len(incl)} incl, {nrays0} rays per incl, fp32",
+        "precision": "fp32", "items": len(its), "fwd_s": fwd[-1],
+        "fwd_items_per_s": len(its) / fwd[-1], "fwd_value": nominal / fwd[-1],
+        "fwd_batches": nb, "fwd_t_traj_ms": st["t_traj_ms"],
+        "fwd_nominal_ray_channels": nominal,
+        "adj_item_s": adj[-1], "adj_value": nrays0 * its[0][2] / adj[-1],
+        "adj_ckpt_interval": K, "adj_chunks": nch,
+        "budget_gb": mem["budget"] / 1e9, "unit": "ray*channel/s",
     }
 
 
@@ -278,6 +347,114 @@ def bench_posvel(args):
     }
 
 
+def host_cores():
+    """CPU threads this process may run on (its affinity set), and the
+    cgroup CPU quota when one is set (cpu.max), for the record."""
+    cores = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    return cores, quota
+
+
+def reference_on_host(cores):
+    """The reference's own trace (oracle/_ref/RADlite_probe, the flang -O2
+    build of the reference sources) on this host's cores: ``cores``
+    processes at once, one per core, each rendering the 100x60 disk
+    (7,041 rays x 50 channels, NONREDUNDANT skipping compiled in, as the
+    reference always runs), as pyradlite runs one process per work item.
+    The probe records the wall time of each calc_line_spectrum call
+    (telescope.F:1465), so set-up and file output are not timed.  The
+    oracle traces the same run with ``cores`` threads and the same
+    skipping.  Returns None where the reference build is absent (it is
+    compiled in the build container from the reference sources)."""
+    import shutil
+    import subprocess
+    import tempfile
+    probe = os.path.join(ROOT, "oracle", "_ref", "RADlite_probe")
+    if not os.access(probe, os.X_OK):
+        return None
+    from radlite_amd import engine, l4io, synth
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyorc
+    base = tempfile.mkdtemp(prefix="rl_refcpu_")
+    try:
+        src = os.path.join(base, "run")
+        synth.write_workload(src, "c1", seed=0)
+        dirs = []
+        for k in range(cores):
+            d = os.path.join(base, f"p{k}")
+            shutil.copytree(src, d)
+            dirs.append(d)
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen([probe], cwd=d, stdout=subprocess.DEVNULL,
+                                  stderr=subprocess.DEVNULL) for d in dirs]
+        rcs = [p.wait(timeout=600) for p in procs]
+        wall = time.perf_counter() - t0
+        if any(rcs):
+            return {"error": f"reference probe exit codes {sorted(set(rcs))}"}
+        tref = [sum(float(ln.split()[1]) for ln in open(os.path.join(d, "probe_time.txt")))
+                for d in dirs]
+        ri = l4io.read_run_dir(src)
+        nfr, _ = engine.passband(ri.vmax, ri.dv)
+        orc = pyorc.Oracle(pyorc.Prepared(ri))
+        torc = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            o = orc.line_spectrum(0, skip_mode=0, nthreads=cores, want_image=False)
+            torc.append(time.perf_counter() - t1)
+        nrc = o["nrays"] * nfr
+        ref_rate = sum(nrc / t for t in tref)          # all processes at once
+        orc_rate = nrc / sorted(torc)[1]
+        return {"workload": f"100x60 disk, {o['nrays']} rays x {nfr} ch, one line, 45 deg",
+                "processes": cores, "wall_s": wall,
+                "trace_s_median": sorted(tref)[len(tref) // 2],
+                "reference_value": ref_rate, "oracle_value": orc_rate,
+                "ref_over_oracle_rate": ref_rate / orc_rate}
+    finally:
+        shutil.rmtree(base, ignore_errors=True)
+
+
+def cpu_baseline(args, ri, incs, skip, nfr, gpu_value):
+    """The oracle (C restatement, OpenMP over rays) on every core of this
+    host on a sample of the headline workload, and the reference's own
+    trace on the same cores (reference_on_host), whose rate ratio to the
+    oracle, measured here, converts the sample into reference-equivalent
+    throughput.  The reference build cannot hold the 400x200 disk itself:
+    its compile-time capacity is 300 radii (configure.h FRSIZE_X)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyorc
+    cores, quota = host_cores()
+    orc = pyorc.Oracle(pyorc.Prepared(ri))
+    t0 = time.perf_counter()
+    nrc = 0
+    for i in range(len(incs)):
+        o = orc.line_spectrum(0, incl_rad=incs[i], skip_mode=1 if skip else 0,
+                              nthreads=cores, ray_stride=args.cpu_stride, want_image=False)
+        nrc += ((o["nrays"] + args.cpu_stride - 1) // args.cpu_stride) * nfr
+    tc = time.perf_counter() - t0
+    res = {
+        "value": nrc / tc, "unit": "ray*channel/s", "cores": cores, "kind": "port",
+        "sample": f"every {args.cpu_stride}th ray of both inclinations of the headline "
+                  f"workload ({nrc} ray*channels, {tc:.1f} s), FP64 C restatement, "
+                  f"skip={args.skip}, {cores} threads"
+                  + (f" (cgroup CPU quota {quota:g})" if quota else ""),
+    }
+    ref = reference_on_host(cores)
+    if ref is not None and "error" not in ref:
+        ratio = ref["ref_over_oracle_rate"]
+        res["reference_equivalent"] = {
+            "value": nrc / tc * ratio, "cores": cores, "ref_over_oracle_rate": ratio,
+            "gpu_speedup": gpu_value / (nrc / tc * ratio), "measured": ref}
+    elif ref is not None:
+        res["reference_equivalent"] = ref
+    return res
+
+
 def copy_probe(nbytes):
     """A plain device-to-device copy of known size: what FETCH_SIZE and
     WRITE_SIZE read for it calibrates the counters against true bytes."""
@@ -312,6 +489,9 @@ def main():
     ap.add_argument("--no-posvel", action="store_true",
                     help="skip the position-velocity cube leg")
     ap.add_argument("--posvel-only", action="store_true")
+    ap.add_argument("--no-c5", action="store_true",
+                    help="skip the 1024x512-disk slice (2 lines x 4 inclinations)")
+    ap.add_argument("--c5-only", action="store_true")
     ap.add_argument("--large-only", action="store_true",
                     help="only the 1024x512-disk, 1000-line plan and measured slice")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -334,6 +514,9 @@ def main():
         return
     if args.large_only:
         print(json.dumps(bench_large(args)))
+        return
+    if args.c5_only:
+        print(json.dumps(bench_c5_slice(args)))
         return
     if args.posvel_only:
         print(json.dumps(bench_posvel(args)))
@@ -376,9 +559,12 @@ def main():
     nbatch, tile_cw, tile_blocks = rt.batch_info()
     formal_s = float(np.mean(t_formal)) / 1e3     # per step: every batch's formal launch
     # per launch (one inclination): algorithmic bytes / mean launch duration
-    achieved_gbs = bytes_per_rc * nominal / formal_s / 1e9
     flops_per_rc = 40.0 * nseg
+    peak_tf = FP32_PEAK_TFLOPS if args.precision == "fp32" else FP64_PEAK_TFLOPS
     achieved_tf = flops_per_rc * nominal / formal_s / 1e12
+    kname = "k_formal_batch" if tile_blocks else "k_formal_tiles"
+    traffic = measured_traffic(args.precision, kname) if args.workload == "c2" else None
+    hbm_gbs = traffic * nbatch / formal_s / 1e9 if traffic is not None else None
 
     out = {
         "metric": "ray*channel integrations/s (forward)",
@@ -404,23 +590,28 @@ def main():
         "stage_ms": {k: stats[k] for k in ("t_traj_ms", "t_nodes_ms", "t_formal_ms",
                                            "t_flux_ms", "t_total_ms")},
         "roofline": {
-            "bound": "hbm",
-            "achieved": achieved_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": (measured_traffic(args.precision, "k_formal_batch" if tile_blocks
-                                         else "k_formal_tiles")
-                        if args.workload == "c2" else None),
-            "kernel": "k_formal_batch" if tile_blocks else "k_formal_tiles",
-            "bytes_per_ray_channel": bytes_per_rc,
+            # the formal kernel is bound by VALU issue and latency, not by HBM:
+            # algorithmic FP32 operations (40 per segment per ray*channel) per
+            # second of the kernel against the MI355X FP32 vector peak
+            "bound": "valu",
+            "achieved": achieved_tf,
+            "peak": peak_tf,
+            "unit": "TFLOP/s",
+            "frac": achieved_tf / peak_tf,
+            "traffic": traffic,
+            "kernel": kname,
             "kernel_ms": formal_s * 1e3 / nbatch,        # mean per launch (one batch)
             "launches_per_step": nbatch,
             "items_per_launch": len(items) // nbatch,
+            "flops_per_ray_channel": flops_per_rc,
             "tile_cw": tile_cw, "persistent_blocks": tile_blocks,
-            "valu_tflops_algorithmic": achieved_tf,
-            "valu_frac_of_fp32_peak": achieved_tf / FP32_PEAK_TFLOPS,
-            # the kernel is VALU-issue bound: measured issue rate over peak
+            # HBM, reported apart: the streaming formulation's bytes (28 B per
+            # segment per ray*channel) and the counter-measured bytes per
+            # launch over the kernel time, against the 8 TB/s peak
+            "hbm_algorithmic_bytes_per_ray_channel": bytes_per_rc,
+            "hbm_measured_gbs": hbm_gbs,
+            "hbm_measured_frac": (hbm_gbs / HBM_PEAK_GBS) if hbm_gbs is not None else None,
+            # measured VALU issue rate over the wave64 FP32 issue peak (SQ counters)
             "valu_issue_frac_measured": (measured_valu_issue(args.precision)
                                          if args.workload == "c2" else None),
         },
@@ -434,33 +625,11 @@ def main():
     if not args.no_posvel:
         out["posvel"] = bench_posvel(args)
 
+    if not args.no_c5:
+        out["c5_slice"] = bench_c5_slice(args)
+
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyorc
-        orc = pyorc.Oracle(pyorc.Prepared(ri))
-        cores = min(os.cpu_count() or 1, 16)
-        t0 = time.perf_counter()
-        nrc = 0
-        for i in range(len(incs)):
-            o = orc.line_spectrum(0, incl_rad=incs[i], skip_mode=1 if skip else 0,
-                                  nthreads=cores, ray_stride=args.cpu_stride,
-                                  want_image=False)
-            nrc += ((o["nrays"] + args.cpu_stride - 1) // args.cpu_stride) * nfr
-        tc = time.perf_counter() - t0
-        out["cpu_baseline"] = {
-            "value": nrc / tc, "unit": "ray*channel/s", "cores": cores, "kind": "port",
-            "sample": f"every {args.cpu_stride}th ray of both inclinations "
-                      f"({nrc} ray*channels, {tc:.1f} s), FP64 C restatement, "
-                      f"skip={args.skip}",
-            # the reference's own trace is REF_OVER_PORT times slower than
-            # the oracle per core (flang build of the reference, one thread
-            # each, 100x60 disk: profiles/cpu_reference_vs_oracle_c1.txt)
-            "reference_equivalent": {
-                "value": nrc / tc / REF_OVER_PORT, "cores": cores,
-                "ref_over_port_time": REF_OVER_PORT,
-                "gpu_speedup": out["value"] / (nrc / tc / REF_OVER_PORT),
-            },
-        }
+        out["cpu_baseline"] = cpu_baseline(args, ri, incs, skip, nfr, out["value"])
     print(json.dumps(out))
 
 

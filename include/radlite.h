@@ -88,8 +88,22 @@ int rl_set_dust_line(rl_ctx *ctx, const double *jd, const double *ad);
 int rl_set_star(rl_ctx *ctx, double rstar_cm, int ncont,
                 const double *cont_nu, const double *istar);
 
-/* Outer boundary: 0 = black, 2 = CMB (telescope.F:3984-4028). */
+/* Outer boundary (radlite.inp, charintline telescope.F:3996-4030):
+ * 0 = black, 2 = CMB at the line centre, 3 = the interstellar radiation
+ * field of rl_set_interstellar_field at each channel frequency; 1 and
+ * other values are the reference's stop 13. */
 int rl_set_outer_bc(rl_ctx *ctx, int type);
+
+/* Interstellar radiation field I_nu on the continuum grid
+ * (interstellfield.inp, read_interstellfield star.F:675-720), for outer
+ * boundary type 3: interpolated to every channel of a circular-image
+ * spectrum as line_init_passband_outerbc (line.F:3855-3902) does,
+ * including its test of the continuum interval against the channel count
+ * (a channel in interval nfr starts from 0).  Position-velocity cubes
+ * start from 0 with type 3, as calc_write_line_posvel leaves the field
+ * unset.  A trace with type 3 and no field is stop 13. */
+int rl_set_interstellar_field(rl_ctx *ctx, int ncont, const double *cont_nu,
+                              const double *isrf);
 
 /* Circular-image ray sets, one per inclination (setup_rays_circular,
  * telescope.F:715-1191 with imethod=1, nrref=10).  Outputs (optional,
@@ -265,6 +279,11 @@ int rl_set_level_data(rl_ctx *ctx, int nlev, const double *E_K, const double *g)
  * line.F:3560-3607 does; 0 outside the table).  Call after rl_set_lines. */
 int rl_set_dust_species(rl_ctx *ctx, int nspec, const double *rho_d,
                         const double *kabs_nu0, const double *kscat_nu0);
+/* The dust scattering/PAH source [nfreq][nr][nth] on the continuum grid
+ * cont_nu (scatsource.dat + quantsource.dat, source.F:494-546), added by
+ * rl_set_physics to j_d at each line centre (line.F:3593-3594); nfreq = 0
+ * removes it.  Call after rl_set_lines. */
+int rl_set_dust_scatter(rl_ctx *ctx, int nfreq, const double *cont_nu, const double *scat);
 /* Per cell [nr][nth]: gas temperature T_gas [K], dust temperature
  * T_dust [nspec][nr][nth] (NULL: T_gas for every species), log-abundance
  * (NULL: keep the abundance of rl_set_medium), turbulent velocity
@@ -298,6 +317,14 @@ int rl_prepare_line_dust(int nr, int nth_half, int nspec, const double *rho_d,
                          const double *t_d, int nfreq, const double *cont_nu,
                          const double *kabs, const double *kscat, int nlines,
                          const double *nu0, double *jd, double *ad);
+/* rl_prepare_line_dust with the dust scattering/PAH source scat
+ * [nfreq][nr][nth] on the continuum grid (scatsource.dat plus
+ * quantsource.dat, read_scatsource source.F:494-546; NULL: none), added
+ * to j at each line centre as line.F:3593-3594 does. */
+int rl_prepare_line_dust_scat(int nr, int nth_half, int nspec, const double *rho_d,
+                              const double *t_d, int nfreq, const double *cont_nu,
+                              const double *kabs, const double *kscat, const double *scat,
+                              int nlines, const double *nu0, double *jd, double *ad);
 /* starspectrum.inp F_nu at 1 pc -> I_nu at the stellar surface. */
 int rl_prepare_star(int nfreq, const double *fnu_1pc, double rstar,
                     double *istar);

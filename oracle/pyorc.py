@@ -28,6 +28,7 @@ class OrcModel(C.Structure):
         ("lev_up", ip), ("lev_low", ip), ("jd", dp), ("ad", dp),
         ("aksmax", C.c_double), ("rstar", C.c_double), ("ncont", C.c_int),
         ("cont_nu", dp), ("cont_istar", dp), ("outer_bc", C.c_int),
+        ("cont_isrf", dp),
     ]
 
 
@@ -113,9 +114,14 @@ class Prepared:
         rd = np.ascontiguousarray(ri.rho_dust); td = np.ascontiguousarray(ri.t_dust)
         nu = np.ascontiguousarray(ri.cont_nu); ka = np.ascontiguousarray(ri.kappa_abs)
         ks = np.ascontiguousarray(ri.kappa_scat)
+        sc = getattr(ri, "scat_src", None)
+        self.scat = None if sc is None else np.ascontiguousarray(sc, dtype=np.float64)
         L.orc_line_dust(C.c_int(self.nr), C.c_int(self.nth), C.c_int(nsp), _p(rd), _p(td),
-                        C.c_int(len(nu)), _p(nu), _p(ka), _p(ks), C.c_int(nlines),
+                        C.c_int(len(nu)), _p(nu), _p(ka), _p(ks),
+                        _p(self.scat) if self.scat is not None else None, C.c_int(nlines),
                         _p(self.nu0), _p(self.jd), _p(self.ad))
+        isrf = getattr(ri, "isrf", None)
+        self.isrf = None if isrf is None else np.ascontiguousarray(isrf, dtype=np.float64)
         self.istar = np.zeros(len(nu))
         fnu = np.ascontiguousarray(ri.star_fnu)
         L.orc_star_intensity(C.c_int(len(nu)), _p(fnu), C.c_double(ri.rstar), _p(self.istar))
@@ -147,7 +153,8 @@ class Oracle:
             nlines=len(p.A), nu0=_p(p.nu0), A=_p(p.A), gu=_p(p.gu), gl=_p(p.gl),
             lev_up=_p(p.lev_up, ip), lev_low=_p(p.lev_low, ip), jd=_p(p.jd),
             ad=_p(p.ad), aksmax=p.aksmax, rstar=p.ri.rstar, ncont=len(p.cont_nu),
-            cont_nu=_p(p.cont_nu), cont_istar=_p(p.istar), outer_bc=p.ri.outer_bc)
+            cont_nu=_p(p.cont_nu), cont_istar=_p(p.istar), outer_bc=p.ri.outer_bc,
+            cont_isrf=_p(p.isrf) if p.isrf is not None else None)
         self.g = OrcGrid()
         rc = L.orc_grid_build(C.byref(self.g), C.c_int(p.nr), _p(p.r), C.c_int(p.nth), _p(p.theta))
         if rc:

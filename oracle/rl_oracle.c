@@ -926,7 +926,7 @@ int orc_line_spectrum(const orc_model *m, const orc_grid *g,
 {
     double nu0 = fabs(m->nu0[il]);
     double passb, nu1, dnu, aksmax_c, *dnu_ch, *velo, *istar, *img;
-    double Bud, Bdu, I_bc = 0.0;
+    double Bud, Bdu, *I_bc;
     int nrays = rays->nrays, k, err = 0, maser = 0;
     long ovs = 0, nint = 0;
     const double D = 1.0 * 3.08572e18;         /* main.F:206,213         */
@@ -949,10 +949,8 @@ int orc_line_spectrum(const orc_model *m, const orc_grid *g,
     Bud = 6.78171833781e46 * m->A[il] / (m->nu0[il] * m->nu0[il] * m->nu0[il]);
     Bdu = Bud * (m->gu[il] / m->gl[il]);
     aksmax_c = m->aksmax / 2.99792458e5;
-    if (m->outer_bc == 2) {
-        double f = m->nu0[il];
-        I_bc = 1.47455253991e-47 * (f * f * f) / (exp(4.7991598e-11 * f / 2.728) - 1.0);
-    }
+    I_bc = (double *)malloc(sizeof(double) * nfr);
+    orc_outer_bc(m, il, nfr, dnu_ch, I_bc);
     img = image ? image : (double *)malloc(sizeof(double) * (size_t)nrays * nfr);
 
 #ifdef _OPENMP
@@ -1001,7 +999,7 @@ int orc_line_spectrum(const orc_model *m, const orc_grid *g,
                          velo[ch] >= (double)mn - 2.0 * aksmax_c);
             if (cmask) cmask[(size_t)iray * nfr + ch] = 0;
             if (doit) {
-                row[ch] = charint(&cache, &L, istar[ch], I_bc, &ovs, &mz);
+                row[ch] = charint(&cache, &L, istar[ch], I_bc[ch], &ovs, &mz);
                 nint++;
                 if (cmask) cmask[(size_t)iray * nfr + ch] = 1;
                 if (ch == 0 && iray > 0 && skip_mode == 0 &&
@@ -1011,7 +1009,7 @@ int orc_line_spectrum(const orc_model *m, const orc_grid *g,
             } else if (cont != 0.0) {
                 row[ch] = cont;
             } else {
-                row[ch] = charint(&cache, &L, istar[ch], I_bc, &ovs, &mz);
+                row[ch] = charint(&cache, &L, istar[ch], I_bc[ch], &ovs, &mz);
                 nint++;
                 cont = row[ch];
             }
@@ -1047,8 +1045,40 @@ int orc_line_spectrum(const orc_model *m, const orc_grid *g,
     if (overshoot_out) *overshoot_out = ovs;
     if (nint_out) *nint_out = nint;
     if (!image) free(img);
-    free(dnu_ch); free(velo); free(istar);
+    free(dnu_ch); free(velo); free(istar); free(I_bc);
     return err;
+}
+
+void orc_outer_bc(const orc_model *m, int il, int nfr, const double *dnu_ch, double *I_bc)
+{
+    const double nu0 = fabs(m->nu0[il]);
+    int k;
+    for (k = 0; k < nfr; k++) I_bc[k] = 0.0;              /* type 0: black */
+    if (m->outer_bc == 2) {                                /* telescope.F:3996-4005 */
+        const double f = m->nu0[il];
+        const double v = 1.47455253991e-47 * (f * f * f) / (exp(4.7991598e-11 * f / 2.728) - 1.0);
+        for (k = 0; k < nfr; k++) I_bc[k] = v;
+    } else if (m->outer_bc == 3 && m->cont_isrf) {
+        /* line_init_passband_outerbc, line.F:3878-3900: the channel
+           frequency freq_nu = nu0 + dnu (line.F:466-467, 518) on the
+           continuum grid.  The upper-end test compares the interval with
+           freq_nr, which line_setup_passband has just set to the
+           passband's channel count (line.F:505); an interval that ends at
+           the last continuum point reads the point after it, which is 0
+           in the reference's (static, zero-filled) arrays. */
+        const double *c = m->cont_nu - 1, *f = m->cont_isrf - 1;
+        const int n = m->ncont;
+        for (k = 0; k < nfr; k++) {
+            const double freq = nu0 + dnu_ch[k];
+            const int i = orc_hunt(c, n, freq, 0);
+            double wgt, c1, f1;
+            if (i == 0 || i == nfr) continue;
+            c1 = i < n ? c[i + 1] : 0.0;
+            f1 = i < n ? f[i + 1] : 0.0;
+            wgt = (freq - c[i]) / (c1 - c[i]);
+            I_bc[k] = (1.0 - wgt) * f[i] + wgt * f1;
+        }
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -1075,8 +1105,8 @@ void orc_line_consts(int nlev, const double *E_cm, int nlines,
 
 int orc_line_dust(int nr, int nth, int nspec, const double *rho_d,
                   const double *t_d, int nf, const double *cont_nu,
-                  const double *kabs, const double *kscat, int nlines,
-                  const double *nu0, double *jd, double *ad)
+                  const double *kabs, const double *kscat, const double *scat,
+                  int nlines, const double *nu0, double *jd, double *ad)
 {
     /* line_dust_compute_src_alp, line.F:3560-3607 (grey per line) */
     const double *c = cont_nu - 1;
@@ -1098,7 +1128,14 @@ int orc_line_dust(int nr, int nth, int nspec, const double *rho_d,
                 src = src + rd * kw * orc_bplanck(t_d[(size_t)isp * nr * nth + cell], freq);
                 alp = alp + rd * kw;
             }
-            src = src + wgt * 0.0 + (1.0 - wgt) * 0.0;   /* no scatsource */
+            /* scattering/PAH source, line.F:3593-3594: scati_src from
+               scatsource.dat + quantsource.dat (source.F:494-546), 0 when
+               neither file exists (COMMON storage starts at 0) */
+            {
+                const double s1 = scat ? scat[(size_t)i * nr * nth + cell] : 0.0;
+                const double s0 = scat ? scat[(size_t)(i - 1) * nr * nth + cell] : 0.0;
+                src = src + wgt * s1 + (1.0 - wgt) * s0;
+            }
             for (isp = 0; isp < nspec; isp++) {
                 const double *ks = kscat + (size_t)isp * nf - 1;
                 double kw = wgt * ks[i + 1] + (1.0 - wgt) * ks[i];

@@ -36,7 +36,8 @@ typedef struct {
     double rstar;
     int ncont;
     const double *cont_nu, *cont_istar;     /* star I_nu at R*           */
-    int outer_bc;             /* 0 = black, 2 = CMB                      */
+    int outer_bc;             /* 0 = black, 2 = CMB, 3 = interstellar    */
+    const double *cont_isrf;  /* [ncont] interstellfield.inp (type 3)    */
 } orc_model;
 
 typedef struct {
@@ -103,10 +104,17 @@ double orc_bplanck(double temp, double nu);
 void orc_line_consts(int nlev, const double *E_cm, int nlines,
                      const int *lev_up, const int *lev_low,
                      double *nu0_out);
+/* scat: the scattering/PAH source [nfreq][nr][nth_half] (scatsource.dat
+ * + quantsource.dat) or NULL */
 int  orc_line_dust(int nr, int nth_half, int nspec, const double *rho_d,
                    const double *t_d, int nfreq, const double *cont_nu,
-                   const double *kabs, const double *kscat, int nlines,
-                   const double *nu0, double *jd, double *ad);
+                   const double *kabs, const double *kscat, const double *scat,
+                   int nlines, const double *nu0, double *jd, double *ad);
+/* Outer boundary intensity of the channels of a passband (charintline,
+ * telescope.F:3996-4026): 0, the CMB at the line centre, or the
+ * interstellar field at each channel frequency
+ * (line_init_passband_outerbc, line.F:3855-3902). */
+void orc_outer_bc(const orc_model *m, int il, int nfr, const double *dnu_ch, double *I_bc);
 void orc_star_intensity(int nfreq, const double *fnu_1pc, double rstar,
                         double *istar);
 double orc_aksmax(int ncell, const double *lw, double fix_lw);

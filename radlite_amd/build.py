@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "librl.so")
 SOURCES = ["csrc/rl_engine.hip", "csrc/rl_prepass.hip", "csrc/rl_resample.hip",
            "csrc/rl_api.cpp"]
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-pthread",
     # FP64 parity mode rounds every a*b+c as two operations, like the
     # reference on x86-64; FP32 code uses explicit intrinsics instead.
     "-ffp-contract=off",

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "rl_device.h"
@@ -111,6 +112,7 @@ struct rl_ctx {
     double rstar = 0.0;
     std::vector<double> cont_nu, istar;
     int outer_bc = 0;
+    std::vector<double> isrf_nu, isrf;     // interstellar field (outer boundary type 3)
     // rays
     std::vector<RaySet> incl;
     // per-item work buffers
@@ -154,6 +156,8 @@ struct rl_ctx {
     int pre_nlev = 0, pre_nspec = 0, pre_dust_from_gas = 0, pre_ready = 0;
     double pre_q = 0.0, pre_cw = 0.0;
     DevBuf d_EK, d_g, d_nu0, d_rhod, d_Td, d_kabs, d_kscat, d_T, d_la, d_vt, d_part;
+    DevBuf d_scat, d_scat_lo, d_scat_w;     // scattering source (rl_set_dust_scatter)
+    int scat_nf = 0;
     DevBuf d_gT, d_gTd, d_gvt, d_gla;
     // stats
     double t_ms[5] = {0, 0, 0, 0, 0};
@@ -377,9 +381,20 @@ extern "C" int rl_set_star(rl_ctx *c, double rstar, int ncont, const double *nu,
 extern "C" int rl_set_outer_bc(rl_ctx *c, int type) {
     if (c) c->gen++;                         // invalidates a tape
     if (!c) return RL_E_ARG;
-    if (type != 0 && type != 2)
-        return fail(c, 13, "outer boundary type %d is not supported (0 or 2)", type);
+    if (type == 1) return fail(c, 13, "outer BC type 1 not allowed for telescope");
+    if (type != 0 && type != 2 && type != 3)
+        return fail(c, 13, "telescope: unknown outer boundary type %d", type);
     c->outer_bc = type;
+    return 0;
+}
+
+extern "C" int rl_set_interstellar_field(rl_ctx *c, int ncont, const double *cont_nu,
+                                         const double *isrf) {
+    if (c) c->gen++;
+    if (!c || ncont < 2 || !cont_nu || !isrf)
+        return fail(c, RL_E_ARG, "rl_set_interstellar_field: bad args");
+    c->isrf_nu.assign(cont_nu, cont_nu + ncont);
+    c->isrf.assign(isrf, isrf + ncont);
     return 0;
 }
 
@@ -858,6 +873,41 @@ static double star_at(const rl_ctx *c, double freq) {      // line.F:3821-3843
     return (1.0 - wgt) * s[lo] + wgt * s[lo + 1];
 }
 
+// Outer boundary intensity of every channel (charintline,
+// telescope.F:3996-4026): 0 (type 0), the CMB at the line centre (type 2),
+// or the interstellar field at each channel frequency nu0 + dnu
+// (line_init_passband_outerbc, line.F:3878-3900) on the circular-image
+// path; the pos-vel path (calc_write_line_posvel) never fills the field,
+// so it starts from 0 there.  The reference tests the upper end of the
+// continuum interval against freq_nr, which line_setup_passband has just
+// set to the channel count (line.F:505), so a channel in interval nfr
+// gets 0; an interval that ends at the last continuum point reads the
+// (zero) point after it.
+static void outer_bc_channels(const rl_ctx *c, int line, int nfr, const double *dnu, bool rect,
+                              double *out) {
+    const double nu0s = c->nu0[line], nu0 = std::fabs(nu0s);
+    double v = 0.0;
+    if (c->outer_bc == 2)
+        v = 1.47455253991e-47 * (nu0s * nu0s * nu0s) / (std::exp(4.7991598e-11 * nu0s / 2.728) - 1.0);
+    for (int k = 0; k < nfr; k++) out[k] = v;
+    if (c->outer_bc != 3 || rect) return;
+    const int n = (int)c->isrf_nu.size();
+    const double *x = c->isrf_nu.data() - 1, *f = c->isrf.data() - 1;
+    const bool asc = x[n] > x[1];
+    for (int k = 0; k < nfr; k++) {
+        const double freq = nu0 + dnu[k];
+        int lo = 0, hi = n + 1;
+        while (hi - lo != 1) {
+            const int m = (hi + lo) >> 1;
+            if ((freq > x[m]) == asc) lo = m; else hi = m;
+        }
+        if (lo == 0 || lo == nfr) continue;
+        const double x1 = lo < n ? x[lo + 1] : 0.0, f1 = lo < n ? f[lo + 1] : 0.0;
+        const double wgt = (freq - x[lo]) / (x1 - x[lo]);
+        out[k] = (1.0 - wgt) * f[lo] + wgt * f1;
+    }
+}
+
 // Per-item set-up shared by the forward and the adjoint: trajectories of
 // the item's inclination, the line's cell table, the channel tables
 // (line_setup_passband, line.F:462-469; star per channel, line.F:3821-3843)
@@ -898,7 +948,7 @@ static int setup_item(rl_ctx *c, const rl_item &it, int ii, ItemSetup &out, hipE
     const double nu1 = 0.0 - passb, dnu = 2.0 * passb / (nfr - 1.0);
     // kept in the context: the copy below is asynchronous
     std::vector<double> &chan = c->h_chan;
-    chan.assign(3 * (size_t)nfr, 0.0);
+    chan.assign(4 * (size_t)nfr, 0.0);
     for (int k = 0; k < nfr; k++) {
         double dn = nu1 + k * dnu;
         chan[k] = dn;
@@ -908,6 +958,7 @@ static int setup_item(rl_ctx *c, const rl_item &it, int ii, ItemSetup &out, hipE
         // line_init_passband_starbc, cf. telescope.F:1557): 0 there
         chan[2 * nfr + k] = rs.rect ? 0.0 : star_at(c, nu0 + dn);
     }
+    outer_bc_channels(c, it.line, nfr, chan.data(), rs.rect, chan.data() + 3 * nfr);
     if (int e = upload(c, c->d_chan, chan.data(), chan.size() * 8)) return e;
     rl::LineConst &lc = out.lc;
     lc.nu0 = nu0s;
@@ -922,16 +973,12 @@ static int setup_item(rl_ctx *c, const rl_item &it, int ii, ItemSetup &out, hipE
         lc.starfract = q * q;
         if (rs.ring_ri[1] < c->rstar) return fail(c, 124, "central beam resolves the star");
     }
-    lc.I0 = 0.0;
-    if (c->outer_bc == 2)
-        lc.I0 = 1.47455253991e-47 * (nu0s * nu0s * nu0s) /
-                (std::exp(4.7991598e-11 * nu0s / 2.728) - 1.0);
     return 0;
 }
 
 // line constants and channel tables of one item (line_setup_passband,
 // line.F:462-469; star per channel, line.F:3821-3843; prepare_lines,
-// line.F:1778-1787); chan receives dnu, dnu/nu0, I_star [3*nfr]
+// line.F:1778-1787); chan receives dnu, dnu/nu0, I_star, I_outer [4*nfr]
 static int item_consts(rl_ctx *c, const rl_item &it, const RaySet &rs, rl::LineConst &lc,
                        double *chan) {
     const int nfr = it.nfr;
@@ -944,6 +991,7 @@ static int item_consts(rl_ctx *c, const rl_item &it, const RaySet &rs, rl::LineC
         chan[nfr + k] = dn / nu0s;
         chan[2 * nfr + k] = star_at(c, nu0 + dn);
     }
+    outer_bc_channels(c, it.line, nfr, chan, rs.rect, chan + 3 * nfr);
     lc.nu0 = nu0s;
     lc.c_aa = 3.33567e-6 * nu0;
     lc.hknu = 5.27296241956e-28 * nu0s;
@@ -953,10 +1001,6 @@ static int item_consts(rl_ctx *c, const rl_item &it, const RaySet &rs, rl::LineC
     double q = c->rstar / rs.ring_ri[1];
     lc.starfract = q * q;
     if (rs.ring_ri[1] < c->rstar) return fail(c, 124, "central beam resolves the star");
-    lc.I0 = 0.0;
-    if (c->outer_bc == 2)
-        lc.I0 = 1.47455253991e-47 * (nu0s * nu0s * nu0s) /
-                (std::exp(4.7991598e-11 * nu0s / 2.728) - 1.0);
     return 0;
 }
 
@@ -1017,7 +1061,7 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
     std::vector<int> slot_line;
     std::vector<double> &chan = c->h_chan;
     size_t nchan = 0;
-    for (int k = 0; k < nb; k++) nchan += 3 * (size_t)items[i0 + k].nfr;
+    for (int k = 0; k < nb; k++) nchan += 4 * (size_t)items[i0 + k].nfr;
     chan.assign(nchan, 0.0);
     long long rec0 = 0, img0 = 0, chan0 = 0, part0 = 0, flux0 = 0;
     int rinfo0 = 0, max_rays = 0, max_nfr = 0, max_rings = 0, any_skip = 0;
@@ -1039,7 +1083,7 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
         d.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
         d.ring_w = rs.d_ringw.as<double>();
         d.center_w = rs.center_w;
-        rec0 += rs.nnodes; img0 += (long long)rs.nrays * it.nfr; chan0 += 3 * it.nfr;
+        rec0 += rs.nnodes; img0 += (long long)rs.nrays * it.nfr; chan0 += 4 * it.nfr;
         part0 += (long long)rs.nrings * it.nfr; flux0 += it.nfr; rinfo0 += rs.nrays;
         max_rays = std::max(max_rays, rs.nrays);
         max_nfr = std::max(max_nfr, it.nfr);
@@ -1261,6 +1305,8 @@ extern "C" int rl_trace_lines_forward_ex(rl_ctx *c, int nitems, const rl_item *i
         return fail(c, RL_E_ARG, "rl_trace_lines_forward: bad args");
     if (!c->have_medium || !c->nlev || !c->nlines || !c->have_dust || c->incl.empty())
         return fail(c, 13, "inputs incomplete: grid, medium, levels, lines, dust, rays");
+    if (c->outer_bc == 3 && c->isrf.empty())    // read_interstellfield, star.F:697-703
+        return fail(c, 13, "outer boundary type 3 without the interstellar field");
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, c->d_err.ensure(4));
     for (int i = 0; i < 5; i++) c->t_ms[i] = 0.0;
@@ -1401,6 +1447,8 @@ extern "C" int rl_trace_lines_posvel(rl_ctx *c, int nitems, const rl_item *items
         return fail(c, RL_E_ARG, "rl_trace_lines_posvel: bad args");
     if (!c->have_medium || !c->nlev || !c->nlines || !c->have_dust || c->incl.empty())
         return fail(c, 13, "inputs incomplete: grid, medium, levels, lines, dust, rays");
+    if (c->outer_bc == 3 && c->isrf.empty())    // read_interstellfield, star.F:697-703
+        return fail(c, 13, "outer boundary type 3 without the interstellar field");
     for (int ii = 0; ii < nitems; ii++) {
         const rl_item &it = items[ii];
         if (it.incl < 0 || it.incl >= (int)c->incl.size() || !c->incl[it.incl].rect)
@@ -1444,6 +1492,7 @@ extern "C" int rl_trace_lines_posvel(rl_ctx *c, int nitems, const rl_item *items
         L.dnu = c->d_chan.as<double>();
         L.velo = L.dnu + nfr;
         L.istar = L.dnu + 2 * nfr;
+        L.i0 = L.dnu + 3 * nfr;
         L.lc = s.lc;
         L.image = c->d_image.as<double>();
         L.extra = L.image + npv;
@@ -1647,6 +1696,7 @@ static void fill_adj(rl_ctx *c, const rl_item &it, const ItemSetup &su, AdjItemB
     a.dnu = c->d_chan.as<double>();
     a.velo = a.dnu + nfr;
     a.istar = a.dnu + 2 * nfr;
+    a.i0 = a.dnu + 3 * nfr;
     a.lc = su.lc;
     a.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
     a.image = c->d_image.as<double>() + (size_t)ch.r0 * nfr;
@@ -1757,6 +1807,8 @@ static int adj_reverse_chunk(rl_ctx *c, const rl_item &it, const ItemSetup &su, 
 static int adj_check(rl_ctx *c, int nitems, const rl_item *items, const char *who) {
     if (!c->have_medium || !c->nlev || !c->nlines || !c->have_dust || c->incl.empty())
         return fail(c, 13, "inputs incomplete: grid, medium, levels, lines, dust, rays");
+    if (c->outer_bc == 3 && c->isrf.empty())    // read_interstellfield, star.F:697-703
+        return fail(c, 13, "outer boundary type 3 without the interstellar field");
     for (int ii = 0; ii < nitems; ii++) {
         if (items[ii].skip_mode != RL_SKIP_NONE)
             return fail(c, 13, "%s: item %d: the adjoint integrates every channel (RL_SKIP_NONE)",
@@ -2058,6 +2110,38 @@ extern "C" int rl_set_dust_species(rl_ctx *c, int nspec, const double *rho_d,
     return 0;
 }
 
+extern "C" int rl_set_dust_scatter(rl_ctx *c, int nfreq, const double *cont_nu,
+                                   const double *scat) {
+    if (c) c->gen++;
+    if (!c || !c->nr || !c->nlines || nfreq < 0 || (nfreq && (nfreq < 2 || !cont_nu || !scat)))
+        return fail(c, RL_E_ARG, "rl_set_dust_scatter: set the grid and lines first; bad args");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->scat_nf = nfreq;
+    if (!nfreq) return 0;
+    const size_t ncell = (size_t)c->nr * c->nh;
+    // each line centre's continuum interval and weight (the hunt of
+    // line.F:3565 on the grid given here)
+    const double *x = cont_nu - 1;
+    const bool asc = x[nfreq] > x[1];
+    std::vector<int> lo_l(c->nlines);
+    std::vector<double> w_l(c->nlines, 0.0);
+    for (int il = 0; il < c->nlines; il++) {
+        const double freq = c->nu0[il];
+        int lo = 0, hi = nfreq + 1;
+        while (hi - lo != 1) {
+            int m = (hi + lo) >> 1;
+            if ((freq > x[m]) == asc) lo = m; else hi = m;
+        }
+        lo_l[il] = lo;
+        if (lo != 0 && lo != nfreq) w_l[il] = (freq - x[lo]) / (x[lo + 1] - x[lo]);
+    }
+    if (int e = upload(c, c->d_scat, scat, (size_t)nfreq * ncell * 8)) return e;
+    if (int e = upload(c, c->d_scat_lo, lo_l.data(), lo_l.size() * 4)) return e;
+    if (int e = upload(c, c->d_scat_w, w_l.data(), w_l.size() * 8)) return e;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 static void prepass_args(rl_ctx *c, rl::PrepassLaunch &a) {
     a.ncell = c->nr * c->nh;
     a.nlev = c->pre_nlev;
@@ -2076,6 +2160,10 @@ static void prepass_args(rl_ctx *c, rl::PrepassLaunch &a) {
     a.Td = c->d_Td.as<double>();
     a.kabs = c->d_kabs.as<double>();
     a.kscat = c->d_kscat.as<double>();
+    a.scat = c->scat_nf ? c->d_scat.as<double>() : nullptr;
+    a.scat_lo = c->d_scat_lo.as<int>();
+    a.scat_w = c->d_scat_w.as<double>();
+    a.scat_nf = c->scat_nf;
     a.pops = c->d_pops.as<double>();
     a.lw = c->d_lw.as<double>();
     a.abund = c->d_abund.as<double>();
@@ -2215,45 +2303,84 @@ static double bplanck(double temp, double nu) {          // setup.F:937-952
     return 1.47455e-47 * nu * nu * nu / (std::exp(4.7989e-11 * nu / temp) - 1.0) + 1.0e-290;
 }
 
-extern "C" int rl_prepare_line_dust(int nr, int nh, int nspec, const double *rho_d,
-                                    const double *t_d, int nf, const double *cn,
-                                    const double *kabs, const double *kscat, int nl,
-                                    const double *nu0, double *jd, double *ad) {
+extern "C" int rl_prepare_line_dust_scat(int nr, int nh, int nspec, const double *rho_d,
+                                         const double *t_d, int nf, const double *cn,
+                                         const double *kabs, const double *kscat,
+                                         const double *scat, int nl, const double *nu0,
+                                         double *jd, double *ad) {
     if (nr < 1 || nh < 1 || nspec < 0 || nf < 1 || nl < 1 || !cn || !nu0 || !jd || !ad)
         return RL_E_ARG;
+    // line_dust_compute_src_alp, line.F:3560-3607: the continuum interval of
+    // each line centre (hunt), its weight and the interpolated opacities are
+    // the same for every cell, so they are formed once per line with the
+    // reference's expressions; cells are split over host threads
     const double *x = cn - 1;
     const bool asc = x[nf] > x[1];
     const size_t ncell = (size_t)nr * nh;
-    for (int il = 0; il < nl; il++) {                   // line.F:3560-3607
+    std::vector<int> lo_l(nl);
+    std::vector<double> wgt_l(nl, 0.0), ka_l((size_t)nl * nspec), ks_l((size_t)nl * nspec);
+    for (int il = 0; il < nl; il++) {
         const double freq = nu0[il];
         int lo = 0, hi = nf + 1;
         while (hi - lo != 1) {
             int m = (hi + lo) >> 1;
             if ((freq > x[m]) == asc) lo = m; else hi = m;
         }
-        for (size_t cell = 0; cell < ncell; cell++) {
-            double src = 0.0, alp = 0.0;
-            if (lo != 0 && lo != nf) {
-                double wgt = (freq - x[lo]) / (x[lo + 1] - x[lo]);
-                for (int s = 0; s < nspec; s++) {
-                    const double *ka = kabs + (size_t)s * nf - 1;
-                    double kw = wgt * ka[lo + 1] + (1.0 - wgt) * ka[lo];
-                    double rd = rho_d[s * ncell + cell];
-                    src = src + rd * kw * bplanck(t_d[s * ncell + cell], freq);
-                    alp = alp + rd * kw;
-                }
-                src = src + wgt * 0.0 + (1.0 - wgt) * 0.0;
-                for (int s = 0; s < nspec; s++) {
-                    const double *ks = kscat + (size_t)s * nf - 1;
-                    double kw = wgt * ks[lo + 1] + (1.0 - wgt) * ks[lo];
-                    alp = alp + rho_d[s * ncell + cell] * kw;
-                }
-            }
-            jd[cell * nl + il] = src;
-            ad[cell * nl + il] = alp;
+        lo_l[il] = lo;
+        if (lo == 0 || lo == nf) continue;
+        const double wgt = (freq - x[lo]) / (x[lo + 1] - x[lo]);
+        wgt_l[il] = wgt;
+        for (int sp = 0; sp < nspec; sp++) {
+            const double *ka = kabs + (size_t)sp * nf - 1, *ks = kscat + (size_t)sp * nf - 1;
+            ka_l[(size_t)il * nspec + sp] = wgt * ka[lo + 1] + (1.0 - wgt) * ka[lo];
+            ks_l[(size_t)il * nspec + sp] = wgt * ks[lo + 1] + (1.0 - wgt) * ks[lo];
         }
     }
+    auto work = [&](size_t c0, size_t c1) {
+        for (size_t cell = c0; cell < c1; cell++) {
+            for (int il = 0; il < nl; il++) {
+                const int lo = lo_l[il];
+                double src = 0.0, alp = 0.0;
+                if (lo != 0 && lo != nf) {
+                    const double freq = nu0[il], wgt = wgt_l[il];
+                    for (int sp = 0; sp < nspec; sp++) {
+                        const double kw = ka_l[(size_t)il * nspec + sp];
+                        const double rd = rho_d[sp * ncell + cell];
+                        src = src + rd * kw * bplanck(t_d[sp * ncell + cell], freq);
+                        alp = alp + rd * kw;
+                    }
+                    // scattering/PAH source (scatsource.dat + quantsource.dat,
+                    // source.F:494-546), line.F:3593-3594; 0 without the files
+                    const double s1 = scat ? scat[(size_t)lo * ncell + cell] : 0.0;
+                    const double s0 = scat ? scat[(size_t)(lo - 1) * ncell + cell] : 0.0;
+                    src = src + wgt * s1 + (1.0 - wgt) * s0;
+                    for (int sp = 0; sp < nspec; sp++)
+                        alp = alp + rho_d[sp * ncell + cell] * ks_l[(size_t)il * nspec + sp];
+                }
+                jd[cell * nl + il] = src;
+                ad[cell * nl + il] = alp;
+            }
+        }
+    };
+    const size_t work_total = ncell * (size_t)nl;
+    unsigned nt = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    if (work_total < ((size_t)1 << 16)) nt = 1;
+    std::vector<std::thread> th;
+    const size_t per = (ncell + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; t++) {
+        const size_t c0 = t * per, c1 = std::min(ncell, c0 + per);
+        if (c0 < c1) th.emplace_back(work, c0, c1);
+    }
+    for (auto &t : th) t.join();
     return 0;
+}
+
+extern "C" int rl_prepare_line_dust(int nr, int nh, int nspec, const double *rho_d,
+                                    const double *t_d, int nf, const double *cn,
+                                    const double *kabs, const double *kscat, int nl,
+                                    const double *nu0, double *jd, double *ad) {
+    return rl_prepare_line_dust_scat(nr, nh, nspec, rho_d, t_d, nf, cn, kabs, kscat, nullptr, nl,
+                                     nu0, jd, ad);
 }
 
 extern "C" int rl_prepare_star(int nf, const double *fnu, double rstar, double *is) {

@@ -32,29 +32,6 @@ struct RayInfo {
 struct LineConst {
     double nu0, c_aa, hknu, A, Bud, Bdu;
     double starfract;         // (R*/rbeam0)^2 for the central beam
-    double I0;                // outer boundary intensity
-};
-
-struct ItemLaunch {
-    int nrays, nfr, nrings, nphi, skip_mode;
-    const long long *off;
-    const GeoNode *geo;
-    const CellRec *cells;
-    void *rec;                // NodeRec<T>[nnodes]
-    RayInfo *rinfo;
-    const double *dnu, *velo, *istar;   // per channel
-    LineConst lc;
-    double aks2;              // 2*aksmax/c
-    double *image;            // [nrays][nfr]
-    unsigned long long *ovs;
-    int *maser;
-    const double *ring_w;     // per ring pi*(ri_{k+1}^2 - ri_k^2)
-    double center_w;
-    int nchunks, rings_per_chunk;
-    double *partial;
-    double d2;                // distance^2
-    double *flux;             // device [nfr]
-    hipEvent_t ev_nodes, ev_formal;
 };
 
 // One item of a batched forward trace (device array, one per item).
@@ -65,7 +42,8 @@ struct ItemDev {
     const GeoNode *geo;
     long long rec0;            // first node record
     long long img0;            // first image value ([ray][channel])
-    long long chan0;           // dnu at chan0, velo at chan0+nfr, istar at chan0+2nfr
+    long long chan0;           // per channel: dnu at chan0, velo at chan0+nfr, the star's
+                               // intensity at chan0+2nfr, the outer boundary's at chan0+3nfr
     long long part0;           // partial flux [ring][channel]
     long long flux0;
     int rinfo0;
@@ -121,6 +99,7 @@ struct AdjLaunch {
     void *ckpt;                        // T[sum blocks][nfr]
     int kblk;                          // checkpoint interval in segments (1..16)
     const double *dnu, *velo, *istar;  // per channel
+    const double *i0;                  // per channel: outer boundary intensity
     LineConst lc;
     double aks2;
     double *image;                     // [nrays][nfr] forward intensities (the chunk's rows;
@@ -178,6 +157,7 @@ struct PosvelLaunch {
     double aks2;
     void *rec;                         // NodeRec<double> (FP64) or FastRec (FP32)
     const double *dnu, *velo, *istar;  // per channel
+    const double *i0;                  // per channel: outer boundary intensity
     LineConst lc;
     double *image, *tau;               // [iy][ix][channel]
     double *extra;                     // [ray - nx*ny][channel] (intensity)
@@ -193,7 +173,6 @@ void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int
                   int lev_low, double molpg, const double *rho, const double *abund,
                   const double *lw, const double *pops, const double *jd, const double *ad,
                   CellRec *out);
-template <typename T> void launch_item(hipStream_t st, const ItemLaunch &it);
 template <typename T> void launch_adjoint_fwd(hipStream_t st, const AdjLaunch &a);
 template <typename T> void launch_adjoint_rev(hipStream_t st, const AdjLaunch &a);
 void launch_adjoint_flux(hipStream_t st, const AdjLaunch &a);
@@ -216,6 +195,10 @@ struct PrepassLaunch {
     const double *nu0;                 // [line]
     const double *rho_d, *Td;          // [species][cell]
     const double *kabs, *kscat;        // at the line centres [species][line]
+    const double *scat;                // scattering source [continuum freq][cell], or null
+    const int *scat_lo;                // per line: its continuum interval (1-based; 0 or
+    const double *scat_w;              // nfreq: none) and the weight of its upper end
+    int scat_nf;
     double *pops, *lw, *abund, *jd, *ad;
     double *part;                      // block maxima of lw, (ncell+255)/256
     // adjoint

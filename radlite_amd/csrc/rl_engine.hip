@@ -10,9 +10,10 @@
 //   k_nodes       one wave per ray: interpolated node records (FP64) +
 //                 the NONREDUNDANT window and continuum channel
 //   k_nodes_fast  the same as scaled records (FP32 mode, adjoint)
-//   k_formal<T>   one lane per (ray, channel): formal solution along the
-//                 ray (charintline, telescope.F:3889-4312)
-//   k_fixup<T>    one thread per ray: the reference's lazy continuum
+//   k_formal_batch<T> / k_formal_tiles<T>  one lane per (ray, channel) of
+//                 many items: formal solution along the ray (charintline,
+//                 telescope.F:3889-4312)
+//   k_fixup_batch<T>  one thread per ray: the reference's lazy continuum
 //                 copies for skipped channels (telescope.F:583-611)
 //   k_flux_*      ring-weighted sums of the image into the spectrum
 //                 (calc_freq_flux_observer, telescope.F:1386-1435)
@@ -456,26 +457,27 @@ template <> struct RecOf<float> { using type = FastRec; };
 template <typename T>
 __device__ inline double trace_dispatch(const typename RecOf<T>::type *rec, long long b, int n,
                                         const LineConst &lc, double dnu, double velo,
-                                        double istar, T sf, int &ovs, int &maser);
+                                        double istar, double I0, T sf, int &ovs, int &maser);
 
 template <>
 __device__ inline double trace_dispatch<double>(const NodeRec<double> *rec, long long b, int n,
                                                 const LineConst &lc, double dnu, double velo,
-                                                double istar, double sf, int &ovs,
+                                                double istar, double I0, double sf, int &ovs,
                                                 int &maser) {
     LineK<double> L;
     L.nu0 = lc.nu0; L.c_aa = lc.c_aa; L.hknu = lc.hknu;
     L.A = lc.A; L.Bud = lc.Bud; L.Bdu = lc.Bdu;
     L.dnu = dnu; L.uch = velo;
-    return trace_one<double>(rec + b, n, L, lc.I0, istar, sf, ovs, maser);
+    return trace_one<double>(rec + b, n, L, I0, istar, sf, ovs, maser);
 }
 
 template <>
 __device__ inline double trace_dispatch<float>(const FastRec *rec, long long b, int n,
                                                const LineConst &lc, double dnu, double velo,
-                                               double istar, float sf, int &ovs, int &maser) {
+                                               double istar, double I0, float sf, int &ovs,
+                                               int &maser) {
     FastLine L{(float)lc.nu0, (float)dnu, (float)velo};
-    return (double)trace_fast<float>(rec + b, n, L, (float)lc.I0, (float)istar, sf, ovs, maser);
+    return (double)trace_fast<float>(rec + b, n, L, (float)I0, (float)istar, sf, ovs, maser);
 }
 
 // image placeholder of a channel left out by NONREDUNDANT skipping (a NaN
@@ -502,7 +504,8 @@ __device__ inline double formal_one(int ray, int ch, const long long *__restrict
                                     const RayInfo *__restrict__ rinfo,
                                     const double *__restrict__ dnu,
                                     const double *__restrict__ velo,
-                                    const double *__restrict__ istar, const LineConst &lc,
+                                    const double *__restrict__ istar,
+                                    const double *__restrict__ i0, const LineConst &lc,
                                     double aks2, int skip_mode, int &ovs, int &maser) {
     const RayInfo ri = rinfo[ray];
     bool doit = true;
@@ -511,70 +514,10 @@ __device__ inline double formal_one(int ray, int ch, const long long *__restrict
     }
     if (!doit) return __longlong_as_double(SKIPPED_BITS);   // filled by k_fixup
     T sf = ray == 0 ? (T)lc.starfract : T(0);
-    return trace_dispatch<T>(rec, off[ray], ri.n, lc, dnu[ch], velo[ch], istar[ch], sf, ovs,
-                             maser);
+    return trace_dispatch<T>(rec, off[ray], ri.n, lc, dnu[ch], velo[ch], istar[ch], i0[ch], sf,
+                             ovs, maser);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_formal(int nrays, int nfr, const long long *__restrict__ off,
-         const typename RecOf<T>::type *__restrict__ rec, const RayInfo *__restrict__ rinfo,
-         const double *__restrict__ dnu, const double *__restrict__ velo,
-         const double *__restrict__ istar, LineConst lc, double aks2, int skip_mode,
-         double *__restrict__ image, unsigned long long *ovs_out, int *maser_out) {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int ovs = 0, maser = 0;
-    if (g < (long long)nrays * nfr) {
-        const int ray = (int)(g / nfr), ch = (int)(g - (long long)ray * nfr);
-        image[g] = formal_one<T>(ray, ch, off, rec, rinfo, dnu, velo, istar, lc, aks2,
-                                 skip_mode, ovs, maser);
-    }
-    // wave-level pre-reduction of the warning counters
-    for (int m = 32; m >= 1; m >>= 1) {
-        ovs += __shfl_xor(ovs, m);
-        maser |= __shfl_xor(maser, m);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        if (ovs) atomicAdd(ovs_out, (unsigned long long)ovs);
-        if (maser) atomicOr(maser_out, 1);
-    }
-}
-
-// lazy continuum (telescope.F:583-611): the first skipped channel of a
-// ray was integrated by k_formal; later skipped channels copy it, unless
-// it is exactly 0, in which case they are integrated in order.
-template <typename T>
-__global__ void k_fixup(int nrays, int nfr, const long long *__restrict__ off,
-                        const typename RecOf<T>::type *__restrict__ rec, const RayInfo *__restrict__ rinfo,
-                        const double *__restrict__ dnu, const double *__restrict__ velo,
-                        const double *__restrict__ istar, LineConst lc, double aks2,
-                        double *__restrict__ image, unsigned long long *ovs_out,
-                        int *maser_out) {
-    int ray = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ray == 0 || ray >= nrays) return;
-    const RayInfo ri = rinfo[ray];
-    if (ri.cont < 0) return;
-    double *row = image + (size_t)ray * nfr;
-    double cont = row[ri.cont];
-    int ovs = 0, maser = 0;
-    for (int ch = ri.cont + 1; ch < nfr; ch++) {
-        bool inwin = velo[ch] <= (double)ri.mx + aks2 && velo[ch] >= (double)ri.mn - aks2;
-        if (inwin) continue;
-        if (cont != 0.0) {
-            row[ch] = cont;
-        } else {
-            row[ch] = trace_dispatch<T>(rec, off[ray], ri.n, lc, dnu[ch], velo[ch], istar[ch],
-                                        T(0), ovs, maser);
-            cont = row[ch];
-        }
-    }
-    if (ovs) atomicAdd(ovs_out, (unsigned long long)ovs);
-    if (maser) atomicOr(maser_out, 1);
-}
-
-// ---------------------------------------------------------------------
-// flux: partial ring sums, then a fixed-order sum of the partials
-// ---------------------------------------------------------------------
 __global__ void k_flux_partial(int nfr, int nrings, int nphi, int rings_per_chunk,
                                const double *__restrict__ image,
                                const double *__restrict__ ring_w, double center_w,
@@ -636,8 +579,8 @@ __global__ void __launch_bounds__(256)
 k_formal_posvel(int nrays, int nfr, int nx, int ny, const long long *__restrict__ off,
                 const typename RecOf<T>::type *__restrict__ rec,
                 const double *__restrict__ dnu, const double *__restrict__ velo,
-                const double *__restrict__ istar, LineConst lc, double *__restrict__ image,
-                double *__restrict__ tau, double *__restrict__ extra,
+                const double *__restrict__ istar, const double *__restrict__ i0, LineConst lc,
+                double *__restrict__ image, double *__restrict__ tau, double *__restrict__ extra,
                 unsigned long long *ovs_out, int *maser_out) {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int ovs = 0, maser = 0;
@@ -652,11 +595,11 @@ k_formal_posvel(int nrays, int nfr, int nx, int ny, const long long *__restrict_
                 L.nu0 = lc.nu0; L.c_aa = lc.c_aa; L.hknu = lc.hknu;
                 L.A = lc.A; L.Bud = lc.Bud; L.Bdu = lc.Bdu;
                 L.dnu = dnu[ch]; L.uch = velo[ch];
-                I = trace_one<double>(rec + b, n, L, lc.I0, istar[ch], 0.0, ovs, maser, &t);
+                I = trace_one<double>(rec + b, n, L, i0[ch], istar[ch], 0.0, ovs, maser, &t);
             } else {
                 FastLine L{(float)lc.nu0, (float)dnu[ch], (float)velo[ch]};
                 float tf = 0.0f;
-                I = (double)trace_fast<float>(rec + b, n, L, (float)lc.I0, (float)istar[ch],
+                I = (double)trace_fast<float>(rec + b, n, L, (float)i0[ch], (float)istar[ch],
                                               0.0f, ovs, maser, &tf);
                 t = (double)tf;
             }
@@ -702,7 +645,7 @@ void launch_posvel(hipStream_t st, const PosvelLaunch &a) {
     hipLaunchKernelGGL(k_formal_posvel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                        st, a.nrays, a.nfr, a.nx, a.ny, a.off,
                        reinterpret_cast<const typename RecOf<T>::type *>(a.rec), a.dnu, a.velo,
-                       a.istar, a.lc, a.image, a.tau, a.extra, a.ovs, a.maser);
+                       a.istar, a.i0, a.lc, a.image, a.tau, a.extra, a.ovs, a.maser);
 }
 template void launch_posvel<double>(hipStream_t, const PosvelLaunch &);
 template void launch_posvel<float>(hipStream_t, const PosvelLaunch &);
@@ -739,34 +682,6 @@ void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int
     hipLaunchKernelGGL(k_cells, dim3((ncell + 255) / 256), dim3(256), 0, st, ncell, nlev,
                        nlines, line, lev_up, lev_low, molpg, rho, abund, lw, pops, jd, ad,
                        out);
-}
-
-template <typename T>
-void launch_item(hipStream_t st, const ItemLaunch &it) {
-    auto *rec = reinterpret_cast<typename RecOf<T>::type *>(it.rec);
-    if constexpr (std::is_same<T, double>::value)
-        hipLaunchKernelGGL(k_nodes, dim3(it.nrays), dim3(64), 0, st, it.nrays, it.off,
-                           it.geo, it.cells, rec, it.rinfo, it.nfr, it.velo, it.aks2,
-                           it.skip_mode);
-    else
-        hipLaunchKernelGGL(k_nodes_fast<float>, dim3(it.nrays), dim3(64), 0, st, it.nrays, it.off,
-                           it.geo, it.cells, rec, 0ll, it.rinfo, it.nfr, it.velo, it.aks2,
-                           it.skip_mode, it.lc);
-    if (it.ev_nodes) (void)hipEventRecord(it.ev_nodes, st);
-    long long total = (long long)it.nrays * it.nfr;
-    hipLaunchKernelGGL(k_formal<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
-                       it.nrays, it.nfr, it.off, rec, it.rinfo, it.dnu, it.velo, it.istar,
-                       it.lc, it.aks2, it.skip_mode, it.image, it.ovs, it.maser);
-    if (it.skip_mode == 0)
-        hipLaunchKernelGGL(k_fixup<T>, dim3((it.nrays + 127) / 128), dim3(128), 0, st,
-                           it.nrays, it.nfr, it.off, rec, it.rinfo, it.dnu, it.velo,
-                           it.istar, it.lc, it.aks2, it.image, it.ovs, it.maser);
-    if (it.ev_formal) (void)hipEventRecord(it.ev_formal, st);
-    dim3 gp((it.nfr + 63) / 64, it.nchunks);
-    hipLaunchKernelGGL(k_flux_partial, gp, dim3(64), 0, st, it.nfr, it.nrings, it.nphi,
-                       it.rings_per_chunk, it.image, it.ring_w, it.center_w, it.partial);
-    hipLaunchKernelGGL(k_flux_final, dim3((it.nfr + 63) / 64), dim3(64), 0, st, it.nfr,
-                       it.nchunks, it.partial, it.d2, it.flux);
 }
 
 // ---------------------------------------------------------------------
@@ -862,8 +777,8 @@ __device__ __forceinline__ void formal_batch_body(RL_FORMAL_BATCH_PARAMS) {
             const long long ix = it.img0 + (long long)ray * it.nfr + ch;
             const double v =
                 formal_one<T>(ray, ch, it.off, rec_base + it.rec0, rinfo_all + it.rinfo0, dnu,
-                              dnu + it.nfr, dnu + 2 * it.nfr, it.lc, it.aks2, it.skip_mode, ovs,
-                              maser);
+                              dnu + it.nfr, dnu + 2 * it.nfr, dnu + 3 * it.nfr, it.lc, it.aks2,
+                              it.skip_mode, ovs, maser);
             image_all[ix] = v;
             if (cmask_all)
                 cmask_all[ix] = window_mask(ray, ch, rinfo_all + it.rinfo0, dnu + it.nfr,
@@ -923,8 +838,8 @@ k_formal_tiles(const ItemDev *__restrict__ items, int nitems,
         const long long ix = it.img0 + (long long)ray * it.nfr + ch;
         const double v =
             formal_one<T>(ray, ch, it.off, rec_base + it.rec0, rinfo_all + it.rinfo0, dnu,
-                          dnu + it.nfr, dnu + 2 * it.nfr, it.lc, it.aks2, it.skip_mode, ovs,
-                          maser);
+                          dnu + it.nfr, dnu + 2 * it.nfr, dnu + 3 * it.nfr, it.lc, it.aks2,
+                          it.skip_mode, ovs, maser);
         image_all[ix] = v;
         if (cmask_all)
             cmask_all[ix] = window_mask(ray, ch, rinfo_all + it.rinfo0, dnu + it.nfr, it.aks2,
@@ -953,7 +868,8 @@ __global__ void k_fixup_batch(const ItemDev *__restrict__ items, const void *rec
     const RayInfo ri = rinfo_all[it.rinfo0 + ray];
     if (ri.cont < 0) return;
     const auto *rec = reinterpret_cast<const typename RecOf<T>::type *>(rec_all) + it.rec0;
-    const double *dnu = chan_all + it.chan0, *velo = dnu + it.nfr, *istar = dnu + 2 * it.nfr;
+    const double *dnu = chan_all + it.chan0, *velo = dnu + it.nfr, *istar = dnu + 2 * it.nfr,
+                 *i0 = dnu + 3 * it.nfr;
     double *row = image_all + it.img0 + (size_t)ray * it.nfr;
     double cont = row[ri.cont];
     int ovs = 0, maser = 0;
@@ -964,7 +880,7 @@ __global__ void k_fixup_batch(const ItemDev *__restrict__ items, const void *rec
             row[ch] = cont;
         } else {
             row[ch] = trace_dispatch<T>(rec, it.off[ray], ri.n, it.lc, dnu[ch], velo[ch],
-                                        istar[ch], T(0), ovs, maser);
+                                        istar[ch], i0[ch], T(0), ovs, maser);
             cont = row[ch];
         }
     }
@@ -1082,18 +998,18 @@ __global__ void __launch_bounds__(256)
 k_adj_fwd(int nrays, int nfr, int ray0, long long node0, const long long *__restrict__ off,
           const long long *__restrict__ ckoff, const FastRecT<T> *__restrict__ rec,
           const double *__restrict__ dnu, const double *__restrict__ velo,
-          const double *__restrict__ istar, LineConst lc, double *__restrict__ image,
-          T *__restrict__ ckpt, int kblk) {
+          const double *__restrict__ istar, const double *__restrict__ i0, LineConst lc,
+          double *__restrict__ image, T *__restrict__ ckpt, int kblk) {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long long)nrays * nfr) return;
     const int ray = (int)(g / nfr), ch = (int)(g - (long long)ray * nfr);
     const long long b = off[ray] - node0;
     const int n = (int)(off[ray + 1] - off[ray]);
-    if (n < 2) { image[g] = lc.I0; return; }
+    if (n < 2) { image[g] = i0[ch]; return; }
     const FastLineT<T> L{(T)lc.nu0, (T)dnu[ch], (T)velo[ch]};
     const T sf = ray0 + ray == 0 ? (T)lc.starfract : T(0);
     const T ist = (T)istar[ch];
-    T I = (T)lc.I0;
+    T I = (T)i0[ch];
     FastCarryT<T> c;
     c.u0 = rec[b].u;
     int ovs = 0;
@@ -1115,23 +1031,6 @@ k_adj_fwd(int nrays, int nfr, int ray0, long long node0, const long long *__rest
     image[g] = (double)I;
 }
 
-// sum over the wave of five values; lanes 0-4 add them to out[0..4]
-template <typename T>
-__device__ inline void wave_add5(double *out, T a0, T a1, T a2, T a3, T a4) {
-    for (int m = 32; m >= 1; m >>= 1) {
-        a0 += __shfl_xor(a0, m);
-        a1 += __shfl_xor(a1, m);
-        a2 += __shfl_xor(a2, m);
-        a3 += __shfl_xor(a3, m);
-        a4 += __shfl_xor(a4, m);
-    }
-    const int lane = threadIdx.x & 63;
-    if (lane < 5) {
-        T v = lane == 0 ? a0 : lane == 1 ? a1 : lane == 2 ? a2 : lane == 3 ? a3 : a4;
-        if (v != T(0)) atomicAdd(out + lane, (double)v);
-    }
-}
-
 // Node gradients of a wave, summed over its lanes in batches of RED_N
 // nodes through LDS: each lane writes its values in row `lane` (RED_W
 // words, odd, so that the writes of 64 lanes fall in distinct banks), then
@@ -1148,13 +1047,21 @@ struct WaveNodeSum {
     long long *node;           // [RED_N] in LDS
     int n = 0;
 
+    // The block that owns a ray writes all of its nodes' slots, and the
+    // same lane handles the same (node, quantity) column in every channel
+    // group (the groups visit the nodes in the same order), so a plain
+    // read-add-write suffices and the groups add in a fixed order: the sums
+    // repeat bit for bit.
     __device__ void flush(double *gnode) {
         __syncthreads();
         const int lane = threadIdx.x;
         for (int p = lane; p < n * 5; p += 64) {
             T acc = T(0);
             for (int i = 0; i < 64; i++) acc += row[i][p];
-            if (acc != T(0)) atomicAdd(gnode + node[p / 5] * 5 + (p % 5), (double)acc);
+            if (acc != T(0)) {
+                double *g = gnode + node[p / 5] * 5 + (p % 5);
+                *g = *g + (double)acc;
+            }
         }
         __syncthreads();
         n = 0;
@@ -1177,31 +1084,28 @@ __device__ inline void carried_in(const FastRecT<T> &a, const FastLineT<T> &L, T
     al = a.dch * phi;
 }
 
-// reverse pass: one wave per (ray, group of 64 channels); per node the
-// gradients on (jd, ad, jc, dc) / sig and per segment on the mean width,
-// summed over the wave's channels into gnode
+// reverse pass: one wave per ray, over its channels in groups of 64 (in
+// order); per node the gradients on (jd, ad, jc, dc) / sig and per segment
+// on the mean width, summed over the channels into gnode (which the block
+// owns for its ray's nodes: no atomics, a fixed summation order)
 // (FP32: held to 128 VGPRs, four waves per SIMD instead of three; it was
 // 130, and the latency of the per-segment record loads and node sums needs
 // the extra wave more than the two registers)
 template <typename T>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
-k_adj_rev(int nrays, int nfr, int nphi, int ray0, long long node0,
-          const long long *__restrict__ off,
-          const long long *__restrict__ ckoff, const FastRecT<T> *__restrict__ rec,
-          const double *__restrict__ dnu, const double *__restrict__ velo,
-          const double *__restrict__ istar, LineConst lc, const double *__restrict__ gflux,
-          const double *__restrict__ ring_w, double center_w, double d2,
-          const T *__restrict__ ckpt, double *__restrict__ gnode, int kblk) {
-    __shared__ T red_row[64][RED_W];
-    __shared__ long long red_node[RED_N];
-    const int ray = blockIdx.x;
+__device__ __forceinline__ void adj_rev_group(
+    int grp, int ray, int nfr, int nphi, int ray0, long long node0,
+    const long long *__restrict__ off, const long long *__restrict__ ckoff,
+    const FastRecT<T> *__restrict__ rec, const double *__restrict__ dnu,
+    const double *__restrict__ velo, const double *__restrict__ istar, const LineConst &lc,
+    const double *__restrict__ gflux, const double *__restrict__ ring_w, double center_w,
+    double d2, const T *__restrict__ ckpt, double *__restrict__ gnode, int kblk,
+    T (*red_row)[RED_W], long long *red_node) {
     const int lane = threadIdx.x;
-    const int ch = blockIdx.y * 64 + lane;
+    const int ch = grp * 64 + lane;
     const bool active = ch < nfr;
     const int chc = active ? ch : nfr - 1;
     const long long b = off[ray] - node0;    // chunk-local node index
     const int n = (int)(off[ray + 1] - off[ray]);
-    if (n < 2) return;                       // uniform over the wave
     const FastLineT<T> L{(T)lc.nu0, (T)dnu[chc], (T)velo[chc]};
     // flux = (center_w I_0 + sum_rings ring_w mean_phi I) / d2 (telescope.F:1386-1418)
     const int rg = ray0 + ray;               // index in the ray set
@@ -1285,6 +1189,25 @@ k_adj_rev(int nrays, int nfr, int nphi, int ray0, long long node0,
     }
     red.push(gnode, b, cur.jd, cur.ad, cur.jc, cur.dc, T(0));
     red.flush(gnode);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+k_adj_rev(int nrays, int nfr, int nphi, int ray0, long long node0,
+          const long long *__restrict__ off,
+          const long long *__restrict__ ckoff, const FastRecT<T> *__restrict__ rec,
+          const double *__restrict__ dnu, const double *__restrict__ velo,
+          const double *__restrict__ istar, LineConst lc, const double *__restrict__ gflux,
+          const double *__restrict__ ring_w, double center_w, double d2,
+          const T *__restrict__ ckpt, double *__restrict__ gnode, int kblk) {
+    __shared__ T red_row[64][RED_W];
+    __shared__ long long red_node[RED_N];
+    const int ray = blockIdx.x;
+    if (off[ray + 1] - off[ray] < 2) return;     // uniform over the wave
+    const int ngrp = (nfr + 63) / 64;
+    for (int grp = 0; grp < ngrp; grp++)
+        adj_rev_group<T>(grp, ray, nfr, nphi, ray0, node0, off, ckoff, rec, dnu, velo, istar,
+                         lc, gflux, ring_w, center_w, d2, ckpt, gnode, kblk, red_row, red_node);
 }
 
 // ---- node -> cell transpose of the interpolation stencil (once per ray set)
@@ -1418,7 +1341,7 @@ void launch_adjoint_fwd(hipStream_t st, const AdjLaunch &a) {
     const long long total = (long long)a.nrays * a.nfr;
     hipLaunchKernelGGL(k_adj_fwd<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
                        a.nrays, a.nfr, a.ray0, a.node0, a.off, a.ckoff, rec, a.dnu, a.velo,
-                       a.istar, a.lc, a.image, ck, a.kblk);
+                       a.istar, a.i0, a.lc, a.image, ck, a.kblk);
 }
 
 // the spectrum of an item from its whole image (every chunk traced)
@@ -1436,7 +1359,7 @@ template <typename T>
 void launch_adjoint_rev(hipStream_t st, const AdjLaunch &a) {
     auto *rec = reinterpret_cast<const FastRecT<T> *>(a.rec);
     auto *ck = reinterpret_cast<const T *>(a.ckpt);
-    hipLaunchKernelGGL(k_adj_rev<T>, dim3(a.nrays, (a.nfr + 63) / 64), dim3(64), 0, st,
+    hipLaunchKernelGGL(k_adj_rev<T>, dim3(a.nrays), dim3(64), 0, st,
                        a.nrays, a.nfr, a.nphi, a.ray0, a.node0, a.off, a.ckoff, rec, a.dnu,
                        a.velo, a.istar, a.lc, a.gflux, a.ring_w, a.center_w, a.d2, ck, a.gnode,
                        a.kblk);
@@ -1456,7 +1379,5 @@ template void launch_adjoint_rev<float>(hipStream_t, const AdjLaunch &);
 int adjoint_block_len() { return ADJ_K; }
 size_t fast_rec_size(int fp64) { return fp64 ? sizeof(FastRecT<double>) : sizeof(FastRecT<float>); }
 
-template void launch_item<double>(hipStream_t, const ItemLaunch &);
-template void launch_item<float>(hipStream_t, const ItemLaunch &);
 
 }  // namespace rl

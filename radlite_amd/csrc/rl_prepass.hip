@@ -16,7 +16,8 @@
 //                values below 1e-99 are 0 (radlite.py:1127)
 //   width        lw = sqrt(v_t^2 + 2 k T / (m_mol m_H)) [km/s]
 //   abundance    exp(log_abund)
-//   dust         j_d = sum_s rho_s kabs_s(nu0) B(nu0, T_s) ; a_d = sum_s rho_s
+//   dust         j_d = sum_s rho_s kabs_s(nu0) B(nu0, T_s) (+ the scattering
+//                source at nu0 when one is set) ; a_d = sum_s rho_s
 //                (kabs_s + kscat_s), with B = 1.47455e-47 nu^3 /
 //                (exp(4.7989e-11 nu/T) - 1) + 1e-290 (0 at T = 0)
 //
@@ -85,8 +86,9 @@ __device__ inline double dbplanck_dt(double temp, double nu) {
 __global__ void k_pre_dust(long long n, int nl, int nspec, long long ncell,
                            const double *__restrict__ nu0, const double *__restrict__ rho_d,
                            const double *__restrict__ Td, const double *__restrict__ kabs,
-                           const double *__restrict__ kscat, double *__restrict__ jd,
-                           double *__restrict__ ad) {
+                           const double *__restrict__ kscat, const double *__restrict__ scat,
+                           const int *__restrict__ scat_lo, const double *__restrict__ scat_w,
+                           int scat_nf, double *__restrict__ jd, double *__restrict__ ad) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const long long c = i / nl;
@@ -97,6 +99,14 @@ __global__ void k_pre_dust(long long n, int nl, int nspec, long long ncell,
         const double kw = kabs[(size_t)s * nl + l], rd = rho_d[s * ncell + c];
         src = src + rd * kw * bplanck_d(Td[s * ncell + c], freq);
         alp = alp + rd * kw;
+    }
+    // scattering/PAH source at the line centre (line.F:3593-3594)
+    if (scat) {
+        const int lo = scat_lo[l];
+        if (lo > 0 && lo < scat_nf) {
+            const double w = scat_w[l];
+            src = src + w * scat[(size_t)lo * ncell + c] + (1.0 - w) * scat[(size_t)(lo - 1) * ncell + c];
+        }
     }
     for (int s = 0; s < nspec; s++) alp = alp + rho_d[s * ncell + c] * kscat[(size_t)s * nl + l];
     jd[i] = src;
@@ -174,7 +184,7 @@ void launch_prepass(hipStream_t st, const PrepassLaunch &a) {
     const long long n = (long long)a.ncell * a.nlines;
     hipLaunchKernelGGL(k_pre_dust, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
                        a.nlines, a.nspec, (long long)a.ncell, a.nu0, a.rho_d, a.Td, a.kabs,
-                       a.kscat, a.jd, a.ad);
+                       a.kscat, a.scat, a.scat_lo, a.scat_w, a.scat_nf, a.jd, a.ad);
     hipLaunchKernelGGL(k_pre_max, dim3((a.ncell + 255) / 256), dim3(256), 0, st, a.ncell, a.lw,
                        a.part);
 }

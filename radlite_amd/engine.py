@@ -98,6 +98,10 @@ def load_library(path=LIB_PATH):
         "rl_prepare_line_freqs": (C.c_int, [C.c_int, _dp, _ip, _ip, _dp]),
         "rl_prepare_line_dust": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int,
                                            _dp, _dp, _dp, C.c_int, _dp, _dp, _dp]),
+        "rl_prepare_line_dust_scat": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int,
+                                                _dp, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp]),
+        "rl_set_interstellar_field": (C.c_int, [vp, C.c_int, _dp, _dp]),
+        "rl_set_dust_scatter": (C.c_int, [vp, C.c_int, _dp, _dp]),
         "rl_prepare_star": (C.c_int, [C.c_int, _dp, C.c_double, _dp]),
         "rl_prepare_aksmax": (C.c_double, [C.c_int, _dp, C.c_double]),
         "rl_passband": (C.c_int, [C.c_double, C.c_double, _ip, _dp]),
@@ -156,11 +160,15 @@ def prepare_tables(ri: l4io.RunInputs) -> LineTables:
     nu, nup = _d(ri.cont_nu)
     ka, kap = _d(ri.kappa_abs)
     ks, ksp = _d(ri.kappa_scat)
-    rc = L.rl_prepare_line_dust(nr, nth, ri.rho_dust.shape[0], rdp, tdp, len(nu), nup, kap,
-                                ksp, nl, nu0.ctypes.data_as(_dp), jd.ctypes.data_as(_dp),
-                                ad.ctypes.data_as(_dp))
+    sc = getattr(ri, "scat_src", None)
+    scp = None
+    if sc is not None:
+        sc, scp = _d(sc)
+    rc = L.rl_prepare_line_dust_scat(nr, nth, ri.rho_dust.shape[0], rdp, tdp, len(nu), nup, kap,
+                                     ksp, scp, nl, nu0.ctypes.data_as(_dp),
+                                     jd.ctypes.data_as(_dp), ad.ctypes.data_as(_dp))
     if rc:
-        raise RadliteError(rc, "rl_prepare_line_dust")
+        raise RadliteError(rc, "rl_prepare_line_dust_scat")
     istar = np.zeros(len(nu))
     fn, fnp = _d(ri.star_fnu)
     rc = L.rl_prepare_star(len(nu), fnp, ri.rstar, istar.ctypes.data_as(_dp))
@@ -253,6 +261,22 @@ class Raytracer:
         s, sp = _d(istar)
         self._chk(self.L.rl_set_star(self.ctx, rstar, len(n), npt, sp), "rl_set_star")
 
+    def set_interstellar_field(self, cont_nu, isrf):
+        nu, nup = _d(cont_nu)
+        f, fp = _d(isrf)
+        self._chk(self.L.rl_set_interstellar_field(self.ctx, len(nu), nup, fp),
+                  "rl_set_interstellar_field")
+
+    def set_dust_scatter(self, cont_nu, scat):
+        """The scattering source [nfreq, nr, nth] for the device pre-pass
+        (None removes it)."""
+        if scat is None:
+            self._chk(self.L.rl_set_dust_scatter(self.ctx, 0, None, None), "rl_set_dust_scatter")
+            return
+        nu, nup = _d(cont_nu)
+        sc, scp = _d(scat)
+        self._chk(self.L.rl_set_dust_scatter(self.ctx, len(nu), nup, scp), "rl_set_dust_scatter")
+
     def set_outer_bc(self, t):
         self._chk(self.L.rl_set_outer_bc(self.ctx, int(t)), "rl_set_outer_bc")
 
@@ -266,6 +290,8 @@ class Raytracer:
         self.set_levels(ri.pops)
         self.set_lines(tb.nu0, tb.A, tb.gu, tb.gl, tb.lev_up, tb.lev_low)
         self.set_dust_line(tb.jd, tb.ad)
+        if getattr(ri, "isrf", None) is not None:
+            self.set_interstellar_field(ri.cont_nu, ri.isrf)
         self.set_outer_bc(ri.outer_bc)
         return tb
 

@@ -80,6 +80,13 @@ module radlite_gpu
             type(c_ptr), value :: ctx
             integer(c_int), value :: itype
         end function
+        integer(c_int) function rl_set_interstellar_field(ctx, ncont, cont_nu, isrf) &
+                bind(C, name="rl_set_interstellar_field")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: ncont
+            real(c_double), intent(in) :: cont_nu(*), isrf(*)
+        end function
         integer(c_int) function rl_setup_rays_circular(ctx, nincl, incl_rad, nphi, nrext, dbdr, &
                 n_rays, n_rings, ring_r, ring_edge) bind(C, name="rl_setup_rays_circular")
             import :: c_ptr, c_int, c_double
@@ -167,6 +174,15 @@ module radlite_gpu
             import :: c_int, c_double
             integer(c_int), value :: nr, nth_half, nspec, nfreq, nlines
             real(c_double), intent(in) :: rho_d(*), t_d(*), cont_nu(*), kabs(*), kscat(*), nu0(*)
+            real(c_double), intent(out) :: jd(*), ad(*)
+        end function
+        integer(c_int) function rl_prepare_line_dust_scat(nr, nth_half, nspec, rho_d, t_d, &
+                nfreq, cont_nu, kabs, kscat, scat, nlines, nu0, jd, ad) &
+                bind(C, name="rl_prepare_line_dust_scat")
+            import :: c_int, c_double, c_ptr
+            integer(c_int), value :: nr, nth_half, nspec, nfreq, nlines
+            real(c_double), intent(in) :: rho_d(*), t_d(*), cont_nu(*), kabs(*), kscat(*), nu0(*)
+            type(c_ptr), value :: scat
             real(c_double), intent(out) :: jd(*), ad(*)
         end function
         integer(c_int) function rl_prepare_star(nfreq, fnu_1pc, rstar, istar) &

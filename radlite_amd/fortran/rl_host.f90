@@ -33,6 +33,9 @@ program rl_host
     real(c_double), allocatable :: rhod(:,:,:), td(:,:,:), cnu(:), kabs(:,:), kscat(:,:)
     real(c_double), allocatable :: fnu(:), istar(:), pops(:,:,:), E(:), g(:), A(:), nu0(:)
     real(c_double), allocatable :: gu(:), gl(:), jd(:,:,:), ad(:,:,:), flux(:), work(:)
+    real(c_double), allocatable, target :: scat(:,:,:)
+    real(c_double), allocatable :: isrf(:)
+    logical :: have_scat, fex
     integer(c_int), allocatable :: lup(:), llo(:), warn(:), nrays(:), nrings(:)
     type(rl_item), allocatable :: items(:)
     character(len=16), allocatable :: rows(:)
@@ -236,11 +239,47 @@ program rl_host
     close(u)
     nlev = k
 
+    ! the dust scattering/PAH source (read_scatsource, source.F:494-546):
+    ! scatsource.dat and quantsource.dat, each optional, summed; one value
+    ! per record, frequency outer, theta, radius inner
+    allocate(scat(nth, nr, nf))
+    scat = 0.0d0
+    have_scat = .false.
+    call read_scat('scatsource.dat')
+    call read_scat('quantsource.dat')
+    ! the interstellar field for outer boundary type 3 (read_interstellfield,
+    ! star.F:675-720)
+    if (outer_bc == 3) then
+        inquire(file=path('interstellfield.inp'), exist=fex)
+        if (.not. fex) then
+            write(*, '(a)') 'ERROR: outer BC type 3 without interstellfield.inp'
+            stop 13
+        end if
+        open(newunit=u, file=path('interstellfield.inp'), status='old', action='read')
+        read(u, *) k
+        if (k /= nf) then
+            write(*, '(a)') 'ERROR: interstellfield.inp inconsistent nr of frequencies'
+            stop 13
+        end if
+        allocate(isrf(nf))
+        do i = 1, nf
+            read(u, *) isrf(i)
+        end do
+        close(u)
+    end if
+
     ! host pre-pass with the reference's arithmetic
     allocate(jd(nlin, nth, nr), ad(nlin, nth, nr))
     call chk(rl_prepare_line_freqs(nlin, E, lup, llo, nu0), 'rl_prepare_line_freqs')
-    call chk(rl_prepare_line_dust(nr, nth, nspec, rhod, td, nf, cnu, kabs, kscat, nlin, nu0, &
-                                  jd, ad), 'rl_prepare_line_dust')
+    if (have_scat) then
+        call chk(rl_prepare_line_dust_scat(nr, nth, nspec, rhod, td, nf, cnu, kabs, kscat, &
+                                           c_loc(scat), nlin, nu0, jd, ad), &
+                 'rl_prepare_line_dust_scat')
+    else
+        call chk(rl_prepare_line_dust_scat(nr, nth, nspec, rhod, td, nf, cnu, kabs, kscat, &
+                                           c_null_ptr, nlin, nu0, jd, ad), &
+                 'rl_prepare_line_dust_scat')
+    end if
     call chk(rl_prepare_star(nf, fnu, rstar, istar), 'rl_prepare_star')
     aks = rl_prepare_aksmax(nr * nth, lw, fix_lw)
     call chk(rl_passband(vmax, dv, nfr, vmax2), 'rl_passband')
@@ -257,6 +296,7 @@ program rl_host
     call chkc(rl_set_levels(ctx, nlev, pops))
     call chkc(rl_set_lines(ctx, nlin, nu0, A, gu, gl, lup, llo))
     call chkc(rl_set_dust_line(ctx, jd, ad))
+    if (outer_bc == 3) call chkc(rl_set_interstellar_field(ctx, nf, cnu, isrf))
     call chkc(rl_set_outer_bc(ctx, outer_bc))
     inc(1) = incl_deg * 0.0174532925199d0
     allocate(nrays(1), nrings(1))
@@ -284,6 +324,26 @@ program rl_host
     call write_spectrum()
 
 contains
+
+    subroutine read_scat(name)
+        character(len=*), intent(in) :: name
+        integer :: v, ifr, jt, jr, h(4)
+        real(c_double) :: val
+        inquire(file=path(name), exist=fex)
+        if (.not. fex) return
+        have_scat = .true.
+        open(newunit=v, file=path(name), status='old', action='read')
+        read(v, *) h
+        do ifr = 1, nf
+            do jt = 1, nth
+                do jr = 1, nr
+                    read(v, *) val
+                    scat(jt, jr, ifr) = scat(jt, jr, ifr) + val
+                end do
+            end do
+        end do
+        close(v)
+    end subroutine
 
     function path(name) result(p)
         character(len=*), intent(in) :: name

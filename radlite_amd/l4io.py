@@ -182,6 +182,22 @@ def write_dust_and_star(path, nu, kappa_abs, kappa_scat, rstar, fnu_1pc):
         f.write("\n")
 
 
+def write_interstellfield(path, isrf):
+    """interstellfield.inp: count, then one intensity per line."""
+    _write_scalar(os.path.join(path, "interstellfield.inp"), f"{len(isrf)}", isrf)
+
+
+def write_scatsource(path, scat, name="scatsource.dat"):
+    """scatsource.dat (or quantsource.dat): ``nfreq nr nth 1``, then one
+    value per line, frequency outer, theta, radius inner; scat is
+    [nfreq, nr, nth]."""
+    nf, nr, nth = scat.shape
+    with open(os.path.join(path, name), "w") as f:
+        f.write(f"{nf} {nr} {nth} 1\n")
+        f.write("\n".join(_fmt(v) for v in np.asarray(scat).transpose(0, 2, 1).ravel()))
+        f.write("\n")
+
+
 # ----------------------------------------------------------------------
 # readers (semantics of the reference's own readers)
 # ----------------------------------------------------------------------
@@ -226,6 +242,88 @@ class RunInputs:
     nlines_run: int
     iline_start: int
     extra: dict = field(default_factory=dict)
+    # interstellar radiation field on the continuum grid (interstellfield.inp,
+    # read for outer boundary type 3, star.F:675-720), or None
+    isrf: np.ndarray | None = None
+    # dust scattering/PAH source on the continuum grid, [nfreq, nr, nth]:
+    # scatsource.dat plus quantsource.dat (source.F:494-546), or None when
+    # neither file is present
+    scat_src: np.ndarray | None = None
+
+
+class _ListDirected:
+    """Fortran list-directed READ of one file: each read takes the values
+    it asks for from the next records (blank records and the ends of
+    records only separate values), and the rest of its last record is
+    skipped, so a following read starts on a new record."""
+
+    def __init__(self, fname):
+        with open(fname) as f:
+            self.lines = f.read().splitlines()
+        self.i = 0
+
+    def firsts(self, n):
+        """n reads of one value each: the first value of each of the next
+        n non-blank records."""
+        out = []
+        while len(out) < n:
+            if self.i >= len(self.lines):
+                raise EOFError(f"end of file after {len(out)} of {n} records")
+            toks = self.lines[self.i].replace(",", " ").split()
+            self.i += 1
+            if toks:
+                out.append(toks[0])
+        return np.array([v.replace("d", "e").replace("D", "e") for v in out], float)
+
+    def read(self, n):
+        vals = []
+        while len(vals) < n:
+            if self.i >= len(self.lines):
+                raise EOFError(f"end of file after {len(vals)} of {n} values")
+            toks = self.lines[self.i].replace(",", " ").split()
+            self.i += 1
+            vals.extend(toks[:n - len(vals)])
+        return [v.replace("d", "e").replace("D", "e") for v in vals]
+
+
+def read_interstellfield(path, nfreq):
+    """interstellfield.inp (read_interstellfield, star.F:675-720): a count
+    equal to frequency.inp's, then one intensity per record.  A missing
+    file or a different count is the reference's stop 13."""
+    fname = os.path.join(path, "interstellfield.inp")
+    if not os.path.exists(fname):
+        raise InputError(13, "outer BC type 3 without interstellfield.inp")
+    rd = _ListDirected(fname)
+    nf = int(rd.read(1)[0])
+    if nf != nfreq:
+        raise InputError(13, f"interstellfield.inp has {nf} frequencies, frequency.inp {nfreq}")
+    return rd.firsts(nf)
+
+
+def read_scatsource(path, nfreq, nr, nth):
+    """scatsource.dat and quantsource.dat (read_scatsource,
+    source.F:494-546), each optional: a header of four integers, then one
+    value per record for every frequency of frequency.inp (outer), theta
+    cell of the half grid, radius (inner); the two files are summed.
+    Returns [nfreq, nr, nth] or None when neither file is present."""
+    out = None
+    for name in ("scatsource.dat", "quantsource.dat"):
+        fname = os.path.join(path, name)
+        if not os.path.exists(fname):
+            continue
+        rd = _ListDirected(fname)
+        rd.read(4)
+        a = rd.firsts(nfreq * nth * nr).reshape(nfreq, nth, nr).transpose(0, 2, 1)
+        out = a.copy() if out is None else out + a
+    return out
+
+
+class InputError(RuntimeError):
+    """An input condition the reference stops on, with its stop code."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"stop {code}: {msg}")
+        self.code = code
 
 
 def read_run_dir(path) -> RunInputs:
@@ -277,16 +375,19 @@ def read_run_dir(path) -> RunInputs:
     # line.inp (line.F:168-279)
     with open(P("line.inp")) as f:
         ll = [ln.split()[0] for ln in f if ln.strip()]
-    umass = float(ll[2])
+    # format 2 has a species count after the format number, format 1 not
+    # (line.F:173-180)
+    o = 1 if int(float(ll[0])) == 2 else 0
+    umass = float(ll[1 + o].replace("d", "e").replace("D", "e"))
     fix_lw = 0.0
     # first line block: comment, iinfo=1, symm, nfreq, width, proftype, imore...
-    j = 5
+    j = 4 + o
     j += 5
     while True:
         imore = int(ll[j]); j += 1
         if imore == 0:
             break
-        fix_lw = float(ll[j]); j += 1
+        fix_lw = float(ll[j].replace("d", "e").replace("D", "e")); j += 1
     # linespectrum.inp (telescope.F:104-238)
     with open(P("linespectrum.inp")) as f:
         ls = [ln.split()[0] for ln in f if ln.strip()]
@@ -317,6 +418,10 @@ def read_run_dir(path) -> RunInputs:
     t = _tokens(P(f"levelpop_{molname}.dat"))
     nlp = int(t[2])
     pops = np.array(t[4 + 2 * nlp:4 + 2 * nlp + nr * nth * nlp], float).reshape(nr, nth, nlp)
+    # optional continuum inputs (setup.F:1149 read_scatsource; main.F:929-941
+    # read_interstellfield for outer boundary type 3)
+    scat = read_scatsource(path, nf, nr, nth)
+    isrf = read_interstellfield(path, nf) if outer_bc == 3 else None
     return RunInputs(
         r=r, theta_half=theta, rho_gas=rho, abund=ab, vel=vel, lw_kms=lw,
         pops=pops, rho_dust=rhod, t_dust=td, cont_nu=nu,
@@ -326,7 +431,7 @@ def read_run_dir(path) -> RunInputs:
         lev_low=np.array(lo), A=np.array(A), molname=molname, nphi=nphi,
         dbdr=dbdr, b_extra=b_extra, outer_bc=outer_bc, vmax=vmax, dv=dv,
         incl_deg=incl_deg, vrad=vrad, nlines_run=nlines_run,
-        iline_start=istart, extra=extra)
+        iline_start=istart, extra=extra, isrf=isrf, scat_src=scat)
 
 
 def write_line_spectrum_file(fname, molname, nfr_max, incl_rad, blocks,

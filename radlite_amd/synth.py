@@ -189,9 +189,29 @@ def passband(vmax_target, nfr):
     return vmax_target, dv
 
 
+def bnu(nu, T):
+    """Planck B_nu(T) [erg/s/cm^2/Hz/sr]."""
+    return 2 * HPLANCK * nu ** 3 / CLIGHT ** 2 / np.expm1(HPLANCK * nu / (KB * T))
+
+
+def scat_source(disk: Disk, nu, seed, kappa=100.0, level=0.5):
+    """A synthetic dust scattering/PAH emissivity [nfreq, nr, nth] of the
+    size of the thermal dust emissivity: level x rho_d kappa B_nu(T_d)
+    times a per-cell lognormal(0.5) factor and a slope in frequency."""
+    rng = np.random.default_rng(seed)
+    T = disk.t_dust
+    th = disk.rho_dust[None] * kappa * bnu(nu[:, None, None], T[None])
+    f = np.exp(rng.normal(0.0, 0.5, size=T.shape))[None]
+    slope = (nu / nu[len(nu) // 2])[:, None, None] ** 0.5
+    return level * th * f * slope
+
+
 def write_run_dir(path, disk: Disk, lines: LineSet, pops, *, nphi, dbdr,
                   b_extra, vmax, dv, incl_deg, nlines_run=None, outer_bc=0,
-                  fix_lw=0.0, umass_av=2.3, nfreq=40, posvel=None):
+                  fix_lw=0.0, umass_av=2.3, nfreq=40, posvel=None, scat=None, isrf=None):
+    """A RADLite run directory.  scat: optional {file name: [nfreq, nr, nth]}
+    written as scatsource.dat / quantsource.dat; isrf: optional
+    function of the continuum frequencies giving interstellfield.inp."""
     os.makedirs(path, exist_ok=True)
     nl = len(lines.A)
     l4io.write_radlite_inp(path, nphi, dbdr, b_extra, outer_bc=outer_bc)
@@ -205,6 +225,10 @@ def write_run_dir(path, disk: Disk, lines: LineSet, pops, *, nphi, dbdr,
                       disk.vel, disk.lw_kms, disk.rho_dust, disk.t_dust)
     nu, ka, ks, rstar, fnu = star_and_dust(nfreq=nfreq)
     l4io.write_dust_and_star(path, nu, ka, ks, rstar, fnu)
+    for name, arr in (scat or {}).items():
+        l4io.write_scatsource(path, arr(nu) if callable(arr) else arr, name)
+    if isrf is not None:
+        l4io.write_interstellfield(path, isrf(nu))
 
 
 # ----------------------------------------------------------------------
@@ -286,16 +310,23 @@ def c3_inputs(seed=0, nlines=64, incl=(10.0, 20.0, 30.0, 40.0, 50.0, 60.0, 70.0,
 
 
 def c5_inputs(seed=0, nlines=1000, incl=(20.0, 40.0, 60.0, 80.0), nr=1024, nth=512, nphi=512,
-              nfr=300, vmax=30.0):
+              nfr=300, vmax=30.0, keep_lines=None):
     """The large workload: nr x nth disk, ``nlines`` ro-vibrational-like
     lines (J = 0-60 ladders, A = 1-100 s^-1, E_up = 100-8000 K), non-LTE
     populations (Boltzmann x lognormal(0.2), renormalised), ``nfr``
     channels per line over +-vmax, nphi azimuths, the given inclinations.
     Returns (RunInputs, items) with items (line, incl index, nfr, vmax) for
-    every line at every inclination, inclination-major."""
+    every line at every inclination, inclination-major.  keep_lines = k
+    keeps only the first k of the drawn lines (same levels, populations and
+    disk): a slice of the same job whose per-line tables are cheap."""
     disk = make_disk(nr, nth, seed=seed, sigma_t=0.1)
     lines = rovib_lines(nlines, seed=seed + 1)
     pops = nonlte_pops(disk, lines, seed=seed + 2)
+    if keep_lines is not None:
+        k = int(keep_lines)
+        lines = LineSet(E_cm=lines.E_cm, g=lines.g, lev_up=lines.lev_up[:k],
+                        lev_low=lines.lev_low[:k], A=lines.A[:k])
+        nlines = k
     v, dv = passband(vmax, nfr)
     ri = run_inputs(disk, lines, pops, nphi=nphi, dbdr=1, b_extra=-1, vmax=v, dv=dv,
                     incl_deg=incl[0])

@@ -29,6 +29,16 @@ CASES = {
                nlines=3),
     "t7": dict(nr=16, nth=12, nphi=16, b_extra=-4, dbdr=2, nfr=30, vmax=30.0, incl=45.0, seed=7,
                t_scale=0.2),
+    # the dust scattering/PAH source: scatsource.dat and quantsource.dat
+    # (summed), interpolated to each of three line centres
+    "t8": dict(nr=10, nth=8, nphi=8, b_extra=-2, dbdr=1, nfr=24, vmax=30.0, incl=35.0, seed=8,
+               nlines=3, scat=True),
+    # outer boundary type 3 (interstellar field per channel): with 20
+    # channels, the lines whose centres fall in continuum interval 20 get a
+    # zero field (the reference compares the interval with freq_nr, the
+    # passband's channel count), the others the interpolated field
+    "t9": dict(nr=10, nth=8, nphi=8, b_extra=-2, dbdr=1, nfr=20, vmax=30.0, incl=50.0, seed=7,
+               nlines=4, outer_bc=3, lam_range=(4.4, 4.9)),
     # the CPU-runnable workload: nr=100 x nth=60, 110 rings x 64 phi, 50 ch, 45 deg
     "c1": dict(nr=100, nth=60, nphi=64, b_extra=-1, dbdr=1, nfr=50, vmax=30.0, incl=45.0, seed=0,
                sigma_t=0.01, image_stride=7),
@@ -52,7 +62,19 @@ POSVEL_CASES = {
                outer_bc=2, posvel=dict(nx=14, ny=14, size_x=230.0 * synth.AU,
                                        size_y=200.0 * synth.AU, phioff=-0.7,
                                        xoff=3.0 * synth.AU, yoff=0.0, addstar=0)),
+    # outer boundary type 3 on the pos-vel path: calc_write_line_posvel never
+    # fills the per-channel field, so rays start from (and pixels off the
+    # grid hold) zero
+    "p4": dict(nr=9, nth=7, nphi=10, b_extra=-2, dbdr=3, nfr=16, vmax=35.0, incl=60.0, seed=5,
+               outer_bc=3, posvel=dict(nx=14, ny=14, size_x=230.0 * synth.AU,
+                                       size_y=200.0 * synth.AU, phioff=-0.7,
+                                       xoff=3.0 * synth.AU, yoff=0.0, addstar=0)),
 }
+
+
+def isrf_field(nu):
+    """A synthetic interstellar field: 0.3 B_nu(1200 K)."""
+    return 0.3 * synth.bnu(nu, 1200.0)
 
 
 def build_case(name, path):
@@ -61,6 +83,9 @@ def build_case(name, path):
                            t_scale=c.get("t_scale", 1.0))
     if c.get("nlines", 1) == 1:
         lines = synth.single_line()
+    elif "lam_range" in c:
+        lines = synth.random_lines(c["nlines"], seed=c["seed"] + 100, Jmax=10,
+                                   lam_range=c["lam_range"])
     else:
         lines = synth.random_lines(c["nlines"], seed=c["seed"] + 100, Jmax=10)
     pops = synth.lte_pops(disk, lines,
@@ -68,7 +93,12 @@ def build_case(name, path):
     vmax, dv = synth.passband(c["vmax"], c["nfr"])
     synth.write_run_dir(path, disk, lines, pops, nphi=c["nphi"], dbdr=c["dbdr"],
                         b_extra=c["b_extra"], vmax=vmax, dv=dv, incl_deg=c["incl"],
-                        outer_bc=c.get("outer_bc", 0), posvel=c.get("posvel"))
+                        outer_bc=c.get("outer_bc", 0), posvel=c.get("posvel"),
+                        scat={"scatsource.dat": lambda nu: synth.scat_source(disk, nu, c["seed"]),
+                              "quantsource.dat": lambda nu: synth.scat_source(
+                                  disk, nu, c["seed"] + 1, level=0.2)}
+                        if c.get("scat") else None,
+                        isrf=isrf_field if c.get("outer_bc", 0) == 3 else None)
     return c
 
 
@@ -79,9 +109,13 @@ INPUT_FILES = ("radlite.inp", "linespectrum.inp", "line.inp", "moldata.dat",
                "starinfo.inp", "starspectrum.inp")
 
 
+OPTIONAL_INPUT_FILES = ("scatsource.dat", "quantsource.dat", "interstellfield.inp")
+
+
 def input_hash(path):
     h = hashlib.sha256()
-    for f in INPUT_FILES:
+    for f in INPUT_FILES + tuple(f for f in OPTIONAL_INPUT_FILES
+                                 if os.path.exists(os.path.join(path, f))):
         with open(os.path.join(path, f), "rb") as fh:
             h.update(fh.read())
     return h.hexdigest()

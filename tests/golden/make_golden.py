@@ -14,6 +14,9 @@ tests/golden/<case>.npz:
   traj_*                trajectories of selected rays (make_trajectory_c)
   input_sha256          hash of the generated input files
 
+Case r1 (radmc_case.py) is rendered from the reference's own tutorial
+RADMC inputs; its file also holds those inputs as read (in_* arrays).
+
 Usage: python tests/golden/make_golden.py [case ...]
 """
 import os
@@ -71,10 +74,22 @@ def posvel_case(name, c, d):
 
 
 def main(names):
+    import radmc_case
+    from radlite_amd import l4io
     for name in names:
-        c = CASES[name] if name in CASES else POSVEL_CASES[name]
+        if name in radmc_case.RADMC_CASES:
+            c = dict(radmc_case.RADMC_CASES[name])
+        else:
+            c = CASES[name] if name in CASES else POSVEL_CASES[name]
         with tempfile.TemporaryDirectory() as d:
-            build_case(name, d)
+            extra = {}
+            if name in radmc_case.RADMC_CASES:
+                radmc_case.build_radmc_case(name, d)
+                ri = l4io.read_run_dir(d)
+                c["nr"] = len(ri.r)
+                extra = radmc_case.inputs_to_arrays(ri)
+            else:
+                build_case(name, d)
             if "posvel" in c:
                 posvel_case(name, c, d)
                 continue
@@ -122,6 +137,7 @@ def main(names):
                 traj_icross=np.concatenate([t["icross"] for t in trajs]).astype(np.int8),
                 traj_ir=np.concatenate([t["ir"] for t in trajs]).astype(np.int16),
                 traj_it=np.concatenate([t["it"] for t in trajs]).astype(np.int16),
+                **extra,
             )
             np.savez_compressed(golden_path(name), **out)
             print(name, "nrays", nrays, "lines", len(recs), "->",

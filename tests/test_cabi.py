@@ -42,7 +42,7 @@ def test_create_without_device_fails_cleanly():
     assert L.rl_create(C.byref(ctx), 0, 7) == 13          # bad precision
 
 
-@pytest.mark.parametrize("name", ["t2", "t6", "c1"])
+@pytest.mark.parametrize("name", ["t2", "t6", "t8", "c1"])
 def test_host_prepass_matches_oracle(name):
     with tempfile.TemporaryDirectory() as d:
         build_case(name, d)

@@ -123,6 +123,34 @@ def test_tape_and_checkpoint_intervals(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_adjoint_repeats_bit_for_bit_many_channels(precision):
+    """300 channels (five groups of 64 per ray in the reverse sweep): two
+    adjoint calls and a tape + reverse call give bit-identical gradients
+    (the channel groups of a ray add their node sums in a fixed order)."""
+    ri = _case("t6")
+    say(f"[gpu] adjoint bit-repeat t6 nfr=300 {precision}")
+    rt, _, vmax = _engine(ri, precision)
+    nfr = 300
+    items = [(il, 0, nfr, vmax, engine.SKIP_NONE) for il in range(len(ri.A))]
+    rng = np.random.default_rng(9)
+    gf = [rng.normal(size=nfr) * 1e20 for _ in items]
+    fa, ga = rt.adjoint(items, gf)
+    fb, gb = rt.adjoint(items, gf)
+    rt.trace_tape(items)
+    fc, gc = rt.adjoint(items, gf)
+    assert rt.adjoint_info()[1] == 1
+    for a, b, c in zip(fa, fb, fc):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    nonzero = 0
+    for k in ga:
+        assert np.array_equal(ga[k], gb[k]), k
+        assert np.array_equal(ga[k], gc[k]), k
+        nonzero += int(np.count_nonzero(ga[k]))
+    assert nonzero > 0
+    rt.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_adjoint_ray_chunks(precision):
     """An item whose buffers do not fit the memory budget is traced and
     reversed a range of rays at a time (the largest checkpoint interval,

@@ -20,7 +20,7 @@ from radlite_amd import build, engine, l4io
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["t1", "t5", "t6", "c1"])
+@pytest.mark.parametrize("name", ["t1", "t5", "t6", "t8", "t9", "c1"])
 def test_fortran_host_writes_the_python_hosts_file(name):
     print(f"[gpu] fortran host {name}", file=sys.stderr, flush=True)
     assert os.path.exists(build.FORTRAN_HOST), "build the Fortran host first"

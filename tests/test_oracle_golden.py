@@ -110,3 +110,49 @@ def test_posvel_cube(name):
         assert np.all((t == 0) == (o["tau"] == 0))
         trel = np.abs(o["tau"] - t) / np.maximum(np.abs(t), 1e-300)
         assert trel.max() <= 1e-13, (il, trel.max())
+
+
+def _oracle_for(name, **change):
+    g = load_golden(name)
+    with tempfile.TemporaryDirectory() as d:
+        build_case(name, d)
+        assert str(g["input_sha256"]) == input_hash(d)
+        ri = l4io.read_run_dir(d)
+    for k, v in change.items():
+        setattr(ri, k, v)
+    return g, ri, pyorc.Oracle(pyorc.Prepared(ri))
+
+
+def test_scattering_source_is_seen():
+    """t8 has scatsource.dat and quantsource.dat: both files are read and
+    summed, and without them the spectra leave the golden by far more than
+    the parity tolerance."""
+    g, ri, _ = _oracle_for("t8")
+    assert ri.scat_src is not None and ri.scat_src.shape == (len(ri.cont_nu),) + ri.rho_gas.shape
+    _, _, orc0 = _oracle_for("t8", scat_src=None)
+    worst = 0.0
+    for il in range(len(g["flux"])):
+        f = orc0.line_spectrum(il, want_image=False)["flux"]
+        worst = max(worst, float(np.max(np.abs(f - g["flux"][il]) / np.abs(g["flux"][il]))))
+    assert worst > 1e-3, worst
+
+
+def test_interstellar_field_channel_count_quirk():
+    """t9 (outer boundary type 3, 20 channels): the lines whose centres lie
+    in continuum interval 20 start from zero exactly as with a black
+    boundary (line.F:3891 compares the interval with freq_nr, which holds
+    the passband's channel count); the others see the interpolated field."""
+    g, ri, orc = _oracle_for("t9")
+    _, _, orc_black = _oracle_for("t9", outer_bc=0, isrf=None)
+    nu = ri.cont_nu
+    intervals = []
+    for il in range(len(g["flux"])):
+        nu0 = abs(orc.p.nu0[il])
+        iv = int(np.searchsorted(nu, nu0))        # ascending grid: 1-based interval
+        intervals.append(iv)
+        fb = orc_black.line_spectrum(il, want_image=False)["flux"]
+        if iv == len(g["dnu"][il]):
+            np.testing.assert_array_equal(fb, g["flux"][il])
+        else:
+            assert np.max(np.abs(fb - g["flux"][il]) / np.abs(g["flux"][il])) > 1e-5
+    assert sorted(set(intervals)) == [19, 20], intervals
