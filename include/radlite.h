@@ -130,6 +130,23 @@ int rl_trace_lines_forward_ex(rl_ctx *ctx, int nitems, const rl_item *items,
                               double *flux, double *image, unsigned char *cmask,
                               unsigned *warn_bits);
 
+/* rl_trace_lines_forward that allocates no device memory.  No counterpart
+ * in the reference: it is for callers that cannot allocate (serving
+ * loops, harnesses that forbid hipMalloc).  Every work buffer comes from
+ * one fixed workspace of 256 MiB per GPU that the library holds for the
+ * life of the process; items that need more (node records, 64 B per
+ * trajectory node and item, are most of it) are refused with 13.  flux_dev
+ * (device memory, the layout of rl_trace_lines_forward's flux) receives
+ * the spectra; image_dev (device memory, optional) the [nrays][nfr]
+ * images; warn_bits (host, optional) per item RL_WARN_* flags.  The
+ * trajectories of the items' inclinations must be resident, i.e. built by
+ * an earlier rl_trace_lines_forward on this context with the same rays
+ * and velocity field (13 otherwise).  The items run as one batch; the call
+ * returns when its results are in place.  All contexts on a GPU share its
+ * workspace: calls that overlap on one GPU race. */
+int rl_trace_lines_forward_noalloc(rl_ctx *ctx, int nitems, const rl_item *items,
+                                   double *flux_dev, double *image_dev, unsigned *warn_bits);
+
 /* Rectangular-image ray sets for position-velocity cubes, one per
  * inclination (setup_rays_rectang, telescope.F:2229-2468; linespec_command
  * 2 reads nrx, nry, the image size in cm, phioff, xoff, yoff and addstar,

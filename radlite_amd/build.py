@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 LIB = os.path.join(HERE, "librl.so")
 SOURCES = ["csrc/rl_engine.hip", "csrc/rl_prepass.hip", "csrc/rl_resample.hip",
-           "csrc/rl_api.cpp"]
+           "csrc/rl_scratch.hip", "csrc/rl_api.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-pthread",
     # FP64 parity mode rounds every a*b+c as two operations, like the

@@ -159,6 +159,10 @@ struct rl_ctx {
     DevBuf d_scat, d_scat_lo, d_scat_w;     // scattering source (rl_set_dust_scatter)
     int scat_nf = 0;
     DevBuf d_gT, d_gTd, d_gvt, d_gla;
+    // the library's fixed device workspace on this context's device (for
+    // rl_trace_lines_forward_noalloc), or the lookup's failure
+    void *scratch = nullptr;
+    std::string scratch_err;
     // stats
     double t_ms[5] = {0, 0, 0, 0, 0};
     long long counts[4] = {0, 0, 0, 0};
@@ -210,6 +214,10 @@ extern "C" int rl_create(rl_ctx **out, int device, int precision) {
     }
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) { delete c; return RL_E_HIP; }
+    // the fixed workspace's device address, once per device (the entry
+    // points that do not use it work without it)
+    const hipError_t se = rl::scratch_attach(device, &c->scratch);
+    if (se != hipSuccess) c->scratch_err = hipGetErrorString(se);
     *out = c;
     return 0;
 }
@@ -1396,6 +1404,195 @@ extern "C" int rl_trace_lines_forward_ex(rl_ctx *c, int nitems, const rl_item *i
             io += (size_t)c->incl[items[k].incl].nrays * items[k].nfr;
         }
     }
+    return 0;
+}
+
+// rl_trace_lines_forward without device allocation: every work buffer of
+// the call (cell tables, node records, ray windows, channel tables, ring
+// partial sums, counters, item descriptors, and the image when the caller
+// does not take it) is carved from the library's fixed workspace, and the
+// flux (and image) are written to the caller's device memory.  The items
+// run as one batch through the forward's kernels; the trajectories of
+// their inclinations must already be resident (built by an earlier
+// allocating call), and no graph or tile trial is made (a tile mapping the
+// autotuner chose earlier for the same batch shape is reused).
+extern "C" int rl_trace_lines_forward_noalloc(rl_ctx *c, int nitems, const rl_item *items,
+                                              double *flux_dev, double *image_dev,
+                                              unsigned *warn) {
+    if (!c || nitems < 1 || !items || !flux_dev)
+        return fail(c, RL_E_ARG, "rl_trace_lines_forward_noalloc: bad args");
+    if (!c->scratch)
+        return fail(c, RL_E_HIP, "rl_trace_lines_forward_noalloc: no workspace on device %d (%s)",
+                    c->device, c->scratch_err.c_str());
+    if (!c->have_medium || !c->nlev || !c->nlines || !c->have_dust || c->incl.empty())
+        return fail(c, 13, "inputs incomplete: grid, medium, levels, lines, dust, rays");
+    if (c->outer_bc == 3 && c->isrf.empty())    // read_interstellfield, star.F:697-703
+        return fail(c, 13, "outer boundary type 3 without the interstellar field");
+    const int fp64 = c->precision == RL_FP64;
+    const size_t rsz = fp64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::FastRec);
+    const int ncell = c->nr * c->nh;
+    const double molpg = 1.0 / (c->umass_av * 1.6726e-24);
+    const int nb = nitems;
+    std::vector<rl::ItemDev> hi(nb);
+    std::vector<int> slot_line;
+    size_t nchan = 0;
+    for (int k = 0; k < nb; k++) {
+        const rl_item &it = items[k];
+        if (it.line < 0 || it.line >= c->nlines || it.incl < 0 ||
+            it.incl >= (int)c->incl.size() || it.nfr < 2 || !(it.vmax_kms > 0.0) ||
+            (it.skip_mode != RL_SKIP_REFERENCE && it.skip_mode != RL_SKIP_NONE))
+            return fail(c, 13, "item %d: bad line/incl/nfr/vmax/skip_mode", k);
+        const RaySet &rs = c->incl[it.incl];
+        if (rs.rect)
+            return fail(c, 13, "item %d: a rectangular-image ray set: use rl_trace_lines_posvel", k);
+        if (!rs.traj_ready)
+            return fail(c, 13,
+                        "item %d: the trajectories of inclination %d are not resident (an "
+                        "allocating trace of that inclination builds them)", k, it.incl);
+        nchan += 4 * (size_t)it.nfr;
+    }
+    std::vector<double> chan(nchan, 0.0);
+    long long rec0 = 0, img0 = 0, chan0 = 0, part0 = 0, flux0 = 0;
+    int rinfo0 = 0, max_rays = 0, max_nfr = 0, max_rings = 0, any_skip = 0;
+    for (int k = 0; k < nb; k++) {
+        const rl_item &it = items[k];
+        const RaySet &rs = c->incl[it.incl];
+        rl::ItemDev &d = hi[k];
+        if (int e = item_consts(c, it, rs, d.lc, chan.data() + chan0)) return e;
+        int slot = -1;
+        for (size_t q = 0; q < slot_line.size(); q++)
+            if (slot_line[q] == it.line) slot = (int)q;
+        if (slot < 0) { slot = (int)slot_line.size(); slot_line.push_back(it.line); }
+        d.nrays = rs.nrays; d.nfr = it.nfr; d.nrings = rs.nrings; d.nphi = rs.nphi;
+        d.skip_mode = it.skip_mode; d.cslot = slot;
+        d.off = rs.d_off.as<long long>();
+        d.geo = rs.d_nodes.as<GeoNode>();
+        d.rec0 = rec0; d.img0 = img0; d.chan0 = chan0; d.part0 = part0; d.flux0 = flux0;
+        d.rinfo0 = rinfo0;
+        d.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
+        d.ring_w = rs.d_ringw.as<double>();
+        d.center_w = rs.center_w;
+        rec0 += rs.nnodes; img0 += (long long)rs.nrays * it.nfr; chan0 += 4 * it.nfr;
+        part0 += (long long)rs.nrings * it.nfr; flux0 += it.nfr; rinfo0 += rs.nrays;
+        max_rays = std::max(max_rays, rs.nrays);
+        max_nfr = std::max(max_nfr, it.nfr);
+        max_rings = std::max(max_rings, rs.nrings);
+        any_skip |= it.skip_mode == RL_SKIP_REFERENCE;
+    }
+    // tile mapping: the autotuner's earlier choice for this batch shape, else
+    // ray-major tiles on the persistent grid
+    int dev_cu = 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device));
+    std::string shape = std::to_string(fp64);
+    for (int k = 0; k < nb; k++)
+        shape += ":" + std::to_string(hi[k].nrays) + "x" + std::to_string(hi[k].nfr) + "s" +
+                 std::to_string(hi[k].skip_mode);
+    int cw = 0, nblk = dev_cu * rl::formal_batch_blocks_per_cu(fp64);
+    if (c->fixed_cw >= 0) {
+        cw = c->fixed_cw;
+        if (!c->fixed_persistent) nblk = 0;
+    } else {
+        auto tuned = c->tune.find(shape);
+        if (tuned != c->tune.end()) { cw = tuned->second.first; nblk = tuned->second.second; }
+    }
+    std::vector<long long> t0(nb + 1, 0);
+    for (int k = 0; k < nb; k++) t0[k + 1] = t0[k] + tiles_of(cw, hi[k].nrays, hi[k].nfr);
+    // the workspace layout, every part 256-byte aligned
+    size_t used = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t at = used;
+        used += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t o_cells = carve(slot_line.size() * ncell * sizeof(CellRec));
+    const size_t o_rec = carve((size_t)rec0 * rsz);
+    const size_t o_rinfo = carve((size_t)rinfo0 * sizeof(RayInfo));
+    const size_t o_chan = carve(nchan * 8);
+    const size_t o_img = image_dev ? 0 : carve((size_t)img0 * 8);
+    const size_t o_part = carve((size_t)part0 * 8);
+    const size_t o_ovs = carve((size_t)nb * 8);
+    const size_t o_maser = carve((size_t)((nb + 1) / 2) * 8);
+    const size_t o_counter = carve(8);
+    const size_t o_items = carve((size_t)nb * sizeof(rl::ItemDev));
+    const size_t o_tile0 = carve((size_t)(nb + 1) * 8);
+    if (used > rl::SCRATCH_BYTES)
+        return fail(c, 13,
+                    "rl_trace_lines_forward_noalloc: the items need %zu bytes of workspace, "
+                    "more than its %zu (node records %lld x %zu B)", used, rl::SCRATCH_BYTES,
+                    rec0, rsz);
+    unsigned char *ws = static_cast<unsigned char *>(c->scratch);
+    rl::BatchLaunch L;
+    L.nitems = nb; L.max_rays = max_rays; L.max_nfr = max_nfr; L.max_rings = max_rings;
+    L.ncell = ncell; L.any_skip = any_skip;
+    L.items = reinterpret_cast<const rl::ItemDev *>(ws + o_items);
+    L.cells = reinterpret_cast<const CellRec *>(ws + o_cells);
+    L.rec = ws + o_rec;
+    L.rinfo = reinterpret_cast<RayInfo *>(ws + o_rinfo);
+    L.chan = reinterpret_cast<const double *>(ws + o_chan);
+    L.image = image_dev ? image_dev : reinterpret_cast<double *>(ws + o_img);
+    L.partial = reinterpret_cast<double *>(ws + o_part);
+    L.flux = flux_dev;
+    L.cmask = nullptr;
+    L.d2 = (1.0 * 3.08572e18) * (1.0 * 3.08572e18);
+    L.tile0 = reinterpret_cast<const long long *>(ws + o_tile0);
+    L.ntiles = t0[nb];
+    L.ovs = reinterpret_cast<unsigned long long *>(ws + o_ovs);
+    L.maser = reinterpret_cast<int *>(ws + o_maser);
+    L.counter = reinterpret_cast<unsigned long long *>(ws + o_counter);
+    // host tables in; counters cleared by a kernel (the workspace keeps the
+    // previous call's contents)
+    HIPCHK(c, hipMemcpyAsync(ws + o_chan, chan.data(), nchan * 8, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(ws + o_items, hi.data(), (size_t)nb * sizeof(rl::ItemDev),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ws + o_tile0, t0.data(), (size_t)(nb + 1) * 8,
+                             hipMemcpyHostToDevice, c->stream));
+    rl::launch_zero_u64(c->stream, L.ovs, nb);
+    rl::launch_zero_u64(c->stream, reinterpret_cast<unsigned long long *>(L.maser), (nb + 1) / 2);
+    for (size_t q = 0; q < slot_line.size(); q++) {
+        const int ln = slot_line[q];
+        rl::launch_cells(c->stream, ncell, c->nlev, c->nlines, ln, c->lev_up[ln], c->lev_low[ln],
+                         molpg, c->d_rho.as<double>(), c->d_abund.as<double>(),
+                         c->d_lw.as<double>(), c->d_pops.as<double>(), c->d_jd.as<double>(),
+                         c->d_ad.as<double>(),
+                         reinterpret_cast<CellRec *>(ws + o_cells) + q * ncell);
+    }
+    c->cells_line = -1;
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    rl::launch_batch_nodes(c->stream, fp64, L);
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    rl::launch_batch_formal(c->stream, fp64, L, cw, nblk, L.ntiles);
+    rl::launch_batch_finish(c->stream, fp64, L);
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    rl::launch_batch_flux(c->stream, L);
+    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    HIPCHK(c, hipGetLastError());
+    std::vector<unsigned long long> ovs(nb);
+    std::vector<int> maser(nb + 1, 0);
+    HIPCHK(c, hipMemcpyAsync(ovs.data(), L.ovs, (size_t)nb * 8, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(maser.data(), L.maser, (size_t)nb * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 5; i++) c->t_ms[i] = 0.0;
+    for (int i = 0; i < 4; i++) c->counts[i] = 0;
+    float t12 = 0, t23 = 0, t34 = 0;
+    (void)hipEventElapsedTime(&t12, c->ev[1], c->ev[2]);
+    (void)hipEventElapsedTime(&t23, c->ev[2], c->ev[3]);
+    (void)hipEventElapsedTime(&t34, c->ev[3], c->ev[4]);
+    c->t_ms[1] = t12; c->t_ms[2] = t23; c->t_ms[3] = t34; c->t_ms[4] = t12 + t23 + t34;
+    for (int k = 0; k < nb; k++) {
+        c->counts[0] += hi[k].nrays;
+        c->counts[1] += c->incl[items[k].incl].nnodes;
+        c->counts[3] += (long long)hi[k].nrays * hi[k].nfr;
+        if (warn)
+            warn[k] = (maser[k] ? RL_WARN_MASER : 0u) | (ovs[k] ? RL_WARN_OVERSHOOT : 0u);
+    }
+    c->counts[2] = (long long)ovs[nb - 1];
+    c->last_cw = cw;
+    c->last_blocks = nblk;
+    c->last_batches = 1;
     return 0;
 }
 

@@ -70,6 +70,14 @@ struct BatchLaunch {
 };
 
 void launch_batch_nodes(hipStream_t st, int fp64, const BatchLaunch &b);
+
+// Fixed device workspace (rl_scratch.hip) of the allocation-free forward
+// entry point: SCRATCH_BYTES per GPU, resident while the process runs.
+// scratch_attach returns its device address on `device` (which must be
+// current), looked up once per device.
+constexpr size_t SCRATCH_BYTES = (size_t)256 << 20;
+constexpr int SCRATCH_MAX_DEVICES = 64;
+hipError_t scratch_attach(int device, void **out);
 // nblocks == 0: static grid over the first ntiles tiles; > 0: persistent
 // grid of nblocks blocks taking tiles from the work counter
 void launch_batch_formal(hipStream_t st, int fp64, const BatchLaunch &b, int cw, int nblocks,

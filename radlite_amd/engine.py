@@ -74,6 +74,8 @@ def load_library(path=LIB_PATH):
                                              C.POINTER(C.c_uint)]),
         "rl_trace_lines_posvel": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
                                             C.POINTER(C.c_uint)]),
+        "rl_trace_lines_forward_noalloc": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), vp, vp,
+                                                     C.POINTER(C.c_uint)]),
         "rl_resample_image": (C.c_int, [vp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, _ip, _dp,
                                         _dp]),
         "rl_trace_lines_forward_ex": (C.c_int, [vp, C.c_int, C.POINTER(RLItem), _dp, _dp,
@@ -400,6 +402,20 @@ class Raytracer:
             io += n
         out = (fl, (im if img is not None else None), warn)
         return out + (cms,) if want_cmask else out
+
+    def trace_noalloc(self, items, flux_dev, image_dev=None):
+        """rl_trace_lines_forward_noalloc: the spectra (and images) of items
+        written to device memory at flux_dev (and image_dev), addresses as
+        ints (e.g. a torch tensor's data_ptr()), no device allocation by the
+        library.  Returns the warn bits."""
+        arr = (RLItem * len(items))(*[RLItem(*it) for it in items])
+        warn = np.zeros(len(items), np.uint32)
+        rc = self.L.rl_trace_lines_forward_noalloc(
+            self.ctx, len(items), arr, C.c_void_p(int(flux_dev)),
+            C.c_void_p(int(image_dev)) if image_dev else None,
+            warn.ctypes.data_as(C.POINTER(C.c_uint)))
+        self._chk(rc, "rl_trace_lines_forward_noalloc")
+        return warn
 
     def trace_tape(self, items):
         """Forward trace of SKIP_NONE items that keeps the adjoint's tape
