@@ -128,3 +128,16 @@ def golden_path(name):
 def load_golden(name):
     z = np.load(golden_path(name), allow_pickle=False)
     return {k: z[k] for k in z.files}
+
+
+def inputs_digest(ri) -> str:
+    """sha256 of the arrays of an in-memory RunInputs (the workload
+    generators' output), so that a fixture made from them detects drift."""
+    h = hashlib.sha256()
+    for k in ("r", "theta_half", "rho_gas", "abund", "vel", "lw_kms", "pops", "rho_dust",
+              "t_dust", "cont_nu", "kappa_abs", "kappa_scat", "star_fnu", "E_cm", "g", "lev_up",
+              "lev_low", "A"):
+        h.update(np.ascontiguousarray(getattr(ri, k), dtype=np.float64).tobytes())
+    h.update(repr((ri.rstar, ri.umass_av, ri.fix_lw, ri.nphi, ri.dbdr, ri.b_extra, ri.outer_bc,
+                   ri.vmax, ri.dv)).encode())
+    return h.hexdigest()

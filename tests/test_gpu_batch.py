@@ -158,3 +158,40 @@ def test_c3_batched_fp32_against_fp64_every_item():
         say(f"    item {k} {its[k][:3]}: sampled-ray max rel vs oracle {rel.max():.2e}")
         assert rel.max() <= 1e-8, (k, its[k], rel.max())
     rt.close()
+
+
+def test_c3_items_fp64_and_fp32_against_oracle_fixture():
+    """16 items of the 64-line x 8-inclination workload (lines 0-7 at 10 and
+    80 deg, 26-344 channels) in one batched call: FP64 against the oracle's
+    spectra (tests/golden/c3_oracle.npz) within 1e-10 per channel; FP32
+    line integrals within 1e-4, the errors stated."""
+    import os
+    from cases import inputs_digest
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "c3_oracle.npz"))
+    ri, items, incl = synth.c3_inputs(0)
+    assert inputs_digest(ri) == str(z["inputs_sha256"]), "workload generator drifted"
+    its = [(int(a), int(b), int(n), float(v), engine.SKIP_NONE)
+           for (a, b, n), v in zip(z["items"], z["vmax"])]
+    incs = [x * 0.0174532925199 for x in incl]
+    for prec in ("fp64", "fp32"):
+        print(f"[gpu] c3 16 items {prec} vs oracle fixture", file=sys.stderr, flush=True)
+        rt = engine.Raytracer(0, prec)
+        rt.load_run(ri)
+        rt.setup_rays(incs, ri.nphi, ri.b_extra, ri.dbdr)
+        fl, _, _ = rt.trace(its)
+        rt.close()
+        o, worst_ch, worst_li = 0, 0.0, 0.0
+        for k, it in enumerate(its):
+            ref = z["flux"][o:o + it[2]]
+            o += it[2]
+            rel = float(np.max(np.abs(fl[k] - ref) / np.abs(ref)))
+            li = float(np.sum(fl[k] - 0.5 * (fl[k][0] + fl[k][-1])))
+            lr = float(np.sum(ref - 0.5 * (ref[0] + ref[-1])))
+            eli = abs(li - lr) / abs(lr)
+            worst_ch, worst_li = max(worst_ch, rel), max(worst_li, eli)
+            if prec == "fp64":
+                assert rel <= 1e-10, (it, rel)
+            else:
+                assert eli <= 1e-4, (it, eli)
+        print(f"    {prec}: worst channel rel {worst_ch:.2e}, worst line integral rel "
+              f"{worst_li:.2e}", file=sys.stderr, flush=True)

@@ -188,3 +188,43 @@ def test_workload_c2_sampled_rays_and_symmetry():
         e = abs(_line_integral(f32[i]) - _line_integral(fl[i])) / abs(_line_integral(fl[i]))
         say(f"    c2 incl {w['incl'][i]}: fp32 line-integral rel err {e:.2e}")
         assert e <= FP32_LINE_TOL
+
+
+def _fixture(name):
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", f"{name}_oracle.npz"))
+    return {k: z[k] for k in z.files}
+
+
+def test_workload_c2_fp64_spectra_match_oracle_fixture():
+    """The 400x200 disk (133,633 rays x 200 channels at 30 and 60 deg):
+    the GPU FP64 spectra against the oracle's (tests/golden/c2_oracle.npz,
+    the oracle pinned bit-exact against the reference on the golden cases),
+    every channel within 1e-10 relative, with every channel integrated and
+    with the reference's NONREDUNDANT skipping; the FP32 error against
+    them stated."""
+    from cases import inputs_digest
+    fx = _fixture("c2")
+    ri, w = synth.workload_inputs("c2", seed=0)
+    assert inputs_digest(ri) == str(fx["inputs_sha256"]), "workload generator drifted"
+    nfr, vmax = engine.passband(ri.vmax, ri.dv)
+    incs = [x * 0.0174532925199 for x in w["incl"]]
+    items = [(0, i, nfr, vmax, mode) for i in range(2) for mode in (engine.SKIP_NONE,
+                                                                     engine.SKIP_REFERENCE)]
+    for prec in ("fp64", "fp32"):
+        say(f"[gpu] c2 {prec} spectra vs oracle fixture")
+        rt = engine.Raytracer(0, prec)
+        rt.load_run(ri)
+        rt.setup_rays(incs, ri.nphi, ri.b_extra, ri.dbdr)
+        fl, _, _ = rt.trace(items)
+        rt.close()
+        for k, (_, i, _, _, mode) in enumerate(items):
+            ref = fx["flux"][i, mode]
+            rel = _rel(fl[k], ref).max()
+            e_line = abs(_line_integral(fl[k]) - _line_integral(ref)) / abs(_line_integral(ref))
+            say(f"    incl {w['incl'][i]} skip={'none' if mode else 'reference'}: max channel "
+                f"rel {rel:.2e}, line integral rel {e_line:.2e}")
+            if prec == "fp64":
+                assert rel <= FP64_TOL, (i, mode, rel)
+            else:
+                assert e_line <= FP32_LINE_TOL and rel <= 1e-4, (i, mode, e_line, rel)
