@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -80,6 +81,11 @@ struct RaySet {
     DevBuf d_bpix;
     std::vector<int> star_rays;        // traced rays with impact parameter <= R*
     DevBuf d_star_rays;
+    // FP32 segment records (rl_fwd32.h), line-independent: built with the
+    // nodes, rebuilt when the line widths change (lw_gen)
+    bool seg_ready = false;
+    unsigned long long seg_lw_gen = 0;
+    DevBuf d_hot, d_aux;
     // node -> cell stencil transpose for the adjoint (built on first use)
     bool st_ready = false;
     DevBuf d_st_vals, d_st_w, d_st_off;
@@ -140,6 +146,9 @@ struct rl_ctx {
     // inputs generation: bumped by every setter, so that a tape taken
     // before a change is not used after it
     unsigned long long gen = 1;
+    // bumped whenever the line-width field changes (the FP32 segment
+    // records depend on it)
+    unsigned long long lw_gen = 1;
     std::map<std::string, std::pair<int, int>> tune;
     std::map<std::string, hipGraphExec_t> graphs;
     // adjoint buffers; the tape of rl_trace_lines_forward_tape
@@ -314,6 +323,7 @@ extern "C" int rl_set_medium(rl_ctx *c, const double *rho, const double *abund,
         if (int e = upload(c, c->d_vel, vel, nc * 24)) return e;
     }
     if (int e = upload(c, c->d_lw, lw, nc * 8)) return e;
+    c->lw_gen++;
     c->umass_av = umass_av;
     c->aksmax = aksmax;
     c->have_medium = true;
@@ -623,13 +633,15 @@ static int check_err(rl_ctx *c) {
 // stencil transpose
 static size_t traj_bytes(const rl_ctx *c, const RaySet &rs) {
     size_t b = (size_t)rs.nnodes * sizeof(GeoNode) + (size_t)(rs.nrays + 1) * 8;
+    if (c->precision == RL_FP32 && !rs.rect)
+        b += (size_t)rs.nnodes * (sizeof(rl::SegHot) + sizeof(rl::SegAux));
     if (rs.st_ready) b += (size_t)rs.nnodes * 4 * 12 + ((size_t)c->nr * c->nh + 1) * 4;
     return b;
 }
 
 // device bytes a ray set holds now
 static size_t held_bytes(const RaySet &rs) {
-    return rs.d_nodes.n + rs.d_st_vals.n + rs.d_st_w.n + rs.d_st_off.n;
+    return rs.d_nodes.n + rs.d_st_vals.n + rs.d_st_w.n + rs.d_st_off.n + rs.d_hot.n + rs.d_aux.n;
 }
 
 static size_t resident_traj(const rl_ctx *c) {
@@ -675,6 +687,9 @@ static void release_adjoint_work(rl_ctx *c) {
 
 static void evict_traj(RaySet &rs) {
     rs.d_nodes = DevBuf();
+    rs.d_hot = DevBuf();
+    rs.d_aux = DevBuf();
+    rs.seg_ready = false;
     rs.d_st_vals = DevBuf();
     rs.d_st_w = DevBuf();
     rs.d_st_off = DevBuf();
@@ -817,6 +832,23 @@ static int build_traj_keep(rl_ctx *c, int idx, const std::vector<int> &keep) {
     if (int e = check_err(c)) return e;
     rs.traj_ready = true;
     rs.st_ready = false;
+    rs.seg_ready = false;
+    return 0;
+}
+
+// the FP32 segment records of a built ray set, current with the line widths
+static int ensure_segrec(rl_ctx *c, RaySet &rs) {
+    if (c->precision != RL_FP32 || rs.rect) return 0;
+    if (!rs.traj_ready) return fail(c, 13, "segment records: trajectories not built");
+    if (rs.seg_ready && rs.seg_lw_gen == c->lw_gen) return 0;
+    HIPCHK(c, rs.d_hot.ensure((size_t)rs.nnodes * sizeof(rl::SegHot)));
+    HIPCHK(c, rs.d_aux.ensure((size_t)rs.nnodes * sizeof(rl::SegAux)));
+    rl::launch_segrec(c->stream, rs.nnodes, rs.d_off.as<long long>(), rs.nrays,
+                      rs.d_nodes.as<GeoNode>(), c->d_lw.as<double>(),
+                      rs.d_hot.as<rl::SegHot>(), rs.d_aux.as<rl::SegAux>());
+    HIPCHK(c, hipGetLastError());
+    rs.seg_ready = true;
+    rs.seg_lw_gen = c->lw_gen;
     return 0;
 }
 
@@ -1015,7 +1047,7 @@ static int item_consts(rl_ctx *c, const rl_item &it, const RaySet &rs, rl::LineC
 // Device bytes one item needs in a batch (node records, image, ring
 // partial sums, channel tables, flux, ray info).
 static size_t item_bytes(const rl_ctx *c, const rl_item &it, const RaySet &rs) {
-    const size_t rsz = c->precision == RL_FP64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::FastRec);
+    const size_t rsz = c->precision == RL_FP64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::NodeLine);
     return (size_t)rs.nnodes * rsz + (size_t)rs.nrays * it.nfr * 8 +
            (size_t)rs.nrings * it.nfr * 8 + (size_t)it.nfr * 32 + (size_t)rs.nrays * 16 + 64;
 }
@@ -1061,7 +1093,7 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
                      size_t io) {
     const int nb = i1 - i0;
     const int fp64 = c->precision == RL_FP64;
-    const size_t rsz = fp64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::FastRec);
+    const size_t rsz = fp64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::NodeLine);
     const int ncell = c->nr * c->nh;
     const double molpg = 1.0 / (c->umass_av * 1.6726e-24);
     // host layout of the batch
@@ -1084,8 +1116,11 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
         if (slot < 0) { slot = (int)slot_line.size(); slot_line.push_back(it.line); }
         d.nrays = rs.nrays; d.nfr = it.nfr; d.nrings = rs.nrings; d.nphi = rs.nphi;
         d.skip_mode = it.skip_mode; d.cslot = slot;
+        if (int e = ensure_segrec(c, rs)) return e;
         d.off = rs.d_off.as<long long>();
         d.geo = rs.d_nodes.as<GeoNode>();
+        d.hot_off = (long long)(uintptr_t)rs.d_hot.p;     // made relative below
+        d.aux_off = (long long)(uintptr_t)rs.d_aux.p;
         d.rec0 = rec0; d.img0 = img0; d.chan0 = chan0; d.part0 = part0; d.flux0 = flux0;
         d.rinfo0 = rinfo0;
         d.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
@@ -1116,6 +1151,10 @@ static int run_batch(rl_ctx *c, const rl_item *items, int i0, int i1, double *fl
     if (cmask) HIPCHK(c, c->d_cmask.ensure((size_t)img0));
     if (int e = upload(c, c->d_chan, chan.data(), nchan * 8)) return e;
     c->cells_line = -1;       // d_cells is rewritten below for this batch's lines
+    for (auto &d : hi) {      // segment records relative to the record buffer
+        d.hot_off -= (long long)(uintptr_t)c->d_rec.p;
+        d.aux_off -= (long long)(uintptr_t)c->d_rec.p;
+    }
     c->h_items = hi;
     if (int e = upload(c, c->d_items, c->h_items.data(), (size_t)nb * sizeof(rl::ItemDev))) return e;
     rl::BatchLaunch L;
@@ -1429,7 +1468,7 @@ extern "C" int rl_trace_lines_forward_noalloc(rl_ctx *c, int nitems, const rl_it
     if (c->outer_bc == 3 && c->isrf.empty())    // read_interstellfield, star.F:697-703
         return fail(c, 13, "outer boundary type 3 without the interstellar field");
     const int fp64 = c->precision == RL_FP64;
-    const size_t rsz = fp64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::FastRec);
+    const size_t rsz = fp64 ? sizeof(rl::NodeRec<double>) : sizeof(rl::NodeLine);
     const int ncell = c->nr * c->nh;
     const double molpg = 1.0 / (c->umass_av * 1.6726e-24);
     const int nb = nitems;
@@ -1445,7 +1484,7 @@ extern "C" int rl_trace_lines_forward_noalloc(rl_ctx *c, int nitems, const rl_it
         const RaySet &rs = c->incl[it.incl];
         if (rs.rect)
             return fail(c, 13, "item %d: a rectangular-image ray set: use rl_trace_lines_posvel", k);
-        if (!rs.traj_ready)
+        if (!rs.traj_ready || (!fp64 && !(rs.seg_ready && rs.seg_lw_gen == c->lw_gen)))
             return fail(c, 13,
                         "item %d: the trajectories of inclination %d are not resident (an "
                         "allocating trace of that inclination builds them)", k, it.incl);
@@ -1467,6 +1506,8 @@ extern "C" int rl_trace_lines_forward_noalloc(rl_ctx *c, int nitems, const rl_it
         d.skip_mode = it.skip_mode; d.cslot = slot;
         d.off = rs.d_off.as<long long>();
         d.geo = rs.d_nodes.as<GeoNode>();
+        d.hot_off = (long long)(uintptr_t)rs.d_hot.p;     // made relative below
+        d.aux_off = (long long)(uintptr_t)rs.d_aux.p;
         d.rec0 = rec0; d.img0 = img0; d.chan0 = chan0; d.part0 = part0; d.flux0 = flux0;
         d.rinfo0 = rinfo0;
         d.aks2 = 2.0 * (c->aksmax / 2.99792458e5);
@@ -1522,6 +1563,10 @@ extern "C" int rl_trace_lines_forward_noalloc(rl_ctx *c, int nitems, const rl_it
                     "more than its %zu (node records %lld x %zu B)", used, rl::SCRATCH_BYTES,
                     rec0, rsz);
     unsigned char *ws = static_cast<unsigned char *>(c->scratch);
+    for (auto &d : hi) {      // segment records relative to the record buffer
+        d.hot_off -= (long long)(uintptr_t)(ws + o_rec);
+        d.aux_off -= (long long)(uintptr_t)(ws + o_rec);
+    }
     rl::BatchLaunch L;
     L.nitems = nb; L.max_rays = max_rays; L.max_nfr = max_nfr; L.max_rings = max_rings;
     L.ncell = ncell; L.any_skip = any_skip;
@@ -2408,6 +2453,7 @@ extern "C" int rl_set_physics(rl_ctx *c, const double *T_gas, const double *T_du
     c->nlev = c->pre_nlev;
     HIPCHK(c, c->d_pops.ensure(ncell * c->nlev * 8));
     HIPCHK(c, c->d_lw.ensure(ncell * 8));
+    c->lw_gen++;                             // the device pre-pass rewrites the widths
     HIPCHK(c, c->d_abund.ensure(ncell * 8));
     HIPCHK(c, c->d_jd.ensure(ncell * c->nlines * 8));
     HIPCHK(c, c->d_ad.ensure(ncell * c->nlines * 8));

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "rl_fwd32.h"
+
 namespace rl {
 
 struct GridDev {
@@ -40,6 +42,8 @@ struct ItemDev {
     int cslot;                 // cell table slot of the item's line in the batch
     const long long *off;      // node CSR of the item's inclination
     const GeoNode *geo;
+    long long hot_off;         // FP32: the inclination's segment records, in bytes from the
+    long long aux_off;         // batch's record buffer (so that the kernels see global memory)
     long long rec0;            // first node record
     long long img0;            // first image value ([ray][channel])
     long long chan0;           // per channel: dnu at chan0, velo at chan0+nfr, the star's
@@ -70,6 +74,9 @@ struct BatchLaunch {
 };
 
 void launch_batch_nodes(hipStream_t st, int fp64, const BatchLaunch &b);
+// FP32: the line-independent segment records of a ray set (rl_fwd32.h)
+void launch_segrec(hipStream_t st, long long nnodes, const long long *off, int nrays,
+                   const GeoNode *geo, const double *lw, SegHot *hot, SegAux *aux);
 
 // Fixed device workspace (rl_scratch.hip) of the allocation-free forward
 // entry point: SCRATCH_BYTES per GPU, resident while the process runs.

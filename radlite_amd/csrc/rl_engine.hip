@@ -23,12 +23,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include <hipcub/hipcub.hpp>
 
 #include "rl_traj.h"
 #include "rl_adjoint.h"
+#include "rl_fwd32.h"
 
 namespace rl {
 
@@ -404,6 +406,109 @@ __device__ inline void nodes_ray_fast(int ray, int lane, const long long *__rest
     finish_window(ray, lane, umin, umax, nfr, velo, aks2, skip_mode, n, rinfo);
 }
 
+// ---------------------------------------------------------------------
+// split records of the FP32 batched forward (rl_fwd32.h)
+// ---------------------------------------------------------------------
+// a cell field at a node by the stencil of get_line_dust_values
+// (line.F:4006-4037), as node_interp forms it
+__device__ inline double interp_field(const GeoNode &gn, const double *__restrict__ f) {
+    const double dr = gn.dr, dt = gn.dt;
+    const int icr = gn.flag & 3;
+    if (icr == 1) return (1.0 - dt) * f[gn.c00] + dt * f[gn.c01];
+    if (icr == 2) return (1.0 - dr) * f[gn.c00] + dr * f[gn.c10];
+    return (1.0 - dr) * ((1.0 - dt) * f[gn.c00] + dt * f[gn.c01]) +
+           dr * ((1.0 - dt) * f[gn.c10] + dt * f[gn.c11]);
+}
+
+// per node (thread): the line-independent segment records of a ray set
+__global__ void __launch_bounds__(256)
+k_segrec(long long nnodes, const long long *__restrict__ off, int nrays,
+         const GeoNode *__restrict__ geo, const double *__restrict__ lw,
+         SegHot *__restrict__ hot, SegAux *__restrict__ aux) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnodes) return;
+    // first node of the ray? (binary search of the CSR)
+    int lo = 0, hi = nrays;
+    while (hi - lo > 1) {
+        const int m = (lo + hi) >> 1;
+        if (off[m] <= k) lo = m; else hi = m;
+    }
+    const bool first = k == off[lo];
+    const GeoNode gn = geo[k];
+    const int fl = gn.flag >> 2;
+    SegAux a;
+    a.u = (float)gn.u;
+    a.flag = fl;
+    a.pad0 = a.pad1 = a.pad2 = 0;
+    SegHot h;
+    if (first) {
+        a.ds = 0.0f; a.dls = -1.0f; a.inv_du = 0.0f;
+        h.hds = -0.0f; h.ag = 0.0f; h.uag = 0.0f; h.ng = 0.0f;
+    } else {
+        const GeoNode gp = geo[k - 1];
+        // the segment's mean width (line.F:4704) and its profile scales
+        // (voigt_profile line.F:2301-2302 without the line's |nu0|)
+        const double lwav = 0.5 * (interp_field(gp, lw) + interp_field(gn, lw));
+        const double inv = 1.0 / (3.33567e-6 * lwav);
+        const double ag = (double)SQRT_LOG2E * inv;
+        const double ds = (fl & NF_RESET) ? 0.0 : gn.ds;     // the hole chord adds nothing
+        const double du = gn.u - gp.u;
+        const double x = fabs(du / (lwav / 2.99792458e5));   // line.F:4706-4715
+        const bool sub = 2.0 * 3.0 * x > 1.0;
+        a.ds = (float)ds;
+        a.dls = sub ? (float)(ds / x) : -1.0f;
+        a.inv_du = (float)(1.0 / du);
+        const float hds = (float)(0.5 * ds);
+        // the sign bit routes the segment to the general path
+        h.hds = (sub || fl != 0) ? __int_as_float(__float_as_int(hds) | (int)0x80000000) : hds;
+        h.ag = (float)ag;
+        h.uag = (float)(gn.u * ag);
+        h.ng = (float)(0.56419583546 * inv);
+    }
+    hot[k] = h;
+    aux[k] = a;
+}
+
+// per item, one wave per ray: the scaled line records and the NONREDUNDANT
+// window (telescope.F:4265-4270) of the FP32 batched forward
+__device__ inline void nodes_ray_line(int ray, int lane, const long long *__restrict__ off,
+                                      const GeoNode *__restrict__ geo,
+                                      const CellRec *__restrict__ cells,
+                                      NodeLine *__restrict__ rec, RayInfo *__restrict__ rinfo,
+                                      int nfr, const double *__restrict__ velo, double aks2,
+                                      int skip_mode, const LineConst &lc) {
+    const long long b = off[ray];
+    const int n = (int)(off[ray + 1] - b);
+    const double inv_nu0 = 1.0 / fabs(lc.nu0);
+    double umin = 1.0e300, umax = -1.0e300;
+    for (int k = lane; k < n; k += 64) {
+        const GeoNode gn = geo[b + k];
+        double v[5];
+        node_interp(gn, cells, v);
+        double sig, jc, dc;
+        scaled_node(v, lc, sig, jc, dc);
+        NodeLine r;
+        const float fs = (float)sig;
+        r.sig = v[CR_AD] > 0.0 ? fs : __int_as_float(__float_as_int(fs) | (int)0x80000000);
+        r.jdh = (float)(v[CR_JD] / sig);
+        r.jcl = (float)(jc / sig * inv_nu0);
+        r.dcl = (float)(dc / sig * inv_nu0);
+        rec[b + k] = r;
+        if (k + 1 < n && v[CR_NU] + v[CR_NL] > LEVTHRES) {
+            umin = fmin(umin, gn.u);
+            umax = fmax(umax, gn.u);
+        }
+    }
+    finish_window(ray, lane, umin, umax, nfr, velo, aks2, skip_mode, n, rinfo);
+}
+
+void launch_segrec(hipStream_t st, long long nnodes, const long long *off, int nrays,
+                   const GeoNode *geo, const double *lw, SegHot *hot, SegAux *aux) {
+    if (nnodes <= 0) return;
+    hipLaunchKernelGGL(k_segrec, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, st,
+                       nnodes, off, nrays, geo, lw, hot, aux);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(64)
 k_nodes_fast(int nrays, const long long *__restrict__ off, const GeoNode *__restrict__ geo,
@@ -451,16 +556,23 @@ __device__ inline T trace_one(const NodeRec<T> *__restrict__ rec, int n, const L
     return I;
 }
 
+// node records of the batched forward: exact FP64 records, or the FP32
+// line records of the split layout (rl_fwd32.h); the position-velocity
+// path keeps the whole scaled record per node in FP32
 template <typename T> struct RecOf { using type = NodeRec<double>; };
-template <> struct RecOf<float> { using type = FastRec; };
+template <> struct RecOf<float> { using type = NodeLine; };
+template <typename T> struct PvRecOf { using type = NodeRec<double>; };
+template <> struct PvRecOf<float> { using type = FastRec; };
 
 template <typename T>
-__device__ inline double trace_dispatch(const typename RecOf<T>::type *rec, long long b, int n,
+__device__ inline double trace_dispatch(const typename RecOf<T>::type *rec, const SegHot *hot,
+                                        const SegAux *aux, long long b, int n,
                                         const LineConst &lc, double dnu, double velo,
                                         double istar, double I0, T sf, int &ovs, int &maser);
 
 template <>
-__device__ inline double trace_dispatch<double>(const NodeRec<double> *rec, long long b, int n,
+__device__ inline double trace_dispatch<double>(const NodeRec<double> *rec, const SegHot *,
+                                                const SegAux *, long long b, int n,
                                                 const LineConst &lc, double dnu, double velo,
                                                 double istar, double I0, double sf, int &ovs,
                                                 int &maser) {
@@ -472,12 +584,13 @@ __device__ inline double trace_dispatch<double>(const NodeRec<double> *rec, long
 }
 
 template <>
-__device__ inline double trace_dispatch<float>(const FastRec *rec, long long b, int n,
-                                               const LineConst &lc, double dnu, double velo,
+__device__ inline double trace_dispatch<float>(const NodeLine *rec, const SegHot *hot,
+                                               const SegAux *aux, long long b, int n,
+                                               const LineConst &, double, double velo,
                                                double istar, double I0, float sf, int &ovs,
                                                int &maser) {
-    FastLine L{(float)lc.nu0, (float)dnu, (float)velo};
-    return (double)trace_fast<float>(rec + b, n, L, (float)I0, (float)istar, sf, ovs, maser);
+    return (double)trace_split(hot + b, aux + b, rec + b, n, (float)velo, (float)I0,
+                               (float)istar, sf, ovs, maser);
 }
 
 // image placeholder of a channel left out by NONREDUNDANT skipping (a NaN
@@ -501,6 +614,8 @@ __device__ inline unsigned char window_mask(int ray, int ch, const RayInfo *__re
 template <typename T>
 __device__ inline double formal_one(int ray, int ch, const long long *__restrict__ off,
                                     const typename RecOf<T>::type *__restrict__ rec,
+                                    const SegHot *__restrict__ hot,
+                                    const SegAux *__restrict__ aux,
                                     const RayInfo *__restrict__ rinfo,
                                     const double *__restrict__ dnu,
                                     const double *__restrict__ velo,
@@ -514,8 +629,8 @@ __device__ inline double formal_one(int ray, int ch, const long long *__restrict
     }
     if (!doit) return __longlong_as_double(SKIPPED_BITS);   // filled by k_fixup
     T sf = ray == 0 ? (T)lc.starfract : T(0);
-    return trace_dispatch<T>(rec, off[ray], ri.n, lc, dnu[ch], velo[ch], istar[ch], i0[ch], sf,
-                             ovs, maser);
+    return trace_dispatch<T>(rec, hot, aux, off[ray], ri.n, lc, dnu[ch], velo[ch], istar[ch],
+                             i0[ch], sf, ovs, maser);
 }
 
 __global__ void k_flux_partial(int nfr, int nrings, int nphi, int rings_per_chunk,
@@ -577,7 +692,7 @@ __global__ void k_star_flags(int nlisted, const int *__restrict__ rays,
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_formal_posvel(int nrays, int nfr, int nx, int ny, const long long *__restrict__ off,
-                const typename RecOf<T>::type *__restrict__ rec,
+                const typename PvRecOf<T>::type *__restrict__ rec,
                 const double *__restrict__ dnu, const double *__restrict__ velo,
                 const double *__restrict__ istar, const double *__restrict__ i0, LineConst lc,
                 double *__restrict__ image, double *__restrict__ tau, double *__restrict__ extra,
@@ -634,7 +749,7 @@ void launch_star_flags(hipStream_t st, int nlisted, const int *rays, const long 
 
 template <typename T>
 void launch_posvel(hipStream_t st, const PosvelLaunch &a) {
-    auto *rec = reinterpret_cast<typename RecOf<T>::type *>(a.rec);
+    auto *rec = reinterpret_cast<typename PvRecOf<T>::type *>(a.rec);
     if constexpr (std::is_same<T, double>::value)
         hipLaunchKernelGGL(k_nodes, dim3(a.nrays), dim3(64), 0, st, a.nrays, a.off, a.geo,
                            a.cells, rec, a.rinfo, a.nfr, a.velo, a.aks2, 1);
@@ -644,7 +759,7 @@ void launch_posvel(hipStream_t st, const PosvelLaunch &a) {
     const long long total = (long long)a.nrays * a.nfr;
     hipLaunchKernelGGL(k_formal_posvel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                        st, a.nrays, a.nfr, a.nx, a.ny, a.off,
-                       reinterpret_cast<const typename RecOf<T>::type *>(a.rec), a.dnu, a.velo,
+                       reinterpret_cast<const typename PvRecOf<T>::type *>(a.rec), a.dnu, a.velo,
                        a.istar, a.i0, a.lc, a.image, a.tau, a.extra, a.ovs, a.maser);
 }
 template void launch_posvel<double>(hipStream_t, const PosvelLaunch &);
@@ -703,15 +818,20 @@ k_nodes_batch(const ItemDev *__restrict__ items, const CellRec *__restrict__ cel
                         reinterpret_cast<NodeRec<double> *>(rec_all) + it.rec0,
                         rinfo_all + it.rinfo0, it.nfr, velo, it.aks2, it.skip_mode);
     else
-        nodes_ray_fast<float>(ray, threadIdx.x, it.off, it.geo, cells,
-                              reinterpret_cast<FastRec *>(rec_all) + it.rec0, 0,
-                              rinfo_all + it.rinfo0, it.nfr, velo, it.aks2, it.skip_mode,
-                              it.lc);
+        nodes_ray_line(ray, threadIdx.x, it.off, it.geo, cells,
+                       reinterpret_cast<NodeLine *>(rec_all) + it.rec0, rinfo_all + it.rinfo0,
+                       it.nfr, velo, it.aks2, it.skip_mode, it.lc);
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
+}
+
+// segment records of an item: a byte offset from the batch's record buffer
+template <typename R>
+__device__ inline const R *seg_ptr(const void *rec_all, long long off) {
+    return reinterpret_cast<const R *>(static_cast<const char *>(rec_all) + off);
 }
 
 // Lane -> (ray, channel) of a tile.  cw == 0: the item's (ray, channel)
@@ -776,7 +896,10 @@ __device__ __forceinline__ void formal_batch_body(RL_FORMAL_BATCH_PARAMS) {
             const double *dnu = chan_all + it.chan0;
             const long long ix = it.img0 + (long long)ray * it.nfr + ch;
             const double v =
-                formal_one<T>(ray, ch, it.off, rec_base + it.rec0, rinfo_all + it.rinfo0, dnu,
+                formal_one<T>(ray, ch, it.off, rec_base + it.rec0,
+                              seg_ptr<SegHot>(rec_all, it.hot_off),
+                              seg_ptr<SegAux>(rec_all, it.aux_off),
+                              rinfo_all + it.rinfo0, dnu,
                               dnu + it.nfr, dnu + 2 * it.nfr, dnu + 3 * it.nfr, it.lc, it.aks2,
                               it.skip_mode, ovs, maser);
             image_all[ix] = v;
@@ -802,11 +925,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_formal_batch(RL_FORMAL_BATCH_PARAMS) {
     formal_batch_body<T>(RL_FORMAL_BATCH_ARGS);
 }
-// FP32: held to 64 VGPRs (eight waves per SIMD; 71 registers unconstrained,
-// seven waves); the spilled values are per-tile state, reloaded once per
-// tile outside the segment loop, so the extra wave costs nothing there
+// FP32 (split records): unconstrained, about 80 VGPRs (six waves per SIMD);
+// the next segment's records are loaded one segment ahead, which needs the
+// registers more than a seventh or eighth wave would
 template <>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ void __launch_bounds__(256)
 k_formal_batch<float>(RL_FORMAL_BATCH_PARAMS) {
     formal_batch_body<float>(RL_FORMAL_BATCH_ARGS);
 }
@@ -837,7 +960,10 @@ k_formal_tiles(const ItemDev *__restrict__ items, int nitems,
         const double *dnu = chan_all + it.chan0;
         const long long ix = it.img0 + (long long)ray * it.nfr + ch;
         const double v =
-            formal_one<T>(ray, ch, it.off, rec_base + it.rec0, rinfo_all + it.rinfo0, dnu,
+            formal_one<T>(ray, ch, it.off, rec_base + it.rec0,
+                          seg_ptr<SegHot>(rec_all, it.hot_off),
+                          seg_ptr<SegAux>(rec_all, it.aux_off),
+                          rinfo_all + it.rinfo0, dnu,
                           dnu + it.nfr, dnu + 2 * it.nfr, dnu + 3 * it.nfr, it.lc, it.aks2,
                           it.skip_mode, ovs, maser);
         image_all[ix] = v;
@@ -879,8 +1005,10 @@ __global__ void k_fixup_batch(const ItemDev *__restrict__ items, const void *rec
         if (cont != 0.0) {
             row[ch] = cont;
         } else {
-            row[ch] = trace_dispatch<T>(rec, it.off[ray], ri.n, it.lc, dnu[ch], velo[ch],
-                                        istar[ch], i0[ch], T(0), ovs, maser);
+            row[ch] = trace_dispatch<T>(rec, seg_ptr<SegHot>(rec_all, it.hot_off),
+                                        seg_ptr<SegAux>(rec_all, it.aux_off), it.off[ray], ri.n,
+                                        it.lc, dnu[ch],
+                                        velo[ch], istar[ch], i0[ch], T(0), ovs, maser);
             cont = row[ch];
         }
     }
@@ -1031,43 +1159,59 @@ k_adj_fwd(int nrays, int nfr, int ray0, long long node0, const long long *__rest
     image[g] = (double)I;
 }
 
-// Node gradients of a wave, summed over its lanes in batches of RED_N
-// nodes through LDS: each lane writes its values in row `lane` (RED_W
-// words, odd, so that the writes of 64 lanes fall in distinct banks), then
-// lane p sums column p (node p/5, quantity p%5) over the 64 rows.  Seven
-// nodes: 64 x 37 x 4 B = 9.3 KB per wave, so that the sixteen waves per CU
-// that the reverse sweep's registers allow also fit in LDS (eight nodes,
-// 10.5 KB, left room for fifteen: 13.7 -> 12.6 ms on the 200x100 disk).
+// Node gradients of a block, summed over its lanes in batches of RED_N
+// nodes through LDS.  A block holds one ray and one wave per channel group
+// (all waves visit the same nodes in the same order).  Each lane writes its
+// values in row `lane` of its wave's [64][RED_W] area (RED_W words, odd, so
+// that the writes of 64 lanes fall in distinct banks); lane p of each wave
+// sums column p (node p/5, quantity p%5) over its 64 rows into the wave's
+// partial sums; wave 0 then adds the waves' partial sums in wave order to
+// gnode.  The block owns its ray's node slots and every sum runs in a fixed
+// order: the gradients repeat bit for bit.  Seven nodes: 64 x 37 x 4 B =
+// 9.3 KB per wave, so that the sixteen waves per CU that the reverse
+// sweep's registers allow also fit in LDS (eight nodes, 10.5 KB, left room
+// for fifteen: 13.7 -> 12.6 ms on the 200x100 disk).
 constexpr int RED_N = 7;
 constexpr int RED_W = RED_N * 5 + 2;
 
 template <typename T>
-struct WaveNodeSum {
-    T (*row)[RED_W];           // [64][RED_W] in LDS
-    long long *node;           // [RED_N] in LDS
+__host__ __device__ inline size_t adj_rev_lds(int nw) {
+    size_t b = (size_t)nw * 64 * RED_W * sizeof(T);
+    b += ((size_t)nw * RED_N * 5 * sizeof(T) + 7) & ~(size_t)7;
+    return b + RED_N * sizeof(long long);
+}
+
+template <typename T>
+struct BlockNodeSum {
+    T (*row)[RED_W];           // this wave's [64][RED_W]
+    T *part;                   // [nw][RED_N * 5] the waves' column sums
+    long long *node;           // [RED_N]
+    int nw, w;                 // waves in the block, this wave
     int n = 0;
 
-    // The block that owns a ray writes all of its nodes' slots, and the
-    // same lane handles the same (node, quantity) column in every channel
-    // group (the groups visit the nodes in the same order), so a plain
-    // read-add-write suffices and the groups add in a fixed order: the sums
-    // repeat bit for bit.
     __device__ void flush(double *gnode) {
         __syncthreads();
-        const int lane = threadIdx.x;
+        const int lane = threadIdx.x & 63;
         for (int p = lane; p < n * 5; p += 64) {
             T acc = T(0);
             for (int i = 0; i < 64; i++) acc += row[i][p];
-            if (acc != T(0)) {
-                double *g = gnode + node[p / 5] * 5 + (p % 5);
-                *g = *g + (double)acc;
-            }
+            part[w * (RED_N * 5) + p] = acc;
         }
         __syncthreads();
+        if (w == 0) {
+            for (int p = lane; p < n * 5; p += 64) {
+                T acc = part[p];
+                for (int v = 1; v < nw; v++) acc += part[v * (RED_N * 5) + p];
+                if (acc != T(0)) {
+                    double *g = gnode + node[p / 5] * 5 + (p % 5);
+                    *g = *g + (double)acc;
+                }
+            }
+        }
         n = 0;
     }
     __device__ void push(double *gnode, long long nd, T a0, T a1, T a2, T a3, T a4) {
-        T *r = row[threadIdx.x] + n * 5;
+        T *r = row[threadIdx.x & 63] + n * 5;
         r[0] = a0; r[1] = a1; r[2] = a2; r[3] = a3; r[4] = a4;
         if (threadIdx.x == 0) node[n] = nd;
         if (++n == RED_N) flush(gnode);
@@ -1099,8 +1243,8 @@ __device__ __forceinline__ void adj_rev_group(
     const double *__restrict__ velo, const double *__restrict__ istar, const LineConst &lc,
     const double *__restrict__ gflux, const double *__restrict__ ring_w, double center_w,
     double d2, const T *__restrict__ ckpt, double *__restrict__ gnode, int kblk,
-    T (*red_row)[RED_W], long long *red_node) {
-    const int lane = threadIdx.x;
+    BlockNodeSum<T> &red) {
+    const int lane = threadIdx.x & 63;
     const int ch = grp * 64 + lane;
     const bool active = ch < nfr;
     const int chc = active ? ch : nfr - 1;
@@ -1119,7 +1263,6 @@ __device__ __forceinline__ void adj_rev_group(
     pg_zero(cur);
     pg_zero(prv);
     T isub[33];                    // 33 entries: kept in scratch memory, not registers
-    WaveNodeSum<T> red{red_row, red_node};
     const int nseg = n - 1;
     // profile at the end node of the segment being reversed, with its
     // constants: computed for the last segment, then carried from the
@@ -1191,23 +1334,58 @@ __device__ __forceinline__ void adj_rev_group(
     red.flush(gnode);
 }
 
+// one block of nw waves per ray: wave w takes channel groups w, w + nw, ...
+// (a group past the last channel runs with every lane idle, so that the
+// block's waves meet at the same node batches)
+#define RL_ADJ_REV_PARAMS                                                                   \
+    int nrays, int nfr, int nphi, int ray0, long long node0,                                \
+        const long long *__restrict__ off, const long long *__restrict__ ckoff,             \
+        const FastRecT<T> *__restrict__ rec, const double *__restrict__ dnu,                \
+        const double *__restrict__ velo, const double *__restrict__ istar, LineConst lc,    \
+        const double *__restrict__ gflux, const double *__restrict__ ring_w, double center_w, \
+        double d2, const T *__restrict__ ckpt, double *__restrict__ gnode, int kblk, int nw
 template <typename T>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
-k_adj_rev(int nrays, int nfr, int nphi, int ray0, long long node0,
-          const long long *__restrict__ off,
-          const long long *__restrict__ ckoff, const FastRecT<T> *__restrict__ rec,
-          const double *__restrict__ dnu, const double *__restrict__ velo,
-          const double *__restrict__ istar, LineConst lc, const double *__restrict__ gflux,
-          const double *__restrict__ ring_w, double center_w, double d2,
-          const T *__restrict__ ckpt, double *__restrict__ gnode, int kblk) {
-    __shared__ T red_row[64][RED_W];
-    __shared__ long long red_node[RED_N];
+__device__ __forceinline__ void adj_rev_body(RL_ADJ_REV_PARAMS) {
+    extern __shared__ __align__(16) unsigned char adj_lds[];
     const int ray = blockIdx.x;
-    if (off[ray + 1] - off[ray] < 2) return;     // uniform over the wave
+    if (off[ray + 1] - off[ray] < 2) return;     // uniform over the block
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
+    BlockNodeSum<T> red;
+    red.row = reinterpret_cast<T (*)[RED_W]>(adj_lds) + (size_t)w * 64;
+    red.part = reinterpret_cast<T *>(adj_lds + (size_t)nw * 64 * RED_W * sizeof(T));
+    red.node = reinterpret_cast<long long *>(
+        adj_lds + adj_rev_lds<T>(nw) - RED_N * sizeof(long long));
+    red.nw = nw;
+    red.w = w;
     const int ngrp = (nfr + 63) / 64;
-    for (int grp = 0; grp < ngrp; grp++)
-        adj_rev_group<T>(grp, ray, nfr, nphi, ray0, node0, off, ckoff, rec, dnu, velo, istar,
-                         lc, gflux, ring_w, center_w, d2, ckpt, gnode, kblk, red_row, red_node);
+    const int npass = (ngrp + nw - 1) / nw;
+    for (int p = 0; p < npass; p++)
+        adj_rev_group<T>(p * nw + w, ray, nfr, nphi, ray0, node0, off, ckoff, rec, dnu, velo,
+                         istar, lc, gflux, ring_w, center_w, d2, ckpt, gnode, kblk, red);
+}
+
+// waves per block at most: FP64 2 (its registers allow two waves per SIMD),
+// FP32 6 (held to 128 VGPRs, four waves per SIMD)
+template <typename T> struct AdjRevWaves { static constexpr int max = 2; };
+template <> struct AdjRevWaves<float> { static constexpr int max = 6; };
+
+template <typename T>
+__global__ void __launch_bounds__(64 * 2) __attribute__((amdgpu_waves_per_eu(2)))
+k_adj_rev(RL_ADJ_REV_PARAMS) {
+    adj_rev_body<T>(nrays, nfr, nphi, ray0, node0, off, ckoff, rec, dnu, velo, istar, lc, gflux,
+                    ring_w, center_w, d2, ckpt, gnode, kblk, nw);
+}
+template <>
+__global__ void __launch_bounds__(64 * 6) __attribute__((amdgpu_waves_per_eu(4)))
+k_adj_rev<float>(int nrays, int nfr, int nphi, int ray0, long long node0,
+                 const long long *__restrict__ off, const long long *__restrict__ ckoff,
+                 const FastRecT<float> *__restrict__ rec, const double *__restrict__ dnu,
+                 const double *__restrict__ velo, const double *__restrict__ istar, LineConst lc,
+                 const double *__restrict__ gflux, const double *__restrict__ ring_w,
+                 double center_w, double d2, const float *__restrict__ ckpt,
+                 double *__restrict__ gnode, int kblk, int nw) {
+    adj_rev_body<float>(nrays, nfr, nphi, ray0, node0, off, ckoff, rec, dnu, velo, istar, lc,
+                        gflux, ring_w, center_w, d2, ckpt, gnode, kblk, nw);
 }
 
 // ---- node -> cell transpose of the interpolation stencil (once per ray set)
@@ -1359,10 +1537,14 @@ template <typename T>
 void launch_adjoint_rev(hipStream_t st, const AdjLaunch &a) {
     auto *rec = reinterpret_cast<const FastRecT<T> *>(a.rec);
     auto *ck = reinterpret_cast<const T *>(a.ckpt);
-    hipLaunchKernelGGL(k_adj_rev<T>, dim3(a.nrays), dim3(64), 0, st,
+    // one wave per channel group, as many as 64 KB of LDS holds
+    const int ngrp = (a.nfr + 63) / 64;
+    int nw = std::min(ngrp, AdjRevWaves<T>::max);
+    while (nw > 1 && adj_rev_lds<T>(nw) > 65536) nw--;
+    hipLaunchKernelGGL(k_adj_rev<T>, dim3(a.nrays), dim3(64 * nw), adj_rev_lds<T>(nw), st,
                        a.nrays, a.nfr, a.nphi, a.ray0, a.node0, a.off, a.ckoff, rec, a.dnu,
                        a.velo, a.istar, a.lc, a.gflux, a.ring_w, a.center_w, a.d2, ck, a.gnode,
-                       a.kblk);
+                       a.kblk, nw);
     hipLaunchKernelGGL(k_node_conv, dim3((unsigned)((a.nnodes + 255) / 256)), dim3(256), 0, st,
                        a.nnodes, a.geo + a.node0, a.cells, a.gnode, a.lc, a.gtrue);
     hipLaunchKernelGGL(k_cell_gather, dim3(a.ncell), dim3(64), 0, st, a.ncell, a.st_off,
