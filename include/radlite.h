@@ -301,15 +301,40 @@ int rl_set_dust_species(rl_ctx *ctx, int nspec, const double *rho_d,
  * rl_set_physics to j_d at each line centre (line.F:3593-3594); nfreq = 0
  * removes it.  Call after rl_set_lines. */
 int rl_set_dust_scatter(rl_ctx *ctx, int nfreq, const double *cont_nu, const double *scat);
+/* The partition sum as a table (ParSum.dat's column of the molecule, read
+ * by _read_psum radlite.py:1354-1394): n >= 2 temperatures T [K],
+ * strictly increasing, and sums Q.  rl_set_physics with q < 0
+ * (RL_Q_TABLE) interpolates it linearly at each cell's gas temperature and
+ * extrapolates past its ends, as interp1d(kind="linear",
+ * fill_value="extrapolate") does at radlite.py:1150-1153.  n = 0 removes
+ * it. */
+#define RL_Q_TABLE (-1.0)
+int rl_set_partition_table(rl_ctx *ctx, int n, const double *T, const double *Q);
+/* pyradlite's turbulence model (_calc_turbulence radlite.py:1592-1631):
+ * with alpha >= 0, rl_set_physics ignores v_turb (which may be NULL) and
+ * sets lw = sqrt((alpha c_s)^2 + 2 k T_d / (m_mol m_p)) with c_s^2 = gamma k
+ * T_d / (mu m_p) and T_d the first dust species' temperature (T_gas when
+ * rl_set_physics has no T_dust); rl_physics_adjoint then carries the
+ * width's gradient to that temperature and returns 0 for v_turb.
+ * alpha < 0 turns it off (explicit v_turb, the default). */
+int rl_set_turbulence_alpha(rl_ctx *ctx, double alpha, double gamma, double mu);
+/* Keplerian velocity about a star of mass mstar [g] (_calc_velocity
+ * radlite.py:1634-1677, mstar from starinfo.inp): per cell v_r = v_theta
+ * = 0, v_phi = sqrt(G mstar / r) [cm/s] with r the cell's radius, formed
+ * on the device in place of rl_set_medium's velocity (trajectories are
+ * rebuilt on the next trace).  Call after rl_set_medium. */
+int rl_set_velocity_kepler(rl_ctx *ctx, double mstar);
 /* Per cell [nr][nth]: gas temperature T_gas [K], dust temperature
  * T_dust [nspec][nr][nth] (NULL: T_gas for every species), log-abundance
  * (NULL: keep the abundance of rl_set_medium), turbulent velocity
- * v_turb [km/s]; molecular mass m_mol [m_H]; partition function Q = T/q
- * (q > 0) or populations renormalised to sum 1 (q = 0).  Sets on the
- * device: n_l = g_l exp(-E_l/T)/Q (values below 1e-99 are 0), lw =
- * sqrt(v_turb^2 + 2kT/(m_mol m_H)), abundance = exp(log-abundance),
- * j_d = sum_s rho_s kabs_s B(nu0, T_s), a_d = sum_s rho_s (kabs_s +
- * kscat_s), and aksmax = max lw. */
+ * v_turb [km/s]; molecular mass m_mol [m_p]; partition function Q = T/q
+ * (q > 0), populations renormalised to sum 1 (q = 0), or the table of
+ * rl_set_partition_table (q < 0).  Sets on the device: n_l = g_l
+ * exp(-E_l/T)/Q (values below 1e-99 are 0), lw = sqrt(v_turb^2 +
+ * 2kT/(m_mol m_p)) (or the alpha model above), abundance =
+ * exp(log-abundance), j_d = sum_s rho_s kabs_s B(nu0, T_s), a_d = sum_s
+ * rho_s (kabs_s + kscat_s), and aksmax = max lw.  k and m_p are
+ * pyradlite's (astropy's CODATA 2018 values). */
 int rl_set_physics(rl_ctx *ctx, const double *T_gas, const double *T_dust,
                    const double *log_abund, const double *v_turb_kms, double m_mol,
                    double q_over);

@@ -131,6 +131,11 @@ struct rl_ctx {
     std::vector<int> x_owner;
     int cells_line = -1;
     std::vector<double> h_chan, h_vel;
+    // node buffers of the ray sets a setup call replaced, reused by the next
+    // builds of the same sizes (counted against the budget; freed first when
+    // room is needed)
+    std::vector<DevBuf> spare;
+    std::vector<DevBuf> x_cnt;               // k_cross node counts per slot
     // batching
     DevBuf d_items, d_tile0, d_counter, d_cmask;
     std::vector<rl::ItemDev> h_items;
@@ -164,6 +169,11 @@ struct rl_ctx {
     // last physical fields (for its adjoint)
     int pre_nlev = 0, pre_nspec = 0, pre_dust_from_gas = 0, pre_ready = 0;
     double pre_q = 0.0, pre_cw = 0.0;
+    int pre_alpha_on = 0;                    // the alpha turbulence model (rl_set_turbulence_alpha)
+    double pre_alpha = 0.0, pre_gamma = 0.0, pre_mu = 0.0;
+    int pre_vt_explicit = 1;                 // the last rl_set_physics took v_turb
+    int pre_nq = 0;                          // partition sum table (rl_set_partition_table)
+    DevBuf d_qx, d_qy;
     DevBuf d_EK, d_g, d_nu0, d_rhod, d_Td, d_kabs, d_kscat, d_T, d_la, d_vt, d_part;
     DevBuf d_scat, d_scat_lo, d_scat_w;     // scattering source (rl_set_dust_scatter)
     int scat_nf = 0;
@@ -503,6 +513,8 @@ static int build_rayset(rl_ctx *c, RaySet &rs, double anginf, int nphi, int nrex
     return 0;
 }
 
+static void keep_spares(rl_ctx *c);
+
 extern "C" int rl_setup_rays_circular(rl_ctx *c, int nincl, const double *incl, int nphi,
                                       int nrext, int dbdr, int *n_rays, int *n_rings,
                                       double *ring_r, double *ring_edge) {
@@ -510,6 +522,7 @@ extern "C" int rl_setup_rays_circular(rl_ctx *c, int nincl, const double *incl, 
     if (!c || !c->nr || nincl < 1 || !incl)
         return fail(c, RL_E_ARG, "rl_setup_rays_circular: set the grid first");
     HIPCHK(c, hipSetDevice(c->device));
+    keep_spares(c);            // launches that still read them precede any reuse (one stream)
     c->incl.clear();
     c->x_owner.assign(c->x_owner.size(), -1);
     c->incl.resize(nincl);
@@ -602,6 +615,7 @@ extern "C" int rl_setup_rays_rectang(rl_ctx *c, int nincl, const double *incl, i
     if (!c || !c->nr || nincl < 1 || !incl)
         return fail(c, RL_E_ARG, "rl_setup_rays_rectang: set the grid first");
     HIPCHK(c, hipSetDevice(c->device));
+    keep_spares(c);            // launches that still read them precede any reuse (one stream)
     c->incl.clear();
     c->x_owner.assign(c->x_owner.size(), -1);
     c->incl.resize(nincl);
@@ -647,7 +661,31 @@ static size_t held_bytes(const RaySet &rs) {
 static size_t resident_traj(const rl_ctx *c) {
     size_t b = 0;
     for (const RaySet &r : c->incl) b += held_bytes(r);
+    for (const DevBuf &d : c->spare) b += d.n;
     return b;
+}
+
+// the node buffers of the current ray sets to the spare pool (before a
+// setup call replaces the sets)
+static void keep_spares(rl_ctx *c) {
+    for (RaySet &r : c->incl)
+        for (DevBuf *d : {&r.d_nodes, &r.d_hot, &r.d_aux})
+            if (d->p) c->spare.push_back(std::move(*d));
+}
+
+// b holds at least `bytes`: a spare buffer if one fits (the smallest of at
+// most twice the size), else a new allocation
+static hipError_t ensure_spare(rl_ctx *c, DevBuf &b, size_t bytes) {
+    if (b.p && b.n >= bytes) return hipSuccess;
+    int best = -1;
+    for (int i = 0; i < (int)c->spare.size(); i++) {
+        const size_t n = c->spare[i].n;
+        if (n >= bytes && n <= 2 * bytes + 4096 && (best < 0 || n < c->spare[best].n)) best = i;
+    }
+    if (best < 0) return b.ensure(bytes);
+    b = std::move(c->spare[best]);
+    c->spare.erase(c->spare.begin() + best);
+    return hipSuccess;
 }
 
 // the memory budget: set by rl_set_memory_budget, else 60% of the device
@@ -701,6 +739,10 @@ static void evict_traj(RaySet &rs) {
 // the resident trajectories plus need fit the budget (or none is left)
 static int make_room(rl_ctx *c, size_t need, const std::vector<int> &keep) {
     const size_t budget = ctx_budget(c);
+    if (resident_traj(c) + need > budget && !c->spare.empty()) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->spare.clear();
+    }
     for (;;) {
         if (resident_traj(c) + need <= budget) return 0;
         int victim = -1;
@@ -731,7 +773,8 @@ static int cross_sets(rl_ctx *c, const std::vector<int> &idxs) {
     }
     rl::CrossSets sets{};
     sets.n = ns;
-    std::vector<DevBuf> cnt(ns);
+    if ((int)c->x_cnt.size() < ns) c->x_cnt.resize(ns);
+    std::vector<DevBuf> &cnt = c->x_cnt;
     for (int k = 0; k < ns; k++) {
         RaySet &rs = c->incl[idxs[k]];
         const int n = rs.nrays;
@@ -795,6 +838,7 @@ static void release_cross_scratch(rl_ctx *c) {
         (void)hipStreamSynchronize(c->stream);
         c->x_scr.clear();
         c->x_cross.clear();
+        c->x_cnt.clear();
         c->x_owner.clear();
     }
 }
@@ -819,11 +863,19 @@ static int build_traj_keep(rl_ctx *c, int idx, const std::vector<int> &keep) {
     const int nmax = traj_nmax(c);
     double *cs = c->x_cross[slot].as<double>();
     int *ccode = reinterpret_cast<int *>(cs + (size_t)rs.nrays * nmax);
-    HIPCHK(c, rs.d_nodes.ensure((size_t)rs.nnodes * sizeof(GeoNode)));
+    HIPCHK(c, ensure_spare(c, rs.d_nodes, (size_t)rs.nnodes * sizeof(GeoNode)));
+    // FP32 circular sets: the segment records in the same pass
+    const bool seg = c->precision == RL_FP32 && !rs.rect && c->d_lw.n;
+    if (seg) {
+        HIPCHK(c, ensure_spare(c, rs.d_hot, (size_t)rs.nnodes * sizeof(rl::SegHot)));
+        HIPCHK(c, ensure_spare(c, rs.d_aux, (size_t)rs.nnodes * sizeof(rl::SegAux)));
+    }
     rl::launch_geo(c->stream, g, c->d_vel.as<double>(), c->nh, rs.nrays, nmax,
                    rs.d_off.as<long long>(), rs.d_x0.as<double>(), rs.d_z0.as<double>(),
                    rs.costh0, rs.sinth0, cs, ccode, rs.d_nodes.as<GeoNode>(),
-                   c->d_err.as<int>());
+                   c->d_err.as<int>(), seg ? c->d_lw.as<double>() : nullptr,
+                   seg ? rs.d_hot.as<rl::SegHot>() : nullptr,
+                   seg ? rs.d_aux.as<rl::SegAux>() : nullptr);
     HIPCHK(c, hipGetLastError());
     if (rs.rect && !rs.star_rays.empty())
         rl::launch_star_flags(c->stream, (int)rs.star_rays.size(), rs.d_star_rays.as<int>(),
@@ -832,7 +884,8 @@ static int build_traj_keep(rl_ctx *c, int idx, const std::vector<int> &keep) {
     if (int e = check_err(c)) return e;
     rs.traj_ready = true;
     rs.st_ready = false;
-    rs.seg_ready = false;
+    rs.seg_ready = seg;
+    rs.seg_lw_gen = c->lw_gen;
     return 0;
 }
 
@@ -2392,10 +2445,14 @@ static void prepass_args(rl_ctx *c, rl::PrepassLaunch &a) {
     a.dust_from_gas = c->pre_dust_from_gas;
     a.T = c->d_T.as<double>();
     a.la = c->d_la.n ? c->d_la.as<double>() : nullptr;
-    a.vt = c->d_vt.as<double>();
+    a.vt = c->pre_vt_explicit ? c->d_vt.as<double>() : nullptr;
+    a.Tw = c->d_Td.as<double>();             // species 0, or the gas temperature's copy
     a.EK = c->d_EK.as<double>();
     a.g = c->d_g.as<double>();
     a.q = c->pre_q;
+    a.nq = c->pre_nq;
+    a.qx = c->d_qx.as<double>();
+    a.qy = c->d_qy.as<double>();
     a.cw = c->pre_cw;
     a.nu0 = c->d_nu0.as<double>();
     a.rho_d = c->d_rhod.as<double>();
@@ -2422,19 +2479,88 @@ static void prepass_args(rl_ctx *c, rl::PrepassLaunch &a) {
     a.gla = c->d_gla.as<double>();
 }
 
+// pyradlite's constants (astropy's CODATA 2018 values, radlite.py:43-51)
+constexpr double PYR_KB = 1.380649e-16, PYR_MP = 1.67262192369e-24, PYR_G = 6.6743e-8;
+
+extern "C" int rl_set_partition_table(rl_ctx *c, int n, const double *T, const double *Q) {
+    if (c) c->gen++;                         // invalidates a tape
+    if (!c || n < 0 || n == 1 || (n > 0 && (!T || !Q)))
+        return fail(c, RL_E_ARG, "rl_set_partition_table: bad args");
+    HIPCHK(c, hipSetDevice(c->device));
+    for (int i = 1; i < n; i++)
+        if (!(T[i] > T[i - 1]))
+            return fail(c, RL_E_ARG, "rl_set_partition_table: temperatures not increasing");
+    c->pre_nq = n;
+    if (n == 0) {
+        c->d_qx = DevBuf();
+        c->d_qy = DevBuf();
+        return 0;
+    }
+    if (int e = upload(c, c->d_qx, T, (size_t)n * 8)) return e;
+    if (int e = upload(c, c->d_qy, Q, (size_t)n * 8)) return e;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int rl_set_turbulence_alpha(rl_ctx *c, double alpha, double gamma, double mu) {
+    if (c) c->gen++;                         // invalidates a tape
+    if (!c) return RL_E_ARG;
+    if (alpha < 0.0) {
+        c->pre_alpha_on = 0;
+        return 0;
+    }
+    if (!(gamma > 0.0) || !(mu > 0.0))
+        return fail(c, RL_E_ARG, "rl_set_turbulence_alpha: bad args");
+    c->pre_alpha_on = 1;
+    c->pre_alpha = alpha;
+    c->pre_gamma = gamma;
+    c->pre_mu = mu;
+    return 0;
+}
+
+extern "C" int rl_set_velocity_kepler(rl_ctx *c, double mstar) {
+    if (c) c->gen++;                         // invalidates a tape
+    if (!c || !(mstar > 0.0)) return fail(c, RL_E_ARG, "rl_set_velocity_kepler: bad args");
+    if (!c->have_medium)
+        return fail(c, 13, "rl_set_velocity_kepler: set the medium first");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int ncell = c->nr * c->nh;
+    HIPCHK(c, c->d_vel.ensure((size_t)ncell * 24));
+    // the radii r_1..r_nr sit at index 2.. of the padded grid (rl_set_grid)
+    rl::launch_kepler(c->stream, ncell, c->nh, c->d_rc.as<double>() + 2, PYR_G * mstar,
+                      c->d_vel.as<double>());
+    HIPCHK(c, hipGetLastError());
+    c->h_vel.clear();                        // the next rl_set_medium uploads its own
+    for (auto &s : c->incl) s.traj_ready = false;        // u depends on the velocity
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 extern "C" int rl_set_physics(rl_ctx *c, const double *T_gas, const double *T_dust,
                               const double *log_abund, const double *vturb_kms, double m_mol,
                               double q_over) {
     if (c) c->gen++;                         // invalidates a tape
-    if (!c || !T_gas || !vturb_kms || !(m_mol > 0.0) || q_over < 0.0)
+    if (!c || !T_gas || (!vturb_kms && !c->pre_alpha_on) || !(m_mol > 0.0))
         return fail(c, RL_E_ARG, "rl_set_physics: bad args");
     if (!c->have_medium || !c->nlines || !c->pre_nlev || !c->d_nu0.n)
         return fail(c, 13, "rl_set_physics: set the medium, lines, level data and dust species "
                            "first");
+    if (q_over < 0.0 && c->pre_nq < 2)
+        return fail(c, 13, "rl_set_physics: a tabulated partition sum (q < 0) needs "
+                           "rl_set_partition_table");
+    if (c->pre_alpha_on && T_dust && c->pre_nspec < 1)
+        return fail(c, 13, "rl_set_physics: the alpha model takes the first dust species' "
+                           "temperature, and there is no dust species");
     HIPCHK(c, hipSetDevice(c->device));
     const size_t ncell = (size_t)c->nr * c->nh, nsp = (size_t)c->pre_nspec;
     if (int e = upload(c, c->d_T, T_gas, ncell * 8)) return e;
-    if (int e = upload(c, c->d_vt, vturb_kms, ncell * 8)) return e;
+    // explicit turbulence unless the alpha model is on
+    c->pre_vt_explicit = c->pre_alpha_on ? 0 : 1;
+    if (c->pre_vt_explicit) {
+        if (int e = upload(c, c->d_vt, vturb_kms, ncell * 8)) return e;
+    } else {
+        HIPCHK(c, c->d_vt.ensure(ncell * 8));
+    }
     if (log_abund) {
         if (int e = upload(c, c->d_la, log_abund, ncell * 8)) return e;
     } else {
@@ -2442,6 +2568,9 @@ extern "C" int rl_set_physics(rl_ctx *c, const double *T_gas, const double *T_du
     }
     HIPCHK(c, c->d_Td.ensure(std::max(nsp, (size_t)1) * ncell * 8));
     c->pre_dust_from_gas = T_dust ? 0 : 1;
+    if (nsp == 0)                            // the width's temperature (alpha model)
+        HIPCHK(c, hipMemcpyAsync(c->d_Td.p, c->d_T.p, ncell * 8, hipMemcpyDeviceToDevice,
+                                 c->stream));
     for (size_t s = 0; s < nsp; s++) {
         if (T_dust)
             HIPCHK(c, hipMemcpyAsync(c->d_Td.as<double>() + s * ncell, T_dust + s * ncell,
@@ -2460,7 +2589,12 @@ extern "C" int rl_set_physics(rl_ctx *c, const double *T_gas, const double *T_du
     const int nblk = (int)((ncell + 255) / 256);
     HIPCHK(c, c->d_part.ensure((size_t)nblk * 8));
     c->pre_q = q_over;
-    c->pre_cw = 2.0 * 1.380649e-16 / (m_mol * 1.6726e-24) / 1e10;   // 2k/(m_mol m_H), (km/s)^2/K
+    // thermal broadening 2k/(m_mol m_p) in (km/s)^2 per K (radlite.py:1620); the
+    // alpha model adds (alpha c_s)^2 / T = alpha^2 gamma k / (mu m_p) (:1617-1618)
+    c->pre_cw = 2.0 * PYR_KB / (m_mol * PYR_MP) / 1e10;
+    if (c->pre_alpha_on)
+        c->pre_cw += c->pre_alpha * c->pre_alpha * (c->pre_gamma * PYR_KB / (c->pre_mu * PYR_MP)) /
+                     1e10;
     rl::PrepassLaunch a;
     prepass_args(c, a);
     rl::launch_prepass(c->stream, a);
@@ -2502,7 +2636,7 @@ extern "C" int rl_physics_adjoint(rl_ctx *c, double *g_T_gas, double *g_T_dust,
     if (g_T_dust && !c->pre_dust_from_gas)
         if (int e = down(g_T_dust, c->d_gTd, nsp * ncell)) return e;
     if (int e = down(g_log_abund, c->d_gla, ncell)) return e;
-    if (int e = down(g_vturb, c->d_gvt, ncell)) return e;
+    if (int e = down(g_vturb, c->d_gvt, ncell)) return e;    // 0 under the alpha model
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }

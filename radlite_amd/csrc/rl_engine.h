@@ -183,7 +183,8 @@ template <typename T> void launch_posvel(hipStream_t st, const PosvelLaunch &a);
 void launch_geo(hipStream_t st, const GridDev &g, const double *vel, int nh, int nrays,
                 int nmax, const long long *off, const double *x0, const double *z0,
                 double costh0, double sinth0, const double *cs, const int *ccode,
-                GeoNode *nodes, int *err);
+                GeoNode *nodes, int *err, const double *lw = nullptr, SegHot *hot = nullptr,
+                SegAux *aux = nullptr);   // hot != null: the FP32 segment records too
 void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int lev_up,
                   int lev_low, double molpg, const double *rho, const double *abund,
                   const double *lw, const double *pops, const double *jd, const double *ad,
@@ -204,9 +205,14 @@ size_t fast_rec_size(int precision_fp64);
 // temperature, log-abundance and turbulence, and its adjoint.
 struct PrepassLaunch {
     int ncell, nlev, nlines, nspec, dust_from_gas;
-    const double *T, *la, *vt;         // [cell]; la may be null (abundance kept)
+    const double *T, *la, *vt;         // [cell]; la may be null (abundance kept); vt null:
+    const double *Tw;                  // the alpha model, whose width follows Tw [cell]
     const double *EK, *g;              // level energies [K] and weights [nlev]
-    double q, cw;                      // Q = T/q (q > 0) or renormalised (q = 0); 2k/(m_mol m_H)/1e10
+    double q;                          // Q = T/q (q > 0), renormalised (q = 0), table (q < 0)
+    int nq;                            // partition sum table: nq temperatures qx (ascending)
+    const double *qx, *qy;             // and sums qy
+    double cw;                         // lw^2 per kelvin: 2k/(m_mol m_p)/1e10 (+ the alpha
+                                       // model's (alpha c_s)^2 / T_d / 1e10)
     const double *nu0;                 // [line]
     const double *rho_d, *Td;          // [species][cell]
     const double *kabs, *kscat;        // at the line centres [species][line]
@@ -221,6 +227,8 @@ struct PrepassLaunch {
     double *gT, *gTd, *gvt, *gla;
 };
 void launch_prepass(hipStream_t st, const PrepassLaunch &a);
+// Keplerian velocity [cell][3] about a mass with G M = gm; r = the radii [nr]
+void launch_kepler(hipStream_t st, int ncell, int nh, const double *r, double gm, double *vel);
 void launch_prepass_adjoint(hipStream_t st, const PrepassLaunch &a);
 
 // Resampling of a circular image onto a grid (rl_resample.hip): out[ch][p]

@@ -111,17 +111,15 @@ k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, i
     count[ray] = n;
 }
 
-__device__ __forceinline__ void geo_node(GridView g, CellView cv, int k, double x0, double z0, double costh0,
-                         double sinth0, const double *__restrict__ cs,
-                         const int *__restrict__ ccode, GeoNode *__restrict__ nodes, int *err);
 
 // Stage 2, one thread per node: position, cell, interpolation stencil,
 // local direction (telescope.F:3682-3718) and projected velocity
 // (get_line_dust_values + omega_dot_v, line.F:4039-4208, 2650-2662),
 // compacted from the per-ray slots into the CSR node array.
-__device__ __forceinline__ void geo_node(GridView g, CellView cv, int k, double x0, double z0, double costh0,
-                         double sinth0, const double *__restrict__ cs,
-                         const int *__restrict__ ccode, GeoNode *__restrict__ nodes, int *err) {
+__device__ __forceinline__ GeoNode geo_node(GridView g, CellView cv, int k, double x0, double z0,
+                                            double costh0, double sinth0,
+                                            const double *__restrict__ cs,
+                                            const int *__restrict__ ccode, int *err) {
     const double pi = PICONST, epsplus = 1.0e1 * TELESC_EPS;
     const int nr = g.nr, ny = g.ny;
     const double sinth02 = sinth0 * sinth0, costh02 = costh0 * costh0;
@@ -208,7 +206,59 @@ __device__ __forceinline__ void geo_node(GridView g, CellView cv, int k, double 
         if (o.ds < 0.0) set_err(err, E_DSNEG);
     }
     o.flag = icross | (flag << 2);
-    nodes[k] = o;
+    return o;
+}
+
+// ---------------------------------------------------------------------
+// split records of the FP32 batched forward (rl_fwd32.h)
+// ---------------------------------------------------------------------
+// a cell field at a node by the stencil of get_line_dust_values
+// (line.F:4006-4037), as node_interp forms it
+__device__ inline double interp_field(const GeoNode &gn, const double *__restrict__ f) {
+    const double dr = gn.dr, dt = gn.dt;
+    const int icr = gn.flag & 3;
+    if (icr == 1) return (1.0 - dt) * f[gn.c00] + dt * f[gn.c01];
+    if (icr == 2) return (1.0 - dr) * f[gn.c00] + dr * f[gn.c10];
+    return (1.0 - dr) * ((1.0 - dt) * f[gn.c00] + dt * f[gn.c01]) +
+           dr * ((1.0 - dt) * f[gn.c10] + dt * f[gn.c11]);
+}
+
+// The line-independent records of the segment ending at node gn, from the
+// previous node's projected velocity up and width lwp and this node's
+// width lwk (first: the ray's first node, which ends no segment).
+__device__ inline void seg_record(const GeoNode &gn, bool first, double up, double lwp,
+                                  double lwk, SegHot &hot, SegAux &aux) {
+    const int fl = gn.flag >> 2;
+    SegAux a;
+    a.u = (float)gn.u;
+    a.flag = fl;
+    a.pad0 = a.pad1 = a.pad2 = 0;
+    SegHot h;
+    if (first) {
+        a.ds = 0.0f; a.dls = -1.0f; a.inv_du = 0.0f;
+        h.hds = -0.0f; h.ag = 0.0f; h.uag = 0.0f; h.ng = 0.0f;
+    } else {
+        // the segment's mean width (line.F:4704) and its profile scales
+        // (voigt_profile line.F:2301-2302 without the line's |nu0|)
+        const double lwav = 0.5 * (lwp + lwk);
+        const double inv = 1.0 / (3.33567e-6 * lwav);
+        const double ag = (double)SQRT_LOG2E * inv;
+        const double ds = (fl & NF_RESET) ? 0.0 : gn.ds;     // the hole chord adds nothing
+        const double du = gn.u - up;
+        const double x = fabs(du / (lwav / 2.99792458e5));   // line.F:4706-4715
+        const bool sub = 2.0 * 3.0 * x > 1.0;
+        a.ds = (float)ds;
+        a.dls = sub ? (float)(ds / x) : -1.0f;
+        a.inv_du = (float)(1.0 / du);
+        const float hds = (float)(0.5 * ds);
+        // the sign bit routes the segment to the general path
+        h.hds = (sub || fl != 0) ? __int_as_float(__float_as_int(hds) | (int)0x80000000) : hds;
+        h.ag = (float)ag;
+        h.uag = (float)(gn.u * ag);
+        h.ng = (float)(0.56419583546 * inv);
+    }
+    hot = h;
+    aux = a;
 }
 
 __global__ void __launch_bounds__(256)
@@ -220,8 +270,42 @@ k_geo(GridView g, CellView cv, int nmax, const long long *__restrict__ off,
     const int k = blockIdx.y * blockDim.x + threadIdx.x;
     const long long o = off[ray];
     if (k >= (int)(off[ray + 1] - o)) return;
-    geo_node(g, cv, k, x0a[ray], z0a[ray], costh0, sinth0, cs_all + (long long)ray * nmax,
-             ccode_all + (long long)ray * nmax, nodes_all + o, err);
+    nodes_all[o + k] = geo_node(g, cv, k, x0a[ray], z0a[ray], costh0, sinth0,
+                                cs_all + (long long)ray * nmax, ccode_all + (long long)ray * nmax,
+                                err);
+}
+
+// FP32: the nodes and, in the same pass, their segment records.  A block
+// writes 255 nodes of a ray; its thread 0 forms the node before them
+// (not written), whose projected velocity and width the block's first
+// segment needs.
+__global__ void __launch_bounds__(256)
+k_geo_seg(GridView g, CellView cv, int nmax, const long long *__restrict__ off,
+          const double *__restrict__ x0a, const double *__restrict__ z0a, double costh0,
+          double sinth0, const double *__restrict__ cs_all, const int *__restrict__ ccode_all,
+          GeoNode *__restrict__ nodes_all, const double *__restrict__ lw,
+          SegHot *__restrict__ hot_all, SegAux *__restrict__ aux_all, int *err) {
+    __shared__ double s_u[256], s_lw[256];
+    const int ray = blockIdx.x;
+    const int t = threadIdx.x;
+    const long long o = off[ray];
+    const int n = (int)(off[ray + 1] - o);
+    if ((int)blockIdx.y * 255 >= n) return;               // uniform over the block
+    const int k = (int)blockIdx.y * 255 + t - 1;
+    const bool valid = k >= 0 && k < n;
+    GeoNode gn;
+    double lwk = 0.0;
+    if (valid) {
+        gn = geo_node(g, cv, k, x0a[ray], z0a[ray], costh0, sinth0,
+                      cs_all + (long long)ray * nmax, ccode_all + (long long)ray * nmax, err);
+        lwk = interp_field(gn, lw);
+        s_u[t] = gn.u;
+        s_lw[t] = lwk;
+    }
+    __syncthreads();
+    if (!valid || t == 0) return;
+    nodes_all[o + k] = gn;
+    seg_record(gn, k == 0, s_u[t - 1], s_lw[t - 1], lwk, hot_all[o + k], aux_all[o + k]);
 }
 
 // ---------------------------------------------------------------------
@@ -409,18 +493,8 @@ __device__ inline void nodes_ray_fast(int ray, int lane, const long long *__rest
 // ---------------------------------------------------------------------
 // split records of the FP32 batched forward (rl_fwd32.h)
 // ---------------------------------------------------------------------
-// a cell field at a node by the stencil of get_line_dust_values
-// (line.F:4006-4037), as node_interp forms it
-__device__ inline double interp_field(const GeoNode &gn, const double *__restrict__ f) {
-    const double dr = gn.dr, dt = gn.dt;
-    const int icr = gn.flag & 3;
-    if (icr == 1) return (1.0 - dt) * f[gn.c00] + dt * f[gn.c01];
-    if (icr == 2) return (1.0 - dr) * f[gn.c00] + dr * f[gn.c10];
-    return (1.0 - dr) * ((1.0 - dt) * f[gn.c00] + dt * f[gn.c01]) +
-           dr * ((1.0 - dt) * f[gn.c10] + dt * f[gn.c11]);
-}
-
 // per node (thread): the line-independent segment records of a ray set
+// whose nodes are built (after the widths change)
 __global__ void __launch_bounds__(256)
 k_segrec(long long nnodes, const long long *__restrict__ off, int nrays,
          const GeoNode *__restrict__ geo, const double *__restrict__ lw,
@@ -435,38 +509,13 @@ k_segrec(long long nnodes, const long long *__restrict__ off, int nrays,
     }
     const bool first = k == off[lo];
     const GeoNode gn = geo[k];
-    const int fl = gn.flag >> 2;
-    SegAux a;
-    a.u = (float)gn.u;
-    a.flag = fl;
-    a.pad0 = a.pad1 = a.pad2 = 0;
-    SegHot h;
-    if (first) {
-        a.ds = 0.0f; a.dls = -1.0f; a.inv_du = 0.0f;
-        h.hds = -0.0f; h.ag = 0.0f; h.uag = 0.0f; h.ng = 0.0f;
-    } else {
+    double up = 0.0, lwp = 0.0;
+    if (!first) {
         const GeoNode gp = geo[k - 1];
-        // the segment's mean width (line.F:4704) and its profile scales
-        // (voigt_profile line.F:2301-2302 without the line's |nu0|)
-        const double lwav = 0.5 * (interp_field(gp, lw) + interp_field(gn, lw));
-        const double inv = 1.0 / (3.33567e-6 * lwav);
-        const double ag = (double)SQRT_LOG2E * inv;
-        const double ds = (fl & NF_RESET) ? 0.0 : gn.ds;     // the hole chord adds nothing
-        const double du = gn.u - gp.u;
-        const double x = fabs(du / (lwav / 2.99792458e5));   // line.F:4706-4715
-        const bool sub = 2.0 * 3.0 * x > 1.0;
-        a.ds = (float)ds;
-        a.dls = sub ? (float)(ds / x) : -1.0f;
-        a.inv_du = (float)(1.0 / du);
-        const float hds = (float)(0.5 * ds);
-        // the sign bit routes the segment to the general path
-        h.hds = (sub || fl != 0) ? __int_as_float(__float_as_int(hds) | (int)0x80000000) : hds;
-        h.ag = (float)ag;
-        h.uag = (float)(gn.u * ag);
-        h.ng = (float)(0.56419583546 * inv);
+        up = gp.u;
+        lwp = interp_field(gp, lw);
     }
-    hot[k] = h;
-    aux[k] = a;
+    seg_record(gn, first, up, lwp, interp_field(gn, lw), hot[k], aux[k]);
 }
 
 // per item, one wave per ray: the scaled line records and the NONREDUNDANT
@@ -782,12 +831,16 @@ void launch_cross(hipStream_t st, const GridDev &gd, const double *tan2, const C
 void launch_geo(hipStream_t st, const GridDev &gd, const double *vel, int nh, int nrays,
                 int nmax, const long long *off, const double *x0, const double *z0,
                 double costh0, double sinth0, const double *cs, const int *ccode,
-                GeoNode *nodes, int *err) {
+                GeoNode *nodes, int *err, const double *lw, SegHot *hot, SegAux *aux) {
     GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar};
     CellView cv{nh, vel};
-    hipLaunchKernelGGL(k_geo, dim3(nrays, (nmax + 255) / 256), dim3(256), 0, st, g, cv, nmax, off,
-                       x0, z0, costh0,
-                       sinth0, cs, ccode, nodes, err);
+    if (hot)
+        hipLaunchKernelGGL(k_geo_seg, dim3(nrays, (nmax + 254) / 255), dim3(256), 0, st, g, cv,
+                           nmax, off, x0, z0, costh0, sinth0, cs, ccode, nodes, lw, hot, aux,
+                           err);
+    else
+        hipLaunchKernelGGL(k_geo, dim3(nrays, (nmax + 255) / 256), dim3(256), 0, st, g, cv, nmax,
+                           off, x0, z0, costh0, sinth0, cs, ccode, nodes, err);
 }
 
 void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int lev_up,

@@ -3,11 +3,16 @@ raytrace as a torch.autograd.Function whose backward is the HIP adjoint.
 
 Chain (all per cell on the half theta grid [nr, nth]):
 
-* LTE populations n_i = g_i exp(-E_i/kT) / Q(T), Q = T/2.77
-  (pyradlite ``_prep_mol_forcore``, radlite.py:1111-1119, with the
-  closed-form partition function of the workloads);
-* line width lw = sqrt(v_turb^2 + 2kT/(m_mol m_H)) in km/s
-  (``_calc_turbulence``, radlite.py:1618-1622);
+* LTE populations n_i = g_i exp(-E_i/kT) / Q(T), Q = T/2.77 (the
+  closed-form partition function of the workloads) or Q interpolated
+  linearly in a table and extrapolated past its ends (pyradlite
+  ``_prep_mol_forcore``, radlite.py:1111-1119, with the ParSum.dat
+  interpolation of radlite.py:1150-1153);
+* line width lw = sqrt(v_turb^2 + 2kT/(m_mol m_p)) in km/s, or the
+  alpha model sqrt((alpha c_s)^2 + 2kT/(m_mol m_p))
+  (``_calc_turbulence``, radlite.py:1613-1622);
+* Keplerian velocity v_phi = sqrt(G M / r) (``_calc_velocity``,
+  radlite.py:1653-1664);
 * abundance = exp(log_abund);
 * dust emissivity at each line centre j_d = rho_d kappa_abs(nu0)
   (B_nu(T) + 1e-290) and extinction a_d = rho_d (kappa_abs + kappa_scat)
@@ -30,17 +35,55 @@ import torch
 from . import engine, synth
 
 
-def lte_populations(T, E_cm, g, q_over=2.77):
-    """n_i [nr, nth, nlev] = g_i exp(-E_i/kT) / (T/q_over); below 1e-99 -> 0."""
+# pyradlite's constants: astropy's CODATA 2018 values (radlite.py:43-51)
+KB_PYR = 1.380649e-16            # erg/K
+MP_PYR = 1.67262192369e-24       # proton mass [g]
+G_PYR = 6.6743e-8                # cm^3 g^-1 s^-2
+
+
+def partition_sum(T, T_tab, Q_tab):
+    """Q(T) from a table as interp1d(kind="linear", bounds_error=False,
+    fill_value="extrapolate") evaluates it (radlite.py:1150-1153): the
+    interval is the first tabulated T >= T, clipped to the first and last
+    intervals, Q = slope (T - T_lo) + Q_lo."""
+    x = torch.as_tensor(np.asarray(T_tab, float), dtype=T.dtype)
+    y = torch.as_tensor(np.asarray(Q_tab, float), dtype=T.dtype)
+    i = torch.searchsorted(x, T.detach().contiguous()).clamp(1, len(x) - 1)
+    lo = i - 1
+    slope = (y[i] - y[lo]) / (x[i] - x[lo])
+    return slope * (T - x[lo]) + y[lo]
+
+
+def lte_populations(T, E_cm, g, q_over=2.77, q_table=None):
+    """n_i [nr, nth, nlev] = g_i exp(-E_i/kT) / Q(T), Q = T/q_over or, with
+    q_table = (T_tab, Q_tab), the interpolated table; below 1e-99 -> 0."""
     EK = torch.as_tensor(np.asarray(E_cm) * synth.K_PER_CM, dtype=T.dtype)
     gg = torch.as_tensor(np.asarray(g, float), dtype=T.dtype)
     Tc = T[..., None]
-    n = gg * torch.exp(-EK / Tc) / (Tc / q_over)
+    Q = (Tc / q_over) if q_table is None else partition_sum(T, *q_table)[..., None]
+    n = gg * torch.exp(-EK / Tc) / Q
     return torch.where(n < 1e-99, torch.zeros_like(n), n)
 
 
-def line_width_kms(T, v_turb_kms, m_mol=28.0):
-    return torch.sqrt(v_turb_kms ** 2 + 2 * synth.KB * T / (m_mol * synth.MH) / 1e10)
+def line_width_kms(T, v_turb_kms, m_mol=28.0, m_p=MP_PYR):
+    return torch.sqrt(v_turb_kms ** 2 + 2 * KB_PYR * T / (m_mol * m_p) / 1e10)
+
+
+def line_width_alpha_kms(T_dust, alpha, gamma=1.4, mu=2.3, m_mol=28.0):
+    """pyradlite's turbulence (radlite.py:1613-1622) in km/s: the alpha
+    fraction of the sound speed and the thermal broadening in quadrature,
+    both at the dust temperature."""
+    cs = torch.sqrt(gamma * KB_PYR * T_dust / (mu * MP_PYR))
+    th = torch.sqrt(2.0 * KB_PYR * T_dust / (m_mol * MP_PYR))
+    return torch.sqrt((alpha * cs) ** 2 + th ** 2) / 1e5
+
+
+def kepler_velocity(r, nth, mstar):
+    """(v_r, v_theta, v_phi) [nr, nth, 3] in cm/s about a star of mass
+    mstar [g] (radlite.py:1653-1664), r the cell radii [nr]."""
+    v = np.zeros((len(r), nth, 3))
+    v[..., 2] = np.sqrt(G_PYR * mstar / np.asarray(r, float))[:, None]
+    return v
 
 
 def kappa_at_lines(cont_nu, kappa, nu0):

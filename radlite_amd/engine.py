@@ -30,6 +30,7 @@ _ip = C.POINTER(C.c_int)
 RL_FP64, RL_FP32 = 0, 1
 OPT_GRAPH, OPT_AUTOTUNE, OPT_TILE, OPT_PERSISTENT, OPT_CKPT = 1, 2, 3, 4, 5
 SKIP_REFERENCE, SKIP_NONE = 0, 1
+Q_TABLE = -1.0          # set_physics: the partition sum from set_partition_table
 WARN_MASER, WARN_OVERSHOOT = 1, 2
 
 
@@ -91,6 +92,9 @@ def load_library(path=LIB_PATH):
         "rl_set_dust_species": (C.c_int, [vp, C.c_int, _dp, _dp, _dp]),
         "rl_set_physics": (C.c_int, [vp, _dp, _dp, _dp, _dp, C.c_double, C.c_double]),
         "rl_physics_adjoint": (C.c_int, [vp, _dp, _dp, _dp, _dp]),
+        "rl_set_partition_table": (C.c_int, [vp, C.c_int, _dp, _dp]),
+        "rl_set_turbulence_alpha": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
+        "rl_set_velocity_kepler": (C.c_int, [vp, C.c_double]),
         "rl_get_tables": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
         "rl_sync": (C.c_int, [vp]),
         "rl_set_memory_budget": (C.c_int, [vp, C.c_double]),
@@ -496,10 +500,35 @@ class Raytracer:
                   "rl_set_dust_species")
         self.nspec = r.shape[0]
 
+    def set_partition_table(self, T, Q):
+        """Tabulated partition sum (T ascending); set_physics(q_over=Q_TABLE)
+        interpolates it linearly, extrapolating past the ends.  T=None
+        removes it."""
+        if T is None:
+            self._chk(self.L.rl_set_partition_table(self.ctx, 0, None, None),
+                      "rl_set_partition_table")
+            return
+        t, tp = _d(T)
+        q, qp = _d(Q)
+        self._chk(self.L.rl_set_partition_table(self.ctx, len(t), tp, qp),
+                  "rl_set_partition_table")
+
+    def set_turbulence_alpha(self, alpha, gamma=1.4, mu=2.3):
+        """pyradlite's alpha turbulence model (alpha < 0: explicit v_turb)."""
+        self._chk(self.L.rl_set_turbulence_alpha(self.ctx, float(alpha), float(gamma), float(mu)),
+                  "rl_set_turbulence_alpha")
+
+    def set_velocity_kepler(self, mstar_g):
+        """Keplerian velocity about a star of mass mstar [g], on the device."""
+        self._chk(self.L.rl_set_velocity_kepler(self.ctx, float(mstar_g)),
+                  "rl_set_velocity_kepler")
+
     def set_physics(self, T_gas, v_turb_kms, log_abund=None, T_dust=None, m_mol=28.0,
                     q_over=2.77):
+        """q_over > 0: Q = T/q_over; 0: populations renormalised; Q_TABLE:
+        the partition table.  v_turb_kms may be None under the alpha model."""
         t, tp = _d(T_gas)
-        v, vp_ = _d(v_turb_kms)
+        v, vp_ = (None, None) if v_turb_kms is None else _d(v_turb_kms)
         la, lap = (None, None) if log_abund is None else _d(log_abund)
         td, tdp = (None, None) if T_dust is None else _d(T_dust)
         self._chk(self.L.rl_set_physics(self.ctx, tp, tdp, lap, vp_, float(m_mol),

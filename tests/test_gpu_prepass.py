@@ -130,3 +130,105 @@ def test_separate_dust_temperature():
     assert _rel(tot, both["g_T"]) <= 1e-12
     assert np.abs(sep["g_T_dust"]).max() > 0
     m.close()
+
+
+def _q_table():
+    Tt = np.array([5.0, 20.0, 50.0, 100.0, 200.0, 500.0, 1000.0, 2000.0, 4000.0])
+    return Tt, 0.3617 * Tt + 2.1e-5 * Tt ** 2 + 0.5
+
+
+def test_partition_table_and_alpha_model_tables():
+    """Populations from a tabulated partition sum (interpolated and
+    extrapolated as interp1d does) and the width of the alpha turbulence
+    model, on the device against the torch pre-pass (pinned by hand values
+    and scipy on the CPU), gas and dust temperatures spanning the table's
+    ends; the adjoint of the alpha width against central differences."""
+    say("[gpu] partition table + alpha model")
+    ri, m, T0, la0, vt0 = _problem("fp64")
+    with torch.no_grad():
+        m.flux_from_physics_device(T0, la0, vt0)       # uploads level and dust data
+    rt = m.rt
+    Tt, Qt = _q_table()
+    rt.set_partition_table(Tt, Qt)
+    rng = np.random.default_rng(8)
+    T = T0.numpy() * np.exp(rng.normal(0.0, 0.8, size=T0.shape))
+    Td = T * np.exp(rng.normal(0.0, 0.2, size=T0.shape))
+    say(f"    T in [{T.min():.3g}, {T.max():.3g}] K, table [{Tt[0]}, {Tt[-1]}]")
+    assert T.min() < Tt[0] or T.max() > Tt[-1]
+    rt.set_turbulence_alpha(0.9, 1.4, 2.3)
+    rt.set_physics(T, None, log_abund=la0.numpy(), T_dust=Td[None], q_over=engine.Q_TABLE)
+    t = rt.get_tables()
+    pops = diff.lte_populations(torch.tensor(T), ri.E_cm, ri.g, q_table=(Tt, Qt)).numpy()
+    lw = diff.line_width_alpha_kms(torch.tensor(Td), 0.9, 1.4, 2.3).numpy()
+    e_p, e_w = _rel(t["pops"], pops), float(np.abs(t["lw"] / lw - 1).max())
+    say(f"    device vs torch: pops {e_p:.2e}, lw {e_w:.2e}")
+    assert e_p <= 1e-13 and e_w <= 1e-14
+    # explicit turbulence again once the model is off
+    rt.set_turbulence_alpha(-1.0)
+    rt.set_physics(T, vt0.numpy(), log_abund=la0.numpy(), q_over=engine.Q_TABLE)
+    t = rt.get_tables()
+    assert _rel(t["lw"], diff.line_width_kms(torch.tensor(T), vt0).numpy()) <= 1e-14
+    # dL/dT_dust of the alpha width by central differences on a few cells
+    rt.set_turbulence_alpha(0.9, 1.4, 2.3)
+    w = [np.random.default_rng(9).normal(size=m.nfr)]
+    rt.set_physics(T, None, log_abund=la0.numpy(), T_dust=Td[None], q_over=engine.Q_TABLE)
+    f0, _ = rt.adjoint(m.items, w, outputs=())
+    g = rt.physics_adjoint(want_dust=True)
+    assert np.all(g["g_v_turb"] == 0.0)
+    gT, gTd = g["g_T"], g["g_T_dust"][0]
+    scale = float(np.sum(np.abs(w[0] * f0[0])))
+    for name, field, grad in (("T_dust", Td, gTd), ("T_gas", T, gT)):
+        cells = np.argsort(np.abs(grad * field).ravel())[::-1][:2]
+        for flat in cells:
+            c = np.unravel_index(flat, field.shape)
+            # as the c4 checks: |g - fd| <= 1e-4 |fd| + floor at a step of 1e-4
+            # relative, again at 1e-5 where the wider step straddles a kink
+            # of the piecewise-smooth forward (sub-grid point counts move
+            # with the width)
+            for rel in (1e-4, 1e-5):
+                h = rel * field[c]
+                Ls = []
+                for sg in (1, -1):
+                    f = field.copy()
+                    f[c] += sg * h
+                    a, b = (T, f) if name == "T_dust" else (f, Td)
+                    rt.set_physics(a, None, log_abund=la0.numpy(), T_dust=b[None],
+                                   q_over=engine.Q_TABLE)
+                    fl, _, _ = rt.trace(m.items)
+                    Ls.append(float(np.dot(w[0], fl[0])))
+                fd = (Ls[0] - Ls[1]) / (2 * h)
+                ok = abs(grad[c] - fd) <= 1e-4 * abs(fd) + 1e-13 * scale / h
+                if ok:
+                    break
+            say(f"    d/d{name} cell {c}: adjoint {grad[c]:.6e}, central difference {fd:.6e}, "
+                f"rel {abs(grad[c] - fd) / abs(fd):.1e} (step {rel:g})")
+            assert ok
+    rt.set_turbulence_alpha(-1.0)
+    rt.set_partition_table(None, None)
+    m.close()
+
+
+def test_kepler_velocity_on_device():
+    """rl_set_velocity_kepler against the same field formed on the host
+    (diff.kepler_velocity) and uploaded: the FP64 spectra agree; the
+    trajectories are rebuilt after the velocity changes."""
+    say("[gpu] Keplerian velocity on the device")
+    ri, w = synth.workload_inputs("c4", seed=0)
+    rt = engine.Raytracer(0, "fp64")
+    rt.load_run(ri)
+    rt.setup_rays([ri.incl_deg * 0.0174532925199], ri.nphi, ri.b_extra, ri.dbdr)
+    nfr, vmax = engine.passband(ri.vmax, ri.dv)
+    item = [(0, 0, nfr, vmax, engine.SKIP_NONE)]
+    mstar = 2.3 * 1.988409870698051e33
+    v = diff.kepler_velocity(ri.r, ri.rho_gas.shape[1], mstar)
+    rt.set_medium(ri.rho_gas, ri.abund, v, ri.lw_kms, ri.umass_av, float(ri.lw_kms.max()))
+    f_host, _, _ = rt.trace(item)
+    rt.set_medium(ri.rho_gas, ri.abund, ri.vel, ri.lw_kms, ri.umass_av, float(ri.lw_kms.max()))
+    f_run, _, _ = rt.trace(item)
+    rt.set_velocity_kepler(mstar)
+    f_dev, _, _ = rt.trace(item)
+    rt.close()
+    e = _rel(f_dev[0], f_host[0])
+    d = _rel(f_run[0], f_host[0])
+    say(f"    device vs host Kepler field: {e:.2e} (the run's own field differs by {d:.2e})")
+    assert e <= 1e-13 and d > 1e-6
