@@ -978,11 +978,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_formal_batch(RL_FORMAL_BATCH_PARAMS) {
     formal_batch_body<T>(RL_FORMAL_BATCH_ARGS);
 }
-// FP32 (split records): unconstrained, about 80 VGPRs (six waves per SIMD);
-// the next segment's records are loaded one segment ahead, which needs the
-// registers more than a seventh or eighth wave would
+// FP32 (split records): held to 64 VGPRs, eight waves per SIMD (79
+// unconstrained, six waves: 61.4 ms on the 400x200 disk's two
+// inclinations; 72, seven waves: 54.1 ms; 64: 52.3 ms).  What spills is
+// per-tile state and the sub-grid path's, outside the fast segment loop.
 template <>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 k_formal_batch<float>(RL_FORMAL_BATCH_PARAMS) {
     formal_batch_body<float>(RL_FORMAL_BATCH_ARGS);
 }

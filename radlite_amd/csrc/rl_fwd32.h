@@ -198,16 +198,7 @@ __device__ inline float trace_split(const SegHot *__restrict__ hot, const SegAux
     // segment 1 starts the ray: the general path reads its start point
     I = seg_general32(I, hot[1], aux[0], aux[1], line[0], line[1], uch, istar, sf, init, c, ovf,
                       maser);
-    // the records of the next segment are requested before the current
-    // segment's arithmetic, so that their latency overlaps it
-    SegHot hn = hot[n > 2 ? 2 : 1];
-    NodeLine qn = line[n > 2 ? 2 : 1];
-    for (int k = 2; k < n; k++) {
-        const SegHot h = hn;
-        const NodeLine q = qn;
-        const int kn = k + 1 < n ? k + 1 : k;
-        hn = hot[kn];
-        qn = line[kn];
+    auto segment = [&](int k, const SegHot &h, const NodeLine &q) {
         if (seg_general(h.hds)) {
             I = seg_general32(I, h, aux[k - 1], aux[k], line[k - 1], q, uch, istar, sf, init, c,
                               ovf, maser);
@@ -219,6 +210,31 @@ __device__ inline float trace_split(const SegHot *__restrict__ hot, const SegAux
             // maser flag; the line coefficient is negative only where the
             // populations are inverted
             if (q.dcl < 0.0f && fabsf(q.sig) * (dcn * e) * (2.0f * h.hds) < -0.01f) maser = 1;
+        }
+    };
+    // Segments 2.. in pairs, the records in two register sets (A: segments
+    // k, k+2, ..; B: k+1, k+3, ..): a set is reloaded right after its
+    // segment is done, so the load overlaps the other set's segment and no
+    // copies rotate the registers.
+    if (n > 2) {
+        int k = 2;
+        SegHot hA = hot[k];
+        NodeLine qA = line[k];
+        const int k1 = k + 1 < n ? k + 1 : k;
+        SegHot hB = hot[k1];
+        NodeLine qB = line[k1];
+        for (;;) {
+            segment(k, hA, qA);
+            if (k + 1 >= n) break;
+            const int ka = k + 2 < n ? k + 2 : n - 1;
+            hA = hot[ka];
+            qA = line[ka];
+            segment(k + 1, hB, qB);
+            if (k + 2 >= n) break;
+            const int kb = k + 3 < n ? k + 3 : n - 1;
+            hB = hot[kb];
+            qB = line[kb];
+            k += 2;
         }
     }
     ovs += ovf ? 1 : 0;
