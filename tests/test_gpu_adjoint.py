@@ -403,4 +403,5 @@ def test_c4_chi2_gradients_fd_and_fp32():
         big = np.abs(b) > 1e-2 * np.abs(b).max()
         cell = (np.abs(a - b)[big] / np.abs(b)[big]).max()
         say(f"    fp32 vs fp64 adjoint d/d{key}: norm rel {rel:.2e}, max over large cells {cell:.2e}")
-        assert rel <= 1e-2 and cell <= 1e-2, key
+        # measured: d/dT 2.5e-8 / 4.5e-7, d/dla 4.0e-6 / 7.3e-5, d/dvt 3.6e-6 / 2.9e-4
+        assert rel <= 1e-3 and cell <= 1e-3, key
