@@ -661,20 +661,29 @@ static size_t held_bytes(const RaySet &rs) {
 static size_t resident_traj(const rl_ctx *c) {
     size_t b = 0;
     for (const RaySet &r : c->incl) b += held_bytes(r);
-    for (const DevBuf &d : c->spare) b += d.n;
     return b;
 }
 
-// the node buffers of the current ray sets to the spare pool (before a
-// setup call replaces the sets)
+// Spare node buffers: those of ray sets that a setup call replaced or that
+// were evicted to make room, kept for the next builds of the same sizes
+// (allocating and freeing tens of GB costs seconds).  They live only
+// between an eviction (or setup) and the builds that follow it: a build
+// that finds none of a fitting size frees them all before it allocates,
+// and release_cross_scratch frees what no build took.
 static void keep_spares(rl_ctx *c) {
     for (RaySet &r : c->incl)
         for (DevBuf *d : {&r.d_nodes, &r.d_hot, &r.d_aux})
             if (d->p) c->spare.push_back(std::move(*d));
 }
 
+static void free_spares(rl_ctx *c) {
+    if (c->spare.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    c->spare.clear();
+}
+
 // b holds at least `bytes`: a spare buffer if one fits (the smallest of at
-// most twice the size), else a new allocation
+// most twice the size), else a new allocation (the spares freed first)
 static hipError_t ensure_spare(rl_ctx *c, DevBuf &b, size_t bytes) {
     if (b.p && b.n >= bytes) return hipSuccess;
     int best = -1;
@@ -682,7 +691,10 @@ static hipError_t ensure_spare(rl_ctx *c, DevBuf &b, size_t bytes) {
         const size_t n = c->spare[i].n;
         if (n >= bytes && n <= 2 * bytes + 4096 && (best < 0 || n < c->spare[best].n)) best = i;
     }
-    if (best < 0) return b.ensure(bytes);
+    if (best < 0) {
+        free_spares(c);
+        return b.ensure(bytes);
+    }
     b = std::move(c->spare[best]);
     c->spare.erase(c->spare.begin() + best);
     return hipSuccess;
@@ -723,10 +735,9 @@ static void release_adjoint_work(rl_ctx *c) {
     }
 }
 
-static void evict_traj(RaySet &rs) {
-    rs.d_nodes = DevBuf();
-    rs.d_hot = DevBuf();
-    rs.d_aux = DevBuf();
+static void evict_traj(rl_ctx *c, RaySet &rs) {
+    for (DevBuf *d : {&rs.d_nodes, &rs.d_hot, &rs.d_aux})
+        if (d->p) c->spare.push_back(std::move(*d));
     rs.seg_ready = false;
     rs.d_st_vals = DevBuf();
     rs.d_st_w = DevBuf();
@@ -739,10 +750,6 @@ static void evict_traj(RaySet &rs) {
 // the resident trajectories plus need fit the budget (or none is left)
 static int make_room(rl_ctx *c, size_t need, const std::vector<int> &keep) {
     const size_t budget = ctx_budget(c);
-    if (resident_traj(c) + need > budget && !c->spare.empty()) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        c->spare.clear();
-    }
     for (;;) {
         if (resident_traj(c) + need <= budget) return 0;
         int victim = -1;
@@ -753,7 +760,7 @@ static int make_room(rl_ctx *c, size_t need, const std::vector<int> &keep) {
         }
         if (victim < 0) return 0;
         HIPCHK(c, hipStreamSynchronize(c->stream));   // no launch may still read it
-        evict_traj(c->incl[victim]);
+        evict_traj(c, c->incl[victim]);
     }
 }
 
@@ -829,12 +836,14 @@ static int count_traj(rl_ctx *c, int idx) {
     return 0;
 }
 
-// the scratch of k_cross is large for big ray sets: not kept past a build
-// when it holds more than a tenth of the budget
+// After the builds of a call: the spares no build took are freed; the
+// scratch of k_cross (25 GB on the 1024x512 disk) is kept for the next
+// build unless it and the resident trajectories exceed the budget.
 static void release_cross_scratch(rl_ctx *c) {
+    free_spares(c);
     size_t b = 0;
     for (size_t k = 0; k < c->x_scr.size(); k++) b += c->x_scr[k].n + c->x_cross[k].n;
-    if (b * 10 > ctx_budget(c)) {
+    if (b + resident_traj(c) > ctx_budget(c)) {
         (void)hipStreamSynchronize(c->stream);
         c->x_scr.clear();
         c->x_cross.clear();
