@@ -1419,9 +1419,12 @@ __device__ __forceinline__ void adj_rev_body(RL_ADJ_REV_PARAMS) {
 }
 
 // waves per block at most: FP64 2 (its registers allow two waves per SIMD),
-// FP32 6 (held to 128 VGPRs, four waves per SIMD)
+// FP32 3.  The block's waves meet at every node batch, each waiting for the
+// slowest; on the 1024x512 disk's 300 channels (five groups) an adjoint
+// item took 3.97 s with one wave per block (the groups in turn), 3.41 s
+// with two, 3.28 s with three and 4.45 s with five.
 template <typename T> struct AdjRevWaves { static constexpr int max = 2; };
-template <> struct AdjRevWaves<float> { static constexpr int max = 6; };
+template <> struct AdjRevWaves<float> { static constexpr int max = 3; };
 
 template <typename T>
 __global__ void __launch_bounds__(64 * 2) __attribute__((amdgpu_waves_per_eu(2)))
@@ -1430,7 +1433,7 @@ k_adj_rev(RL_ADJ_REV_PARAMS) {
                     ring_w, center_w, d2, ckpt, gnode, kblk, nw);
 }
 template <>
-__global__ void __launch_bounds__(64 * 6) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(64 * 3) __attribute__((amdgpu_waves_per_eu(4)))
 k_adj_rev<float>(int nrays, int nfr, int nphi, int ray0, long long node0,
                  const long long *__restrict__ off, const long long *__restrict__ ckoff,
                  const FastRecT<float> *__restrict__ rec, const double *__restrict__ dnu,
