@@ -555,7 +555,10 @@ def main():
     nodes = stats["nodes"]
     rays = stats["rays"]
     nseg = (nodes - rays) / rays                      # mean segments per ray
-    bytes_per_rc = (28.0 if args.precision == "fp32" else 56.0) * nseg
+    # records read per segment and ray-channel without reuse across channels:
+    # FP32 the split records (SegHot 16 B + NodeLine 16 B), FP64 the 64-byte
+    # NodeRec
+    bytes_per_rc = (32.0 if args.precision == "fp32" else 64.0) * nseg
     nbatch, tile_cw, tile_blocks = rt.batch_info()
     formal_s = float(np.mean(t_formal)) / 1e3     # per step: every batch's formal launch
     # per launch (one inclination): algorithmic bytes / mean launch duration
