@@ -882,7 +882,7 @@ static int build_traj_keep(rl_ctx *c, int idx, const std::vector<int> &keep) {
     rl::launch_geo(c->stream, g, c->d_vel.as<double>(), c->nh, rs.nrays, nmax,
                    rs.d_off.as<long long>(), rs.d_x0.as<double>(), rs.d_z0.as<double>(),
                    rs.costh0, rs.sinth0, cs, ccode, rs.d_nodes.as<GeoNode>(),
-                   c->d_err.as<int>(), seg ? c->d_lw.as<double>() : nullptr,
+                   c->d_err.as<int>(), rs.nnodes, seg ? c->d_lw.as<double>() : nullptr,
                    seg ? rs.d_hot.as<rl::SegHot>() : nullptr,
                    seg ? rs.d_aux.as<rl::SegAux>() : nullptr);
     HIPCHK(c, hipGetLastError());

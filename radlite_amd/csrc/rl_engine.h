@@ -183,7 +183,8 @@ template <typename T> void launch_posvel(hipStream_t st, const PosvelLaunch &a);
 void launch_geo(hipStream_t st, const GridDev &g, const double *vel, int nh, int nrays,
                 int nmax, const long long *off, const double *x0, const double *z0,
                 double costh0, double sinth0, const double *cs, const int *ccode,
-                GeoNode *nodes, int *err, const double *lw = nullptr, SegHot *hot = nullptr,
+                GeoNode *nodes, int *err, long long nnodes = 0, const double *lw = nullptr,
+                SegHot *hot = nullptr,
                 SegAux *aux = nullptr);   // hot != null: the FP32 segment records too
 void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int lev_up,
                   int lev_low, double molpg, const double *rho, const double *abund,

@@ -275,27 +275,43 @@ k_geo(GridView g, CellView cv, int nmax, const long long *__restrict__ off,
                                 err);
 }
 
-// FP32: the nodes and, in the same pass, their segment records.  A block
-// writes 255 nodes of a ray; its thread 0 forms the node before them
-// (not written), whose projected velocity and width the block's first
-// segment needs.
+// FP32: the nodes and, in the same pass, their segment records, on a grid
+// over the ray set's nodes (no block past a ray's last node).  A block
+// writes nodes 255 b .. 255 b + 254 of the CSR; its thread 0 forms the node
+// before them (not written), whose projected velocity and width the
+// block's first segment needs.  A block spans a few rays: thread 0 finds
+// the first one by bisection of the offsets, the others step from it.
 __global__ void __launch_bounds__(256)
-k_geo_seg(GridView g, CellView cv, int nmax, const long long *__restrict__ off,
-          const double *__restrict__ x0a, const double *__restrict__ z0a, double costh0,
-          double sinth0, const double *__restrict__ cs_all, const int *__restrict__ ccode_all,
+k_geo_seg(GridView g, CellView cv, int nmax, int nrays, long long nnodes,
+          const long long *__restrict__ off, const double *__restrict__ x0a,
+          const double *__restrict__ z0a, double costh0, double sinth0,
+          const double *__restrict__ cs_all, const int *__restrict__ ccode_all,
           GeoNode *__restrict__ nodes_all, const double *__restrict__ lw,
           SegHot *__restrict__ hot_all, SegAux *__restrict__ aux_all, int *err) {
     __shared__ double s_u[256], s_lw[256];
-    const int ray = blockIdx.x;
+    __shared__ int s_ray0;
     const int t = threadIdx.x;
-    const long long o = off[ray];
-    const int n = (int)(off[ray + 1] - o);
-    if ((int)blockIdx.y * 255 >= n) return;               // uniform over the block
-    const int k = (int)blockIdx.y * 255 + t - 1;
-    const bool valid = k >= 0 && k < n;
+    const long long base = (long long)blockIdx.x * 255 - 1;       // node of thread 0
+    if (t == 0) {
+        // the ray of node max(base, 0): the last r with off[r] <= it
+        const long long g0 = base < 0 ? 0 : base;
+        int lo = 0, hi = nrays;
+        while (hi - lo > 1) {
+            const int m = (lo + hi) >> 1;
+            if (off[m] <= g0) lo = m; else hi = m;
+        }
+        s_ray0 = lo;
+    }
+    __syncthreads();
+    const long long gnode = base + t;
+    const bool valid = gnode >= 0 && gnode < nnodes;
     GeoNode gn;
     double lwk = 0.0;
+    int k = 0;
     if (valid) {
+        int ray = s_ray0;
+        while (off[ray + 1] <= gnode) ray++;
+        k = (int)(gnode - off[ray]);
         gn = geo_node(g, cv, k, x0a[ray], z0a[ray], costh0, sinth0,
                       cs_all + (long long)ray * nmax, ccode_all + (long long)ray * nmax, err);
         lwk = interp_field(gn, lw);
@@ -304,8 +320,8 @@ k_geo_seg(GridView g, CellView cv, int nmax, const long long *__restrict__ off,
     }
     __syncthreads();
     if (!valid || t == 0) return;
-    nodes_all[o + k] = gn;
-    seg_record(gn, k == 0, s_u[t - 1], s_lw[t - 1], lwk, hot_all[o + k], aux_all[o + k]);
+    nodes_all[gnode] = gn;
+    seg_record(gn, k == 0, s_u[t - 1], s_lw[t - 1], lwk, hot_all[gnode], aux_all[gnode]);
 }
 
 // ---------------------------------------------------------------------
@@ -831,13 +847,14 @@ void launch_cross(hipStream_t st, const GridDev &gd, const double *tan2, const C
 void launch_geo(hipStream_t st, const GridDev &gd, const double *vel, int nh, int nrays,
                 int nmax, const long long *off, const double *x0, const double *z0,
                 double costh0, double sinth0, const double *cs, const int *ccode,
-                GeoNode *nodes, int *err, const double *lw, SegHot *hot, SegAux *aux) {
+                GeoNode *nodes, int *err, long long nnodes, const double *lw, SegHot *hot,
+                SegAux *aux) {
     GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar};
     CellView cv{nh, vel};
     if (hot)
-        hipLaunchKernelGGL(k_geo_seg, dim3(nrays, (nmax + 254) / 255), dim3(256), 0, st, g, cv,
-                           nmax, off, x0, z0, costh0, sinth0, cs, ccode, nodes, lw, hot, aux,
-                           err);
+        hipLaunchKernelGGL(k_geo_seg, dim3((unsigned)((nnodes + 1 + 254) / 255)), dim3(256), 0,
+                           st, g, cv, nmax, nrays, nnodes, off, x0, z0, costh0, sinth0, cs,
+                           ccode, nodes, lw, hot, aux, err);
     else
         hipLaunchKernelGGL(k_geo, dim3(nrays, (nmax + 255) / 256), dim3(256), 0, st, g, cv, nmax,
                            off, x0, z0, costh0, sinth0, cs, ccode, nodes, err);
