@@ -787,7 +787,7 @@ static int cross_sets(rl_ctx *c, const std::vector<int> &idxs) {
         const int n = rs.nrays;
         c->x_owner[k] = -1;
         HIPCHK(c, c->x_scr[k].ensure((size_t)(iyeq + 1) * n * 20));
-        HIPCHK(c, c->x_cross[k].ensure((size_t)n * nmax * 12));
+        HIPCHK(c, c->x_cross[k].ensure((size_t)n * nmax * sizeof(rl::CrossRec)));
         HIPCHK(c, cnt[k].ensure((size_t)n * 4));
         rl::CrossSet &q = sets.s[k];
         q.nrays = n;
@@ -801,8 +801,7 @@ static int cross_sets(rl_ctx *c, const std::vector<int> &idxs) {
         q.s2 = q.s1 + (size_t)(iyeq + 1) * n;
         q.iy = reinterpret_cast<int *>(q.s2 + (size_t)(iyeq + 1) * n);
         q.count = cnt[k].as<int>();
-        q.cs = c->x_cross[k].as<double>();
-        q.ccode = reinterpret_cast<int *>(q.cs + (size_t)n * nmax);
+        q.cr = c->x_cross[k].as<rl::CrossRec>();
     }
     HIPCHK(c, hipMemsetAsync(c->d_err.p, 0, 4, c->stream));
     rl::launch_cross(c->stream, g, c->d_tan2.as<double>(), sets, nmax, c->d_err.as<int>());
@@ -870,8 +869,7 @@ static int build_traj_keep(rl_ctx *c, int idx, const std::vector<int> &keep) {
     }
     rl::GridDev g{c->nr, 2 * c->nh, c->d_rc.as<double>(), c->d_tc.as<double>(), c->rstar};
     const int nmax = traj_nmax(c);
-    double *cs = c->x_cross[slot].as<double>();
-    int *ccode = reinterpret_cast<int *>(cs + (size_t)rs.nrays * nmax);
+    const rl::CrossRec *cr = c->x_cross[slot].as<rl::CrossRec>();
     HIPCHK(c, ensure_spare(c, rs.d_nodes, (size_t)rs.nnodes * sizeof(GeoNode)));
     // FP32 circular sets: the segment records in the same pass
     const bool seg = c->precision == RL_FP32 && !rs.rect && c->d_lw.n;
@@ -881,7 +879,7 @@ static int build_traj_keep(rl_ctx *c, int idx, const std::vector<int> &keep) {
     }
     rl::launch_geo(c->stream, g, c->d_vel.as<double>(), c->nh, rs.nrays, nmax,
                    rs.d_off.as<long long>(), rs.d_x0.as<double>(), rs.d_z0.as<double>(),
-                   rs.costh0, rs.sinth0, cs, ccode, rs.d_nodes.as<GeoNode>(),
+                   rs.costh0, rs.sinth0, cr, rs.d_nodes.as<GeoNode>(),
                    c->d_err.as<int>(), rs.nnodes, seg ? c->d_lw.as<double>() : nullptr,
                    seg ? rs.d_hot.as<rl::SegHot>() : nullptr,
                    seg ? rs.d_aux.as<rl::SegAux>() : nullptr);

@@ -135,6 +135,14 @@ struct AdjLaunch {
 
 // One ray set's crossing pass: rays, direction, and where its crossings go
 // (per-ray slots of nmax entries) and its scratch.
+// a crossing of the crossing pass in its ray's slot: s and the packed code
+// (icross | rext << 2 | node flags << 3 | index << 5), one 16-byte record so
+// that a ray's successive crossings fill whole cache lines
+struct CrossRec {
+    double s;
+    int code, pad;
+};
+
 struct CrossSet {
     int nrays, central_blend;
     const double *x0, *z0;
@@ -144,8 +152,7 @@ struct CrossSet {
     double *s1, *s2;
     int *iy;
     int *count;
-    double *cs;
-    int *ccode;
+    CrossRec *cr;              // [ray][nmax]
 };
 constexpr int CROSS_SETS_MAX = 8;
 struct CrossSets {
@@ -182,7 +189,7 @@ struct PosvelLaunch {
 template <typename T> void launch_posvel(hipStream_t st, const PosvelLaunch &a);
 void launch_geo(hipStream_t st, const GridDev &g, const double *vel, int nh, int nrays,
                 int nmax, const long long *off, const double *x0, const double *z0,
-                double costh0, double sinth0, const double *cs, const int *ccode,
+                double costh0, double sinth0, const CrossRec *cr,
                 GeoNode *nodes, int *err, long long nnodes = 0, const double *lw = nullptr,
                 SegHot *hot = nullptr,
                 SegAux *aux = nullptr);   // hot != null: the FP32 segment records too

@@ -75,8 +75,7 @@ k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, i
     double *scr_s1 = q.s1, *scr_s2 = q.s2;
     int *scr_iy = q.iy;
     int *__restrict__ count = q.count;
-    double *__restrict__ cs = q.cs;
-    int *__restrict__ ccode = q.ccode;
+    CrossRec *__restrict__ cr = q.cr;
     int ray = blockIdx.x * blockDim.x + threadIdx.x;
     if (ray >= nrays) return;
     if (q.bpix && !(q.bpix[ray] < q.bmax)) {     // telescope.F:2118: outside, not traced
@@ -100,8 +99,7 @@ k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, i
                 star_done = 1;
             }
         }
-        cs[base + k] = c.s;
-        ccode[base + k] = c.icross | (c.rext << 2) | (flag << 3) | (c.idx << 5);
+        cr[base + k] = CrossRec{c.s, c.icross | (c.rext << 2) | (flag << 3) | (c.idx << 5), 0};
         icr_prev = c.icross;
         idx_prev = c.idx;
     };
@@ -118,13 +116,13 @@ k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, i
 // compacted from the per-ray slots into the CSR node array.
 __device__ __forceinline__ GeoNode geo_node(GridView g, CellView cv, int k, double x0, double z0,
                                             double costh0, double sinth0,
-                                            const double *__restrict__ cs,
-                                            const int *__restrict__ ccode, int *err) {
+                                            const CrossRec *__restrict__ cr, int *err) {
     const double pi = PICONST, epsplus = 1.0e1 * TELESC_EPS;
     const int nr = g.nr, ny = g.ny;
     const double sinth02 = sinth0 * sinth0, costh02 = costh0 * costh0;
-    const double s = cs[k];
-    const int code = ccode[k];
+    const CrossRec ck = cr[k];
+    const double s = ck.s;
+    const int code = ck.code;
     const int icross = code & 3, rext = (code >> 2) & 1, flag = (code >> 3) & 3, idx = code >> 5;
     auto thr = [&](double q) { return sqrt(x0 * x0 + z0 * z0 + q * q + 2.0 * z0 * costh0 * q); };
     auto theta_of_s = [&](double q) {
@@ -202,7 +200,7 @@ __device__ __forceinline__ GeoNode geo_node(GridView g, CellView cv, int k, doub
     if (k == 0) {
         o.ds = 0.0;
     } else {
-        o.ds = s - cs[k - 1];
+        o.ds = s - cr[k - 1].s;
         if (o.ds < 0.0) set_err(err, E_DSNEG);
     }
     o.flag = icross | (flag << 2);
@@ -264,15 +262,14 @@ __device__ inline void seg_record(const GeoNode &gn, bool first, double up, doub
 __global__ void __launch_bounds__(256)
 k_geo(GridView g, CellView cv, int nmax, const long long *__restrict__ off,
       const double *__restrict__ x0a, const double *__restrict__ z0a, double costh0,
-      double sinth0, const double *__restrict__ cs_all, const int *__restrict__ ccode_all,
-      GeoNode *__restrict__ nodes_all, int *err) {
+      double sinth0, const CrossRec *__restrict__ cr_all, GeoNode *__restrict__ nodes_all,
+      int *err) {
     const int ray = blockIdx.x;
     const int k = blockIdx.y * blockDim.x + threadIdx.x;
     const long long o = off[ray];
     if (k >= (int)(off[ray + 1] - o)) return;
     nodes_all[o + k] = geo_node(g, cv, k, x0a[ray], z0a[ray], costh0, sinth0,
-                                cs_all + (long long)ray * nmax, ccode_all + (long long)ray * nmax,
-                                err);
+                                cr_all + (long long)ray * nmax, err);
 }
 
 // FP32: the nodes and, in the same pass, their segment records, on a grid
@@ -285,8 +282,8 @@ __global__ void __launch_bounds__(256)
 k_geo_seg(GridView g, CellView cv, int nmax, int nrays, long long nnodes,
           const long long *__restrict__ off, const double *__restrict__ x0a,
           const double *__restrict__ z0a, double costh0, double sinth0,
-          const double *__restrict__ cs_all, const int *__restrict__ ccode_all,
-          GeoNode *__restrict__ nodes_all, const double *__restrict__ lw,
+          const CrossRec *__restrict__ cr_all, GeoNode *__restrict__ nodes_all,
+          const double *__restrict__ lw,
           SegHot *__restrict__ hot_all, SegAux *__restrict__ aux_all, int *err) {
     __shared__ double s_u[256], s_lw[256];
     __shared__ int s_ray0;
@@ -313,7 +310,7 @@ k_geo_seg(GridView g, CellView cv, int nmax, int nrays, long long nnodes,
         while (off[ray + 1] <= gnode) ray++;
         k = (int)(gnode - off[ray]);
         gn = geo_node(g, cv, k, x0a[ray], z0a[ray], costh0, sinth0,
-                      cs_all + (long long)ray * nmax, ccode_all + (long long)ray * nmax, err);
+                      cr_all + (long long)ray * nmax, err);
         lwk = interp_field(gn, lw);
         s_u[t] = gn.u;
         s_lw[t] = lwk;
@@ -846,18 +843,18 @@ void launch_cross(hipStream_t st, const GridDev &gd, const double *tan2, const C
 
 void launch_geo(hipStream_t st, const GridDev &gd, const double *vel, int nh, int nrays,
                 int nmax, const long long *off, const double *x0, const double *z0,
-                double costh0, double sinth0, const double *cs, const int *ccode,
+                double costh0, double sinth0, const CrossRec *cr,
                 GeoNode *nodes, int *err, long long nnodes, const double *lw, SegHot *hot,
                 SegAux *aux) {
     GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar};
     CellView cv{nh, vel};
     if (hot)
         hipLaunchKernelGGL(k_geo_seg, dim3((unsigned)((nnodes + 1 + 254) / 255)), dim3(256), 0,
-                           st, g, cv, nmax, nrays, nnodes, off, x0, z0, costh0, sinth0, cs,
-                           ccode, nodes, lw, hot, aux, err);
+                           st, g, cv, nmax, nrays, nnodes, off, x0, z0, costh0, sinth0, cr,
+                           nodes, lw, hot, aux, err);
     else
         hipLaunchKernelGGL(k_geo, dim3(nrays, (nmax + 255) / 256), dim3(256), 0, st, g, cv, nmax,
-                           off, x0, z0, costh0, sinth0, cs, ccode, nodes, err);
+                           off, x0, z0, costh0, sinth0, cr, nodes, err);
 }
 
 void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int lev_up,
