@@ -836,7 +836,7 @@ static int count_traj(rl_ctx *c, int idx) {
 }
 
 // After the builds of a call: the spares no build took are freed; the
-// scratch of k_cross (25 GB on the 1024x512 disk) is kept for the next
+// scratch of k_cross (32 GB on the 1024x512 disk) is kept for the next
 // build unless it and the resident trajectories exceed the budget.
 static void release_cross_scratch(rl_ctx *c) {
     free_spares(c);
