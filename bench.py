@@ -501,8 +501,8 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding of (line, inclination) items is described in DESIGN.md)",
+        print(f"bench.py: the raytracer runs on one GPU; --gpus {args.gpus} is not supported "
+              "(several GPUs would each take a share of the (line, inclination) items)",
               file=sys.stderr)
         sys.exit(2)
 
