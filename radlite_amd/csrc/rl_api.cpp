@@ -62,7 +62,9 @@ struct RaySet {
     double theta0 = 0.0, costh0 = 1.0, sinth0 = 0.0;
     int nrays = 0, nrings = 0, nphi = 0;
     std::vector<double> x0, z0, ring_r, ring_ri;
-    DevBuf d_x0, d_z0;
+    std::vector<double> raytab;       // circular: ring radii and directions (k_rays_circular)
+    std::vector<double> ring_w;       // flux weights of the rings (uploaded from here)
+    DevBuf d_x0, d_z0, d_raytab;
     // trajectory CSR (built lazily, shared by every line of this inclination;
     // the nodes may be evicted to make room and are then built again)
     bool counted = false;             // off / nnodes known
@@ -461,18 +463,13 @@ static int build_rayset(rl_ctx *c, RaySet &rs, double anginf, int nphi, int nrex
     rs.nrings = nrings;
     rs.nphi = nphi;
     rs.nrays = 1 + nrings * nphi;
-    rs.x0.assign(rs.nrays, 0.0);
-    rs.z0.assign(rs.nrays, 0.0);
+    rs.x0.clear();
+    rs.z0.clear();
     rs.ring_r.assign(nrings + 1, 0.0);
     rs.ring_ri.assign(nrings + 2, 0.0);
-    int ir = 1, irad = 1;
-    auto ring = [&](double r_c) {
-        rs.ring_r[irad++] = r_c;
-        for (int k = 0; k < nphi; k++, ir++) {
-            rs.x0[ir] = r_c * xh[k];
-            rs.z0[ir] = r_c * zh[k];
-        }
-    };
+    // the rays of ring ir are r_ir (xh_k, zh_k), formed on the device
+    int irad = 1;
+    auto ring = [&](double r_c) { rs.ring_r[irad++] = r_c; };
     double ref1 = 0.5, ref2 = 0.0, rstar = c->rstar;
     for (int ix = 1; ix <= nrrextra + nrref; ix++) {
         if (ix <= nrrextra) {
@@ -506,9 +503,17 @@ static int build_rayset(rl_ctx *c, RaySet &rs, double anginf, int nphi, int nrex
     for (int k = 1; k <= nrings; k++)
         w[k - 1] = 3.14159265359 * (ri[k + 1] * ri[k + 1] - ri[k] * ri[k]);
     rs.center_w = 3.14159265359 * (ri[1] * ri[1]);
-    if (int e = upload(c, rs.d_x0, rs.x0.data(), rs.x0.size() * 8)) return e;
-    if (int e = upload(c, rs.d_z0, rs.z0.data(), rs.z0.size() * 8)) return e;
-    if (int e = upload(c, rs.d_ringw, w.data(), w.size() * 8)) return e;
+    rs.raytab.assign(rs.ring_r.begin(), rs.ring_r.end());
+    rs.raytab.insert(rs.raytab.end(), xh.begin(), xh.end());
+    rs.raytab.insert(rs.raytab.end(), zh.begin(), zh.end());
+    if (int e = upload(c, rs.d_raytab, rs.raytab.data(), rs.raytab.size() * 8)) return e;
+    HIPCHK(c, rs.d_x0.ensure((size_t)rs.nrays * 8));
+    HIPCHK(c, rs.d_z0.ensure((size_t)rs.nrays * 8));
+    rl::launch_rays_circular(c->stream, nrings, nphi, rs.d_raytab.as<double>(),
+                             rs.d_x0.as<double>(), rs.d_z0.as<double>());
+    HIPCHK(c, hipGetLastError());
+    rs.ring_w = w;
+    if (int e = upload(c, rs.d_ringw, rs.ring_w.data(), w.size() * 8)) return e;
     rs.traj_ready = rs.counted = false;
     return 0;
 }

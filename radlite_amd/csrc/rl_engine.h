@@ -92,6 +92,8 @@ void launch_batch_formal(hipStream_t st, int fp64, const BatchLaunch &b, int cw,
 void launch_batch_finish(hipStream_t st, int fp64, const BatchLaunch &b);
 void launch_batch_flux(hipStream_t st, const BatchLaunch &b);
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
+void launch_rays_circular(hipStream_t st, int nrings, int nphi, const double *tab, double *x0,
+                          double *z0);
 int formal_batch_blocks_per_cu(int fp64);
 
 // One item of the adjoint (scaled records of precision T, all channels).

@@ -1168,6 +1168,35 @@ void launch_batch_flux(hipStream_t st, const BatchLaunch &b) {
                        b.flux);
 }
 
+// The ray positions of a circular image (setup_rays_circular,
+// telescope.F:913-1075): ray 0 the central one at the origin, then ring
+// ir = 1..nrings, direction k = 0..nphi-1 at x0 = r_ir xh_k, z0 = r_ir zh_k.
+// tab: the ring radii [nrings + 1] (entry 0 unused), xh [nphi], zh [nphi],
+// formed on the host (their trigonometry needs the host's libm for the
+// reference's bits; each product is one IEEE multiply on either side).
+__global__ void k_rays_circular(int nrings, int nphi, const double *__restrict__ tab,
+                                double *__restrict__ x0, double *__restrict__ z0) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 1 + (long long)nrings * nphi) return;
+    if (i == 0) {
+        x0[0] = 0.0;
+        z0[0] = 0.0;
+        return;
+    }
+    const int ir = (int)((i - 1) / nphi) + 1;
+    const int k = (int)(i - 1 - (long long)(ir - 1) * nphi);
+    const double r = tab[ir];
+    x0[i] = r * tab[nrings + 1 + k];
+    z0[i] = r * tab[nrings + 1 + nphi + k];
+}
+
+void launch_rays_circular(hipStream_t st, int nrings, int nphi, const double *tab, double *x0,
+                          double *z0) {
+    const long long n = 1 + (long long)nrings * nphi;
+    hipLaunchKernelGGL(k_rays_circular, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       nrings, nphi, tab, x0, z0);
+}
+
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n) {
     if (n > 0) hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
 }
