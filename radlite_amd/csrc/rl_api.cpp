@@ -103,6 +103,7 @@ struct rl_ctx {
     // grid
     int nr = 0, nh = 0;
     std::vector<double> rc, tc;
+    int t_sorted = 0;                      // the mirrored theta grid is non-decreasing
     DevBuf d_rc, d_tc, d_tan2;
     // medium
     bool have_medium = false;
@@ -301,6 +302,9 @@ extern "C" int rl_set_grid(rl_ctx *c, int nr, const double *r, int nh, const dou
     T(-1) = -T(2);
     T(ny + 1) = (double)twopi_sp - T(ny);
     T(ny + 2) = (double)twopi_sp - T(ny - 1);
+    c->t_sorted = 1;
+    for (int i = 1; i < ny; i++)
+        if (!(T(i + 1) >= T(i))) c->t_sorted = 0;
     if (int e = upload(c, c->d_rc, c->rc.data(), c->rc.size() * 8)) return e;
     if (int e = upload(c, c->d_tc, c->tc.data(), c->tc.size() * 8)) return e;
     // tan^2 of the cone angles (telescope.F:2973), ray independent
@@ -637,8 +641,12 @@ extern "C" int rl_setup_rays_rectang(rl_ctx *c, int nincl, const double *incl, i
 // trace
 // ---------------------------------------------------------------------
 // upper bound of the nodes of one ray: every R crossing, every cone
-// crossing and every extra point (telescope.F:3279, 3196, RAYEXPT)
-static int traj_nmax(const rl_ctx *c) { return 2 * (c->nr + 1) + 2 * c->nh + 40; }
+// crossing and every extra point (telescope.F:3279, 3196, RAYEXPT),
+// rounded up to 8 records so that each ray's crossing slot starts on a
+// 128-byte line (the crossing pass writes whole lines)
+static int traj_nmax(const rl_ctx *c) {
+    return (2 * (c->nr + 1) + 2 * c->nh + 40 + 7) / 8 * 8;
+}
 
 static int check_err(rl_ctx *c) {
     int e = 0;
@@ -776,7 +784,8 @@ static int make_room(rl_ctx *c, size_t need, const std::vector<int> &keep) {
 static int cross_sets(rl_ctx *c, const std::vector<int> &idxs) {
     const int ns = (int)idxs.size();
     if (ns < 1 || ns > rl::CROSS_SETS_MAX) return fail(c, RL_E_ARG, "crossing: %d ray sets", ns);
-    rl::GridDev g{c->nr, 2 * c->nh, c->d_rc.as<double>(), c->d_tc.as<double>(), c->rstar};
+    rl::GridDev g{c->nr, 2 * c->nh, c->d_rc.as<double>(), c->d_tc.as<double>(), c->rstar,
+                  c->t_sorted};
     const int iyeq = c->nh, nmax = traj_nmax(c);
     if ((int)c->x_scr.size() < ns) {
         c->x_scr.resize(ns);
@@ -872,7 +881,8 @@ static int build_traj_keep(rl_ctx *c, int idx, const std::vector<int> &keep) {
         if (int e = cross_traj(c, idx)) return e;
         slot = 0;
     }
-    rl::GridDev g{c->nr, 2 * c->nh, c->d_rc.as<double>(), c->d_tc.as<double>(), c->rstar};
+    rl::GridDev g{c->nr, 2 * c->nh, c->d_rc.as<double>(), c->d_tc.as<double>(), c->rstar,
+                  c->t_sorted};
     const int nmax = traj_nmax(c);
     const rl::CrossRec *cr = c->x_cross[slot].as<rl::CrossRec>();
     HIPCHK(c, ensure_spare(c, rs.d_nodes, (size_t)rs.nnodes * sizeof(GeoNode)));

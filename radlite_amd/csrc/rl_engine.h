@@ -11,6 +11,8 @@ struct GridDev {
     int nr, ny;
     const double *rc, *tc;    // ghost-extended, index i+1
     double rstar;
+    int t_sorted;             // tc non-decreasing over 1..ny: the node pass may take its
+                              // theta cell from the crossing pass's guess
 };
 
 // Channel-independent geometry of one trajectory node (FP64).
@@ -137,12 +139,14 @@ struct AdjLaunch {
 
 // One ray set's crossing pass: rays, direction, and where its crossings go
 // (per-ray slots of nmax entries) and its scratch.
-// a crossing of the crossing pass in its ray's slot: s and the packed code
-// (icross | rext << 2 | node flags << 3 | index << 5), one 16-byte record so
-// that a ray's successive crossings fill whole cache lines
+// a crossing of the crossing pass in its ray's slot: s, the packed code
+// (icross | rext << 2 | node flags << 3 | index << 5) and the cell guesses
+// for the node pass: the indices of the last sphere and the last cone the
+// ray crossed before this crossing, each plus one, in the low and high 16
+// bits (0: none); 16 bytes, so that 8 records fill a 128-byte line
 struct CrossRec {
     double s;
-    int code, pad;
+    int code, guess;
 };
 
 struct CrossSet {

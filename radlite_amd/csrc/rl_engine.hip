@@ -50,9 +50,20 @@ struct CellView {
 // Several ray sets (inclinations) go in one launch, blockIdx.y choosing the
 // set: a set has one thread per ray, a long serial chain of FP64 work, so
 // sets launched together keep more waves resident to hide its latency.
-__global__ void __launch_bounds__(128)
+// A thread's records go through LDS and out 4 at a time, 64 aligned bytes
+// of its slot (slots start on 128-byte lines: nmax is a multiple of 8).
+// Written one 16-byte record per crossing, the lines of the ~34 MB of slots
+// being filled at once were evicted part-written: 4.8 times the records'
+// bytes were written, and about as many read back.  Eight records (whole
+// lines) would need 16 KB of LDS per block, and at ten blocks per CU (the
+// two inclinations' 2088 blocks resident at once) the grid tables leave
+// room for 8 KB.
+constexpr int CROSS_BLOCK = 128, CROSS_LINE = 4;
+
+__global__ void __launch_bounds__(CROSS_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
 k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, int *err,
         int grid_in_lds) {
+    __shared__ CrossRec stage[CROSS_LINE][CROSS_BLOCK];
     // the radii, cone angles and tan^2 table (a few KB) staged in LDS: the
     // merge reads them at every step
     extern __shared__ double lds_grid[];
@@ -86,7 +97,13 @@ k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, i
            scr_iy + ray, nrays);
     const long long base = (long long)ray * nmax;
     int icr_prev = -1, idx_prev = -1, star_done = 0;
+    int last_r = 0, last_t = 0;                  // last sphere / cone crossed, plus one
     const bool central = ray == 0 && central_blend;
+    const int tid = threadIdx.x;
+    auto flush = [&](int k0, int m) {            // records k0 .. k0+m-1 from the stage
+        for (int j = 0; j < m; j++)
+            if (k0 + j < nmax) cr[base + k0 + j] = stage[j][tid];
+    };
     auto sink = [&](int k, const Crossing &c) {
         if (k >= nmax) { set_err(err, E_CAP); return; }
         int flag = 0;
@@ -99,13 +116,19 @@ k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, i
                 star_done = 1;
             }
         }
-        cr[base + k] = CrossRec{c.s, c.icross | (c.rext << 2) | (flag << 3) | (c.idx << 5), 0};
+        const int guess = (last_r <= 0xFFFF ? last_r : 0) | ((last_t <= 0xFFFF ? last_t : 0) << 16);
+        stage[k & (CROSS_LINE - 1)][tid] =
+            CrossRec{c.s, c.icross | (c.rext << 2) | (flag << 3) | (c.idx << 5), guess};
+        if ((k & (CROSS_LINE - 1)) == CROSS_LINE - 1) flush(k - (CROSS_LINE - 1), CROSS_LINE);
         icr_prev = c.icross;
         idx_prev = c.idx;
+        if (c.icross == 1) last_r = c.idx + 1;
+        else if (c.icross == 2) last_t = c.idx + 1;
     };
     int n = trajectory(t, sink);
     if (n < 0) { set_err(err, -n); n = 0; }
     if (n > nmax) { set_err(err, E_CAP); n = nmax; }
+    flush(n & ~(CROSS_LINE - 1), n & (CROSS_LINE - 1));
     count[ray] = n;
 }
 
@@ -124,14 +147,24 @@ __device__ __forceinline__ GeoNode geo_node(GridView g, CellView cv, int k, doub
     const double s = ck.s;
     const int code = ck.code;
     const int icross = code & 3, rext = (code >> 2) & 1, flag = (code >> 3) & 3, idx = code >> 5;
+    // the last sphere / cone crossed before this node: the cell is at most
+    // one from it (else the bisection)
+    const int gr = (ck.guess & 0xFFFF) - 1, gt = (ck.guess >> 16) - 1;
     auto thr = [&](double q) { return sqrt(x0 * x0 + z0 * z0 + q * q + 2.0 * z0 * costh0 * q); };
     auto theta_of_s = [&](double q) {
         double th = atan(sqrt(x0 * x0 + sinth02 * q * q) / (z0 + costh0 * q));
         if (th < 0.0) th = th + pi;
         return th;
     };
-    auto hunt_r = [&](double x) { return hunt_bisect([&](int i) { return g.R(i); }, nr, x); };
-    auto hunt_t = [&](double x) { return hunt_bisect([&](int i) { return g.TH(i); }, ny, x); };
+    auto hunt_r = [&](double x) {
+        auto xx = [&](int i) { return g.R(i); };
+        return gr >= 0 && gr <= nr ? hunt_near(xx, nr, x, gr) : hunt_bisect(xx, nr, x);
+    };
+    auto hunt_t = [&](double x) {
+        auto xx = [&](int i) { return g.TH(i); };
+        return g.t_sorted && gt >= 0 && gt <= ny ? hunt_near(xx, ny, x, gt)
+                                                 : hunt_bisect(xx, ny, x);
+    };
     double r, theta;
     int ir, it;
     if (icross == 1) {
@@ -276,8 +309,10 @@ k_geo(GridView g, CellView cv, int nmax, const long long *__restrict__ off,
 // over the ray set's nodes (no block past a ray's last node).  A block
 // writes nodes 255 b .. 255 b + 254 of the CSR; its thread 0 forms the node
 // before them (not written), whose projected velocity and width the
-// block's first segment needs.  A block spans a few rays: thread 0 finds
-// the first one by bisection of the offsets, the others step from it.
+// block's first segment needs.  A block spans a few rays: its first wave
+// finds the first one by a 64-way search of the offsets (four rounds of
+// one load per lane for 10^5 rays, where a bisection is eighteen dependent
+// loads), the other threads step from it.
 __global__ void __launch_bounds__(256)
 k_geo_seg(GridView g, CellView cv, int nmax, int nrays, long long nnodes,
           const long long *__restrict__ off, const double *__restrict__ x0a,
@@ -289,15 +324,20 @@ k_geo_seg(GridView g, CellView cv, int nmax, int nrays, long long nnodes,
     __shared__ int s_ray0;
     const int t = threadIdx.x;
     const long long base = (long long)blockIdx.x * 255 - 1;       // node of thread 0
-    if (t == 0) {
-        // the ray of node max(base, 0): the last r with off[r] <= it
+    if (t < 64) {
+        // the ray of node max(base, 0): the last r with off[r] <= it.  The
+        // offsets do not decrease, so the lanes whose candidate passes are
+        // a prefix; off[lo] <= g0 and the answer is below hi throughout.
         const long long g0 = base < 0 ? 0 : base;
         int lo = 0, hi = nrays;
         while (hi - lo > 1) {
-            const int m = (lo + hi) >> 1;
-            if (off[m] <= g0) lo = m; else hi = m;
+            const int step = (hi - lo + 63) / 64;
+            const int r = lo + t * step;
+            const unsigned long long m = __ballot(r < hi && off[r] <= g0);
+            lo += (63 - __clzll((long long)m)) * step;
+            hi = min(hi, lo + step);
         }
-        s_ray0 = lo;
+        if (t == 0) s_ray0 = lo;
     }
     __syncthreads();
     const long long gnode = base + t;
@@ -832,10 +872,10 @@ template void launch_posvel<float>(hipStream_t, const PosvelLaunch &);
 // ---------------------------------------------------------------------
 void launch_cross(hipStream_t st, const GridDev &gd, const double *tan2, const CrossSets &sets,
                   int nmax, int *err) {
-    GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar};
+    GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar, gd.t_sorted};
     int most = 0;
     for (int i = 0; i < sets.n; i++) most = max(most, sets.s[i].nrays);
-    dim3 blk(128), grd((most + 127) / 128, sets.n);
+    dim3 blk(CROSS_BLOCK), grd((most + CROSS_BLOCK - 1) / CROSS_BLOCK, sets.n);
     const size_t lds = (size_t)(gd.nr + 4 + gd.ny + 4 + gd.ny / 2 + 1) * sizeof(double);
     const int in_lds = lds <= 48 * 1024 ? 1 : 0;
     hipLaunchKernelGGL(k_cross, grd, blk, in_lds ? lds : 0, st, g, tan2, sets, nmax, err, in_lds);
@@ -846,7 +886,7 @@ void launch_geo(hipStream_t st, const GridDev &gd, const double *vel, int nh, in
                 double costh0, double sinth0, const CrossRec *cr,
                 GeoNode *nodes, int *err, long long nnodes, const double *lw, SegHot *hot,
                 SegAux *aux) {
-    GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar};
+    GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar, gd.t_sorted};
     CellView cv{nh, vel};
     if (hot)
         hipLaunchKernelGGL(k_geo_seg, dim3((unsigned)((nnodes + 1 + 254) / 255)), dim3(256), 0,

@@ -22,6 +22,7 @@ struct GridView {
     const double *rc;         // rc[i+1] = R_i, i=-1..nr+2
     const double *tc;         // tc[i+1] = Theta_i, i=-1..ny+2
     double rstar;
+    int t_sorted;             // Theta_1..Theta_ny non-decreasing
     __device__ double R(int i) const { return rc[i + 1]; }
     __device__ double TH(int i) const { return tc[i + 1]; }
 };
