@@ -103,10 +103,10 @@ struct Traj {
     }
     __device__ int IY(int is) const { return iy0 > 0 ? iy0 + is - 1 : iyar[(size_t)is * stride]; }
 
-    // th list entry iss (1-based): ordering of telescope.F:3024-3194
-    __device__ void th_entry(int iss, double &s, int &itheta) const {
-        if (iss > ith_amount) { s = 1.0e30; itheta = 0; return; }
-        int is, root, upq;
+    // th list entry iss (1 <= iss <= ith_amount): its cone is, root (1: the
+    // smaller) and cone number itheta, ordering of telescope.F:3024-3194
+    __device__ void th_index(int iss, int &is, int &root, int &itheta) const {
+        int upq;
         const int n = isnr, d = isdblnr;
         if (branchA) {
             if (iss <= n - d) { is = d + iss; root = 1; upq = 0; }
@@ -119,10 +119,16 @@ struct Traj {
             else if (iss <= n + d) { is = d + 1 + (iss - 2 * d - 1); root = 1; upq = 0; }
             else { is = n - (iss - n - d - 1); root = 2; upq = 1; }
         }
-        s = root == 1 ? S1(is) : S2(is);
-        int iy = IY(is);
+        const int iy = IY(is);
         if (upq) itheta = iup == 1 ? iy : iyend + 1 - iy;
         else itheta = iup == 1 ? iyend + 1 - iy : iy;
+    }
+    // th list entry iss (1-based)
+    __device__ void th_entry(int iss, double &s, int &itheta) const {
+        if (iss > ith_amount) { s = 1.0e30; itheta = 0; return; }
+        int is, root;
+        th_index(iss, is, root, itheta);
+        s = root == 1 ? S1(is) : S2(is);
     }
     __device__ double th_s(int iss) const {
         double s; int it;
@@ -144,6 +150,58 @@ struct Traj {
         double s, r; int ix;
         r_entry(k, s, r, ix);
         return s;
+    }
+    // The merge's next entry: th_entry(k) into (ths, thit) when th, else
+    // r_entry(k) (k >= 1) into (rs, rr, rix), with the same arithmetic.
+    // Both lists' roots are one quadratic, a s^2 + b s + c, with the kind's
+    // coefficients (a sphere: a = 1 and the eps widening of the
+    // discriminant), so a wave whose lanes advance different lists forms one
+    // square root and one quotient per lane, not both kinds' in turn.
+    __device__ void next_entry(bool th, int k, double &ths, int &thit, double &rs, double &rr,
+                               int &rix) const {
+        double a = 1.0, b = 0.0, c = -1.0;       // an exhausted list's stand-in, overridden
+        double r = 0.0, kept = 0.0;
+        bool minus = true, widen = false, use_kept = false;
+        int itheta = 0, ix = 0;
+        const bool valid = th ? k <= ith_amount : k <= ir_amount;
+        if (th) {
+            if (valid) {
+                int is, root;
+                th_index(k, is, root, itheta);
+                if (iy0 > 0) {
+                    const double tanth2 = tan2[iy0 + is - 1];
+                    a = tanth2 * costh02 - sinth02;
+                    b = 2.0 * tanth2 * costh0 * z0;
+                    c = tanth2 * z0 * z0 - x0 * x0;
+                    minus = (root == 1) == (a > 0.0);
+                } else {
+                    kept = root == 1 ? sar1[(size_t)is * stride] : sar2[(size_t)is * stride];
+                    use_kept = true;
+                }
+            }
+        } else if (valid) {
+            const int is = k <= half ? k : ir_amount + 1 - k;
+            ix = g.nr - is + 1;
+            r = ix == 0 ? g.rstar : g.R(ix);
+            b = 2.0 * costh0 * z0;
+            c = x0 * x0 + z0 * z0 - r * r;
+            minus = k <= half;
+            widen = true;
+        }
+        double disc = b * b - 4.0 * a * c;
+        if (widen) disc = disc + TELESC_EPS * b * b;
+        const double sd = sqrt(disc);
+        double s = minus ? (-b - sd) / (2.0 * a) : (-b + sd) / (2.0 * a);
+        if (use_kept) s = kept;
+        if (!valid) s = 1.0e30;
+        if (th) {
+            ths = s;
+            thit = itheta;
+        } else {
+            rs = s;
+            rr = r;
+            rix = ix;
+        }
     }
 };
 
@@ -329,20 +387,17 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
             isex++;
             es = isex <= nex ? ex_s[isex] : 1.0e30;
         }
-        if (ths < rs) {
-            double r = t.thr(ths);
-            if (r <= rmaxt && r >= rmint && (ths - sprev) > eps * r &&
-                ths >= sbeg && ths <= send)
-                emit(2, ths, thit, 0);
-            ist++;
-            t.th_entry(ist, ths, thit);
-        } else {
-            if (rr_ <= rmaxr && rr_ >= rminr && (rs - sprev) > eps * rr_ &&
-                rs >= sbeg && rs <= send)
-                emit(1, rs, rix, 0);
-            isr++;
-            t.r_entry(isr, rs, rr_, rix);
-        }
+        // the nearer of the next cone and the next sphere crossing
+        const bool th = ths < rs;
+        double r = rr_;
+        if (th) r = t.thr(ths);
+        const double s = th ? ths : rs;
+        const bool inside = th ? (r <= rmaxt && r >= rmint) : (r <= rmaxr && r >= rminr);
+        if (inside && (s - sprev) > eps * r && s >= sbeg && s <= send)
+            emit(th ? 2 : 1, s, th ? thit : rix, 0);
+        if (th) ist++;
+        else isr++;
+        t.next_entry(th, th ? ist : isr, ths, thit, rs, rr_, rix);
     }
     return nout;
 }
