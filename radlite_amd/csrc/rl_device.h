@@ -47,22 +47,29 @@ __device__ inline int hunt_bisect(F xx, int n, double x) {
     return jlo;
 }
 
-// hunt_bisect's result from a guess g (0 <= g <= n) that is near it.  With
-// P(j) = (x > xx(j)) == ascnd, P(0) true and P(n+1) false, the bisection
-// keeps P(jlo) and !P(jhi) and returns the jlo of the pair jhi = jlo + 1.
-// On a monotone table (ascending or descending) P is true up to one index
-// and false after it, so that pair is unique: a candidate j with P(j) and
-// !P(j+1) is the bisection's answer.  g-1, g, g+1 are tried (four
-// independent loads), else the bisection runs.
-template <typename F>
-__device__ inline int hunt_near(F xx, int n, double x, int g) {
-    const bool ascnd = xx(n) > xx(1);
-    auto P = [&](int j) { return j < 1 ? true : (j > n ? false : (x > xx(j)) == ascnd); };
-    const bool p0 = P(g - 1), p1 = P(g), p2 = P(g + 1), p3 = P(g + 2);
-    if (p1 && !p2) return g;
-    if (p0 && !p1) return g - 1;
-    if (p2 && !p3) return g + 1;
-    return hunt_bisect(xx, n, x);
+// hunt_bisect over the table xx(i) = tb[i], i = 1..n, from a guess g
+// (0 <= g <= n) near the answer when use.  With P(j) = (x > xx(j)) ==
+// ascnd, P(0) true and P(n+1) false, the bisection keeps P(jlo) and !P(jhi)
+// and returns the jlo of the pair jhi = jlo + 1.  On a monotone table
+// (ascending or descending) P is true up to one index and false after it,
+// so that pair is unique: a candidate j with P(j) and !P(j+1) is the
+// bisection's answer.  g-1, g, g+1 are tried, their loads at clamped
+// indices so that they issue together without branches; else (or without
+// use) the bisection runs.
+__device__ inline int hunt_table(const double *__restrict__ tb, int n, double x, int g,
+                                 bool use) {
+    if (use) {
+        const bool ascnd = tb[n] > tb[1];
+        auto P = [&](int j) {
+            const double v = tb[j < 1 ? 1 : (j > n ? n : j)];
+            return j < 1 || (j <= n && (x > v) == ascnd);
+        };
+        const bool p0 = P(g - 1), p1 = P(g), p2 = P(g + 1), p3 = P(g + 2);
+        if (p1 && !p2) return g;
+        if (p0 && !p1) return g - 1;
+        if (p2 && !p3) return g + 1;
+    }
+    return hunt_bisect([&](int i) { return tb[i]; }, n, x);
 }
 
 // make_index theta re-index (interpol.F:87-100)

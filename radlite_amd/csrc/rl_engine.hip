@@ -147,47 +147,40 @@ __device__ __forceinline__ GeoNode geo_node(GridView g, CellView cv, int k, doub
     const double s = ck.s;
     const int code = ck.code;
     const int icross = code & 3, rext = (code >> 2) & 1, flag = (code >> 3) & 3, idx = code >> 5;
-    // the last sphere / cone crossed before this node: the cell is at most
-    // one from it (else the bisection)
-    const int gr = (ck.guess & 0xFFFF) - 1, gt = (ck.guess >> 16) - 1;
-    auto thr = [&](double q) { return sqrt(x0 * x0 + z0 * z0 + q * q + 2.0 * z0 * costh0 * q); };
-    auto theta_of_s = [&](double q) {
-        double th = atan(sqrt(x0 * x0 + sinth02 * q * q) / (z0 + costh0 * q));
-        if (th < 0.0) th = th + pi;
-        return th;
-    };
-    auto hunt_r = [&](double x) {
-        auto xx = [&](int i) { return g.R(i); };
-        return gr >= 0 && gr <= nr ? hunt_near(xx, nr, x, gr) : hunt_bisect(xx, nr, x);
-    };
-    auto hunt_t = [&](double x) {
-        auto xx = [&](int i) { return g.TH(i); };
-        return g.t_sorted && gt >= 0 && gt <= ny ? hunt_near(xx, ny, x, gt)
-                                                 : hunt_bisect(xx, ny, x);
-    };
-    double r, theta;
-    int ir, it;
-    if (icross == 1) {
-        ir = idx;
-        r = idx == 0 ? g.rstar : g.R(idx);
-        theta = theta_of_s(s);
-        it = hunt_t(theta);
-    } else if (icross == 2) {
-        r = thr(s);
-        ir = r < g.R(1) ? 0 : (r > g.R(nr) ? nr : hunt_r(r));
-        it = idx;
-        theta = g.TH(idx);
-    } else {
-        if (rext) {
-            r = sqrt(x0 * x0 + z0 * z0 * (1.0 - costh02));
-            ir = idx;
-        } else {
-            r = thr(s);
-            ir = hunt_r(r);
-        }
-        theta = theta_of_s(s);
-        it = hunt_t(theta);
+    // The position and cells.  A sphere crossing (icross 1) knows its r and
+    // radial cell, a cone crossing (2) its theta and theta cell; an extra
+    // point (3) knows neither (its r when it is the R-extremum, rext).  The
+    // lanes of a wave hold nodes of every kind, so each quantity is formed
+    // once for the lanes that need it (one square root for r, one atan for
+    // theta, one table search), rather than once per kind in turn.
+    const bool need_t = icross != 2, need_r = icross != 1;
+    double r = 0.0, theta = 0.0;
+    if (icross == 1) r = idx == 0 ? g.rstar : g.R(idx);
+    if (icross == 2) theta = g.TH(idx);
+    if (need_r) {
+        r = sqrt(rext ? x0 * x0 + z0 * z0 * (1.0 - costh02)
+                      : x0 * x0 + z0 * z0 + s * s + 2.0 * z0 * costh0 * s);
     }
+    if (need_t) {
+        theta = atan(sqrt(x0 * x0 + sinth02 * s * s) / (z0 + costh0 * s));
+        if (theta < 0.0) theta = theta + pi;
+    }
+    // the searches (hunt): theta where it is not known, else r (a cone
+    // crossing: below R_1 the search gives 0, above R_nr nr); an extra point
+    // that is not the R-extremum searches r as well.  The crossing pass's
+    // guess, the last sphere / cone crossed before this node, is at most one
+    // cell off; a guess that is not is left to the bisection.
+    const int gr = (ck.guess & 0xFFFF) - 1, gt = (ck.guess >> 16) - 1;
+    int ir = idx, it = idx;
+    {
+        const bool on_t = need_t;
+        const int n = on_t ? ny : nr, gs = on_t ? gt : gr;
+        const bool use = gs >= 0 && gs <= n && (!on_t || g.t_sorted);
+        const int found = hunt_table(on_t ? g.tc + 1 : g.rc + 1, n, on_t ? theta : r, gs, use);
+        if (on_t) it = found;
+        else ir = found;
+    }
+    if (icross == 3 && !rext) ir = hunt_table(g.rc + 1, nr, r, gr, gr >= 0 && gr <= nr);
     GeoNode o;
     if (ir == 0) { set_err(err, E_HOLE); ir = 1; }
     double dr = (r - g.R(ir)) / (g.R(ir + 1) - g.R(ir));
