@@ -210,15 +210,22 @@ __device__ __forceinline__ GeoNode geo_node(GridView g, CellView cv, int k, doub
     const double sx = dummy > 0.0 ? (bnew * bnew * costh0 - znew * snew) / dummy : epsplus;
     const double cx = sqrt(1.0 - sx * sx);
     const double sin_phi = sx, cos_phi = x0 < 0.0 ? cx : -cx;
+    // the velocity at the node: a sphere or cone crossing interpolates
+    // between two cells (in theta, resp. r), an extra point between four
     double v[3];
     const double *V = cv.vel;
+    const bool two = icross == 1 || icross == 2;
+    const int cb = icross == 1 ? o.c01 : o.c10;
+    const double w = icross == 1 ? dt : dr;
     for (int c = 0; c < 3; c++) {
-        double a00 = V[o.c00 * 3 + c], a01 = V[o.c01 * 3 + c];
-        double a10 = V[o.c10 * 3 + c], a11 = V[o.c11 * 3 + c];
-        if (icross == 1) v[c] = (1.0 - dt) * a00 + dt * a01;
-        else if (icross == 2) v[c] = (1.0 - dr) * a00 + dr * a10;
-        else v[c] = (1.0 - dr) * ((1.0 - dt) * a00 + dt * a01) +
-                    dr * ((1.0 - dt) * a10 + dt * a11);
+        const double a00 = V[o.c00 * 3 + c], ab = V[cb * 3 + c];
+        if (two) {
+            v[c] = (1.0 - w) * a00 + w * ab;
+        } else {
+            const double a01 = V[o.c01 * 3 + c], a11 = V[o.c11 * 3 + c];
+            v[c] = (1.0 - dr) * ((1.0 - dt) * a00 + dt * a01) +
+                   dr * ((1.0 - dt) * ab + dt * a11);
+        }
     }
     if (mu > 1.0) set_err(err, 393);
     o.u = 3.335668e-11 * (mu * v[0] + sqrt(1.0 - mu * mu) *
@@ -238,13 +245,15 @@ __device__ __forceinline__ GeoNode geo_node(GridView g, CellView cv, int k, doub
 // ---------------------------------------------------------------------
 // a cell field at a node by the stencil of get_line_dust_values
 // (line.F:4006-4037), as node_interp forms it
+// (two cells for a sphere or cone crossing, four for an extra point; the
+// loads follow, so that a wave of mixed nodes loads two cells per lane)
 __device__ inline double interp_field(const GeoNode &gn, const double *__restrict__ f) {
     const double dr = gn.dr, dt = gn.dt;
     const int icr = gn.flag & 3;
-    if (icr == 1) return (1.0 - dt) * f[gn.c00] + dt * f[gn.c01];
-    if (icr == 2) return (1.0 - dr) * f[gn.c00] + dr * f[gn.c10];
-    return (1.0 - dr) * ((1.0 - dt) * f[gn.c00] + dt * f[gn.c01]) +
-           dr * ((1.0 - dt) * f[gn.c10] + dt * f[gn.c11]);
+    const double a = f[gn.c00], b = f[icr == 1 ? gn.c01 : gn.c10];
+    if (icr == 1) return (1.0 - dt) * a + dt * b;
+    if (icr == 2) return (1.0 - dr) * a + dr * b;
+    return (1.0 - dr) * ((1.0 - dt) * a + dt * f[gn.c01]) + dr * ((1.0 - dt) * b + dt * f[gn.c11]);
 }
 
 // The line-independent records of the segment ending at node gn, from the
@@ -380,15 +389,21 @@ __global__ void k_cells(int ncell, int nlev, int nlines, int line, int lev_up,
 // table {jd, ad, lw, N_up, N_low} at one node
 __device__ inline void node_interp(const GeoNode &gn, const CellRec *__restrict__ cells,
                                    double v[5]) {
-    const CellRec c00 = cells[gn.c00], c01 = cells[gn.c01];
-    const CellRec c10 = cells[gn.c10], c11 = cells[gn.c11];
+    // a sphere crossing interpolates in theta between c00 and c01, a cone
+    // crossing in r between c00 and c10: two cells loaded per lane; only an
+    // extra point needs c01 and c11 besides
     const double dr = gn.dr, dt = gn.dt;
     const int icr = gn.flag & 3;
-    for (int f = 0; f < 5; f++) {
-        if (icr == 1) v[f] = (1.0 - dt) * c00.v[f] + dt * c01.v[f];
-        else if (icr == 2) v[f] = (1.0 - dr) * c00.v[f] + dr * c10.v[f];
-        else v[f] = (1.0 - dr) * ((1.0 - dt) * c00.v[f] + dt * c01.v[f]) +
-                    dr * ((1.0 - dt) * c10.v[f] + dt * c11.v[f]);
+    const CellRec c00 = cells[gn.c00];
+    const CellRec cb = cells[icr == 1 ? gn.c01 : gn.c10];
+    if (icr == 1 || icr == 2) {
+        const double w = icr == 1 ? dt : dr;
+        for (int f = 0; f < 5; f++) v[f] = (1.0 - w) * c00.v[f] + w * cb.v[f];
+    } else {
+        const CellRec c01 = cells[gn.c01], c11 = cells[gn.c11];
+        for (int f = 0; f < 5; f++)
+            v[f] = (1.0 - dr) * ((1.0 - dt) * c00.v[f] + dt * c01.v[f]) +
+                   dr * ((1.0 - dt) * cb.v[f] + dt * c11.v[f]);
     }
 }
 
