@@ -109,6 +109,8 @@ struct rl_ctx {
     bool have_medium = false;
     double umass_av = 2.3, aksmax = 0.0;
     DevBuf d_rho, d_abund, d_vel, d_lw;
+    DevBuf d_vlw, d_xz;                    // velocity and width per cell, x0 and z0 per ray,
+                                           // packed per node pass
     int nlev = 0;
     DevBuf d_pops;
     // lines
@@ -891,13 +893,17 @@ static int build_traj_keep(rl_ctx *c, int idx, const std::vector<int> &keep) {
     if (seg) {
         HIPCHK(c, ensure_spare(c, rs.d_hot, (size_t)rs.nnodes * sizeof(rl::SegHot)));
         HIPCHK(c, ensure_spare(c, rs.d_aux, (size_t)rs.nnodes * sizeof(rl::SegAux)));
+        HIPCHK(c, c->d_vlw.ensure((size_t)c->nr * c->nh * sizeof(rl::VelLw)));
+        HIPCHK(c, c->d_xz.ensure((size_t)rs.nrays * sizeof(rl::RayXZ)));
     }
     rl::launch_geo(c->stream, g, c->d_vel.as<double>(), c->nh, rs.nrays, nmax,
                    rs.d_off.as<long long>(), rs.d_x0.as<double>(), rs.d_z0.as<double>(),
                    rs.costh0, rs.sinth0, cr, rs.d_nodes.as<GeoNode>(),
                    c->d_err.as<int>(), rs.nnodes, seg ? c->d_lw.as<double>() : nullptr,
                    seg ? rs.d_hot.as<rl::SegHot>() : nullptr,
-                   seg ? rs.d_aux.as<rl::SegAux>() : nullptr);
+                   seg ? rs.d_aux.as<rl::SegAux>() : nullptr,
+                   seg ? c->d_vlw.as<rl::VelLw>() : nullptr,
+                   seg ? c->d_xz.as<rl::RayXZ>() : nullptr);
     HIPCHK(c, hipGetLastError());
     if (rs.rect && !rs.star_rays.empty())
         rl::launch_star_flags(c->stream, (int)rs.star_rays.size(), rs.d_star_rays.as<int>(),

@@ -25,6 +25,10 @@ struct GeoNode {
 };
 
 enum { CR_JD = 0, CR_AD = 1, CR_LW = 2, CR_NU = 3, CR_NL = 4 };
+// velocity and line width of a cell, and a ray's start position, packed
+// for the FP32 node pass
+struct alignas(16) VelLw { double vx, vy, vz, lw; };
+struct alignas(16) RayXZ { double x0, z0; };
 struct CellRec { double v[5]; };
 
 struct RayInfo {
@@ -197,8 +201,10 @@ void launch_geo(hipStream_t st, const GridDev &g, const double *vel, int nh, int
                 int nmax, const long long *off, const double *x0, const double *z0,
                 double costh0, double sinth0, const CrossRec *cr,
                 GeoNode *nodes, int *err, long long nnodes = 0, const double *lw = nullptr,
-                SegHot *hot = nullptr,
-                SegAux *aux = nullptr);   // hot != null: the FP32 segment records too
+                SegHot *hot = nullptr, SegAux *aux = nullptr,
+                VelLw *vlw = nullptr,     // hot != null: the FP32 segment records too, the
+                RayXZ *xz = nullptr);     // velocities and widths packed into vlw [cell],
+                                          // the rays' x0, z0 into xz [ray]
 void launch_cells(hipStream_t st, int ncell, int nlev, int nlines, int line, int lev_up,
                   int lev_low, double molpg, const double *rho, const double *abund,
                   const double *lw, const double *pops, const double *jd, const double *ad,

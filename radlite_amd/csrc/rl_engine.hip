@@ -40,6 +40,7 @@ namespace rl {
 struct CellView {
     int nh;                   // nth_half
     const double *vel;        // [cell][3]
+    const VelLw *vlw;         // [cell]: velocity and width together (the FP32 node pass)
     __device__ int cell(int ir, int it) const { return (ir - 1) * nh + (it - 1); }
 };
 
@@ -137,9 +138,15 @@ k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, i
 // local direction (telescope.F:3682-3718) and projected velocity
 // (get_line_dust_values + omega_dot_v, line.F:4039-4208, 2650-2662),
 // compacted from the per-ray slots into the CSR node array.
+// PACKED: the velocity and the width both from cv.vlw (two 16-byte loads
+// per cell instead of four 8-byte ones), the width interpolated into *lwk
+// as interp_field forms it; ds is left to the caller, which has the
+// previous node's s (*s_out: this node's).
+template <bool PACKED>
 __device__ __forceinline__ GeoNode geo_node(GridView g, CellView cv, int k, double x0, double z0,
                                             double costh0, double sinth0,
-                                            const CrossRec *__restrict__ cr, int *err) {
+                                            const CrossRec *__restrict__ cr, int *err,
+                                            double *lwk = nullptr, double *s_out = nullptr) {
     const double pi = PICONST, epsplus = 1.0e1 * TELESC_EPS;
     const int nr = g.nr, ny = g.ny;
     const double sinth02 = sinth0 * sinth0, costh02 = costh0 * costh0;
@@ -213,24 +220,46 @@ __device__ __forceinline__ GeoNode geo_node(GridView g, CellView cv, int k, doub
     // the velocity at the node: a sphere or cone crossing interpolates
     // between two cells (in theta, resp. r), an extra point between four
     double v[3];
-    const double *V = cv.vel;
     const bool two = icross == 1 || icross == 2;
     const int cb = icross == 1 ? o.c01 : o.c10;
     const double w = icross == 1 ? dt : dr;
-    for (int c = 0; c < 3; c++) {
-        const double a00 = V[o.c00 * 3 + c], ab = V[cb * 3 + c];
+    if constexpr (PACKED) {
+        const VelLw p00 = cv.vlw[o.c00], pb = cv.vlw[cb];
+        const double f00[4] = {p00.vx, p00.vy, p00.vz, p00.lw};
+        const double fb[4] = {pb.vx, pb.vy, pb.vz, pb.lw};
+        double f[4];
         if (two) {
-            v[c] = (1.0 - w) * a00 + w * ab;
+            for (int q = 0; q < 4; q++) f[q] = (1.0 - w) * f00[q] + w * fb[q];
         } else {
-            const double a01 = V[o.c01 * 3 + c], a11 = V[o.c11 * 3 + c];
-            v[c] = (1.0 - dr) * ((1.0 - dt) * a00 + dt * a01) +
-                   dr * ((1.0 - dt) * ab + dt * a11);
+            const VelLw p01 = cv.vlw[o.c01], p11 = cv.vlw[o.c11];
+            const double f01[4] = {p01.vx, p01.vy, p01.vz, p01.lw};
+            const double f11[4] = {p11.vx, p11.vy, p11.vz, p11.lw};
+            for (int q = 0; q < 4; q++)
+                f[q] = (1.0 - dr) * ((1.0 - dt) * f00[q] + dt * f01[q]) +
+                       dr * ((1.0 - dt) * fb[q] + dt * f11[q]);
+        }
+        v[0] = f[0]; v[1] = f[1]; v[2] = f[2];
+        *lwk = f[3];
+    } else {
+        const double *V = cv.vel;
+        for (int c = 0; c < 3; c++) {
+            const double a00 = V[o.c00 * 3 + c], ab = V[cb * 3 + c];
+            if (two) {
+                v[c] = (1.0 - w) * a00 + w * ab;
+            } else {
+                const double a01 = V[o.c01 * 3 + c], a11 = V[o.c11 * 3 + c];
+                v[c] = (1.0 - dr) * ((1.0 - dt) * a00 + dt * a01) +
+                       dr * ((1.0 - dt) * ab + dt * a11);
+            }
         }
     }
     if (mu > 1.0) set_err(err, 393);
     o.u = 3.335668e-11 * (mu * v[0] + sqrt(1.0 - mu * mu) *
                                          (v[1] * sin_phi + v[2] * cos_phi));
-    if (k == 0) {
+    if constexpr (PACKED) {
+        o.ds = 0.0;
+        *s_out = s;
+    } else if (k == 0) {
         o.ds = 0.0;
     } else {
         o.ds = s - cr[k - 1].s;
@@ -326,7 +355,7 @@ k_geo(GridView g, CellView cv, int nmax, const long long *__restrict__ off,
     }
     const int k = blockIdx.y * blockDim.x + threadIdx.x;
     if (k >= cnt) return;
-    nodes_all[o + k] = geo_node(g, cv, k, x0a[ray], z0a[ray], costh0, sinth0,
+    nodes_all[o + k] = geo_node<false>(g, cv, k, x0a[ray], z0a[ray], costh0, sinth0,
                                 cr_all + (long long)ray * nmax, err);
 }
 
@@ -341,12 +370,10 @@ k_geo(GridView g, CellView cv, int nmax, const long long *__restrict__ off,
 template <bool GRID_LDS>
 __global__ void __launch_bounds__(256)
 k_geo_seg(GridView g, CellView cv, int nmax, int nrays, long long nnodes,
-          const long long *__restrict__ off, const double *__restrict__ x0a,
-          const double *__restrict__ z0a, double costh0, double sinth0,
-          const CrossRec *__restrict__ cr_all, GeoNode *__restrict__ nodes_all,
-          const double *__restrict__ lw,
+          const long long *__restrict__ off, const RayXZ *__restrict__ xz, double costh0,
+          double sinth0, const CrossRec *__restrict__ cr_all, GeoNode *__restrict__ nodes_all,
           SegHot *__restrict__ hot_all, SegAux *__restrict__ aux_all, int *err) {
-    __shared__ double s_u[256], s_lw[256];
+    __shared__ double s_u[256], s_lw[256], s_s[256];
     __shared__ int s_ray0;
     extern __shared__ double lds_tab[];
     const int t = threadIdx.x;
@@ -371,20 +398,26 @@ k_geo_seg(GridView g, CellView cv, int nmax, int nrays, long long nnodes,
     const long long gnode = base + t;
     const bool valid = gnode >= 0 && gnode < nnodes;
     GeoNode gn;
-    double lwk = 0.0;
+    double lwk = 0.0, sk = 0.0;
     int k = 0;
     if (valid) {
         int ray = s_ray0;
         while (off[ray + 1] <= gnode) ray++;
         k = (int)(gnode - off[ray]);
-        gn = geo_node(g, cv, k, x0a[ray], z0a[ray], costh0, sinth0,
-                      cr_all + (long long)ray * nmax, err);
-        lwk = interp_field(gn, lw);
+        const RayXZ p = xz[ray];
+        gn = geo_node<true>(g, cv, k, p.x0, p.z0, costh0, sinth0,
+                            cr_all + (long long)ray * nmax, err, &lwk, &sk);
         s_u[t] = gn.u;
         s_lw[t] = lwk;
+        s_s[t] = sk;
     }
     __syncthreads();
     if (!valid || t == 0) return;
+    // the path step from the previous node, thread t-1's (the same ray when k > 0)
+    if (k > 0) {
+        gn.ds = sk - s_s[t - 1];
+        if (gn.ds < 0.0) set_err(err, E_DSNEG);
+    }
     nodes_all[gnode] = gn;
     seg_record(gn, k == 0, s_u[t - 1], s_lw[t - 1], lwk, hot_all[gnode], aux_all[gnode]);
 }
@@ -915,22 +948,36 @@ void launch_cross(hipStream_t st, const GridDev &gd, const double *tan2, const C
     hipLaunchKernelGGL(k_cross, grd, blk, in_lds ? lds : 0, st, g, tan2, sets, nmax, err, in_lds);
 }
 
+// velocity [cell][3] and width [cell] into one record per cell; the rays'
+// x0 and z0 into one record per ray
+__global__ void k_pack_vlw(int ncell, const double *__restrict__ vel,
+                           const double *__restrict__ lw, VelLw *__restrict__ out, int nrays,
+                           const double *__restrict__ x0, const double *__restrict__ z0,
+                           RayXZ *__restrict__ xz) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncell) out[c] = VelLw{vel[3 * c], vel[3 * c + 1], vel[3 * c + 2], lw[c]};
+    if (c < nrays) xz[c] = RayXZ{x0[c], z0[c]};
+}
+
 void launch_geo(hipStream_t st, const GridDev &gd, const double *vel, int nh, int nrays,
                 int nmax, const long long *off, const double *x0, const double *z0,
                 double costh0, double sinth0, const CrossRec *cr,
                 GeoNode *nodes, int *err, long long nnodes, const double *lw, SegHot *hot,
-                SegAux *aux) {
+                SegAux *aux, VelLw *vlw, RayXZ *xz) {
     GridView g{gd.nr, gd.ny, gd.rc, gd.tc, gd.rstar, gd.t_sorted};
-    CellView cv{nh, vel};
+    CellView cv{nh, vel, vlw};
     const size_t tab = (size_t)(gd.nr + 4 + gd.ny + 4) * sizeof(double);
     if (hot) {
+        const int ncell = gd.nr * nh, npk = max(ncell, nrays);
+        hipLaunchKernelGGL(k_pack_vlw, dim3((npk + 255) / 256), dim3(256), 0, st, ncell, vel,
+                           lw, vlw, nrays, x0, z0, xz);
         const dim3 grd((unsigned)((nnodes + 1 + 254) / 255));
         if (tab <= GEO_GRID_LDS_MAX)
             hipLaunchKernelGGL(k_geo_seg<true>, grd, dim3(256), tab, st, g, cv, nmax, nrays,
-                               nnodes, off, x0, z0, costh0, sinth0, cr, nodes, lw, hot, aux, err);
+                               nnodes, off, xz, costh0, sinth0, cr, nodes, hot, aux, err);
         else
             hipLaunchKernelGGL(k_geo_seg<false>, grd, dim3(256), 0, st, g, cv, nmax, nrays,
-                               nnodes, off, x0, z0, costh0, sinth0, cr, nodes, lw, hot, aux, err);
+                               nnodes, off, xz, costh0, sinth0, cr, nodes, hot, aux, err);
     } else {
         const dim3 grd(nrays, (nmax + 255) / 256);
         if (tab <= GEO_GRID_LDS_MAX)
