@@ -286,15 +286,19 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
     }
 
     // extra points (telescope.F:3417-3606)
-    double ex_s[EXMAX + 1], ex_r[EXMAX + 1];
-    int ex_ir[EXMAX + 1], ex_it[EXMAX + 1];
+    // an extra point is its s and a code (idx + 1) << 1 | rext; its r is
+    // formed again where the merge reaches it (the impact parameter for the
+    // R-extremum, rext; thr(s) for every other point, as it was formed
+    // here), so that the sort below moves two arrays, not four
+    double ex_s[EXMAX + 1];
+    int ex_c[EXMAX + 1];
     int nex = 0;
     if (bimpact > g.R(1)) {
         double s2 = 0.0 - z0 * costh0;
         if (nex + 1 + 4 * RAYADPT > EXMAX) return -E_EXOVF;
         nex++;
-        ex_s[nex] = s2; ex_r[nex] = bimpact;
-        ex_ir[nex] = t.ir_min - 1; ex_it[nex] = 1;
+        ex_s[nex] = s2;
+        ex_c[nex] = (t.ir_min << 1) | 1;           // idx ir_min - 1, rext
         double s3 = s2;
         int isrt = hunt_bisect([&](int i) { return t.r_s(i); }, t.ir_amount, s2);
         for (int irng = 1; irng <= 4; irng++) {
@@ -307,8 +311,8 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
             for (int iad = 1; iad <= RAYADPT; iad++) {
                 double s0 = iad * ds + s1;
                 nex++;
-                ex_s[nex] = s0; ex_r[nex] = t.thr(s0);
-                ex_ir[nex] = 0; ex_it[nex] = 0;
+                ex_s[nex] = s0;
+                ex_c[nex] = 1 << 1;
             }
         }
     }
@@ -320,8 +324,8 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
             if (ixh == 0 || ixh == nr) return -E_HUNT;
             if (nex + 1 + 4 * RAYADPT > EXMAX) return -E_EXOVF;
             nex++;
-            ex_s[nex] = s2; ex_r[nex] = rr;
-            ex_ir[nex] = 0; ex_it[nex] = 0;
+            ex_s[nex] = s2;                         // r = rr = thr(s2)
+            ex_c[nex] = 1 << 1;
             double s3 = s2;
             int isrt = hunt_bisect([&](int i) { return t.th_s(i); }, t.ith_amount, s3);
             if (!(isrt - RAYRNPT + 1 < 1 || isrt + RAYRNPT > t.ith_amount)) {
@@ -339,8 +343,8 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
                         int ixx = t.hunt_r(r2);
                         if (ixx == 0 || ixx == nr) continue;        // :3580
                         nex++;
-                        ex_s[nex] = s0; ex_r[nex] = r2;
-                        ex_ir[nex] = 0; ex_it[nex] = 0;
+                        ex_s[nex] = s0;             // r = r2 = thr(s0)
+                        ex_c[nex] = 1 << 1;
                     }
                 }
             }
@@ -348,14 +352,14 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
     }
     // ray_sort (nrecip.F:833): ascending in s, insertion sort (<= 34)
     for (int i = 2; i <= nex; i++) {
-        double ks = ex_s[i], kr = ex_r[i];
-        int ki = ex_ir[i], kj = ex_it[i], j;
+        double ks = ex_s[i];
+        int kc = ex_c[i], j;
         for (j = i - 1; j >= 1 && ex_s[j] > ks; j--) {
-            ex_s[j + 1] = ex_s[j]; ex_r[j + 1] = ex_r[j];
-            ex_ir[j + 1] = ex_ir[j]; ex_it[j + 1] = ex_it[j];
+            ex_s[j + 1] = ex_s[j];
+            ex_c[j + 1] = ex_c[j];
         }
-        ex_s[j + 1] = ks; ex_r[j + 1] = kr;
-        ex_ir[j + 1] = ki; ex_it[j + 1] = kj;
+        ex_s[j + 1] = ks;
+        ex_c[j + 1] = kc;
     }
 
     // merge (telescope.F:3610-3677)
@@ -381,9 +385,11 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
         for (;;) {
             double m = ths < rs ? ths : rs;
             if (!(es < m)) break;
-            if (ex_r[isex] <= rmaxt && ex_r[isex] >= rmint &&
-                (es - sprev) > eps * ex_r[isex] && es >= sbeg && es <= send)
-                emit(3, es, ex_ir[isex], ex_it[isex]);
+            const int code = ex_c[isex];
+            const double er = (code & 1) ? t.bimpact : t.thr(es);
+            if (er <= rmaxt && er >= rmint && (es - sprev) > eps * er && es >= sbeg &&
+                es <= send)
+                emit(3, es, (code >> 1) - 1, code & 1);
             isex++;
             es = isex <= nex ? ex_s[isex] : 1.0e30;
         }
