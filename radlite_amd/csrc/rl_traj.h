@@ -285,81 +285,116 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
         if (b * b - 4.0 * c + eps * b * b < 0.0) return -E_INPUT;
     }
 
-    // extra points (telescope.F:3417-3606)
-    // an extra point is its s and a code (idx + 1) << 1 | rext; its r is
-    // formed again where the merge reaches it (the impact parameter for the
-    // R-extremum, rext; thr(s) for every other point, as it was formed
-    // here), so that the sort below moves two arrays, not four
+    // extra points (telescope.F:3417-3606).  An extra point is its s and a
+    // code (idx + 1) << 1 | rext; its r is formed again where the merge
+    // reaches it (the impact parameter for the R-extremum, rext; thr(s) for
+    // every other point, as it was formed here).
+    // The reference sorts them (ray_sort, nrecip.F:833: an insertion sort,
+    // stable).  Each group -- the R-extremum and the 4 x RAYADPT points of
+    // its four intervals, then the theta-extremum and its points -- is
+    // written here already in that order: with the intervals' ends in
+    // order, the points of interval 4 lie below those of 2, then the
+    // extremum, then 1, then 3, each interval's points ascending.  Points of
+    // equal s and equal code are interchangeable; the R-extremum (rext) goes
+    // before the points of interval 2 that equal it, as the stable sort puts
+    // it.  A first group whose interval ends are not in that order (the
+    // outermost rings, whose lowest interval starts at the r_s(0) = 0 of the
+    // reference build) is sorted as the reference sorts it.  The merge below
+    // takes the two sorted groups in turn, the first group's point first on
+    // equal s, which is the order of the stable sort of both.
     double ex_s[EXMAX + 1];
     int ex_c[EXMAX + 1];
-    int nex = 0;
-    if (bimpact > g.R(1)) {
-        double s2 = 0.0 - z0 * costh0;
-        if (nex + 1 + 4 * RAYADPT > EXMAX) return -E_EXOVF;
+    int nex = 0, n1 = 0;
+    auto put = [&](double v, int c) {
         nex++;
-        ex_s[nex] = s2;
-        ex_c[nex] = (t.ir_min << 1) | 1;           // idx ir_min - 1, rext
-        double s3 = s2;
-        int isrt = hunt_bisect([&](int i) { return t.r_s(i); }, t.ir_amount, s2);
-        for (int irng = 1; irng <= 4; irng++) {
-            double s1;
-            if (irng == 1) { s1 = s3; s2 = t.r_s(isrt + 1); }
-            else if (irng == 2) { s1 = t.r_s(isrt); s2 = s3; }
-            else if (irng == 3) { s1 = t.r_s(isrt + 1); s2 = t.r_s(isrt + 2); }
-            else { s1 = t.r_s(isrt - 1); s2 = t.r_s(isrt); }
-            double ds = (s2 - s1) / (1.0 + 1.0 * RAYADPT);
-            for (int iad = 1; iad <= RAYADPT; iad++) {
-                double s0 = iad * ds + s1;
-                nex++;
-                ex_s[nex] = s0;
-                ex_c[nex] = 1 << 1;
+        ex_s[nex] = v;
+        ex_c[nex] = c;
+    };
+    if (bimpact > g.R(1)) {
+        const double s3 = 0.0 - z0 * costh0;
+        if (nex + 1 + 4 * RAYADPT > EXMAX) return -E_EXOVF;
+        const int code_ext = (t.ir_min << 1) | 1;          // idx ir_min - 1, rext
+        const int isrt = hunt_bisect([&](int i) { return t.r_s(i); }, t.ir_amount, s3);
+        // the intervals irng = 1..4 of the reference: (s3, rB), (rA, s3), (rB, rC), (rZ, rA)
+        const double rZ = t.r_s(isrt - 1), rA = t.r_s(isrt), rB = t.r_s(isrt + 1),
+                     rC = t.r_s(isrt + 2);
+        auto points = [&](double s1, double s2) {
+            const double ds = (s2 - s1) / (1.0 + 1.0 * RAYADPT);
+            for (int iad = 1; iad <= RAYADPT; iad++) put(iad * ds + s1, 1 << 1);
+        };
+        if (isrt >= 2 && isrt + 2 <= t.ir_amount && rZ <= rA && rA < s3 && s3 <= rB &&
+            rB <= rC) {
+            points(rZ, rA);                                 // irng 4
+            {                                               // irng 2, split at s3
+                const double ds = (s3 - rA) / (1.0 + 1.0 * RAYADPT);
+                int iad = 1;
+                for (; iad <= RAYADPT; iad++) {
+                    const double s0 = iad * ds + rA;
+                    if (!(s0 < s3)) break;
+                    put(s0, 1 << 1);
+                }
+                put(s3, code_ext);
+                for (; iad <= RAYADPT; iad++) put(iad * ds + rA, 1 << 1);
+            }
+            points(s3, rB);                                 // irng 1
+            points(rB, rC);                                 // irng 3
+        } else {
+            put(s3, code_ext);
+            points(s3, rB);
+            points(rA, s3);
+            points(rB, rC);
+            points(rZ, rA);
+            // ray_sort (nrecip.F:833): ascending in s, insertion sort
+            for (int i = 2; i <= nex; i++) {
+                const double ks = ex_s[i];
+                const int kc = ex_c[i];
+                int j;
+                for (j = i - 1; j >= 1 && ex_s[j] > ks; j--) {
+                    ex_s[j + 1] = ex_s[j];
+                    ex_c[j + 1] = ex_c[j];
+                }
+                ex_s[j + 1] = ks;
+                ex_c[j + 1] = kc;
             }
         }
     }
+    n1 = nex;
     {
-        double s2 = x0 * x0 * costh0 / (z0 * sinth02);
-        double rr = t.thr(s2);
+        const double s3 = x0 * x0 * costh0 / (z0 * sinth02);
+        const double rr = t.thr(s3);
         if (rr > g.R(1) && rr < g.R(nr)) {
             int ixh = t.hunt_r(rr);
             if (ixh == 0 || ixh == nr) return -E_HUNT;
             if (nex + 1 + 4 * RAYADPT > EXMAX) return -E_EXOVF;
-            nex++;
-            ex_s[nex] = s2;                         // r = rr = thr(s2)
-            ex_c[nex] = 1 << 1;
-            double s3 = s2;
-            int isrt = hunt_bisect([&](int i) { return t.th_s(i); }, t.ith_amount, s3);
+            const int isrt = hunt_bisect([&](int i) { return t.th_s(i); }, t.ith_amount, s3);
             if (!(isrt - RAYRNPT + 1 < 1 || isrt + RAYRNPT > t.ith_amount)) {
-                for (int irng = 1; irng <= 4; irng++) {
-                    double s1;
-                    if (irng == 1) { s1 = s3; s2 = t.th_s(isrt + 1); }
-                    else if (irng == 2) { s1 = t.th_s(isrt); s2 = s3; }
-                    else if (irng == 3) { s1 = t.th_s(isrt + 1); s2 = t.th_s(isrt + 2); }
-                    else { s1 = t.th_s(isrt - 1); s2 = t.th_s(isrt); }
-                    if (s2 == s1) return -E_EXEQ;
-                    if (s2 < s1) return -E_EXORD;
-                    double ds = (s2 - s1) / (1.0 + 1.0 * RAYADPT);
-                    for (int iad = 1; iad <= RAYADPT; iad++) {
-                        double s0 = iad * ds + s1, r2 = t.thr(s0);
-                        int ixx = t.hunt_r(r2);
-                        if (ixx == 0 || ixx == nr) continue;        // :3580
-                        nex++;
-                        ex_s[nex] = s0;             // r = r2 = thr(s0)
-                        ex_c[nex] = 1 << 1;
-                    }
+                const double tZ = t.th_s(isrt - 1), tA = t.th_s(isrt), tB = t.th_s(isrt + 1),
+                             tC = t.th_s(isrt + 2);
+                // the reference's checks, in its interval order irng = 1..4;
+                // passed, they put the ends in order: tZ < tA < s3 < tB < tC
+                const double e1[4] = {s3, tA, tB, tZ}, e2[4] = {tB, s3, tC, tA};
+                for (int q = 0; q < 4; q++) {
+                    if (e2[q] == e1[q]) return -E_EXEQ;
+                    if (e2[q] < e1[q]) return -E_EXORD;
                 }
+                auto points = [&](double s1, double s2) {
+                    const double ds = (s2 - s1) / (1.0 + 1.0 * RAYADPT);
+                    for (int iad = 1; iad <= RAYADPT; iad++) {
+                        const double s0 = iad * ds + s1, r2 = t.thr(s0);
+                        const int ixx = t.hunt_r(r2);
+                        if (ixx == 0 || ixx == nr) continue;        // :3580
+                        put(s0, 1 << 1);                            // r = r2 = thr(s0)
+                    }
+                };
+                points(tZ, tA);                                     // irng 4
+                points(tA, s3);                                     // irng 2
+                put(s3, 1 << 1);                                    // r = rr = thr(s3)
+                points(s3, tB);                                     // irng 1
+                points(tB, tC);                                     // irng 3
+            } else {
+                put(s3, 1 << 1);
             }
         }
-    }
-    // ray_sort (nrecip.F:833): ascending in s, insertion sort (<= 34)
-    for (int i = 2; i <= nex; i++) {
-        double ks = ex_s[i];
-        int kc = ex_c[i], j;
-        for (j = i - 1; j >= 1 && ex_s[j] > ks; j--) {
-            ex_s[j + 1] = ex_s[j];
-            ex_c[j + 1] = ex_c[j];
-        }
-        ex_s[j + 1] = ks;
-        ex_c[j + 1] = kc;
     }
 
     // merge (telescope.F:3610-3677)
@@ -367,7 +402,7 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
     const double rmint = g.R(1) * (1.0 + epsplus), rminr = g.R(1) * (1.0 - epsplus);
     const double sbeg = -1.0e30;
     double sprev = -1.0e30;
-    int nout = 0, isex = 1, ist = 1, isr = 1;
+    int nout = 0, ist = 1, isr = 1;
     double ths, rs, rr_;
     int thit, rix;
     t.th_entry(ist, ths, thit);
@@ -380,18 +415,27 @@ __device__ int trajectory(Traj &t, Sink &&sink) {
         sprev = s;
     };
     const int tr_total = t.ir_amount + t.ith_amount;
-    double es = nex >= 1 ? ex_s[1] : 1.0e30;       // the next extra point, kept in a register
+    // the next point of each group of extra points, kept in registers
+    int i1 = 1, i2 = n1 + 1;
+    double es1 = n1 >= 1 ? ex_s[1] : 1.0e30, es2 = nex > n1 ? ex_s[n1 + 1] : 1.0e30;
     for (int iss = 1; iss <= tr_total; iss++) {
         for (;;) {
+            const bool second = es2 < es1;
+            const double es = second ? es2 : es1;
             double m = ths < rs ? ths : rs;
             if (!(es < m)) break;
-            const int code = ex_c[isex];
+            const int code = ex_c[second ? i2 : i1];
             const double er = (code & 1) ? t.bimpact : t.thr(es);
             if (er <= rmaxt && er >= rmint && (es - sprev) > eps * er && es >= sbeg &&
                 es <= send)
                 emit(3, es, (code >> 1) - 1, code & 1);
-            isex++;
-            es = isex <= nex ? ex_s[isex] : 1.0e30;
+            if (second) {
+                i2++;
+                es2 = i2 <= nex ? ex_s[i2] : 1.0e30;
+            } else {
+                i1++;
+                es1 = i1 <= n1 ? ex_s[i1] : 1.0e30;
+            }
         }
         // the nearer of the next cone and the next sphere crossing
         const bool th = ths < rs;
