@@ -15,14 +15,18 @@ struct GridDev {
                               // theta cell from the crossing pass's guess
 };
 
-// Channel-independent geometry of one trajectory node (FP64).
+// Channel-independent geometry of one trajectory node (FP64), 48 bytes.
 struct GeoNode {
     double ds;                // s_k - s_{k-1} (0 for the first node)
     double u;                 // Omega.v/c at the node (line.F:2650-2662)
     double dr, dt;            // interpolation fractions (telescope.F:4088-4091)
-    int c00, c01, c10, c11;   // stencil cells (r0,t0) (r0,t1) (r1,t0) (r1,t1)
+    int c00, c01, c10;        // stencil cells (r0,t0) (r0,t1) (r1,t0)
     int flag;                 // icross | node flags << 2
+    // the fourth cell (r1,t1): cell(ir, it) = (ir-1) nh + (it-1) is linear,
+    // so cell(r1,t1) = cell(r1,t0) + cell(r0,t1) - cell(r0,t0)
+    __host__ __device__ int c11() const { return c10 + c01 - c00; }
 };
+static_assert(sizeof(GeoNode) == 48, "GeoNode layout");
 
 enum { CR_JD = 0, CR_AD = 1, CR_LW = 2, CR_NU = 3, CR_NL = 4 };
 // velocity and line width of a cell, and a ray's start position, packed

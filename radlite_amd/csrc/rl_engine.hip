@@ -203,7 +203,6 @@ __device__ __forceinline__ GeoNode geo_node(GridView g, CellView cv, int k, doub
     o.c00 = cv.cell(ir, t0);
     o.c01 = cv.cell(ir, t1);
     o.c10 = cv.cell(r1, t0);
-    o.c11 = cv.cell(r1, t1);
     o.dr = dr;
     o.dt = dt;
     // local direction, telescope.F:3682-3718
@@ -231,7 +230,7 @@ __device__ __forceinline__ GeoNode geo_node(GridView g, CellView cv, int k, doub
         if (two) {
             for (int q = 0; q < 4; q++) f[q] = (1.0 - w) * f00[q] + w * fb[q];
         } else {
-            const VelLw p01 = cv.vlw[o.c01], p11 = cv.vlw[o.c11];
+            const VelLw p01 = cv.vlw[o.c01], p11 = cv.vlw[o.c11()];
             const double f01[4] = {p01.vx, p01.vy, p01.vz, p01.lw};
             const double f11[4] = {p11.vx, p11.vy, p11.vz, p11.lw};
             for (int q = 0; q < 4; q++)
@@ -247,7 +246,7 @@ __device__ __forceinline__ GeoNode geo_node(GridView g, CellView cv, int k, doub
             if (two) {
                 v[c] = (1.0 - w) * a00 + w * ab;
             } else {
-                const double a01 = V[o.c01 * 3 + c], a11 = V[o.c11 * 3 + c];
+                const double a01 = V[o.c01 * 3 + c], a11 = V[o.c11() * 3 + c];
                 v[c] = (1.0 - dr) * ((1.0 - dt) * a00 + dt * a01) +
                        dr * ((1.0 - dt) * ab + dt * a11);
             }
@@ -282,7 +281,7 @@ __device__ inline double interp_field(const GeoNode &gn, const double *__restric
     const double a = f[gn.c00], b = f[icr == 1 ? gn.c01 : gn.c10];
     if (icr == 1) return (1.0 - dt) * a + dt * b;
     if (icr == 2) return (1.0 - dr) * a + dr * b;
-    return (1.0 - dr) * ((1.0 - dt) * a + dt * f[gn.c01]) + dr * ((1.0 - dt) * b + dt * f[gn.c11]);
+    return (1.0 - dr) * ((1.0 - dt) * a + dt * f[gn.c01]) + dr * ((1.0 - dt) * b + dt * f[gn.c11()]);
 }
 
 // The line-independent records of the segment ending at node gn, from the
@@ -459,7 +458,7 @@ __device__ inline void node_interp(const GeoNode &gn, const CellRec *__restrict_
         const double w = icr == 1 ? dt : dr;
         for (int f = 0; f < 5; f++) v[f] = (1.0 - w) * c00.v[f] + w * cb.v[f];
     } else {
-        const CellRec c01 = cells[gn.c01], c11 = cells[gn.c11];
+        const CellRec c01 = cells[gn.c01], c11 = cells[gn.c11()];
         for (int f = 0; f < 5; f++)
             v[f] = (1.0 - dr) * ((1.0 - dt) * c00.v[f] + dt * c01.v[f]) +
                    dr * ((1.0 - dt) * cb.v[f] + dt * c11.v[f]);
@@ -1642,7 +1641,7 @@ __global__ void k_stencil_entries(long long nnodes, const GeoNode *__restrict__ 
     const GeoNode gn = geo[k];
     double wk[4];
     stencil_weights(gn, wk);
-    const int c[4] = {gn.c00, gn.c01, gn.c10, gn.c11};
+    const int c[4] = {gn.c00, gn.c01, gn.c10, gn.c11()};
     for (int j = 0; j < 4; j++) {
         const long long e = 4 * k + j;
         keys[e] = wk[j] != 0.0 ? c[j] : ncell;
