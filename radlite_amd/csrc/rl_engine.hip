@@ -293,10 +293,9 @@ __device__ inline void seg_record(const GeoNode &gn, bool first, double up, doub
     SegAux a;
     a.u = (float)gn.u;
     a.flag = fl;
-    a.pad0 = a.pad1 = a.pad2 = 0;
     SegHot h;
     if (first) {
-        a.ds = 0.0f; a.dls = -1.0f; a.inv_du = 0.0f;
+        a.dls = -1.0f; a.inv_du = 0.0f;
         h.hds = -0.0f; h.ag = 0.0f; h.uag = 0.0f; h.ng = 0.0f;
     } else {
         // the segment's mean width (line.F:4704) and its profile scales
@@ -308,7 +307,6 @@ __device__ inline void seg_record(const GeoNode &gn, bool first, double up, doub
         const double du = gn.u - up;
         const double x = fabs(du / (lwav / 2.99792458e5));   // line.F:4706-4715
         const bool sub = 2.0 * 3.0 * x > 1.0;
-        a.ds = (float)ds;
         a.dls = sub ? (float)(ds / x) : -1.0f;
         a.inv_du = (float)(1.0 / du);
         const float hds = (float)(0.5 * ds);

@@ -36,13 +36,13 @@ struct alignas(16) SegHot {
     float ng;    // 0.56419583546 / (3.33567e-6 lwav)
 };
 
-struct SegAux {
-    float ds;      // s_k - s_{k-1} (0 on a hole chord)
+// (ds is SegHot's: 2 |hds| is the single-precision ds exactly, the
+// halving and doubling being exact in binary)
+struct alignas(16) SegAux {
     float u;       // Omega.v/c at node k
     float dls;     // sub-grid line length ds/x (line.F:4733-4738), < 0: never
     float inv_du;  // 1 / (u_k - u_{k-1})
     int flag;      // NF_* bits of node k
-    int pad0, pad1, pad2;
 };
 
 // j = |sig| (jdh + jcl ng e), alpha = |sig| (adh + dcl ng e); sig is the
@@ -135,7 +135,7 @@ __device__ inline float seg_general32(float I, const SegHot &h, const SegAux &a0
                                       const SegAux &a1, const NodeLine &q0, const NodeLine &q1,
                                       float uch, float istar, float sf, int &init, Car32 &c,
                                       bool &ovf, int &maser) {
-    float ds = a1.ds;
+    float ds = 2.0f * fabsf(h.hds);                     // (float)ds, hds = (float)(ds/2)
     const int fl = a1.flag;
     if (fl & NF_RESET) { ds = 0.0f; init = 1; }
     if (fl & NF_BLEND) { I = (1.0f - sf) * I + sf * istar; init = 1; }
