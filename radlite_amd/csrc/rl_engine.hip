@@ -56,12 +56,14 @@ struct CellView {
 // Written one 16-byte record per crossing, the lines of the ~34 MB of slots
 // being filled at once were evicted part-written: 4.8 times the records'
 // bytes were written, and about as many read back.  Eight records (whole
-// lines) would need 16 KB of LDS per block, and at ten blocks per CU (the
-// two inclinations' 2088 blocks resident at once) the grid tables leave
-// room for 8 KB.
+// lines) would need 16 KB of LDS per block, which with the grid tables
+// leaves too few blocks per CU.  Four waves per SIMD (128 registers) keep
+// the merge's state out of scratch; the few blocks that do not fit at once
+// are the last, the outermost rings' short rays, which start as the first
+// blocks finish (five waves, with per-step spill reloads, were slower).
 constexpr int CROSS_BLOCK = 128, CROSS_LINE = 4;
 
-__global__ void __launch_bounds__(CROSS_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(CROSS_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, int *err,
         int grid_in_lds) {
     __shared__ CrossRec stage[CROSS_LINE][CROSS_BLOCK];
@@ -97,9 +99,13 @@ k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, i
     Traj t(g, tan2, x0[ray], z0[ray], theta0, costh0, sinth0, scr_s1 + ray, scr_s2 + ray,
            scr_iy + ray, nrays);
     const long long base = (long long)ray * nmax;
-    int icr_prev = -1, idx_prev = -1, star_done = 0;
-    int last_r = 0, last_t = 0;                  // last sphere / cone crossed, plus one
-    const bool central = ray == 0 && central_blend;
+    // the sink's state, packed (the merge holds many values per lane):
+    // bit 0 of st, the previous crossing was on R_1; bit 1, the central
+    // beam's star blend is still to come; gs, the cell guesses of the next
+    // record (the last sphere and cone crossed, plus one, 16 bits each; 0:
+    // none or beyond 16 bits)
+    int st = ray == 0 && central_blend ? 2 : 0;
+    int gs = 0;
     const int tid = threadIdx.x;
     auto flush = [&](int k0, int m) {            // records k0 .. k0+m-1 from the stage
         for (int j = 0; j < m; j++)
@@ -107,24 +113,24 @@ k_cross(GridView g, const double *__restrict__ tan2, CrossSets sets, int nmax, i
     };
     auto sink = [&](int k, const Crossing &c) {
         if (k >= nmax) { set_err(err, E_CAP); return; }
+        const bool on_r1 = c.icross == 1 && c.idx == 1;
         int flag = 0;
         if (k > 0) {
             // hole chord (telescope.F:4095-4104): two consecutive crossings of R_1
-            if (c.icross == 1 && c.idx == 1 && icr_prev == 1 && idx_prev == 1) flag |= NF_RESET;
+            if (on_r1 && (st & 1)) flag |= NF_RESET;
             // central-beam star blend at the first node on R_1 (telescope.F:4144)
-            if (central && star_done == 0 && c.icross == 1 && c.idx == 1) {
+            if (on_r1 && (st & 2)) {
                 flag |= NF_BLEND;
-                star_done = 1;
+                st &= ~2;
             }
         }
-        const int guess = (last_r <= 0xFFFF ? last_r : 0) | ((last_t <= 0xFFFF ? last_t : 0) << 16);
         stage[k & (CROSS_LINE - 1)][tid] =
-            CrossRec{c.s, c.icross | (c.rext << 2) | (flag << 3) | (c.idx << 5), guess};
+            CrossRec{c.s, c.icross | (c.rext << 2) | (flag << 3) | (c.idx << 5), gs};
         if ((k & (CROSS_LINE - 1)) == CROSS_LINE - 1) flush(k - (CROSS_LINE - 1), CROSS_LINE);
-        icr_prev = c.icross;
-        idx_prev = c.idx;
-        if (c.icross == 1) last_r = c.idx + 1;
-        else if (c.icross == 2) last_t = c.idx + 1;
+        st = (st & ~1) | (on_r1 ? 1 : 0);
+        const int v = c.idx + 1 <= 0xFFFF ? c.idx + 1 : 0;
+        if (c.icross == 1) gs = (gs & (int)0xFFFF0000) | v;
+        else if (c.icross == 2) gs = (gs & 0xFFFF) | (v << 16);
     };
     int n = trajectory(t, sink);
     if (n < 0) { set_err(err, -n); n = 0; }
