@@ -45,6 +45,17 @@ struct alignas(16) SegAux {
     int flag;      // NF_* bits of node k
 };
 
+// one 16-byte load (read field by field, the record took four)
+__device__ inline SegAux load_aux(const SegAux *p) {
+    const float4 v = *reinterpret_cast<const float4 *>(p);
+    SegAux a;
+    a.u = v.x;
+    a.dls = v.y;
+    a.inv_du = v.z;
+    a.flag = __float_as_int(v.w);
+    return a;
+}
+
 // j = |sig| (jdh + jcl ng e), alpha = |sig| (adh + dcl ng e); sig is the
 // node's scale (its dust opacity where there is dust, adh = 1); the sign
 // bit of sig is set where there is no dust (adh = 0).  jcl and dcl are the
@@ -196,12 +207,12 @@ __device__ inline float trace_split(const SegHot *__restrict__ hot, const SegAux
     int init = 1;
     bool ovf = false;
     // segment 1 starts the ray: the general path reads its start point
-    I = seg_general32(I, hot[1], aux[0], aux[1], line[0], line[1], uch, istar, sf, init, c, ovf,
-                      maser);
+    I = seg_general32(I, hot[1], aux[0], load_aux(aux + 1), line[0], line[1], uch, istar, sf,
+                      init, c, ovf, maser);
     auto segment = [&](int k, const SegHot &h, const NodeLine &q) {
         if (seg_general(h.hds)) {
-            I = seg_general32(I, h, aux[k - 1], aux[k], line[k - 1], q, uch, istar, sf, init, c,
-                              ovf, maser);
+            I = seg_general32(I, h, aux[k - 1], load_aux(aux + k), line[k - 1], q, uch, istar, sf,
+                              init, c, ovf, maser);
         } else {
             const float z = fmaf(uch, h.ag, -h.uag);
             const float e = ex2(-(z * z));
